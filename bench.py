@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Time to lambda_max within 1e-3 relative error: AMM vs the exact power
+method on one MI355X.
+
+Workload: A in R^{16384 x 65536} FP32 (4 GiB), i.i.d. N(0,1) entries with
+each column scaled by a lognormal(0, 1.5) factor, plus a planted rank-8
+term (theta_k = 50 (sqrt d + sqrt n) 0.9^(k-1)).  Ground truth lambda* of
+A A^T: FP64 Lanczos on the GPU.
+
+A "step" is one complete power-method run (200 iterations, one sampling
+seed) of the AMM configuration; its time to tolerance is the cumulative
+device time (device wall clock, from the run's first kernel, including
+column statistics, CDF, z = A alpha and sampling) at the first iteration
+after which the Rayleigh quotient stays within 1e-3 of lambda*.  `value` is
+the median over the timed steps.  The FP32 exact power method on the same
+GPU is timed the same way and reported beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--sweep]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+METRIC = "time to λ_max within 1e-3 rel err; AMM vs exact power method, 1 MI355X"
+TOL = 1e-3
+HBM_PEAK_GBS = 8000.0
+
+# headline AMM configuration (chosen from the sweep)
+HEADLINE = dict(kernel="proposedAMM_v2", c=4096, vr_period=10)
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def settled_index(eig: np.ndarray, lam: float, tol: float = TOL) -> int:
+    """First i with |eig_j - lam| <= tol*lam for every j >= i (-1: never)."""
+    ok = np.abs(eig - lam) <= tol * lam
+    if not ok[-1]:
+        return -1
+    bad = np.nonzero(~ok)[0]
+    return 0 if bad.size == 0 else int(bad[-1] + 1)
+
+
+def first_index(eig: np.ndarray, lam: float, tol: float = TOL) -> int:
+    ok = np.nonzero(np.abs(eig - lam) <= tol * lam)[0]
+    return int(ok[0]) if ok.size else -1
+
+
+def run_config(ctx, RunParams, lam, seeds, step, **cfg):
+    out = []
+    for s in seeds:
+        r = ctx.power_method(RunParams(step=step, seed=s, **cfg))
+        i = settled_index(r.eig, lam)
+        f = first_index(r.eig, lam)
+        out.append(dict(seed=s, t_settled=(r.time_ms[i] if i >= 0 else None), i_settled=i,
+                        t_first=(r.time_ms[f] if f >= 0 else None), i_first=f, wall_ms=r.wall_ms,
+                        run_ms=float(r.time_ms[-1]), final_rel=float(abs(r.eig[-1] - lam) / lam)))
+    return out
+
+
+def summarize(rows):
+    ts = [r["t_settled"] for r in rows if r["t_settled"] is not None]
+    return dict(median_ms=(statistics.median(ts) if ts else None),
+                p10_ms=(float(np.percentile(ts, 10)) if ts else None),
+                p90_ms=(float(np.percentile(ts, 90)) if ts else None),
+                not_reached=1.0 - len(ts) / len(rows),
+                median_first_ms=statistics.median([r["t_first"] for r in rows if r["t_first"] is not None])
+                if any(r["t_first"] is not None for r in rows) else None,
+                median_iters=statistics.median([r["i_settled"] for r in rows if r["i_settled"] >= 0])
+                if ts else None,
+                median_run_ms=statistics.median([r["run_ms"] for r in rows]))
+
+
+def cpu_baseline(A_cols_host, d, lam, cfg, seed, iters_needed, budget_s=30.0):
+    """The oracle (single-thread C port of the same algorithm) on the host
+    cores: one run of the headline configuration, cut at the iteration where
+    the GPU run of the same seed settled (same samples, same trajectory)."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O
+    step = max(1, iters_needed + 1)
+    t0 = time.perf_counter()
+    r = O.power_method(A_cols_host, d, seed=seed, step=step, **cfg)
+    dt = time.perf_counter() - t0
+    rel_err = abs(r["eig"][-1] - lam) / lam
+    return dict(value=dt * 1e3, unit="ms", cores=1, kind="port",
+                sample=(f"oracle C power method ({cfg['kernel']}, c={cfg.get('c')}, "
+                        f"vr={cfg.get('vr_period')}), seed {seed}, {step} iterations on the full "
+                        f"16384x65536 matrix incl. column statistics; final rel err {rel_err:.2e}"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10, help="timed AMM runs (seeds)")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--iterations", type=int, default=200)
+    ap.add_argument("--exact-seeds", type=int, default=3)
+    ap.add_argument("--sweep", action="store_true", help="also sweep kernel x c x vr (stderr table)")
+    ap.add_argument("--sweep-seeds", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--d", type=int, default=16384)
+    ap.add_argument("--n", type=int, default=65536)
+    ap.add_argument("--kernel-iters", type=int, default=200)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the work would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from eigenvaluecomputation_by_amm_amd import AmmContext, RunParams
+    from eigenvaluecomputation_by_amm_amd.workloads import heavy_tailed, lambda_max_lanczos
+
+    d, n = args.d, args.n
+    log(f"generating {d}x{n} FP32 heavy-tailed matrix on {torch.cuda.get_device_name(0)}")
+    A = heavy_tailed(d, n)
+    torch.cuda.synchronize()
+    log("FP64 Lanczos ground truth")
+    lam, change = lambda_max_lanczos(A, d, k=64)
+    log(f"lambda* = {lam:.12e} (Lanczos change over last 8 steps {change:.1e})")
+    torch.cuda.synchronize()
+
+    ctx = AmmContext(A)
+    iters = args.iterations
+
+    # exact FP32 power method (comparator)
+    log("exact power method")
+    exact_rows = run_config(ctx, RunParams, lam, list(range(100, 100 + args.exact_seeds)), iters,
+                            kernel="exact")
+    ex = summarize(exact_rows)
+    log(f"exact: {ex}")
+
+    if args.sweep:
+        log("sweep kernel x c x vr")
+        for kern in ["previousAMM", "proposedAMM", "proposedAMM_v2"]:
+            for c in [1024, 4096, 16384]:
+                for vr in [0, 5, 10, 20]:
+                    if kern == "proposedAMM_v2" and vr == 0:
+                        continue
+                    rows = run_config(ctx, RunParams, lam, list(range(500, 500 + args.sweep_seeds)), iters,
+                                      kernel=kern, c=c, vr_period=vr)
+                    s = summarize(rows)
+                    log(f"sweep {kern:15s} c={c:6d} vr={vr:3d}: median {s['median_ms']} ms "
+                        f"(first-hit {s['median_first_ms']}) iters {s['median_iters']} "
+                        f"not_reached {s['not_reached']:.2f} run {s['median_run_ms']:.2f} ms "
+                        f"final_rel_med {statistics.median(r['final_rel'] for r in rows):.2e}")
+
+    cfg = dict(HEADLINE)
+    log(f"headline {cfg}: warmup {args.warmup}")
+    run_config(ctx, RunParams, lam, list(range(900, 900 + args.warmup)), iters, **cfg)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rows = run_config(ctx, RunParams, lam, list(range(1, 1 + args.steps)), iters, **cfg)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    am = summarize(rows)
+    log(f"headline: {am}")
+
+    # dominant kernel of the sampled product, HIP events on the context stream
+    kname = "proposedAMM" if cfg["kernel"] != "previousAMM" else "previousAMM"
+    ms_dot, ms_axpy, ucols = ctx.time_product(kname, cfg["c"], seed=1, iters=args.kernel_iters)
+    ex_dot, ex_axpy, _ = ctx.time_product("exact", 0, iters=20)
+    bytes_col = d * A.element_size()
+    dom = ("gather_dot", ms_dot) if ms_dot >= ms_axpy else ("gather_axpy", ms_axpy)
+    achieved = ucols * bytes_col / (dom[1] * 1e-3) / 1e9
+    log(f"kernels: gather_dot {ms_dot*1e3:.1f} us, gather_axpy {ms_axpy*1e3:.1f} us over {ucols} columns; "
+        f"exact dot {ex_dot*1e3:.1f} us axpy {ex_axpy*1e3:.1f} us "
+        f"({2*d*n*4/((ex_dot+ex_axpy)*1e-3)/1e9:.0f} GB/s)")
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel=f"k_gather_{'dot' if dom[0]=='gather_dot' else 'axpy'}<float> (list)",
+                bytes_per_launch=ucols * bytes_col, avg_launch_us=round(dom[1] * 1e3, 2),
+                other_kernel_us=round((ms_axpy if dom[0] == "gather_dot" else ms_dot) * 1e3, 2),
+                exact_gemv_pair_GBs=round(2 * d * n * 4 / ((ex_dot + ex_axpy) * 1e-3) / 1e9, 1))
+
+    cpu = None
+    if not args.no_cpu and am["median_ms"] is not None:
+        seed0 = rows[0]["seed"]
+        need = rows[0]["i_settled"]
+        if need >= 0:
+            log(f"cpu baseline: oracle port, seed {seed0}, {need + 1} iterations")
+            Ah = A.cpu().numpy()
+            with torch.cuda.device(0):
+                _, nrm2, cnrm2, _ = ctx.column_stats()
+            cpu = cpu_baseline(Ah, d, lam, dict(cfg), seed0, need)
+            del Ah
+            log(f"cpu: {cpu}")
+
+    value = am["median_ms"]
+    speed = (ex["median_ms"] / value) if (value and ex["median_ms"]) else None
+    line = {
+        "metric": METRIC, "value": (round(value, 4) if value is not None else None), "unit": "ms",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{d}x{n} FP32 heavy-tailed columns (lognormal 1.5) + planted rank 8",
+                   "kernel": cfg["kernel"], "c": cfg["c"], "vr_period": cfg["vr_period"],
+                   "iterations": iters, "tolerance": TOL, "lambda_star": lam,
+                   "amm_p10_ms": am["p10_ms"], "amm_p90_ms": am["p90_ms"],
+                   "amm_not_reached": am["not_reached"], "amm_median_iters": am["median_iters"],
+                   "amm_first_hit_ms": am["median_first_ms"],
+                   "exact_ms": ex["median_ms"], "exact_not_reached": ex["not_reached"],
+                   "exact_median_iters": ex["median_iters"],
+                   "speedup_vs_exact": (round(speed, 3) if speed else None)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,100 @@
+"""ctypes binding of libamm.so (the C ABI declared in include/amm.h).
+
+The library is loaded from the package directory.  There is no fallback:
+if libamm.so is missing or fails to load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libamm.so"
+
+AMM_OK = 0
+AMM_DTYPE_F32 = 0
+AMM_DTYPE_BF16 = 1
+AMM_KERNEL_EXACT = 0
+AMM_KERNEL_PREVIOUS = 1
+AMM_KERNEL_PROPOSED = 2
+AMM_KERNEL_PROPOSED_V2 = 3
+AMM_FLAG_NO_GRAPH = 1
+AMM_FLAG_KEEP_STATS = 2
+
+KERNEL_NAMES = {
+    "exact": AMM_KERNEL_EXACT,
+    "previousAMM": AMM_KERNEL_PREVIOUS,
+    "proposedAMM": AMM_KERNEL_PROPOSED,
+    "proposedAMM_v2": AMM_KERNEL_PROPOSED_V2,
+}
+
+# symbol -> (restype, argtypes); mirrors include/amm.h
+_vp = C.c_void_p
+_SIGS = {
+    "amm_last_error": (C.c_char_p, []),
+    "amm_abi_version": (C.c_int, []),
+    "amm_ctx_create": (C.c_int, [_vp, C.c_int32, C.c_uint64, C.c_uint64, C.c_uint64, _vp, C.POINTER(_vp)]),
+    "amm_ctx_destroy": (C.c_int, [_vp]),
+    "amm_set_exact_eigenvector": (C.c_int, [_vp, _vp, C.c_uint64]),
+    "amm_column_stats": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "amm_build_cdf": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp]),
+    "amm_sample": (C.c_int, [_vp, C.c_int32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _vp, _vp, _vp]),
+    "amm_square_of_A": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_uint32, C.c_uint64, C.c_uint32,
+                                  C.c_uint32, _vp, _vp, _vp, _vp]),
+    "amm_exact_product": (C.c_int, [_vp, _vp, _vp]),
+    "amm_power_method": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "amm_last_modes": (C.c_int, [_vp, _vp, C.c_uint64]),
+    "amm_get_vector": (C.c_int, [_vp, C.c_int32, _vp, C.c_uint64]),
+    "amm_last_run_ms": (C.c_int, [_vp, _vp]),
+    "amm_time_product": (C.c_int, [_vp, C.c_int32, C.c_uint32, C.c_uint64, C.c_uint32, _vp, _vp, _vp]),
+}
+
+
+class amm_params(C.Structure):
+    """struct amm_params of include/amm.h."""
+
+    _fields_ = [
+        ("kernel", C.c_int32),
+        ("c", C.c_uint32),
+        ("step", C.c_uint64),
+        ("vr_period", C.c_uint64),
+        ("vr_adaptive_threshold", C.c_double),
+        ("seed", C.c_uint64),
+        ("trial", C.c_uint32),
+        ("sanity", C.c_int32),
+        ("flags", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class AmmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"amm error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libamm.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run `python -m eigenvaluecomputation_by_amm_amd.build`")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def check(rc: int) -> None:
+    if rc != AMM_OK:
+        raise AmmError(rc, load().amm_last_error().decode())

@@ -1,0 +1,254 @@
+"""Python host mirror of the reference's power-method interface.
+
+The reference drives everything from `AMM.cu` through
+`powerMethod_by_{exact,previous,proposed,proposed_v2}(host_memory&,
+device_memory&, bool is_output)` (powerMethod_by_exact.cuh:4-7,
+powerMethod_by_previous.cuh:100-103, powerMethod_by_proposed.cuh:105-108,
+powerMethod_by_proposed_v2.cuh:199-202) and `square_of_A_by_X(device_memory&,
+bool sanity_check)` for one sampled product.  Here `AmmContext` plays the
+part of `device_memory` (it owns every work buffer; the matrix stays the
+caller's) and `RunParams` that of `host_memory`'s run fields.  All compute
+goes through libamm.so; torch is used only to hold device memory.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import AmmError, check  # noqa: F401
+
+SAMPLE_SEG = 8192
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+@dataclass
+class RunParams:
+    """Run fields of host_memory (host_memory.h:26-50)."""
+
+    kernel: str = "exact"
+    c: int = 0
+    step: int = 10
+    vr_period: int = 0
+    vr_adaptive_threshold: float = 0.0
+    seed: int = 0
+    trial: int = 0
+    sanity: bool = False
+    use_graph: bool = True
+    keep_stats: bool = False
+
+    def to_c(self) -> _lib.amm_params:
+        if self.kernel not in _lib.KERNEL_NAMES:
+            raise ValueError(f"'-kernel {self.kernel}' is not valid.")
+        flags = 0
+        if not self.use_graph:
+            flags |= _lib.AMM_FLAG_NO_GRAPH
+        if self.keep_stats:
+            flags |= _lib.AMM_FLAG_KEEP_STATS
+        return _lib.amm_params(
+            kernel=_lib.KERNEL_NAMES[self.kernel], c=int(self.c), step=int(self.step),
+            vr_period=int(self.vr_period), vr_adaptive_threshold=float(self.vr_adaptive_threshold),
+            seed=int(self.seed) & 0xFFFFFFFFFFFFFFFF, trial=int(self.trial) & 0xFFFFFFFF,
+            sanity=1 if self.sanity else 0, flags=flags, reserved=0)
+
+
+@dataclass
+class RunResult:
+    time_ms: np.ndarray  # cumulative device time per iteration
+    eig: np.ndarray      # raw Rayleigh quotient y^T x of A A^T per iteration
+    err: np.ndarray      # y^T v* / ||y|| per iteration
+    wall_ms: float       # HIP-event time of the whole run
+
+    def eigenvalue(self, diagonal_offset: float = 0.0) -> np.ndarray:
+        """Printed Eigenvalue column: max(0, sqrt(max(0, eig)) - delta)
+        (powerMethod_by_exact.cuh:83)."""
+        return np.maximum(0.0, np.sqrt(np.maximum(0.0, self.eig)) - diagonal_offset)
+
+    def norm_error(self) -> np.ndarray:
+        """Printed NormError column: 1 - err^2 (powerMethod_by_exact.cuh:84)."""
+        return 1.0 - self.err * self.err
+
+
+class AmmContext:
+    """Context over a device-resident d x n column-major matrix.
+
+    `A` is either a torch tensor on the GPU holding the matrix as an
+    (n, ld) row-major array (row j = column j of A, i.e. column-major A),
+    or a raw device pointer with explicit d, n, ld and dtype.
+    """
+
+    def __init__(self, A, d: int | None = None, n: int | None = None, ld: int | None = None,
+                 dtype: str | None = None, stream: int = 0):
+        self._lib = _lib.load()
+        self._keep = A
+        if hasattr(A, "data_ptr"):
+            import torch
+            if A.dim() != 2:
+                raise ValueError("A must be 2-D (n, ld): one row per column of the matrix")
+            if not A.is_cuda:
+                raise ValueError("A must be a device tensor")
+            if A.stride(1) != 1:
+                raise ValueError("A rows must be contiguous")
+            n_ = A.shape[0]
+            ld_ = A.stride(0)
+            d_ = A.shape[1] if d is None else d
+            if A.dtype == torch.float32:
+                dt = _lib.AMM_DTYPE_F32
+            elif A.dtype == torch.bfloat16:
+                dt = _lib.AMM_DTYPE_BF16
+            else:
+                raise ValueError("A must be float32 or bfloat16")
+            ptr = A.data_ptr()
+            d, n, ld = d_, n_, ld_
+            if stream == 0:
+                stream = torch.cuda.current_stream(A.device).cuda_stream
+        else:
+            ptr = int(A)
+            dt = {"f32": _lib.AMM_DTYPE_F32, "bf16": _lib.AMM_DTYPE_BF16}[dtype or "f32"]
+        self.d, self.n, self.ld, self.dtype = int(d), int(n), int(ld), dt
+        h = C.c_void_p()
+        check(self._lib.amm_ctx_create(C.c_void_p(ptr), dt, self.d, self.n, self.ld,
+                                       C.c_void_p(stream) if stream else None, C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.amm_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- reference eigenvector (device_memory.h:246-254) --------------------
+    def set_exact_eigenvector(self, v: np.ndarray | None) -> None:
+        if v is None:
+            check(self._lib.amm_set_exact_eigenvector(self._h, None, 0))
+            return
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        check(self._lib.amm_set_exact_eigenvector(self._h, C.c_void_p(_ptr(v)), v.size))
+
+    # -- get_norms (device_memory.h:34-59) ---------------------------------
+    def column_stats(self):
+        alpha = np.empty(self.n, np.float32)
+        nrm2 = np.empty(self.n, np.float64)
+        cnrm2 = np.empty(self.n, np.float64)
+        a2 = np.zeros(1, np.float64)
+        check(self._lib.amm_column_stats(self._h, C.c_void_p(_ptr(alpha)), C.c_void_p(_ptr(nrm2)),
+                                         C.c_void_p(_ptr(cnrm2)), C.c_void_p(_ptr(a2))))
+        return alpha, nrm2, cnrm2, float(a2[0])
+
+    # -- set_weights + InclusiveSum (common.h:34-40) ------------------------
+    def build_cdf(self, centred: bool = False):
+        cdf = np.empty(self.n, np.uint64)
+        tot = np.zeros(1, np.uint64)
+        lp = np.zeros(1, np.uint64)
+        check(self._lib.amm_build_cdf(self._h, 1 if centred else 0, C.c_void_p(_ptr(cdf)),
+                                      C.c_void_p(_ptr(tot)), C.c_void_p(_ptr(lp))))
+        return cdf, int(tot[0]), int(lp[0])
+
+    # -- pick_index + SortKeys + update_weights (device_memory.h:519-545) ---
+    def sample(self, c: int, iteration: int, seed: int, trial: int = 0, centred: bool = False):
+        """Returns per-segment (idx, wgt) lists: unique sorted columns and
+        their weights mult * total / (c * q_j)."""
+        idx = np.zeros(c, np.uint32)
+        wgt = np.zeros(c, np.float32)
+        nseg = (c + SAMPLE_SEG - 1) // SAMPLE_SEG
+        segc = np.zeros(nseg, np.uint32)
+        check(self._lib.amm_sample(self._h, 1 if centred else 0, seed & 0xFFFFFFFFFFFFFFFF, iteration,
+                                   trial, c, C.c_void_p(_ptr(idx)), C.c_void_p(_ptr(wgt)),
+                                   C.c_void_p(_ptr(segc))))
+        return idx, wgt, segc
+
+    # -- square_of_A_by_previous / _proposed --------------------------------
+    def square_of_A(self, kernel: str, x, c: int, iteration: int, seed: int, trial: int = 0,
+                    x_memo=None, y_memo=None, sanity: bool = False, y=None):
+        import torch
+        k = _lib.KERNEL_NAMES[kernel]
+        if y is None:
+            y = torch.empty(self.d, dtype=torch.float32, device=x.device)
+        for t in (x, x_memo, y_memo, y):
+            if t is not None and (t.dtype != torch.float32 or t.numel() != self.d or not t.is_contiguous()):
+                raise ValueError("vectors must be contiguous float32 of length d")
+        p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+        check(self._lib.amm_square_of_A(self._h, k, 1 if sanity else 0, c, seed & 0xFFFFFFFFFFFFFFFF,
+                                        iteration, trial, p(x), p(x_memo), p(y_memo), p(y)))
+        return y
+
+    def exact_product(self, x, y=None):
+        import torch
+        if y is None:
+            y = torch.empty(self.d, dtype=torch.float32, device=x.device)
+        check(self._lib.amm_exact_product(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())))
+        return y
+
+    # -- powerMethod_by_X ----------------------------------------------------
+    def power_method(self, params: RunParams) -> RunResult:
+        p = params.to_c()
+        S = int(params.step)
+        t = np.zeros(S, np.float64)
+        e = np.zeros(S, np.float64)
+        r = np.zeros(S, np.float64)
+        check(self._lib.amm_power_method(self._h, C.byref(p), C.c_void_p(_ptr(t)), C.c_void_p(_ptr(e)),
+                                         C.c_void_p(_ptr(r))))
+        ms = np.zeros(1, np.float64)
+        check(self._lib.amm_last_run_ms(self._h, C.c_void_p(_ptr(ms))))
+        return RunResult(t, e, r, float(ms[0]))
+
+    def last_modes(self, step: int) -> np.ndarray:
+        m = np.zeros(step, np.uint32)
+        check(self._lib.amm_last_modes(self._h, C.c_void_p(_ptr(m)), step))
+        return m
+
+    def time_product(self, kernel: str, c: int = 0, seed: int = 0, iters: int = 100):
+        """Mean duration (ms) of the two contraction kernels of one product
+        (HIP events on the context's stream) and the columns one launch reads."""
+        k = _lib.KERNEL_NAMES[kernel]
+        t = np.zeros(2, np.float64)
+        u = np.zeros(1, np.uint64)
+        check(self._lib.amm_time_product(self._h, k, c, seed & 0xFFFFFFFFFFFFFFFF, iters,
+                                         C.c_void_p(_ptr(t)), C.c_void_p(_ptr(t) + 8),
+                                         C.c_void_p(_ptr(u))))
+        return float(t[0]), float(t[1]), int(u[0])
+
+    def get_vector(self, which: str = "x") -> np.ndarray:
+        v = np.empty(self.d, np.float32)
+        check(self._lib.amm_get_vector(self._h, 0 if which == "x" else 1, C.c_void_p(_ptr(v)), self.d))
+        return v
+
+
+def _run(kernel: str, ctx: AmmContext, **kw) -> RunResult:
+    return ctx.power_method(RunParams(kernel=kernel, **kw))
+
+
+def powerMethod_by_exact(ctx: AmmContext, **kw) -> RunResult:
+    """powerMethod_by_exact.cuh:4-100"""
+    return _run("exact", ctx, **kw)
+
+
+def powerMethod_by_previous(ctx: AmmContext, **kw) -> RunResult:
+    """powerMethod_by_previous.cuh:100-210"""
+    return _run("previousAMM", ctx, **kw)
+
+
+def powerMethod_by_proposed(ctx: AmmContext, **kw) -> RunResult:
+    """powerMethod_by_proposed.cuh:105-235"""
+    return _run("proposedAMM", ctx, **kw)
+
+
+def powerMethod_by_proposed_v2(ctx: AmmContext, **kw) -> RunResult:
+    """powerMethod_by_proposed_v2.cuh:199-377"""
+    return _run("proposedAMM_v2", ctx, **kw)

@@ -1,0 +1,102 @@
+"""Build recipe: compiles the HIP library (libamm.so), the AMM command-line
+driver and the CPU oracle in-tree.
+
+    python -m eigenvaluecomputation_by_amm_amd.build
+
+Every HIP translation unit is compiled for gfx950 only, with packed-FP32
+instructions disabled (``-target-feature -packed-fp32-ops``).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+INCLUDE = ROOT / "include"
+ORACLE = ROOT / "oracle"
+
+LIB = PKG / "libamm.so"
+CLI = PKG / "AMM"
+ORACLE_LIB = ORACLE / "liboracle.so"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+    f"-I{INCLUDE}",
+    f"-I{CSRC}",
+]
+
+LIB_SOURCES = ["amm_kernels.hip", "amm_capi.hip"]
+CLI_SOURCES = ["amm_cli.cpp"]
+
+
+def _run(cmd: list[str]) -> None:
+    print("[build]", " ".join(str(c) for c in cmd), flush=True)
+    subprocess.run([str(c) for c in cmd], check=True)
+
+
+def _newer(target: Path, deps: list[Path]) -> bool:
+    if not target.exists():
+        return False
+    t = target.stat().st_mtime
+    return all(d.stat().st_mtime <= t for d in deps)
+
+
+def build_lib(force: bool = True) -> Path:
+    objdir = PKG / "_build"
+    objdir.mkdir(exist_ok=True)
+    deps = [CSRC / s for s in LIB_SOURCES] + list(CSRC.glob("*.hpp")) + [INCLUDE / "amm.h"]
+    if not force and _newer(LIB, deps):
+        return LIB
+    objs = []
+    for src in LIB_SOURCES:
+        obj = objdir / (Path(src).stem + ".o")
+        _run([HIPCC, *HIP_FLAGS, "-c", CSRC / src, "-o", obj])
+        objs.append(obj)
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_cli(force: bool = True) -> Path:
+    deps = [CSRC / s for s in CLI_SOURCES] + list(CSRC.glob("*.hpp")) + [INCLUDE / "amm.h", LIB]
+    if not force and _newer(CLI, deps):
+        return CLI
+    srcs = [CSRC / s for s in CLI_SOURCES]
+    _run([HIPCC, *HIP_FLAGS, *srcs, "-o", CLI, f"-L{PKG}", "-lamm", "-Wl,-rpath,$ORIGIN"])
+    return CLI
+
+
+def build_oracle(force: bool = True) -> Path:
+    src = ORACLE / "amm_oracle.c"
+    if not force and _newer(ORACLE_LIB, [src]):
+        return ORACLE_LIB
+    cc = shutil.which("gcc") or "cc"
+    tmp = ORACLE_LIB.with_suffix(".so.tmp")
+    _run([cc, "-O2", "-std=c11", "-ffp-contract=off", "-fPIC", "-shared", src, "-o", tmp, "-lm"])
+    os.replace(tmp, ORACLE_LIB)
+    return ORACLE_LIB
+
+
+def build_all(force: bool = True) -> None:
+    build_oracle(force)
+    build_lib(force)
+    if all((CSRC / s).exists() for s in CLI_SOURCES):
+        build_cli(force)
+
+
+if __name__ == "__main__":
+    build_all(force="--lazy" not in sys.argv)

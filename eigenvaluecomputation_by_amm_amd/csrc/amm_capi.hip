@@ -1,0 +1,928 @@
+// amm_capi.hip -- C ABI of the AMM power method (include/amm.h): context,
+// buffer ownership, launch planning, the power-iteration schedules and the
+// hipGraph that carries a whole run.
+//
+// Schedules restate powerMethod_by_exact.cuh:4-100,
+// powerMethod_by_previous.cuh:100-210, powerMethod_by_proposed.cuh:105-235 and
+// powerMethod_by_proposed_v2.cuh:199-377.  Unlike the reference there is no
+// host round trip inside a run: samples of every iteration are drawn up
+// front (the Philox stream is keyed by iteration), the Rayleigh quotient,
+// normalisation and the adaptive variance-reduction decision stay on the
+// device, per-iteration time stamps come from the device wall clock, and the
+// whole run is one hipGraph launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "amm.h"
+#include "amm_device.hpp"
+#include "amm_launch.hpp"
+
+using amm::DevState;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(AMM_E_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int grow(T*& p, size_t& cap, size_t need) {
+  if (need <= cap && p) return AMM_OK;
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+  const size_t n = need ? need : 1;
+  if (hipMalloc((void**)&p, n * sizeof(T)) != hipSuccess) {
+    p = nullptr;
+    return fail(AMM_E_NOMEM, "device allocation of " + std::to_string(n * sizeof(T)) + " bytes failed");
+  }
+  if (hipMemset(p, 0, n * sizeof(T)) != hipSuccess) return fail(AMM_E_HIP, "hipMemset failed");
+  cap = n;
+  return AMM_OK;
+}
+
+uint32_t next_pow2(uint32_t v) {
+  uint32_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+constexpr uint32_t kTargetBlocks = 1024;  // >> 256 CUs
+constexpr uint32_t kMaxChunk = 2048;
+
+struct DotPlan {
+  uint32_t R, rows_per_split;
+};
+struct AxpyPlan {
+  uint32_t cs, S;
+};
+
+}  // namespace
+
+struct amm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  const void* A = nullptr;
+  int dtype = AMM_DTYPE_F32;
+  uint64_t d = 0, n = 0, ld = 0, dpad = 0;
+  int V = 4;
+  int wall_khz = 100000;
+
+  float* alpha = nullptr;
+  double* nrm2 = nullptr;
+  double* cnrm2 = nullptr;
+  uint64_t* cdf[2] = {nullptr, nullptr};
+  float *z = nullptr, *v = nullptr, *x = nullptr, *y = nullptr, *xm = nullptr, *ym = nullptr;
+  float* scratch_vec = nullptr;
+  double* eigvec = nullptr;
+  bool has_eigvec = false;
+  DevState* st = nullptr;
+  double* BP = nullptr;
+
+  uint32_t* mode = nullptr;
+  size_t mode_cap = 0;
+  double* eig = nullptr;
+  size_t eig_cap = 0;
+  double* err = nullptr;
+  size_t err_cap = 0;
+  int64_t* ts = nullptr;
+  size_t ts_cap = 0;
+  uint32_t* idx = nullptr;
+  size_t idx_cap = 0;
+  float* wgt = nullptr;
+  size_t wgt_cap = 0;
+  uint32_t* segc = nullptr;
+  size_t segc_cap = 0;
+  float* P1 = nullptr;
+  size_t P1_cap = 0;
+  float* P2 = nullptr;
+  size_t P2_cap = 0;
+  double* P2s = nullptr;
+  size_t P2s_cap = 0;
+
+  bool stats_valid = false;
+
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool graph_valid = false;
+  amm_params graph_key{};
+
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  uint64_t last_step = 0;
+  std::vector<uint32_t> host_modes;
+  // host staging of small uploads (kept alive until the copies complete)
+  uint64_t h_seed = 0;
+  uint32_t h_trial = 0;
+  uint32_t h_mode = 0;
+
+  // launch plans
+  DotPlan dot_dense{}, dot_list{};
+  AxpyPlan axpy_dense{}, axpy_list{};
+  uint32_t list_c = 0;
+};
+
+namespace {
+
+uint32_t rowblocks(const amm_ctx* c) { return (uint32_t)((c->dpad + 256u * c->V - 1) / (256u * c->V)); }
+
+DotPlan plan_dot(const amm_ctx* c, uint64_t entries, int cpw) {
+  const uint64_t groups = (entries + 4 * cpw - 1) / (4 * cpw);
+  const uint32_t step = 64u * c->V * 2;
+  uint64_t R = groups >= kTargetBlocks ? 1 : (kTargetBlocks + groups - 1) / groups;
+  const uint64_t maxR = std::max<uint64_t>(1, (c->d + step - 1) / step);
+  R = std::min(R, maxR);
+  uint64_t rps = (c->d + R - 1) / R;
+  rps = (rps + step - 1) / step * step;
+  R = (c->d + rps - 1) / rps;
+  return DotPlan{(uint32_t)R, (uint32_t)rps};
+}
+
+AxpyPlan plan_axpy(const amm_ctx* c, uint64_t entries, bool pow2) {
+  const uint32_t rb = rowblocks(c);
+  const uint64_t S_t = std::max<uint64_t>(1, (kTargetBlocks + rb - 1) / rb);
+  uint64_t cs = std::max<uint64_t>(1, (entries + S_t - 1) / S_t);
+  if (pow2) cs = next_pow2((uint32_t)std::min<uint64_t>(cs, kMaxChunk));
+  cs = std::min<uint64_t>(cs, kMaxChunk);
+  const uint64_t S = (entries + cs - 1) / cs;
+  return AxpyPlan{(uint32_t)cs, (uint32_t)S};
+}
+
+int set_device(amm_ctx* c) {
+  HIPCHK(hipSetDevice(c->device));
+  return AMM_OK;
+}
+
+int ensure_plans(amm_ctx* c, uint32_t list_c) {
+  c->dot_dense = plan_dot(c, c->n, 4);
+  c->axpy_dense = plan_axpy(c, c->n, false);
+  c->list_c = list_c;
+  if (list_c) {
+    c->dot_list = plan_dot(c, list_c, 2);
+    c->axpy_list = plan_axpy(c, list_c, true);
+  } else {
+    c->dot_list = DotPlan{1, 0};
+    c->axpy_list = AxpyPlan{1, 0};
+  }
+  size_t p1 = (size_t)c->dot_dense.R * c->n;
+  if (list_c) p1 = std::max(p1, (size_t)c->dot_list.R * list_c);
+  const uint32_t S = std::max(c->axpy_dense.S, c->axpy_list.S);
+  int rc;
+  if ((rc = grow(c->P1, c->P1_cap, p1))) return rc;
+  if ((rc = grow(c->P2, c->P2_cap, (size_t)S * c->dpad))) return rc;
+  if ((rc = grow(c->P2s, c->P2s_cap, (size_t)S))) return rc;
+  return AMM_OK;
+}
+
+int ensure_step(amm_ctx* c, uint64_t step) {
+  int rc;
+  if ((rc = grow(c->mode, c->mode_cap, step))) return rc;
+  if ((rc = grow(c->eig, c->eig_cap, step))) return rc;
+  if ((rc = grow(c->err, c->err_cap, step))) return rc;
+  if ((rc = grow(c->ts, c->ts_cap, step))) return rc;
+  return AMM_OK;
+}
+
+int ensure_lists(amm_ctx* c, uint64_t nit, uint32_t cc) {
+  const uint64_t nseg = (cc + amm::kSampleSeg - 1) / amm::kSampleSeg;
+  int rc;
+  if ((rc = grow(c->idx, c->idx_cap, (size_t)nit * cc))) return rc;
+  if ((rc = grow(c->wgt, c->wgt_cap, (size_t)nit * cc))) return rc;
+  if ((rc = grow(c->segc, c->segc_cap, (size_t)nit * nseg))) return rc;
+  return AMM_OK;
+}
+
+// ---- enqueue helpers (all asynchronous on c->stream) ----------------------
+int enq_stats(amm_ctx* c) {
+  HIPCHK(amm::launch_colstats(c->dtype, c->A, c->ld, (uint32_t)c->d, c->n, c->alpha, c->nrm2,
+                              c->cnrm2, c->stream));
+  return AMM_OK;
+}
+
+int enq_cdf(amm_ctx* c, int which, bool unit) {
+  HIPCHK(amm::launch_cdf(which ? c->cnrm2 : c->nrm2, unit ? 1 : 0, c->n, c->cdf[which], c->st, which,
+                         which ? c->alpha : nullptr, c->stream));
+  return AMM_OK;
+}
+
+int enq_sample(amm_ctx* c, int which, uint32_t it0, uint32_t nit, uint32_t buf_it0, uint32_t cc) {
+  HIPCHK(amm::launch_sample(c->cdf[which], c->n, c->st, which, it0, nit, buf_it0, cc, c->idx, c->wgt,
+                            c->segc, c->stream));
+  return AMM_OK;
+}
+
+// z = A alpha, v = z - ||alpha||^2 (powerMethod_by_proposed.cuh:132-145)
+int enq_z(amm_ctx* c) {
+  amm::GatherAxpyArgs a{};
+  a.A = c->A;
+  a.ld = c->ld;
+  a.d = (uint32_t)c->d;
+  a.dpad = c->dpad;
+  a.cap = (uint32_t)c->n;
+  a.cs = c->axpy_dense.cs;
+  a.S = c->axpy_dense.S;
+  a.tdirect = c->alpha;
+  a.st = c->st;
+  a.P2 = c->P2;
+  a.P2s = c->P2s;
+  HIPCHK(amm::launch_gather_axpy(c->dtype, a, c->stream));
+  HIPCHK(amm::launch_sum_rows(c->P2, a.S, c->dpad, (uint32_t)c->d, c->z, c->v, c->st, c->stream));
+  return AMM_OK;
+}
+
+// exact product: P1 = A^T x (dense), P2 = A P1
+int enq_exact(amm_ctx* c, uint32_t it, const uint32_t* gate_mode, uint32_t gate_mask) {
+  amm::GatherDotArgs g{};
+  g.A = c->A;
+  g.ld = c->ld;
+  g.d = (uint32_t)c->d;
+  g.cap = (uint32_t)c->n;
+  g.x = c->x;
+  g.xm = nullptr;
+  g.R = c->dot_dense.R;
+  g.rows_per_split = c->dot_dense.rows_per_split;
+  g.P1 = c->P1;
+  g.mode = gate_mode;
+  g.it = it;
+  g.gate_mask = gate_mask;
+  HIPCHK(amm::launch_gather_dot(c->dtype, g, c->stream));
+  amm::GatherAxpyArgs a{};
+  a.A = c->A;
+  a.ld = c->ld;
+  a.d = (uint32_t)c->d;
+  a.dpad = c->dpad;
+  a.cap = (uint32_t)c->n;
+  a.cs = c->axpy_dense.cs;
+  a.S = c->axpy_dense.S;
+  a.P1 = c->P1;
+  a.R = c->dot_dense.R;
+  a.st = c->st;
+  a.P2 = c->P2;
+  a.P2s = c->P2s;
+  a.mode = gate_mode;
+  a.it = it;
+  a.gate_mask = gate_mask;
+  HIPCHK(amm::launch_gather_axpy(c->dtype, a, c->stream));
+  return AMM_OK;
+}
+
+// sampled product with iteration `it`'s list at list slot `slot`
+int enq_amm(amm_ctx* c, uint32_t it, uint32_t slot, bool proposed, bool sanity,
+            const uint32_t* gate_mode, uint32_t gate_mask) {
+  const uint32_t cc = c->list_c;
+  const uint64_t nseg = (cc + amm::kSampleSeg - 1) / amm::kSampleSeg;
+  const DotPlan dp = sanity ? c->dot_dense : c->dot_list;
+  const AxpyPlan ap = sanity ? c->axpy_dense : c->axpy_list;
+  amm::GatherDotArgs g{};
+  g.A = c->A;
+  g.ld = c->ld;
+  g.d = (uint32_t)c->d;
+  g.idx = sanity ? nullptr : c->idx + (size_t)slot * cc;
+  g.seg_count = sanity ? nullptr : c->segc + (size_t)slot * nseg;
+  g.cap = sanity ? (uint32_t)c->n : cc;
+  g.x = c->x;
+  g.xm = c->xm;
+  g.R = dp.R;
+  g.rows_per_split = dp.rows_per_split;
+  g.P1 = c->P1;
+  g.mode = gate_mode;
+  g.it = it;
+  g.gate_mask = gate_mask;
+  HIPCHK(amm::launch_gather_dot(c->dtype, g, c->stream));
+  amm::GatherAxpyArgs a{};
+  a.A = c->A;
+  a.ld = c->ld;
+  a.d = (uint32_t)c->d;
+  a.dpad = c->dpad;
+  a.idx = g.idx;
+  a.wgt = sanity ? nullptr : c->wgt + (size_t)slot * cc;
+  a.seg_count = g.seg_count;
+  a.cap = g.cap;
+  a.cs = ap.cs;
+  a.S = ap.S;
+  a.P1 = c->P1;
+  a.R = dp.R;
+  a.alpha = proposed ? c->alpha : nullptr;
+  a.st = c->st;
+  a.P2 = c->P2;
+  a.P2s = c->P2s;
+  a.mode = gate_mode;
+  a.it = it;
+  a.gate_mask = gate_mask;
+  HIPCHK(amm::launch_gather_axpy(c->dtype, a, c->stream));
+  return AMM_OK;
+}
+
+int enq_finish(amm_ctx* c, uint32_t it, bool sanity, float* y_out, const float* ym_base,
+               uint32_t adapt_kernel, uint64_t period, double thr, uint32_t step) {
+  amm::FinishArgs f{};
+  f.P2 = c->P2;
+  f.S_exact = c->axpy_dense.S;
+  f.S_amm = sanity ? c->axpy_dense.S : c->axpy_list.S;
+  f.dpad = c->dpad;
+  f.d = (uint32_t)c->d;
+  f.x = c->x;
+  f.y = y_out;
+  f.xm = c->xm;
+  f.ym = const_cast<float*>(ym_base);
+  f.v = c->v;
+  f.P2s = c->P2s;
+  f.eigvec = c->has_eigvec ? c->eigvec : nullptr;
+  f.BP = c->BP;
+  f.st = c->st;
+  f.mode = c->mode;
+  f.it = it;
+  f.eig = c->eig;
+  f.err = c->err;
+  f.ts = c->ts;
+  f.adapt_kernel = adapt_kernel;
+  f.period = period;
+  f.thr = thr;
+  f.step = step;
+  HIPCHK(amm::launch_finish(f, c->stream));
+  return AMM_OK;
+}
+
+// Step kind of iteration i for a static schedule; mirrors orc_is_exact_step
+// (powerMethod_by_previous.cuh:135-138, powerMethod_by_proposed.cuh:156-161,
+// powerMethod_by_proposed_v2.cuh:234-326).
+std::vector<uint32_t> static_modes(const amm_params& p) {
+  std::vector<uint32_t> m(p.step);
+  uint64_t last_exact = 0;
+  const uint64_t P = p.vr_period;
+  for (uint64_t i = 0; i < p.step; ++i) {
+    uint32_t md;
+    const bool due = P > 0 && (i + 1) % P == 0;
+    if (p.kernel == AMM_KERNEL_EXACT) {
+      md = amm::kModeExactNoMemo;
+    } else if (p.kernel == AMM_KERNEL_PREVIOUS) {
+      md = due ? amm::kModeExactMemo : amm::kModeAmmPrevious;
+    } else if (p.kernel == AMM_KERNEL_PROPOSED) {
+      md = (i > last_exact + 1 && due) ? amm::kModeExactMemo : amm::kModeAmmProposed;
+    } else {
+      if (i + 1 < P) md = amm::kModeAmmPrevious;
+      else if (i + 1 == P) md = amm::kModeExactNoMemo;
+      else md = due ? amm::kModeExactMemo : amm::kModeAmmProposed;
+    }
+    if (md == amm::kModeExactMemo || md == amm::kModeExactNoMemo) last_exact = i;
+    m[i] = md;
+  }
+  return m;
+}
+
+bool is_adaptive(const amm_params& p) {
+  return p.vr_adaptive_threshold > 0.0 &&
+         (p.kernel == AMM_KERNEL_PREVIOUS || p.kernel == AMM_KERNEL_PROPOSED);
+}
+
+int validate(const amm_ctx* c, const amm_params* p) {
+  if (!p) return fail(AMM_E_INVALID, "params is NULL");
+  if (p->kernel < AMM_KERNEL_EXACT || p->kernel > AMM_KERNEL_PROPOSED_V2)
+    return fail(AMM_E_INVALID, "unknown kernel " + std::to_string(p->kernel));
+  if (p->step < 1 || p->step > (1ull << 31)) return fail(AMM_E_INVALID, "step must be in [1, 2^31]");
+  if (p->kernel != AMM_KERNEL_EXACT) {
+    if (p->c < 1 || p->c > c->n)
+      return fail(AMM_E_INVALID, "c must satisfy 1 <= c <= n (host_memory.h:166-167)");
+    if ((uint64_t)p->c * p->step > (1ull << 34))
+      return fail(AMM_E_INVALID, "c * step exceeds the sample-list limit 2^34");
+  }
+  if (p->kernel == AMM_KERNEL_PROPOSED_V2 && (p->vr_period < 1 || p->vr_period > p->step))
+    return fail(AMM_E_INVALID,
+                "proposedAMM_v2 needs 1 <= vr_period <= step (powerMethod_by_proposed_v2.cuh:234)");
+  if (!(p->vr_adaptive_threshold >= 0.0)) return fail(AMM_E_INVALID, "vr_adaptive_threshold < 0");
+  return AMM_OK;
+}
+
+int upload_run_state(amm_ctx* c, uint64_t seed, uint32_t trial) {
+  c->h_seed = seed;
+  c->h_trial = trial;
+  HIPCHK(hipMemcpyAsync((char*)c->st + offsetof(DevState, seed), &c->h_seed, sizeof(uint64_t),
+                        hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync((char*)c->st + offsetof(DevState, trial), &c->h_trial, sizeof(uint32_t),
+                        hipMemcpyHostToDevice, c->stream));
+  return AMM_OK;
+}
+
+// All kernels of one run, in stream order.
+int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& modes) {
+  int rc;
+  const int K = p.kernel;
+  const uint64_t P = p.vr_period;
+  const bool adaptive = is_adaptive(p);
+  const bool sanity = K != AMM_KERNEL_EXACT && p.sanity && p.c == c->n;
+  const uint32_t step = (uint32_t)p.step;
+  HIPCHK(amm::launch_begin(c->xm, c->ym, c->dpad, c->st, c->mode, adaptive ? modes[0] : 0, c->stream));
+  if (!(p.flags & AMM_FLAG_KEEP_STATS) || !c->stats_valid) {
+    if ((rc = enq_stats(c))) return rc;
+  }
+  const bool need_plain = K == AMM_KERNEL_PREVIOUS || K == AMM_KERNEL_PROPOSED_V2;
+  const bool need_cent = K == AMM_KERNEL_PROPOSED || K == AMM_KERNEL_PROPOSED_V2;
+  if (need_plain && !sanity)
+    if ((rc = enq_cdf(c, 0, false))) return rc;
+  if (need_cent)
+    if ((rc = enq_cdf(c, 1, false))) return rc;  // also ||alpha||^2
+  if (K == AMM_KERNEL_PROPOSED)
+    if ((rc = enq_z(c))) return rc;
+  if (!sanity && K != AMM_KERNEL_EXACT) {
+    if (K == AMM_KERNEL_PREVIOUS) {
+      if ((rc = enq_sample(c, 0, 0, step, 0, p.c))) return rc;
+    } else if (K == AMM_KERNEL_PROPOSED) {
+      if ((rc = enq_sample(c, 1, 0, step, 0, p.c))) return rc;
+    } else {
+      const uint32_t np = (uint32_t)(P - 1);
+      if ((rc = enq_sample(c, 0, 0, np, 0, p.c))) return rc;
+      if (step > P)
+        if ((rc = enq_sample(c, 1, (uint32_t)P, step - (uint32_t)P, 0, p.c))) return rc;
+    }
+  }
+  // x0 (curandGenerateNormal + normalize<512>)
+  HIPCHK(amm::launch_x0(c->y, (uint32_t)c->d, c->dpad, c->st, c->BP, c->stream));
+  HIPCHK(amm::launch_normalize(c->y, c->x, (uint32_t)c->d, c->dpad, c->st, c->xm,
+                               K == AMM_KERNEL_PROPOSED ? c->z : nullptr, c->BP, c->stream));
+  const uint32_t gate_exact = 1u << amm::kModeExactMemo;
+  const uint32_t gate_amm = (1u << amm::kModeAmmPrevious) | (1u << amm::kModeAmmProposed);
+  for (uint32_t i = 0; i < step; ++i) {
+    bool z_after = false;
+    if (adaptive) {
+      if ((rc = enq_exact(c, i, c->mode, gate_exact))) return rc;
+      if ((rc = enq_amm(c, i, i, K == AMM_KERNEL_PROPOSED, sanity, c->mode, gate_amm))) return rc;
+    } else {
+      const uint32_t md = modes[i];
+      if (md == amm::kModeExactMemo || md == amm::kModeExactNoMemo) {
+        if ((rc = enq_exact(c, i, nullptr, 0))) return rc;
+        z_after = K == AMM_KERNEL_PROPOSED_V2 && i + 1 == P;
+      } else {
+        if ((rc = enq_amm(c, i, i, md == amm::kModeAmmProposed, sanity, nullptr, 0))) return rc;
+      }
+    }
+    if ((rc = enq_finish(c, i, sanity, c->y, c->ym, adaptive ? (uint32_t)K : 0u, P,
+                         p.vr_adaptive_threshold, step)))
+      return rc;
+    if (z_after)
+      if ((rc = enq_z(c))) return rc;  // [z, .] = A [alpha, .] at the v2 transition
+    const bool want_sq = K == AMM_KERNEL_PROPOSED || (K == AMM_KERNEL_PROPOSED_V2 && i + 1 >= P);
+    HIPCHK(amm::launch_normalize(c->y, c->x, (uint32_t)c->d, c->dpad, c->st, c->xm,
+                                 want_sq ? c->z : nullptr, c->BP, c->stream));
+  }
+  return AMM_OK;
+}
+
+// Reads and clears the sticky device status after a synchronisation.
+int check_status(amm_ctx* c) {
+  uint32_t stt = 0;
+  HIPCHK(hipMemcpy(&stt, (char*)c->st + offsetof(DevState, status), sizeof(uint32_t),
+                   hipMemcpyDeviceToHost));
+  if (stt) {
+    const uint32_t zero = 0;
+    HIPCHK(hipMemcpy((char*)c->st + offsetof(DevState, status), &zero, sizeof(uint32_t),
+                     hipMemcpyHostToDevice));
+    if (stt & amm::kStatusNonFinite)
+      return fail(AMM_E_INVALID, "the matrix has non-finite entries (NaN or Inf column norms)");
+    return fail(AMM_E_STATE, "device status " + std::to_string(stt));
+  }
+  return AMM_OK;
+}
+
+bool same_key(const amm_params& a, const amm_params& b) {
+  return a.kernel == b.kernel && a.c == b.c && a.step == b.step && a.vr_period == b.vr_period &&
+         a.vr_adaptive_threshold == b.vr_adaptive_threshold && a.sanity == b.sanity &&
+         a.flags == b.flags;
+}
+
+void drop_graph(amm_ctx* c) {
+  if (c->exec) (void)hipGraphExecDestroy(c->exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->exec = nullptr;
+  c->graph = nullptr;
+  c->graph_valid = false;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* amm_last_error(void) { return g_err.c_str(); }
+
+int amm_abi_version(void) { return AMM_ABI_VERSION; }
+
+int amm_ctx_create(const void* A, int32_t dtype, uint64_t d, uint64_t n, uint64_t ld, void* stream,
+                   amm_ctx** out) {
+  if (!out) return fail(AMM_E_INVALID, "out is NULL");
+  *out = nullptr;
+  if (!A) return fail(AMM_E_INVALID, "A is NULL");
+  if (dtype != AMM_DTYPE_F32 && dtype != AMM_DTYPE_BF16) return fail(AMM_E_INVALID, "unknown dtype");
+  if (d < 1 || n < 1) return fail(AMM_E_INVALID, "d and n must be >= 1");
+  if (d >= (1ull << 31) || n >= (1ull << 31)) return fail(AMM_E_INVALID, "d and n must be < 2^31");
+  if (ld < d) return fail(AMM_E_INVALID, "ld must be >= d");
+  const int V = dtype == AMM_DTYPE_BF16 ? 8 : 4;
+  if (ld % V) return fail(AMM_E_INVALID, "ld must be a multiple of 16 bytes of elements");
+  if (((uintptr_t)A) & 15u) return fail(AMM_E_INVALID, "A must be 16-byte aligned");
+  amm_ctx* c = new amm_ctx();
+  c->A = A;
+  c->dtype = dtype;
+  c->d = d;
+  c->n = n;
+  c->ld = ld;
+  c->dpad = (d + 7) / 8 * 8;
+  c->V = V;
+  int rc = AMM_OK;
+  auto bail = [&](int r) {
+    amm_ctx_destroy(c);
+    return r;
+  };
+  if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(AMM_E_HIP, "hipGetDevice failed"));
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+  } else {
+    // blocking stream: ordered against work on the null stream (the caller's
+    // uploads of A and of the vectors)
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamDefault) != hipSuccess)
+      return bail(fail(AMM_E_HIP, "hipStreamCreate failed"));
+    c->own_stream = true;
+  }
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) == hipSuccess && khz > 0)
+    c->wall_khz = khz;
+  size_t cap = 0;
+  const size_t dp = c->dpad;
+  if ((rc = grow(c->alpha, cap, n)) || (cap = 0, rc = grow(c->nrm2, cap, n)) ||
+      (cap = 0, rc = grow(c->cnrm2, cap, n)) || (cap = 0, rc = grow(c->cdf[0], cap, n)) ||
+      (cap = 0, rc = grow(c->cdf[1], cap, n)) || (cap = 0, rc = grow(c->z, cap, dp)) ||
+      (cap = 0, rc = grow(c->v, cap, dp)) || (cap = 0, rc = grow(c->x, cap, dp)) ||
+      (cap = 0, rc = grow(c->y, cap, dp)) || (cap = 0, rc = grow(c->xm, cap, dp)) ||
+      (cap = 0, rc = grow(c->ym, cap, dp)) || (cap = 0, rc = grow(c->scratch_vec, cap, dp)) ||
+      (cap = 0, rc = grow(c->eigvec, cap, dp)) || (cap = 0, rc = grow(c->st, cap, 1)) ||
+      (cap = 0, rc = grow(c->BP, cap, 4 * (dp / 256 + 8))))
+    return bail(rc);
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+    return bail(fail(AMM_E_HIP, "hipEventCreate failed"));
+  if ((rc = ensure_plans(c, 0))) return bail(rc);
+  *out = c;
+  return AMM_OK;
+}
+
+int amm_ctx_destroy(amm_ctx* c) {
+  if (!c) return AMM_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  void* bufs[] = {c->alpha, c->nrm2, c->cnrm2, c->cdf[0], c->cdf[1], c->z,    c->v,
+                  c->x,     c->y,    c->xm,    c->ym,     c->scratch_vec, c->eigvec, c->st,
+                  c->BP,    c->mode, c->eig,   c->err,    c->ts,   c->idx, c->wgt,
+                  c->segc,  c->P1,   c->P2,    c->P2s};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return AMM_OK;
+}
+
+int amm_set_exact_eigenvector(amm_ctx* c, const double* v, uint64_t len) {
+  if (!c) return fail(AMM_E_INVALID, "ctx is NULL");
+  int rc;
+  if ((rc = set_device(c))) return rc;
+  if (!v) {
+    c->has_eigvec = false;
+    return AMM_OK;
+  }
+  if (len != c->d) return fail(AMM_E_INVALID, "eigenvector length must equal d");
+  HIPCHK(hipMemcpy(c->eigvec, v, sizeof(double) * len, hipMemcpyHostToDevice));
+  c->has_eigvec = true;
+  drop_graph(c);
+  return AMM_OK;
+}
+
+int amm_column_stats(amm_ctx* c, float* alpha_h, double* nrm2_h, double* cnrm2_h, double* alpha2_h) {
+  if (!c) return fail(AMM_E_INVALID, "ctx is NULL");
+  int rc;
+  if ((rc = set_device(c))) return rc;
+  if ((rc = enq_stats(c))) return rc;
+  if ((rc = enq_cdf(c, 1, false))) return rc;
+  c->stats_valid = true;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (alpha_h) HIPCHK(hipMemcpy(alpha_h, c->alpha, sizeof(float) * c->n, hipMemcpyDeviceToHost));
+  if (nrm2_h) HIPCHK(hipMemcpy(nrm2_h, c->nrm2, sizeof(double) * c->n, hipMemcpyDeviceToHost));
+  if (cnrm2_h) HIPCHK(hipMemcpy(cnrm2_h, c->cnrm2, sizeof(double) * c->n, hipMemcpyDeviceToHost));
+  if (alpha2_h)
+    HIPCHK(hipMemcpy(alpha2_h, (char*)c->st + offsetof(DevState, alpha2), sizeof(double),
+                     hipMemcpyDeviceToHost));
+  return AMM_OK;
+}
+
+int amm_build_cdf(amm_ctx* c, int32_t centred, uint64_t* cdf_h, uint64_t* total_h, uint64_t* last_h) {
+  if (!c) return fail(AMM_E_INVALID, "ctx is NULL");
+  int rc;
+  if ((rc = set_device(c))) return rc;
+  const int which = centred ? 1 : 0;
+  if (!c->stats_valid) {
+    if ((rc = enq_stats(c))) return rc;
+    c->stats_valid = true;
+  }
+  if ((rc = enq_cdf(c, which, false))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if ((rc = check_status(c))) return rc;
+  if (cdf_h) HIPCHK(hipMemcpy(cdf_h, c->cdf[which], sizeof(uint64_t) * c->n, hipMemcpyDeviceToHost));
+  if (total_h)
+    HIPCHK(hipMemcpy(total_h, (char*)c->st + offsetof(DevState, total) + sizeof(uint64_t) * which,
+                     sizeof(uint64_t), hipMemcpyDeviceToHost));
+  if (last_h)
+    HIPCHK(hipMemcpy(last_h, (char*)c->st + offsetof(DevState, last_pos) + sizeof(uint64_t) * which,
+                     sizeof(uint64_t), hipMemcpyDeviceToHost));
+  return AMM_OK;
+}
+
+int amm_sample(amm_ctx* c, int32_t centred, uint64_t seed, uint32_t iter, uint32_t trial, uint32_t cc,
+               uint32_t* idx_h, float* wgt_h, uint32_t* segc_h) {
+  if (!c) return fail(AMM_E_INVALID, "ctx is NULL");
+  if (cc < 1 || cc > c->n) return fail(AMM_E_INVALID, "c must satisfy 1 <= c <= n");
+  int rc;
+  if ((rc = set_device(c))) return rc;
+  drop_graph(c);
+  if ((rc = ensure_lists(c, 1, cc))) return rc;
+  if (!c->stats_valid) {
+    if ((rc = enq_stats(c))) return rc;
+    c->stats_valid = true;
+  }
+  const int which = centred ? 1 : 0;
+  if ((rc = enq_cdf(c, which, false))) return rc;
+  if ((rc = upload_run_state(c, seed, trial))) return rc;
+  if ((rc = enq_sample(c, which, iter, 1, iter, cc))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if ((rc = check_status(c))) return rc;
+  const size_t nseg = (cc + amm::kSampleSeg - 1) / amm::kSampleSeg;
+  if (idx_h) HIPCHK(hipMemcpy(idx_h, c->idx, sizeof(uint32_t) * cc, hipMemcpyDeviceToHost));
+  if (wgt_h) HIPCHK(hipMemcpy(wgt_h, c->wgt, sizeof(float) * cc, hipMemcpyDeviceToHost));
+  if (segc_h) HIPCHK(hipMemcpy(segc_h, c->segc, sizeof(uint32_t) * nseg, hipMemcpyDeviceToHost));
+  return AMM_OK;
+}
+
+int amm_square_of_A(amm_ctx* c, int32_t kernel, int32_t sanity, uint32_t cc, uint64_t seed, uint32_t iter,
+                    uint32_t trial, const float* x_dev, const float* xm_dev, const float* ym_dev,
+                    float* y_dev) {
+  if (!c) return fail(AMM_E_INVALID, "ctx is NULL");
+  if (kernel != AMM_KERNEL_PREVIOUS && kernel != AMM_KERNEL_PROPOSED)
+    return fail(AMM_E_INVALID, "kernel must be previousAMM or proposedAMM");
+  if (cc < 1 || cc > c->n) return fail(AMM_E_INVALID, "c must satisfy 1 <= c <= n");
+  if (!x_dev || !y_dev) return fail(AMM_E_INVALID, "x and y must be device pointers");
+  int rc;
+  if ((rc = set_device(c))) return rc;
+  drop_graph(c);
+  const bool prop = kernel == AMM_KERNEL_PROPOSED;
+  const bool san = sanity && cc == c->n;
+  const int which = prop ? 1 : 0;
+  if ((rc = ensure_plans(c, cc))) return rc;
+  if ((rc = ensure_lists(c, 1, cc))) return rc;
+  if ((rc = ensure_step(c, 1))) return rc;
+  const size_t db = sizeof(float) * c->d;
+  HIPCHK(hipMemcpyAsync(c->x, x_dev, db, hipMemcpyDeviceToDevice, c->stream));
+  if (xm_dev) HIPCHK(hipMemcpyAsync(c->xm, xm_dev, db, hipMemcpyDeviceToDevice, c->stream));
+  else HIPCHK(hipMemsetAsync(c->xm, 0, sizeof(float) * c->dpad, c->stream));
+  if (ym_dev) HIPCHK(hipMemcpyAsync(c->ym, ym_dev, db, hipMemcpyDeviceToDevice, c->stream));
+  else HIPCHK(hipMemsetAsync(c->ym, 0, sizeof(float) * c->dpad, c->stream));
+  if (!c->stats_valid) {
+    if ((rc = enq_stats(c))) return rc;
+    c->stats_valid = true;
+  }
+  if ((rc = enq_cdf(c, 1, false))) return rc;  // ||alpha||^2
+  if (!prop && !san)
+    if ((rc = enq_cdf(c, 0, false))) return rc;
+  if (prop) {
+    if ((rc = enq_z(c))) return rc;
+    HIPCHK(amm::launch_dots(c->x, c->xm, c->z, c->dpad, c->st, c->BP, c->stream));
+  }
+  if ((rc = upload_run_state(c, seed, trial))) return rc;
+  if (!san)
+    if ((rc = enq_sample(c, which, iter, 1, iter, cc))) return rc;
+  c->h_mode = prop ? amm::kModeAmmProposed : amm::kModeAmmPrevious;
+  HIPCHK(hipMemcpyAsync(c->mode, &c->h_mode, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  if ((rc = enq_amm(c, 0, 0, prop, san, nullptr, 0))) return rc;
+  if ((rc = enq_finish(c, 0, san, c->y, c->ym, 0, 0, 0.0, 1))) return rc;
+  HIPCHK(hipMemcpyAsync(y_dev, c->y, db, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return check_status(c);
+}
+
+int amm_exact_product(amm_ctx* c, const float* x_dev, float* y_dev) {
+  if (!c) return fail(AMM_E_INVALID, "ctx is NULL");
+  if (!x_dev || !y_dev) return fail(AMM_E_INVALID, "x and y must be device pointers");
+  int rc;
+  if ((rc = set_device(c))) return rc;
+  drop_graph(c);
+  if ((rc = ensure_step(c, 1))) return rc;
+  const size_t db = sizeof(float) * c->d;
+  HIPCHK(hipMemcpyAsync(c->x, x_dev, db, hipMemcpyDeviceToDevice, c->stream));
+  c->h_mode = amm::kModeExactNoMemo;
+  HIPCHK(hipMemcpyAsync(c->mode, &c->h_mode, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  if ((rc = enq_exact(c, 0, nullptr, 0))) return rc;
+  if ((rc = enq_finish(c, 0, false, c->y, c->ym, 0, 0, 0.0, 1))) return rc;
+  HIPCHK(hipMemcpyAsync(y_dev, c->y, db, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return AMM_OK;
+}
+
+int amm_power_method(amm_ctx* c, const amm_params* p, double* time_ms, double* eig, double* err) {
+  if (!c) return fail(AMM_E_INVALID, "ctx is NULL");
+  int rc;
+  if ((rc = validate(c, p))) return rc;
+  if ((rc = set_device(c))) return rc;
+  const bool use_graph = !(p->flags & AMM_FLAG_NO_GRAPH);
+  const uint32_t cc = p->kernel == AMM_KERNEL_EXACT ? 0 : p->c;
+  const bool rebuild = !c->graph_valid || !same_key(c->graph_key, *p) || c->list_c != cc;
+  if (rebuild) {
+    drop_graph(c);
+    if ((rc = ensure_plans(c, cc))) return rc;
+    if ((rc = ensure_step(c, p->step))) return rc;
+    if (cc)
+      if ((rc = ensure_lists(c, p->step, cc))) return rc;
+    c->host_modes = static_modes(*p);
+    if (is_adaptive(*p)) {
+      // iteration 0: prev_cos = 0, last_exact = 0
+      const uint64_t P = p->vr_period;
+      const bool due0 = P > 0 && 1 % P == 0;
+      const bool ex0 = p->kernel == AMM_KERNEL_PREVIOUS ? due0 : false;
+      c->host_modes[0] = ex0 ? amm::kModeExactMemo
+                             : (p->kernel == AMM_KERNEL_PROPOSED ? amm::kModeAmmProposed
+                                                                  : amm::kModeAmmPrevious);
+    }
+  }
+  HIPCHK(hipMemcpyAsync(c->mode, c->host_modes.data(), sizeof(uint32_t) * p->step,
+                        hipMemcpyHostToDevice, c->stream));
+  if ((rc = upload_run_state(c, p->seed, p->trial))) return rc;
+  if (use_graph && rebuild) {
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_run(c, *p, c->host_modes);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (ec != hipSuccess) return fail(AMM_E_HIP, std::string("capture failed: ") + hipGetErrorString(ec));
+    c->graph = g;
+    HIPCHK(hipGraphInstantiate(&c->exec, g, nullptr, nullptr, 0));
+    c->graph_valid = true;
+    c->graph_key = *p;
+  }
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  if (use_graph) {
+    HIPCHK(hipGraphLaunch(c->exec, c->stream));
+  } else {
+    if ((rc = enqueue_run(c, *p, c->host_modes))) return rc;
+  }
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  c->last_step = p->step;
+  c->stats_valid = true;
+  if ((rc = check_status(c))) return rc;
+  const size_t S = p->step;
+  if (eig) HIPCHK(hipMemcpy(eig, c->eig, sizeof(double) * S, hipMemcpyDeviceToHost));
+  if (err) HIPCHK(hipMemcpy(err, c->err, sizeof(double) * S, hipMemcpyDeviceToHost));
+  if (time_ms) {
+    std::vector<int64_t> ts(S);
+    int64_t t0 = 0;
+    HIPCHK(hipMemcpy(ts.data(), c->ts, sizeof(int64_t) * S, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&t0, (char*)c->st + offsetof(DevState, t_start), sizeof(int64_t),
+                     hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < S; ++i) time_ms[i] = (double)(ts[i] - t0) / (double)c->wall_khz;
+  }
+  return AMM_OK;
+}
+
+int amm_last_modes(amm_ctx* c, uint32_t* modes_h, uint64_t len) {
+  if (!c || !modes_h) return fail(AMM_E_INVALID, "NULL argument");
+  if (len > c->mode_cap) return fail(AMM_E_INVALID, "len exceeds the last run's step");
+  int rc;
+  if ((rc = set_device(c))) return rc;
+  HIPCHK(hipMemcpy(modes_h, c->mode, sizeof(uint32_t) * len, hipMemcpyDeviceToHost));
+  return AMM_OK;
+}
+
+int amm_get_vector(amm_ctx* c, int32_t which, float* host, uint64_t len) {
+  if (!c || !host) return fail(AMM_E_INVALID, "NULL argument");
+  if (len != c->d) return fail(AMM_E_INVALID, "len must equal d");
+  int rc;
+  if ((rc = set_device(c))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(host, which ? c->y : c->x, sizeof(float) * len, hipMemcpyDeviceToHost));
+  return AMM_OK;
+}
+
+int amm_last_run_ms(amm_ctx* c, double* ms) {
+  if (!c || !ms) return fail(AMM_E_INVALID, "NULL argument");
+  *ms = c->last_ms;
+  return AMM_OK;
+}
+
+int amm_time_product(amm_ctx* c, int32_t kernel, uint32_t cc, uint64_t seed, uint32_t iters,
+                     double* ms_dot, double* ms_axpy, uint64_t* columns) {
+  if (!c) return fail(AMM_E_INVALID, "ctx is NULL");
+  if (kernel < AMM_KERNEL_EXACT || kernel > AMM_KERNEL_PROPOSED)
+    return fail(AMM_E_INVALID, "kernel must be exact, previousAMM or proposedAMM");
+  const bool exact = kernel == AMM_KERNEL_EXACT;
+  if (!exact && (cc < 1 || cc > c->n)) return fail(AMM_E_INVALID, "c must satisfy 1 <= c <= n");
+  if (iters < 1 || iters > 100000) return fail(AMM_E_INVALID, "iters must be in [1, 100000]");
+  int rc;
+  if ((rc = set_device(c))) return rc;
+  drop_graph(c);
+  const bool prop = kernel == AMM_KERNEL_PROPOSED;
+  if ((rc = ensure_plans(c, exact ? 0 : cc))) return rc;
+  if (!exact && (rc = ensure_lists(c, 1, cc))) return rc;
+  if ((rc = ensure_step(c, 1))) return rc;
+  if (!c->stats_valid) {
+    if ((rc = enq_stats(c))) return rc;
+    c->stats_valid = true;
+  }
+  if ((rc = enq_cdf(c, 1, false))) return rc;
+  if (!exact) {
+    if (!prop && (rc = enq_cdf(c, 0, false))) return rc;
+    if ((rc = upload_run_state(c, seed, 0))) return rc;
+    if ((rc = enq_sample(c, prop ? 1 : 0, 0, 1, 0, cc))) return rc;
+  }
+  const int dtype = c->dtype;
+  // the two launches of one product, as enq_exact / enq_amm issue them
+  amm::GatherDotArgs g{};
+  g.A = c->A;
+  g.ld = c->ld;
+  g.d = (uint32_t)c->d;
+  g.idx = exact ? nullptr : c->idx;
+  g.seg_count = exact ? nullptr : c->segc;
+  g.cap = exact ? (uint32_t)c->n : cc;
+  g.x = c->x;
+  g.xm = exact ? nullptr : c->xm;
+  g.R = exact ? c->dot_dense.R : c->dot_list.R;
+  g.rows_per_split = exact ? c->dot_dense.rows_per_split : c->dot_list.rows_per_split;
+  g.P1 = c->P1;
+  amm::GatherAxpyArgs a{};
+  a.A = c->A;
+  a.ld = c->ld;
+  a.d = (uint32_t)c->d;
+  a.dpad = c->dpad;
+  a.idx = g.idx;
+  a.wgt = exact ? nullptr : c->wgt;
+  a.seg_count = g.seg_count;
+  a.cap = g.cap;
+  a.cs = exact ? c->axpy_dense.cs : c->axpy_list.cs;
+  a.S = exact ? c->axpy_dense.S : c->axpy_list.S;
+  a.P1 = c->P1;
+  a.R = g.R;
+  a.alpha = prop ? c->alpha : nullptr;
+  a.st = c->st;
+  a.P2 = c->P2;
+  a.P2s = c->P2s;
+  HIPCHK(amm::launch_gather_dot(dtype, g, c->stream));
+  HIPCHK(amm::launch_gather_axpy(dtype, a, c->stream));
+  hipEvent_t e2 = nullptr;
+  HIPCHK(hipEventCreate(&e2));
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  for (uint32_t i = 0; i < iters; ++i) HIPCHK(amm::launch_gather_dot(dtype, g, c->stream));
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  for (uint32_t i = 0; i < iters; ++i) HIPCHK(amm::launch_gather_axpy(dtype, a, c->stream));
+  HIPCHK(hipEventRecord(e2, c->stream));
+  const hipError_t se = hipEventSynchronize(e2);
+  float t1 = 0.0f, t2 = 0.0f;
+  hipError_t q1 = hipEventElapsedTime(&t1, c->ev0, c->ev1);
+  hipError_t q2 = hipEventElapsedTime(&t2, c->ev1, e2);
+  (void)hipEventDestroy(e2);
+  if (se != hipSuccess || q1 != hipSuccess || q2 != hipSuccess)
+    return fail(AMM_E_HIP, "event timing failed");
+  if (ms_dot) *ms_dot = t1 / iters;
+  if (ms_axpy) *ms_axpy = t2 / iters;
+  if (columns) {
+    if (exact) {
+      *columns = c->n;
+    } else {
+      const size_t nseg = (cc + amm::kSampleSeg - 1) / amm::kSampleSeg;
+      std::vector<uint32_t> sc(nseg);
+      HIPCHK(hipMemcpy(sc.data(), c->segc, sizeof(uint32_t) * nseg, hipMemcpyDeviceToHost));
+      uint64_t u = 0;
+      for (uint32_t v : sc) u += v;
+      *columns = u;
+    }
+  }
+  return check_status(c);
+}
+
+}  // extern "C"

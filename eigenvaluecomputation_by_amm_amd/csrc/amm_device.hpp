@@ -1,0 +1,206 @@
+// amm_device.hpp -- device helpers shared by the AMM power-method kernels
+// (gfx950, wave64).  Philox4x32-10 follows Salmon et al., SC'11; the
+// reductions are fixed butterflies so every result is reproducible run to
+// run.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+namespace amm {
+
+constexpr int kWave = 64;
+
+// Philox counter word 3 names the stream.
+constexpr uint32_t kStreamSample = 1u;
+constexpr uint32_t kStreamX0 = 2u;
+
+// Samples are deduplicated in segments of this many slots (one workgroup
+// sorts one segment in LDS: 8192 x 4 B = 32 KiB).
+constexpr uint32_t kSampleSeg = 8192u;
+
+// Per-iteration step kinds (device mode array).
+enum StepMode : uint32_t {
+  kModeExactMemo = 1,    // exact A(A^T x), snapshot [x, y] -> [x_memo, y_memo]
+  kModeAmmPrevious = 2,  // sampled A_S W A_S^T (x - x_memo) + y_memo
+  kModeAmmProposed = 3,  // mean-shifted sampled product + rank-2 part
+  kModeExactNoMemo = 4,  // exact product, memo untouched
+};
+
+// Status bits of DevState::status.
+constexpr uint32_t kStatusNonFinite = 1u;  // a column weight was NaN or Inf
+
+struct alignas(16) DevState {
+  uint64_t total[2];    // integer CDF totals: [0] plain ||a_j||^2, [1] centred
+  uint64_t last_pos[2]; // last positive-weight column per CDF
+  double alpha2;        // ||alpha||^2
+  double scale;         // 1 / ||y|| of the latest product
+  double s, q;          // proposed: 1^T dx and z^T dx for the next product
+  double cos_prev;      // y^T x / ||y|| of the latest iteration
+  uint64_t last_exact;  // last exact iteration (proposed gate)
+  uint64_t seed;        // Philox key of the run
+  uint32_t trial;       // Philox counter word 2 of the run
+  uint32_t counter[6];  // last-block-done counters (reset by the last block)
+  uint32_t status;      // kStatus* bits, sticky until the host clears them
+  int64_t t_start;      // wall clock at the run's first kernel
+};
+
+__device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) {
+  return __umulhi(a, b);
+}
+
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = mulhi32(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+    const uint32_t hi1 = mulhi32(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+// t = floor(u * total / 2^64) for the 64-bit uniform u = (a << 32) | b.
+__device__ __forceinline__ uint64_t draw_target(uint32_t a, uint32_t b, uint64_t total) {
+  return __umul64hi(((uint64_t)a << 32) | (uint64_t)b, total);
+}
+
+__device__ __forceinline__ int ceil_log2_u64(uint64_t n) {
+  int L = 0;
+  while (L < 63 && (1ull << L) < n) ++L;
+  return L;
+}
+
+// 53-bit uniform in [0,1): ((a>>5)*2^26 + (b>>6)) * 2^-53, exact.
+__device__ __forceinline__ double uniform53(uint32_t a, uint32_t b) {
+  const uint64_t v = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
+  return (double)v * (1.0 / 9007199254740992.0);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Sum over a 256-thread block (4 waves); result valid in thread 0.
+template <typename T>
+__device__ __forceinline__ T block256_sum(T v, T* lds4) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) lds4[wave] = v;
+  __syncthreads();
+  T r = v;
+  if (threadIdx.x == 0) r = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ float bf16_bits_to_f32(uint32_t h16) {
+  return __uint_as_float(h16 << 16);
+}
+
+// Element-type traits: VEC elements per 16-byte load.
+template <typename T>
+struct Elem;
+
+template <>
+struct Elem<float> {
+  static constexpr int VEC = 4;
+  __device__ __forceinline__ static void load_full(const float* p, float (&v)[4]) {
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+  }
+  __device__ __forceinline__ static float load_one(const float* p) { return *p; }
+};
+
+template <>
+struct Elem<__hip_bfloat16> {
+  static constexpr int VEC = 8;
+  __device__ __forceinline__ static void load_full(const __hip_bfloat16* p, float (&v)[8]) {
+    const uint4 q = *reinterpret_cast<const uint4*>(p);
+    v[0] = bf16_bits_to_f32(q.x & 0xFFFFu);
+    v[1] = bf16_bits_to_f32(q.x >> 16);
+    v[2] = bf16_bits_to_f32(q.y & 0xFFFFu);
+    v[3] = bf16_bits_to_f32(q.y >> 16);
+    v[4] = bf16_bits_to_f32(q.z & 0xFFFFu);
+    v[5] = bf16_bits_to_f32(q.z >> 16);
+    v[6] = bf16_bits_to_f32(q.w & 0xFFFFu);
+    v[7] = bf16_bits_to_f32(q.w >> 16);
+  }
+  __device__ __forceinline__ static float load_one(const __hip_bfloat16* p) {
+    return bf16_bits_to_f32(*reinterpret_cast<const uint16_t*>(p));
+  }
+};
+
+// Load VEC consecutive elements starting at row `row` of a column; rows at
+// or beyond d read as zero.
+template <typename T>
+__device__ __forceinline__ void load_col(const T* col, uint32_t row, uint32_t d,
+                                         float (&v)[Elem<T>::VEC]) {
+  constexpr int V = Elem<T>::VEC;
+  if (row + V <= d) {
+    Elem<T>::load_full(col + row, v);
+  } else {
+#pragma unroll
+    for (int k = 0; k < V; ++k) v[k] = (row + k < d) ? Elem<T>::load_one(col + row + k) : 0.0f;
+  }
+}
+
+// Load VEC floats of a padded FP32 vector (length a multiple of 8).
+template <int V>
+__device__ __forceinline__ void load_vecf(const float* p, float (&v)[V]) {
+#pragma unroll
+  for (int k = 0; k < V; k += 4) {
+    const float4 q = *reinterpret_cast<const float4*>(p + k);
+    v[k] = q.x;
+    v[k + 1] = q.y;
+    v[k + 2] = q.z;
+    v[k + 3] = q.w;
+  }
+}
+
+template <int V>
+__device__ __forceinline__ void store_vecf(float* p, const float (&v)[V]) {
+#pragma unroll
+  for (int k = 0; k < V; k += 4)
+    *reinterpret_cast<float4*>(p + k) = make_float4(v[k], v[k + 1], v[k + 2], v[k + 3]);
+}
+
+// Last-block-done: every block calls this once after writing its partials;
+// returns true in exactly one block (the last to arrive), which then owns the
+// final reduction.  The counter is reset by that block for the next launch.
+__device__ __forceinline__ bool last_block_arrive(uint32_t* counter, bool* lds_flag) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t total = gridDim.x * gridDim.y * gridDim.z;
+    const uint32_t prev =
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    *lds_flag = (prev == total - 1);
+  }
+  __syncthreads();
+  const bool last = *lds_flag;
+  if (last) __threadfence();
+  return last;
+}
+
+__device__ __forceinline__ void reset_counter(uint32_t* counter) {
+  __hip_atomic_store(counter, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace amm

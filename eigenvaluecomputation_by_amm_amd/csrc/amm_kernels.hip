@@ -1,0 +1,900 @@
+// amm_kernels.hip -- HIP kernels of the AMM power method for lambda_max(A A^T)
+// on gfx950 (wave64).
+//
+// Matrix layout: A is d x n, column-major, leading dimension ld (column j is
+// the contiguous run A + ld*j, as in the reference's "A + m*idx" addressing,
+// core_previousAMM.cuh:47,219).  Vectors of length d are FP32 and padded to
+// dpad = roundup(d, 8) with zeros, so 16-byte vector loads never leave them.
+//
+// One power iteration is four launches:
+//   gather_dot   P1[r][e] = a_{col(e)}[rows of split r]^T (x - x_memo)[rows]
+//   gather_axpy  t_e = wgt_e (sum_r P1[r][e] - alpha_col s);
+//                P2[s][rows] = sum_{e in chunk s} a_{col(e)}[rows] t_e
+//   finish       y = base + sum_s P2[s]; y^T x, y^T y, y^T v* (FP64);
+//                the last block to arrive writes lambda_i, err_i, cos_i
+//   normalize    x = y / ||y||  (+ s = 1^T dx, q = z^T dx for proposed)
+// The exact step uses the same two contraction kernels over all n columns
+// with unit weights (the sanity-mode identity of device_memory.h:500-551).
+// Every reduction runs in a fixed order, so results do not depend on timing
+// (the reference's kernel2 uses float atomics, core_previousAMM.cuh:259-262).
+#include "amm_device.hpp"
+#include "amm_launch.hpp"
+
+namespace amm {
+
+// ---------------------------------------------------------------------------
+// Column statistics (get_norms, device_memory.h:34-59): alpha_j = mean,
+// nrm2_j = ||a_j||^2, cnrm2_j = ||a_j - alpha_j 1||^2.  One wave per column,
+// coalesced 16-byte loads down the column, FP64 accumulation.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_colstats(const T* __restrict__ A, uint64_t ld, uint32_t d,
+                                                  uint64_t n, float* __restrict__ alpha,
+                                                  double* __restrict__ nrm2,
+                                                  double* __restrict__ cnrm2) {
+  constexpr int V = Elem<T>::VEC;
+  const int lane = threadIdx.x & 63;
+  const uint64_t j = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= n) return;
+  const T* col = A + j * ld;
+  double s = 0.0, ss = 0.0;
+  uint32_t row = lane * V;
+  // full steps: every lane of the wave has V in-range rows
+  for (; row + 64u * V * 3 + V <= d; row += 64u * V * 4) {
+    float a0[V], a1[V], a2[V], a3[V];
+    Elem<T>::load_full(col + row, a0);
+    Elem<T>::load_full(col + row + 64 * V, a1);
+    Elem<T>::load_full(col + row + 128 * V, a2);
+    Elem<T>::load_full(col + row + 192 * V, a3);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const double b0 = a0[k], b1 = a1[k], b2 = a2[k], b3 = a3[k];
+      s += (b0 + b1) + (b2 + b3);
+      ss += (b0 * b0 + b1 * b1) + (b2 * b2 + b3 * b3);
+    }
+  }
+  for (; row < d; row += 64u * V) {
+    float a0[V];
+    load_col<T>(col, row, d, a0);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const double b0 = a0[k];
+      s += b0;
+      ss += b0 * b0;
+    }
+  }
+  s = wave_sum(s);
+  ss = wave_sum(ss);
+  if (lane == 0) {
+    const double mean = s / (double)d;
+    alpha[j] = (float)mean;
+    nrm2[j] = ss;
+    const double c = ss - s * mean;
+    cnrm2[j] = c > 0.0 ? c : 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sampling CDF (set_weights + cub::DeviceScan::InclusiveSum,
+// common.h:34-40, powerMethod_by_previous.cuh:26-32) in exact integer
+// arithmetic, one workgroup of 1024 lanes:
+//   scale = 2^(62 - ceil(log2 n)) / max_j w_j,  q_j = floor(w_j * scale),
+//   cdf_j = q_0 + ... + q_j  (uint64).
+// Integer sums are associative, so this parallel scan (lane sums,
+// Hillis-Steele over lanes in LDS, running sums in the lane) equals any
+// sequential replay bit for bit; the CDF is monotone and q_j = 0 columns can
+// never be drawn.  weights: w_j, or 1 for every column when unit != 0.
+// Also ||alpha||^2 when alpha is given (fixed-order FP64 tree).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_cdf(const double* __restrict__ w, int unit, uint64_t n,
+                                              uint64_t* __restrict__ cdf, DevState* st, int which,
+                                              const float* __restrict__ alpha) {
+  __shared__ double dred[1024];
+  __shared__ uint64_t buf[2][1024];
+  const int t = threadIdx.x;
+  const uint64_t ch = (n + 1023) / 1024;
+  const uint64_t b = (uint64_t)t * ch;
+  const uint64_t e = b + ch < n ? b + ch : n;
+  // 1. max weight, finiteness, ||alpha||^2
+  double mx = 0.0, a2 = 0.0;
+  int bad = 0;
+  for (uint64_t j = b; j < e; ++j) {
+    const double wj = unit ? 1.0 : w[j];
+    if (!(wj >= 0.0) || isinf(wj)) bad = 1;
+    mx = wj > mx ? wj : mx;
+    if (alpha) {
+      const double a = alpha[j];
+      a2 += a * a;
+    }
+  }
+  bad = __syncthreads_or(bad);
+  dred[t] = mx;
+  __syncthreads();
+  for (int off = 512; off > 0; off >>= 1) {
+    if (t < off) dred[t] = dred[t] > dred[t + off] ? dred[t] : dred[t + off];
+    __syncthreads();
+  }
+  mx = dred[0];
+  __syncthreads();
+  const double scale = (!bad && mx > 0.0) ? ldexp(1.0, 62 - ceil_log2_u64(n)) / mx : 0.0;
+  if (bad && t == 0) atomicOr(&st->status, kStatusNonFinite);
+  // 2. lane sums of q_j, inclusive scan over lanes
+  uint64_t s = 0;
+  for (uint64_t j = b; j < e; ++j) s += (uint64_t)((unit ? 1.0 : w[j]) * scale);
+  buf[0][t] = s;
+  __syncthreads();
+  int cur = 0;
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint64_t v = t >= off ? buf[cur][t] + buf[cur][t - off] : buf[cur][t];
+    buf[cur ^ 1][t] = v;
+    cur ^= 1;
+    __syncthreads();
+  }
+  // 3. running sums in the lane
+  uint64_t run = t == 0 ? 0 : buf[cur][t - 1];
+  uint64_t lp = 0;
+  for (uint64_t j = b; j < e; ++j) {
+    const uint64_t q = (uint64_t)((unit ? 1.0 : w[j]) * scale);
+    run += q;
+    cdf[j] = run;
+    if (q) lp = j;
+  }
+  if (b < n && e == n) st->total[which] = run;
+  __syncthreads();
+  buf[0][t] = lp;  // last positive column: max over lanes
+  __syncthreads();
+  for (int off = 512; off > 0; off >>= 1) {
+    if (t < off) buf[0][t] = buf[0][t] > buf[0][t + off] ? buf[0][t] : buf[0][t + off];
+    __syncthreads();
+  }
+  if (t == 0) st->last_pos[which] = buf[0][0];
+  if (alpha) {
+    dred[t] = a2;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+      if (t < off) dred[t] = dred[t] + dred[t + off];
+      __syncthreads();
+    }
+    if (t == 0) st->alpha2 = dred[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sampler (get_rand + pick_index + cub SortKeys + update_weights,
+// device_memory.h:506-545, powerMethod_by_previous.cuh:56-70,
+// common.h:42-53).  Slot k of iteration it takes the 64-bit uniform u from
+// Philox counter (k>>1, it, trial, kStreamSample), words 2(k&1), 2(k&1)+1;
+// t = floor(u * total / 2^64); column = min{j : cdf_j > t}.  One workgroup
+// per (segment, iteration) sorts its slots in LDS (bitonic), merges
+// duplicates and writes the unique columns in ascending order with weight
+// mult * total / (c * q_j), q_j = cdf_j - cdf_{j-1}.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t cdf_search(const uint64_t* __restrict__ cdf, uint64_t n,
+                                               uint64_t t, uint64_t last_pos) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = lo + ((hi - lo) >> 1);
+    if (cdf[mid] > t) hi = mid; else lo = mid + 1;
+  }
+  return (uint32_t)(lo < n ? lo : last_pos);
+}
+
+__global__ __launch_bounds__(1024) void k_sample(const uint64_t* __restrict__ cdf, uint64_t n,
+                                                 const DevState* st, int which, uint32_t it0,
+                                                 uint32_t buf_it0, uint32_t c,
+                                                 uint32_t* __restrict__ idx_out,
+                                                 float* __restrict__ wgt_out,
+                                                 uint32_t* __restrict__ seg_count) {
+  __shared__ uint32_t keys[kSampleSeg];
+  __shared__ uint32_t scan[2][1024];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t g = blockIdx.x;
+  const uint32_t it = it0 + blockIdx.y;
+  const uint32_t nseg = (c + kSampleSeg - 1) / kSampleSeg;
+  const uint32_t b = g * kSampleSeg;
+  const uint32_t cnt = (c - b) < kSampleSeg ? (c - b) : kSampleSeg;
+  const uint64_t total = st->total[which];
+  const uint64_t last_pos = st->last_pos[which];
+  const uint64_t seed = st->seed;
+  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  const uint32_t trial = st->trial;
+
+  // draw: one Philox call serves two consecutive slots
+  for (uint32_t p = tid; 2 * p < kSampleSeg; p += 1024) {
+    const uint32_t l0 = 2 * p;
+    if (l0 < cnt) {
+      const uint32_t k = b + l0;
+      uint32_t r[4];
+      philox4x32_10(k >> 1, it, trial, kStreamSample, k0, k1, r);
+      keys[l0] = cdf_search(cdf, n, draw_target(r[0], r[1], total), last_pos);
+      keys[l0 + 1] = (l0 + 1 < cnt) ? cdf_search(cdf, n, draw_target(r[2], r[3], total), last_pos)
+                                    : 0xFFFFFFFFu;
+    } else {
+      keys[l0] = 0xFFFFFFFFu;
+      keys[l0 + 1] = 0xFFFFFFFFu;
+    }
+  }
+  __syncthreads();
+
+  // bitonic sort of the first N = next_pow2(cnt) keys (the rest are sentinels)
+  uint32_t N = 1;
+  while (N < cnt) N <<= 1;
+  for (uint32_t k = 2; k <= N; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = tid; i < N; i += 1024) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const uint32_t a = keys[i], bb = keys[ixj];
+          const bool up = (i & k) == 0;
+          if ((a > bb) == up) {
+            keys[i] = bb;
+            keys[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // run starts: thread tid owns positions [tid*8, tid*8+8)
+  constexpr uint32_t E = kSampleSeg / 1024;
+  uint32_t flags = 0, nf = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < E; ++q) {
+    const uint32_t p = tid * E + q;
+    const bool f = p < cnt && (p == 0 || keys[p] != keys[p - 1]);
+    flags |= (f ? 1u : 0u) << q;
+    nf += f ? 1u : 0u;
+  }
+  scan[0][tid] = nf;
+  __syncthreads();
+  int cur = 0;
+  for (uint32_t off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = tid >= off ? scan[cur][tid] + scan[cur][tid - off] : scan[cur][tid];
+    scan[cur ^ 1][tid] = v;
+    cur ^= 1;
+    __syncthreads();
+  }
+  uint32_t pos = scan[cur][tid] - nf;  // exclusive prefix
+  const uint64_t base = (uint64_t)(it - buf_it0) * c + b;
+#pragma unroll
+  for (uint32_t q = 0; q < E; ++q) {
+    if (flags & (1u << q)) {
+      const uint32_t p = tid * E + q;
+      const uint32_t key = keys[p];
+      // end of the run: first position in [p, cnt) whose key differs
+      uint32_t lo = p, hi = cnt;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (keys[mid] > key) hi = mid; else lo = mid + 1;
+      }
+      const uint32_t mult = lo - p;
+      const uint64_t qj = cdf[key] - (key ? cdf[key - 1] : 0ull);
+      const double num = (double)mult * (double)total;
+      const double den = (double)c * (double)qj;
+      idx_out[base + pos] = key;
+      wgt_out[base + pos] = den > 0.0 ? (float)(num / den) : 0.0f;
+      ++pos;
+    }
+  }
+  if (tid == 1023) seg_count[(uint64_t)(it - buf_it0) * nseg + g] = scan[cur][1023];
+}
+
+// ---------------------------------------------------------------------------
+// x0 ~ N(0, I) (curandGenerateNormal, powerMethod_by_exact.cuh:32): Box-Muller
+// on Philox counter (m, 0, trial, kStreamX0) for rows 2m, 2m+1.  Writes the
+// raw vector into y and 1/||y|| into st->scale; k_normalize then forms x
+// (normalize<512>, common.h:156-183).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_x0(float* __restrict__ y, uint32_t d, uint64_t dpad,
+                                            DevState* st, double* __restrict__ BP) {
+  __shared__ double red[4];
+  __shared__ bool last;
+  const uint64_t m = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const uint64_t seed = st->seed;
+  double acc = 0.0;
+  if (2 * m < dpad) {
+    float g0 = 0.0f, g1 = 0.0f;
+    if (2 * m < d) {
+      uint32_t r[4];
+      philox4x32_10((uint32_t)m, 0u, st->trial, kStreamX0, (uint32_t)seed, (uint32_t)(seed >> 32),
+                    r);
+      const double u1 = uniform53(r[0], r[1]);
+      const double u2 = uniform53(r[2], r[3]);
+      const double rad = sqrt(-2.0 * log(1.0 - u1));
+      const double th = 6.283185307179586 * u2;
+      g0 = (float)(rad * cos(th));
+      if (2 * m + 1 < d) g1 = (float)(rad * sin(th));
+    }
+    y[2 * m] = g0;
+    y[2 * m + 1] = g1;
+    acc = (double)g0 * (double)g0 + (double)g1 * (double)g1;
+  }
+  acc = block256_sum(acc, red);
+  if (threadIdx.x == 0) BP[blockIdx.x] = acc;
+  if (last_block_arrive(&st->counter[2], &last)) {
+    double s = 0.0;
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += 256) s += BP[i];
+    s = block256_sum(s, red);
+    if (threadIdx.x == 0) {
+      st->scale = s > 0.0 ? 1.0 / sqrt(s) : 0.0;
+      reset_counter(&st->counter[2]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Run prologue: zero x_memo / y_memo (zero_clear, device_memory.h:345-355),
+// reset the scalar state and stamp the start time.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_begin(float* __restrict__ xm, float* __restrict__ ym,
+                                               uint64_t dpad, DevState* st,
+                                               uint32_t* __restrict__ mode, uint32_t mode0) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->t_start = wall_clock64();
+    st->cos_prev = 0.0;
+    st->last_exact = 0;
+    st->s = 0.0;
+    st->q = 0.0;
+    st->status = 0;
+    if (mode0) mode[0] = mode0;
+  }
+  if (i < dpad) {
+    xm[i] = 0.0f;
+    ym[i] = 0.0f;
+  }
+}
+
+__device__ __forceinline__ bool gate_open(const uint32_t* mode, uint32_t it, uint32_t mask) {
+  return mode == nullptr || ((mask >> mode[it]) & 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Contraction 1 (previousAMM_kernel1 / proposedAMM_kernel1,
+// core_previousAMM.cuh:4-175, core_proposedAMM.cuh:17-213): for list entry e,
+// P1[r][e] = a_{col(e)}[split r]^T (x - x_memo)[split r].  One wave per CPW
+// entries; lanes stride down the column with 16-byte loads; wave butterfly.
+// LIST: col(e) = idx[e], valid when (e % seg) < seg_count[e / seg];
+// dense: col(e) = e (exact step, sanity mode).
+// ---------------------------------------------------------------------------
+template <typename T, int CPW, bool LIST>
+__global__ __launch_bounds__(256) void k_gather_dot(
+    const T* __restrict__ A, uint64_t ld, uint32_t d, const uint32_t* __restrict__ idx,
+    const uint32_t* __restrict__ seg_count, uint32_t cap, const float* __restrict__ x,
+    const float* __restrict__ xm, uint32_t rows_per_split, float* __restrict__ P1,
+    const uint32_t* __restrict__ mode, uint32_t it, uint32_t gate_mask) {
+  if (!gate_open(mode, it, gate_mask)) return;
+  constexpr int V = Elem<T>::VEC;
+  const int lane = threadIdx.x & 63;
+  const uint32_t e0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * CPW;
+  const uint32_t r = blockIdx.y;
+  const uint32_t rb = r * rows_per_split;
+  const uint32_t re = (rb + rows_per_split < d) ? rb + rows_per_split : d;
+  const T* col[CPW];
+  bool valid[CPW];
+  bool any = false;
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) {
+    const uint32_t e = e0 + c;
+    bool v = e < cap;
+    uint32_t j = e;
+    if (LIST && v) {
+      v = (e % kSampleSeg) < seg_count[e / kSampleSeg];
+      if (v) j = idx[e];
+    }
+    valid[c] = v;
+    col[c] = A + (uint64_t)(v ? j : 0) * ld;
+    any |= v;
+  }
+  if (!any) return;
+  float acc[CPW];
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) acc[c] = 0.0f;
+  uint32_t row = rb + lane * V;
+  constexpr uint32_t STEP = 64u * V;
+  // two wave-steps per trip: all loads of both issued before the FMAs
+  for (; row + STEP + V <= re; row += 2 * STEP) {
+    float x0[V], x1[V], m0[V], m1[V];
+    load_vecf<V>(x + row, x0);
+    load_vecf<V>(x + row + STEP, x1);
+    if (xm) {
+      load_vecf<V>(xm + row, m0);
+      load_vecf<V>(xm + row + STEP, m1);
+#pragma unroll
+      for (int k = 0; k < V; ++k) {
+        x0[k] -= m0[k];
+        x1[k] -= m1[k];
+      }
+    }
+    float a0[CPW][V], a1[CPW][V];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      if (valid[c]) {
+        Elem<T>::load_full(col[c] + row, a0[c]);
+        Elem<T>::load_full(col[c] + row + STEP, a1[c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      if (valid[c]) {
+        float s0 = acc[c];
+#pragma unroll
+        for (int k = 0; k < V; ++k) s0 = fmaf(a0[c][k], x0[k], s0);
+#pragma unroll
+        for (int k = 0; k < V; ++k) s0 = fmaf(a1[c][k], x1[k], s0);
+        acc[c] = s0;
+      }
+    }
+  }
+  for (; row < re; row += STEP) {
+    float x0[V];
+    load_vecf<V>(x + row, x0);
+    if (xm) {
+      float m0[V];
+      load_vecf<V>(xm + row, m0);
+#pragma unroll
+      for (int k = 0; k < V; ++k) x0[k] -= m0[k];
+    }
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      if (valid[c]) {
+        float a0[V];
+        load_col<T>(col[c], row, re, a0);
+#pragma unroll
+        for (int k = 0; k < V; ++k) acc[c] = fmaf(a0[k], x0[k], acc[c]);
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) {
+    const float s = wave_sum(acc[c]);
+    if (lane == 0 && valid[c]) P1[(uint64_t)r * cap + e0 + c] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Contraction 2 (previousAMM_kernel2 / proposedAMM_kernel2,
+// core_previousAMM.cuh:177-263, core_proposedAMM.cuh:215-313): workgroup
+// (row block, chunk s) forms t_e for the chunk's entries in LDS and
+// accumulates P2[s][rows] = sum_e a_{col(e)}[rows] t_e.  The proposed
+// mean shift (a_e - alpha_e 1) t_e is split into the raw product plus the
+// scalar sum_e alpha_e t_e, written per chunk to P2s[s].
+//   t_e = wgt_e * (sum_r P1[r][e] - alpha_col * s)     (P1 source)
+//   t_e = tdirect[col]                                  (direct, z = A alpha)
+// ---------------------------------------------------------------------------
+constexpr uint32_t kMaxChunk = 2048;
+
+template <typename T, bool LIST>
+__global__ __launch_bounds__(256) void k_gather_axpy(
+    const T* __restrict__ A, uint64_t ld, uint32_t d, uint64_t dpad,
+    const uint32_t* __restrict__ idx, const float* __restrict__ wgt,
+    const uint32_t* __restrict__ seg_count, uint32_t cap, uint32_t cs, const float* __restrict__ P1,
+    uint32_t R, const float* __restrict__ tdirect, const float* __restrict__ alpha,
+    const DevState* __restrict__ st, float* __restrict__ P2, double* __restrict__ P2s,
+    const uint32_t* __restrict__ mode, uint32_t it, uint32_t gate_mask) {
+  if (!gate_open(mode, it, gate_mask)) return;
+  constexpr int V = Elem<T>::VEC;
+  __shared__ float tl[kMaxChunk];
+  __shared__ uint32_t cl[kMaxChunk];
+  __shared__ double red[4];
+  const uint32_t s = blockIdx.y;
+  const uint32_t e0 = s * cs;
+  uint32_t e1 = e0 + cs < cap ? e0 + cs : cap;
+  if (LIST && e0 < cap) {
+    const uint32_t g = e0 / kSampleSeg;
+    const uint32_t vend = g * kSampleSeg + seg_count[g];
+    e1 = e1 < vend ? e1 : vend;
+  }
+  const uint32_t nv = e1 > e0 ? e1 - e0 : 0;
+  const double sx = alpha ? st->s : 0.0;
+  double corr = 0.0;
+  for (uint32_t k = threadIdx.x; k < nv; k += 256) {
+    const uint32_t e = e0 + k;
+    const uint32_t j = LIST ? idx[e] : e;
+    float t;
+    if (tdirect) {
+      t = tdirect[j];
+    } else {
+      double rr = 0.0;
+      for (uint32_t q = 0; q < R; ++q) rr += (double)P1[(uint64_t)q * cap + e];
+      const double aj = alpha ? (double)alpha[j] : 0.0;
+      const double wj = (LIST && wgt) ? (double)wgt[e] : 1.0;
+      const double tt = wj * (rr - aj * sx);
+      t = (float)tt;
+      corr += aj * (double)t;
+    }
+    tl[k] = t;
+    cl[k] = j;
+  }
+  if (alpha && blockIdx.x == 0) {
+    corr = block256_sum(corr, red);
+    if (threadIdx.x == 0) P2s[s] = corr;
+  }
+  __syncthreads();
+  const uint64_t row64 = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * V;
+  if (row64 >= dpad) return;
+  const uint32_t row = (uint32_t)row64;
+  float acc[V];
+#pragma unroll
+  for (int k = 0; k < V; ++k) acc[k] = 0.0f;
+  const T* Ar = A + row;
+  if (row + V <= d) {
+    uint32_t k = 0;
+    for (; k + 8 <= nv; k += 8) {
+      float a[8][V];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) Elem<T>::load_full(Ar + (uint64_t)cl[k + u] * ld, a[u]);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const float t = tl[k + u];
+#pragma unroll
+        for (int q = 0; q < V; ++q) acc[q] = fmaf(a[u][q], t, acc[q]);
+      }
+    }
+    for (; k < nv; ++k) {
+      float a[V];
+      Elem<T>::load_full(Ar + (uint64_t)cl[k] * ld, a);
+      const float t = tl[k];
+#pragma unroll
+      for (int q = 0; q < V; ++q) acc[q] = fmaf(a[q], t, acc[q]);
+    }
+  } else {
+    for (uint32_t k = 0; k < nv; ++k) {
+      float a[V];
+      load_col<T>(A + (uint64_t)cl[k] * ld, row, d, a);
+      const float t = tl[k];
+#pragma unroll
+      for (int q = 0; q < V; ++q) acc[q] = fmaf(a[q], t, acc[q]);
+    }
+  }
+  store_vecf<V>(P2 + (uint64_t)s * dpad + row, acc);
+}
+
+// ---------------------------------------------------------------------------
+// Finish one product and its Rayleigh quotient (dotProduct_and_generate_x,
+// common.h:89-154): y = base + sum_s P2[s] with
+//   exact:    base = 0
+//   previous: base = y_memo
+//   proposed: base = y_memo + s v + q - sum_s P2s[s]
+// (core_proposedAMM.cuh:198-212), then FP64 y^T x, y^T y, y^T v*.  Exact
+// steps with memo snapshot [x, y] into [x_memo, y_memo]
+// (powerMethod_by_previous.cuh:157).  The last block writes lambda_i = y^T x,
+// err_i = y^T v* / ||y||, cos_i = y^T x / ||y||, 1/||y||, the time stamp, and
+// (adaptive variance reduction) the next iteration's step kind.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool exact_step_due(uint32_t kernel, uint64_t i, uint64_t period,
+                                               double thr, double prev_cos, uint64_t last_exact) {
+  // mirrors orc_is_exact_step: powerMethod_by_previous.cuh:135-138,
+  // powerMethod_by_proposed.cuh:159-161
+  const bool vr = period > 0 || thr > 0.0;
+  const bool due = (period > 0 && (i + 1) % period == 0) || (thr > 0.0 && prev_cos > 1.0 - thr);
+  if (kernel == 1) return vr && due;
+  if (kernel == 2) return (i > last_exact + 1) && vr && due;
+  return false;
+}
+
+__global__ __launch_bounds__(256) void k_finish(
+    const float* __restrict__ P2, uint32_t S_exact, uint32_t S_amm, uint64_t dpad, uint32_t d,
+    const float* __restrict__ x, float* __restrict__ y, float* __restrict__ xm, float* __restrict__ ym,
+    const float* __restrict__ v, const double* __restrict__ P2s, const double* __restrict__ eigvec,
+    double* __restrict__ BP, DevState* st, uint32_t* __restrict__ mode, uint32_t it,
+    double* __restrict__ eig, double* __restrict__ err, int64_t* __restrict__ ts,
+    uint32_t adapt_kernel, uint64_t period, double thr, uint32_t step) {
+  __shared__ double red[4];
+  __shared__ double corr_l;
+  __shared__ bool last;
+  const uint32_t md = mode[it];
+  const bool prop = md == kModeAmmProposed;
+  const bool memo_base = md == kModeAmmPrevious || md == kModeAmmProposed;
+  const uint32_t S = memo_base ? S_amm : S_exact;
+  if (prop) {
+    double c = 0.0;
+    for (uint32_t i = threadIdx.x; i < S; i += 256) c += P2s[i];
+    c = block256_sum(c, red);
+    if (threadIdx.x == 0) corr_l = c;
+    __syncthreads();
+  }
+  const double sx = prop ? st->s : 0.0, qx = prop ? st->q : 0.0;
+  const double corr = prop ? corr_l : 0.0;
+  double yx = 0.0, yy = 0.0, yv = 0.0;
+  const uint64_t r0 = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * 4;
+  if (r0 < dpad) {
+    double acc[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[k] = 0.0;
+    if (memo_base) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc[k] = ym[r0 + k];
+    }
+    if (prop) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc[k] += sx * (double)v[r0 + k] + qx - corr;
+    }
+    for (uint32_t s = 0; s < S; ++s) {
+      const float4 p = *reinterpret_cast<const float4*>(P2 + (uint64_t)s * dpad + r0);
+      acc[0] += p.x;
+      acc[1] += p.y;
+      acc[2] += p.z;
+      acc[3] += p.w;
+    }
+    float yf[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      yf[k] = (r0 + k < d) ? (float)acc[k] : 0.0f;
+      const double yd = yf[k];
+      yx += yd * (double)x[r0 + k];
+      yy += yd * yd;
+      if (eigvec && r0 + k < d) yv += yd * eigvec[r0 + k];
+    }
+    *reinterpret_cast<float4*>(y + r0) = make_float4(yf[0], yf[1], yf[2], yf[3]);
+    if (md == kModeExactMemo) {
+      *reinterpret_cast<float4*>(xm + r0) = *reinterpret_cast<const float4*>(x + r0);
+      *reinterpret_cast<float4*>(ym + r0) = make_float4(yf[0], yf[1], yf[2], yf[3]);
+    }
+  }
+  yx = block256_sum(yx, red);
+  yy = block256_sum(yy, red);
+  yv = block256_sum(yv, red);
+  if (threadIdx.x == 0) {
+    BP[3 * blockIdx.x + 0] = yx;
+    BP[3 * blockIdx.x + 1] = yy;
+    BP[3 * blockIdx.x + 2] = yv;
+  }
+  if (last_block_arrive(&st->counter[0], &last)) {
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += 256) {
+      a += BP[3 * i];
+      b += BP[3 * i + 1];
+      c += BP[3 * i + 2];
+    }
+    a = block256_sum(a, red);
+    b = block256_sum(b, red);
+    c = block256_sum(c, red);
+    if (threadIdx.x == 0) {
+      const double scale = b > 0.0 ? 1.0 / sqrt(b) : 0.0;
+      eig[it] = a;
+      err[it] = c * scale;
+      st->scale = scale;
+      const double cs = a * scale;
+      st->cos_prev = cs;
+      uint64_t le = st->last_exact;
+      if (md == kModeExactMemo || md == kModeExactNoMemo) {
+        le = it;
+        st->last_exact = le;
+      }
+      if (adapt_kernel && it + 1 < step) {
+        const bool ex = exact_step_due(adapt_kernel, it + 1, period, thr, cs, le);
+        mode[it + 1] = ex ? kModeExactMemo : (adapt_kernel == 2 ? kModeAmmProposed : kModeAmmPrevious);
+      }
+      ts[it] = wall_clock64();
+      reset_counter(&st->counter[0]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x = y / ||y|| (the generate-x half of dotProduct_and_generate_x,
+// common.h:146-153); with z given, also s = 1^T (x - x_memo) and
+// q = z^T (x - x_memo) for the next proposed product (dots<128>,
+// common.h:55-87).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_normalize(const float* __restrict__ y,
+                                                   float* __restrict__ x, uint32_t d,
+                                                   uint64_t dpad, DevState* st,
+                                                   const float* __restrict__ xm,
+                                                   const float* __restrict__ z,
+                                                   double* __restrict__ BP) {
+  __shared__ double red[4];
+  __shared__ bool last;
+  const double scale = st->scale;
+  double ps = 0.0, pq = 0.0;
+  const uint64_t r0 = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * 4;
+  if (r0 < dpad) {
+    const float4 yv = *reinterpret_cast<const float4*>(y + r0);
+    float xv[4] = {(float)(scale * (double)yv.x), (float)(scale * (double)yv.y),
+                   (float)(scale * (double)yv.z), (float)(scale * (double)yv.w)};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (r0 + k >= d) xv[k] = 0.0f;
+    *reinterpret_cast<float4*>(x + r0) = make_float4(xv[0], xv[1], xv[2], xv[3]);
+    if (z) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double dx = (double)xv[k] - (double)xm[r0 + k];
+        ps += dx;
+        pq += (double)z[r0 + k] * dx;
+      }
+    }
+  }
+  if (!z) return;
+  ps = block256_sum(ps, red);
+  pq = block256_sum(pq, red);
+  if (threadIdx.x == 0) {
+    BP[2 * blockIdx.x] = ps;
+    BP[2 * blockIdx.x + 1] = pq;
+  }
+  if (last_block_arrive(&st->counter[1], &last)) {
+    double a = 0.0, b = 0.0;
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += 256) {
+      a += BP[2 * i];
+      b += BP[2 * i + 1];
+    }
+    a = block256_sum(a, red);
+    b = block256_sum(b, red);
+    if (threadIdx.x == 0) {
+      st->s = a;
+      st->q = b;
+      reset_counter(&st->counter[1]);
+    }
+  }
+}
+
+// s = 1^T (x - x_memo), q = z^T (x - x_memo) for a stand-alone proposed
+// product (dots<128>, common.h:55-87); FP64, fixed-order reduction.
+__global__ __launch_bounds__(256) void k_dots(const float* __restrict__ x,
+                                              const float* __restrict__ xm,
+                                              const float* __restrict__ z, uint64_t dpad,
+                                              DevState* st, double* __restrict__ BP) {
+  __shared__ double red[4];
+  __shared__ bool last;
+  double ps = 0.0, pq = 0.0;
+  const uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r < dpad) {
+    const double dx = (double)x[r] - (double)xm[r];
+    ps = dx;
+    pq = (double)z[r] * dx;
+  }
+  ps = block256_sum(ps, red);
+  pq = block256_sum(pq, red);
+  if (threadIdx.x == 0) {
+    BP[2 * blockIdx.x] = ps;
+    BP[2 * blockIdx.x + 1] = pq;
+  }
+  if (last_block_arrive(&st->counter[3], &last)) {
+    double a = 0.0, b = 0.0;
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += 256) {
+      a += BP[2 * i];
+      b += BP[2 * i + 1];
+    }
+    a = block256_sum(a, red);
+    b = block256_sum(b, red);
+    if (threadIdx.x == 0) {
+      st->s = a;
+      st->q = b;
+      reset_counter(&st->counter[3]);
+    }
+  }
+}
+
+// z = sum_s P2[s] (z = A alpha, powerMethod_by_proposed.cuh:132-139) and
+// v = z - ||alpha||^2 (generate_v, core_proposedAMM.cuh:4-15).
+__global__ __launch_bounds__(256) void k_sum_rows(const float* __restrict__ P2, uint32_t S,
+                                                  uint64_t dpad, uint32_t d, float* __restrict__ z,
+                                                  float* __restrict__ v, const DevState* st) {
+  const uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= dpad) return;
+  double acc = 0.0;
+  for (uint32_t s = 0; s < S; ++s) acc += (double)P2[(uint64_t)s * dpad + r];
+  const float zf = r < d ? (float)acc : 0.0f;
+  z[r] = zf;
+  if (v) v[r] = r < d ? (float)((double)zf - st->alpha2) : 0.0f;
+}
+
+// Plain fill kernel (graph-safe replacement for memset nodes).
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t val) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = val;
+}
+
+// ===========================================================================
+// Host launchers
+// ===========================================================================
+static inline unsigned grid1(uint64_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
+
+hipError_t launch_colstats(int dtype, const void* A, uint64_t ld, uint32_t d, uint64_t n,
+                           float* alpha, double* nrm2, double* cnrm2, hipStream_t s) {
+  const dim3 g(grid1(n, 4));
+  if (dtype == 1)
+    k_colstats<__hip_bfloat16><<<g, 256, 0, s>>>((const __hip_bfloat16*)A, ld, d, n, alpha, nrm2, cnrm2);
+  else
+    k_colstats<float><<<g, 256, 0, s>>>((const float*)A, ld, d, n, alpha, nrm2, cnrm2);
+  return hipGetLastError();
+}
+
+hipError_t launch_cdf(const double* w, int unit, uint64_t n, uint64_t* cdf, DevState* st, int which,
+                      const float* alpha, hipStream_t s) {
+  k_cdf<<<1, 1024, 0, s>>>(w, unit, n, cdf, st, which, alpha);
+  return hipGetLastError();
+}
+
+hipError_t launch_sample(const uint64_t* cdf, uint64_t n, const DevState* st,
+                         int which, uint32_t it0, uint32_t nit, uint32_t buf_it0, uint32_t c,
+                         uint32_t* idx, float* wgt, uint32_t* seg_count, hipStream_t s) {
+  if (nit == 0) return hipSuccess;
+  const dim3 g((c + kSampleSeg - 1) / kSampleSeg, nit);
+  k_sample<<<g, 1024, 0, s>>>(cdf, n, st, which, it0, buf_it0, c, idx, wgt, seg_count);
+  return hipGetLastError();
+}
+
+hipError_t launch_x0(float* y, uint32_t d, uint64_t dpad, DevState* st, double* BP, hipStream_t s) {
+  k_x0<<<grid1(dpad / 2, 256), 256, 0, s>>>(y, d, dpad, st, BP);
+  return hipGetLastError();
+}
+
+hipError_t launch_begin(float* xm, float* ym, uint64_t dpad, DevState* st, uint32_t* mode,
+                        uint32_t mode0, hipStream_t s) {
+  k_begin<<<grid1(dpad, 256), 256, 0, s>>>(xm, ym, dpad, st, mode, mode0);
+  return hipGetLastError();
+}
+
+template <typename T, int CPW, bool LIST>
+static void gd(const GatherDotArgs& a, hipStream_t s) {
+  const dim3 g((a.cap + 4 * CPW - 1) / (4 * CPW), a.R);
+  k_gather_dot<T, CPW, LIST><<<g, 256, 0, s>>>((const T*)a.A, a.ld, a.d, a.idx, a.seg_count, a.cap,
+                                               a.x, a.xm, a.rows_per_split, a.P1, a.mode, a.it,
+                                               a.gate_mask);
+}
+
+hipError_t launch_gather_dot(int dtype, const GatherDotArgs& a, hipStream_t s) {
+  const bool list = a.idx != nullptr;
+  if (dtype == 1) {
+    if (list) gd<__hip_bfloat16, 2, true>(a, s); else gd<__hip_bfloat16, 4, false>(a, s);
+  } else {
+    if (list) gd<float, 2, true>(a, s); else gd<float, 4, false>(a, s);
+  }
+  return hipGetLastError();
+}
+
+template <typename T, bool LIST>
+static void ga(const GatherAxpyArgs& a, hipStream_t s) {
+  constexpr int V = Elem<T>::VEC;
+  const dim3 g(grid1(a.dpad, 256 * V), a.S);
+  k_gather_axpy<T, LIST><<<g, 256, 0, s>>>((const T*)a.A, a.ld, a.d, a.dpad, a.idx, a.wgt,
+                                           a.seg_count, a.cap, a.cs, a.P1, a.R, a.tdirect, a.alpha,
+                                           a.st, a.P2, a.P2s, a.mode, a.it, a.gate_mask);
+}
+
+hipError_t launch_gather_axpy(int dtype, const GatherAxpyArgs& a, hipStream_t s) {
+  if (a.cs == 0 || a.cs > kMaxChunk) return hipErrorInvalidValue;
+  const bool list = a.idx != nullptr;
+  if (dtype == 1) {
+    if (list) ga<__hip_bfloat16, true>(a, s); else ga<__hip_bfloat16, false>(a, s);
+  } else {
+    if (list) ga<float, true>(a, s); else ga<float, false>(a, s);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_finish(const FinishArgs& a, hipStream_t s) {
+  k_finish<<<grid1(a.dpad, 1024), 256, 0, s>>>(a.P2, a.S_exact, a.S_amm, a.dpad, a.d, a.x, a.y, a.xm, a.ym, a.v,
+                                               a.P2s, a.eigvec, a.BP, a.st, a.mode, a.it, a.eig,
+                                               a.err, a.ts, a.adapt_kernel, a.period, a.thr, a.step);
+  return hipGetLastError();
+}
+
+hipError_t launch_normalize(const float* y, float* x, uint32_t d, uint64_t dpad, DevState* st,
+                            const float* xm, const float* z, double* BP, hipStream_t s) {
+  k_normalize<<<grid1(dpad, 1024), 256, 0, s>>>(y, x, d, dpad, st, xm, z, BP);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_rows(const float* P2, uint32_t S, uint64_t dpad, uint32_t d, float* z, float* v,
+                           const DevState* st, hipStream_t s) {
+  k_sum_rows<<<grid1(dpad, 256), 256, 0, s>>>(P2, S, dpad, d, z, v, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_dots(const float* x, const float* xm, const float* z, uint64_t dpad, DevState* st,
+                       double* BP, hipStream_t s) {
+  k_dots<<<grid1(dpad, 256), 256, 0, s>>>(x, xm, z, dpad, st, BP);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t val, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_fill_u32<<<grid1(n, 256), 256, 0, s>>>(p, n, val);
+  return hipGetLastError();
+}
+
+}  // namespace amm

@@ -1,0 +1,98 @@
+// amm_launch.hpp -- host-side launchers of the AMM kernels (amm_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace amm {
+
+struct DevState;
+
+struct GatherDotArgs {
+  const void* A;
+  uint64_t ld;
+  uint32_t d;
+  const uint32_t* idx;        // null: dense (column e for entry e)
+  const uint32_t* seg_count;  // per-segment unique counts (list mode)
+  uint32_t cap;               // list capacity (c) or n
+  const float* x;
+  const float* xm;            // null: no memo
+  uint32_t rows_per_split;
+  uint32_t R;
+  float* P1;                  // [R][cap]
+  const uint32_t* mode;       // null: always run
+  uint32_t it;
+  uint32_t gate_mask;
+};
+
+struct GatherAxpyArgs {
+  const void* A;
+  uint64_t ld;
+  uint32_t d;
+  uint64_t dpad;
+  const uint32_t* idx;
+  const float* wgt;
+  const uint32_t* seg_count;
+  uint32_t cap;
+  uint32_t cs;                // entries per chunk (power of two in list mode)
+  uint32_t S;                 // number of chunks
+  const float* P1;
+  uint32_t R;
+  const float* tdirect;       // t_e = tdirect[col] when non-null
+  const float* alpha;         // proposed mean shift when non-null
+  const DevState* st;
+  float* P2;                  // [S][dpad]
+  double* P2s;                // [S]
+  const uint32_t* mode;
+  uint32_t it;
+  uint32_t gate_mask;
+};
+
+struct FinishArgs {
+  const float* P2;
+  uint32_t S_exact;  // chunks of the exact product
+  uint32_t S_amm;    // chunks of the sampled product
+  uint64_t dpad;
+  uint32_t d;
+  const float* x;
+  float* y;
+  float* xm;
+  float* ym;
+  const float* v;
+  const double* P2s;
+  const double* eigvec;
+  double* BP;
+  DevState* st;
+  uint32_t* mode;
+  uint32_t it;
+  double* eig;
+  double* err;
+  int64_t* ts;
+  uint32_t adapt_kernel;  // 0: static schedule; 1 previous / 2 proposed: decide next step on device
+  uint64_t period;
+  double thr;
+  uint32_t step;
+};
+
+hipError_t launch_colstats(int dtype, const void* A, uint64_t ld, uint32_t d, uint64_t n,
+                           float* alpha, double* nrm2, double* cnrm2, hipStream_t s);
+hipError_t launch_cdf(const double* w, int unit, uint64_t n, uint64_t* cdf, DevState* st, int which,
+                      const float* alpha, hipStream_t s);
+hipError_t launch_sample(const uint64_t* cdf, uint64_t n, const DevState* st, int which, uint32_t it0, uint32_t nit,
+                         uint32_t buf_it0, uint32_t c, uint32_t* idx, float* wgt,
+                         uint32_t* seg_count, hipStream_t s);
+hipError_t launch_dots(const float* x, const float* xm, const float* z, uint64_t dpad, DevState* st,
+                       double* BP, hipStream_t s);
+hipError_t launch_x0(float* y, uint32_t d, uint64_t dpad, DevState* st, double* BP, hipStream_t s);
+hipError_t launch_begin(float* xm, float* ym, uint64_t dpad, DevState* st, uint32_t* mode,
+                        uint32_t mode0, hipStream_t s);
+hipError_t launch_gather_dot(int dtype, const GatherDotArgs& a, hipStream_t s);
+hipError_t launch_gather_axpy(int dtype, const GatherAxpyArgs& a, hipStream_t s);
+hipError_t launch_finish(const FinishArgs& a, hipStream_t s);
+hipError_t launch_normalize(const float* y, float* x, uint32_t d, uint64_t dpad, DevState* st,
+                            const float* xm, const float* z, double* BP, hipStream_t s);
+hipError_t launch_sum_rows(const float* P2, uint32_t S, uint64_t dpad, uint32_t d, float* z,
+                           float* v, const DevState* st, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t val, hipStream_t s);
+
+}  // namespace amm

@@ -1,0 +1,84 @@
+"""Synthetic workloads (generated on the GPU with torch) and the FP64 Lanczos
+ground truth used by the benchmark.
+
+Matrices are returned in the library's layout: a (n, ld) tensor whose row j
+is column j of the d x n matrix A.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+
+def _haar(rows: int, cols: int, gen: torch.Generator, device) -> torch.Tensor:
+    G = torch.randn((rows, cols), generator=gen, device=device, dtype=torch.float64)
+    Q, R = torch.linalg.qr(G)
+    return Q * torch.sign(torch.diagonal(R))[None, :]
+
+
+def heavy_tailed(d: int = 16384, n: int = 65536, seed: int = 2024, sigma: float = 1.5,
+                 rank: int = 8, strength: float = 50.0, device="cuda") -> torch.Tensor:
+    """A_ij = N(0,1) * l_j with l_j ~ lognormal(0, sigma), plus a planted
+    rank-`rank` term sum_k theta_k u_k v_k^T, theta_k = strength *
+    (sqrt(d) + sqrt(n)) * 0.9^(k-1), u, v Haar.  FP32, (n, d) layout.
+    (16384 x 65536: the single-large-matrix workload, 4 GiB.)"""
+    gen = torch.Generator(device=device).manual_seed(seed)
+    ell = torch.exp(sigma * torch.randn(n, generator=gen, device=device, dtype=torch.float64))
+    A = torch.randn((n, d), generator=gen, device=device, dtype=torch.float32)
+    A.mul_(ell.float()[:, None])
+    U = _haar(d, rank, gen, device)
+    V = _haar(n, rank, gen, device)
+    th = strength * (math.sqrt(d) + math.sqrt(n)) * (0.9 ** torch.arange(rank, device=device,
+                                                                         dtype=torch.float64))
+    A.addmm_((V * th).float(), U.T.float().contiguous())
+    return A
+
+
+def lowrank_plus_noise(d: int = 1024, n: int = 2048, seed: int = 0, noise: float = 0.01,
+                       device="cuda") -> torch.Tensor:
+    """A = U diag(s) V^T + noise*G, s_i = 10/i, Haar U, V (the 1024 x 2048
+    CPU-runnable workload).  FP32, (n, d) layout."""
+    gen = torch.Generator(device=device).manual_seed(seed)
+    r = min(d, n)
+    U = _haar(d, r, gen, device)
+    V = _haar(n, r, gen, device)
+    s = 10.0 / torch.arange(1, r + 1, device=device, dtype=torch.float64)
+    A = (V * s) @ U.T + noise * torch.randn((n, d), generator=gen, device=device, dtype=torch.float64)
+    return A.float().contiguous()
+
+
+@torch.no_grad()
+def lambda_max_lanczos(A_cols: torch.Tensor, d: int | None = None, k: int = 64, seed: int = 0):
+    """Largest eigenvalue of A A^T in FP64 by Lanczos with full
+    re-orthogonalisation; A_cols is the (n, ld) layout (any float dtype).
+    Returns (lambda, change over the last 8 Lanczos steps)."""
+    d = A_cols.shape[1] if d is None else d
+    A64 = A_cols[:, :d].double()
+    dev = A64.device
+    gen = torch.Generator(device=dev).manual_seed(seed)
+    q = torch.randn(d, generator=gen, device=dev, dtype=torch.float64)
+    q /= torch.linalg.norm(q)
+    Q = torch.zeros((k + 1, d), device=dev, dtype=torch.float64)
+    Q[0] = q
+    al, be, hist = [], [], []
+    for j in range(k):
+        w = A64.T @ (A64 @ Q[j])
+        a = float(Q[j] @ w)
+        w -= Q[: j + 1].T @ (Q[: j + 1] @ w)
+        w -= Q[: j + 1].T @ (Q[: j + 1] @ w)
+        al.append(a)
+        b = float(torch.linalg.norm(w))
+        T = torch.diag(torch.tensor(al, dtype=torch.float64))
+        if be:
+            off = torch.tensor(be, dtype=torch.float64)
+            T += torch.diag(off, 1) + torch.diag(off, -1)
+        hist.append(float(torch.linalg.eigvalsh(T)[-1]))
+        if b < 1e-300 or j == k - 1:
+            break
+        be.append(b)
+        Q[j + 1] = w / b
+    lam = hist[-1]
+    change = abs(hist[-1] - hist[-9]) / lam if len(hist) > 8 else float("inf")
+    del A64
+    return lam, change

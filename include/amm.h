@@ -1,0 +1,152 @@
+/*
+ * amm.h -- C ABI of the MI355X (gfx950) AMM power method for the largest
+ * eigenvalue of A*A^T.
+ *
+ * A is d x n, column-major (column j at A + ld*j), FP32 or BF16, resident in
+ * device memory and owned by the caller.  Every function returns 0 on
+ * success or a negative AMM_E* code; amm_last_error() then holds a message
+ * (per thread).  No function throws.  A context belongs to one stream and is
+ * not shared between threads.
+ *
+ * Each entry point names the reference interface it replaces
+ * (Takuya-Okuyama/eigenvalueComputation_by_AMM, file:line).  The reference
+ * has no FFI; these are the operator boundaries of its hot path.
+ */
+#ifndef AMM_H_
+#define AMM_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMM_ABI_VERSION 1
+
+/* status codes */
+#define AMM_OK 0
+#define AMM_E_INVALID (-1)   /* bad argument */
+#define AMM_E_HIP (-2)       /* HIP runtime error */
+#define AMM_E_NOMEM (-3)     /* device allocation failed */
+#define AMM_E_STATE (-4)     /* call out of order */
+
+/* element types */
+#define AMM_DTYPE_F32 0
+#define AMM_DTYPE_BF16 1
+
+/* kernels (the -kernel flag, host_memory.h:132-141) */
+#define AMM_KERNEL_EXACT 0
+#define AMM_KERNEL_PREVIOUS 1
+#define AMM_KERNEL_PROPOSED 2
+#define AMM_KERNEL_PROPOSED_V2 3
+
+/* flags of amm_params.flags */
+#define AMM_FLAG_NO_GRAPH 1u     /* launch kernel by kernel instead of one hipGraph */
+#define AMM_FLAG_KEEP_STATS 2u   /* reuse column statistics of an earlier run */
+
+typedef struct amm_ctx amm_ctx;
+
+/* Run parameters: the reference's host_memory fields (host_memory.h:26-50). */
+typedef struct amm_params {
+  int32_t kernel;               /* AMM_KERNEL_* */
+  uint32_t c;                   /* sampled columns per product (ignored by exact) */
+  uint64_t step;                /* power iterations */
+  uint64_t vr_period;           /* -vr N: exact step every N iterations (0: off) */
+  double vr_adaptive_threshold; /* -vr_adaptive thr: exact step when cos > 1 - thr */
+  uint64_t seed;                /* Philox key */
+  uint32_t trial;               /* Philox counter word: repetition / trial index */
+  int32_t sanity;               /* -sanity: with c == n every product is exact */
+  uint32_t flags;               /* AMM_FLAG_* */
+  uint32_t reserved;
+} amm_params;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *amm_last_error(void);
+int amm_abi_version(void);
+
+/* Create a context over a device-resident matrix.  stream: a hipStream_t
+ * (NULL: the context creates its own).  Replaces the constructor of
+ * struct device_memory (device_memory.h:172-276).  Requirements: d, n >= 1,
+ * ld >= d, A 16-byte aligned, ld a multiple of 4 (FP32) / 8 (BF16). */
+int amm_ctx_create(const void *A, int32_t dtype, uint64_t d, uint64_t n, uint64_t ld,
+                   void *stream, amm_ctx **out);
+int amm_ctx_destroy(amm_ctx *ctx);
+
+/* Reference eigenvector v* (host, FP64, length d) for NormError
+ * err = 1 - (y^T v* / ||y||)^2; replaces the read of data/exact_eigenvector
+ * (device_memory.h:246-254).  NULL clears it (err then reads 1). */
+int amm_set_exact_eigenvector(amm_ctx *ctx, const double *v_host, uint64_t len);
+
+/* Column statistics alpha_j (mean), ||a_j||^2, ||a_j - alpha_j 1||^2.
+ * Replaces get_norms (device_memory.h:34-59) and cublasSnrm2 of alpha
+ * (device_memory.h:335-336).  Outputs are host arrays of length n (each may
+ * be NULL); alpha2_host receives ||alpha||^2 (may be NULL). */
+int amm_column_stats(amm_ctx *ctx, float *alpha_host, double *nrm2_host, double *cnrm2_host,
+                     double *alpha2_host);
+
+/* Sampling CDF over w_j = ||a_j||^2 (centred = 0) or ||a_j - alpha_j 1||^2
+ * (centred = 1), exact in integers: q_j = floor(w_j 2^(62 - ceil(log2 n)) /
+ * max w), cdf_j = q_0 + ... + q_j.  Replaces set_weights +
+ * cub::DeviceScan::InclusiveSum (common.h:34-40,
+ * powerMethod_by_previous.cuh:26-32).  Host outputs. */
+int amm_build_cdf(amm_ctx *ctx, int32_t centred, uint64_t *cdf_host, uint64_t *total_host,
+                  uint64_t *last_pos_host);
+
+/* Draw the c samples of iteration `iter` and merge duplicates.  Replaces
+ * pick_index + cub::DeviceRadixSort::SortKeys + update_weights
+ * (device_memory.h:519-545, powerMethod_by_previous.cuh:56-70,
+ * common.h:42-53).  Segment g (slots [8192 g, 8192 g + 8192)) writes its
+ * unique columns, ascending, at idx_host[8192 g ...] with weights
+ * mult * total / (c * q_j), q_j = cdf_j - cdf_{j-1}, and its count in
+ * seg_count_host[g]. */
+int amm_sample(amm_ctx *ctx, int32_t centred, uint64_t seed, uint32_t iter, uint32_t trial,
+               uint32_t c, uint32_t *idx_host, float *wgt_host, uint32_t *seg_count_host);
+
+/* One sampled product y = AMM(A A^T)(x - x_memo) + y_memo (previous) or the
+ * mean-shifted form (proposed), with iteration `iter`'s samples.  Device
+ * pointers, length d (x_memo, y_memo may be NULL).  Replaces
+ * square_of_A_by_previous / square_of_A_by_proposed
+ * (powerMethod_by_previous.cuh:7-98, powerMethod_by_proposed.cuh:7-103). */
+int amm_square_of_A(amm_ctx *ctx, int32_t kernel, int32_t sanity, uint32_t c, uint64_t seed,
+                    uint32_t iter, uint32_t trial, const float *x_dev, const float *x_memo_dev,
+                    const float *y_memo_dev, float *y_dev);
+
+/* Exact y = A (A^T x), device pointers of length d.  Replaces the two
+ * cublasSgemv calls (powerMethod_by_exact.cuh:37-53). */
+int amm_exact_product(amm_ctx *ctx, const float *x_dev, float *y_dev);
+
+/* A full power-method run; host outputs of length p->step: cumulative device
+ * time in ms from the run's first kernel, raw Rayleigh quotient
+ * lambda_i = y^T x, and y^T v* / ||y||.  Replaces powerMethod_by_exact /
+ * _previous / _proposed / _proposed_v2 (powerMethod_by_exact.cuh:4-100,
+ * powerMethod_by_previous.cuh:100-210, powerMethod_by_proposed.cuh:105-235,
+ * powerMethod_by_proposed_v2.cuh:199-377).  Any output may be NULL. */
+int amm_power_method(amm_ctx *ctx, const amm_params *p, double *time_ms, double *eig,
+                     double *err);
+
+/* Step kinds of the last run (1 exact+memo, 2 previous, 3 proposed, 4 exact),
+ * host array of length step. */
+int amm_last_modes(amm_ctx *ctx, uint32_t *modes_host, uint64_t len);
+
+/* Copy the current iterate x (which = 0) or the last product y (which = 1)
+ * to host (length d). */
+int amm_get_vector(amm_ctx *ctx, int32_t which, float *host, uint64_t len);
+
+/* Wall time of the last amm_power_method, from HIP events around it (ms). */
+int amm_last_run_ms(amm_ctx *ctx, double *ms);
+
+/* Measurement helper (the reference times whole iterations only,
+ * powerMethod_by_exact.cuh:64-69): launch the two contraction kernels of one
+ * product `iters` times each on the context's stream, bracketed by HIP
+ * events, and report the mean duration of each (ms) and the number of
+ * distinct columns one launch reads (n for the exact product).
+ * kernel: AMM_KERNEL_EXACT, _PREVIOUS or _PROPOSED; samples of iteration 0
+ * of `seed`. */
+int amm_time_product(amm_ctx *ctx, int32_t kernel, uint32_t c, uint64_t seed, uint32_t iters,
+                     double *ms_dot, double *ms_axpy, uint64_t *columns);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AMM_H_ */

@@ -1,0 +1,275 @@
+"""Oracle: CPU restatement of the AMM power method (amm_oracle.c via ctypes)
+plus the FP64 eigenvalue oracle.  TEST INFRASTRUCTURE ONLY -- imported by
+tests/, by the smoke check and by bench.py's CPU-baseline leg, never by the
+product library.
+
+FP64 oracle: numpy.linalg.eigvalsh of A A^T, the computation of the
+reference's calculate_largest_eigenvalue.py:9-13 (which uses eigh in FP64 on
+the square matrix), or svdvals / a Lanczos iteration with full
+re-orthogonalisation where the Gram matrix is large.
+
+Pins of every function are in tests/test_oracle_*.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+SAMPLE_SEG = 8192
+
+_vp = C.c_void_p
+_u64 = C.c_uint64
+_u32 = C.c_uint32
+_SIGS = {
+    "orc_philox4x32_10": (None, [_vp, _vp, _vp]),
+    "orc_uniform53": (C.c_double, [_u32, _u32]),
+    "orc_x0_raw": (None, [_u64, _u32, _u64, _vp]),
+    "orc_x0": (None, [_u64, _u32, _u64, _vp]),
+    "orc_colstats_f32": (None, [_vp, _u64, _u64, _u64, _vp, _vp, _vp]),
+    "orc_colstats_bf16": (None, [_vp, _u64, _u64, _u64, _vp, _vp, _vp]),
+    "orc_ceil_log2": (C.c_int, [_u64]),
+    "orc_cdf_scale": (C.c_double, [_vp, _u64]),
+    "orc_cdf": (None, [_vp, _u64, _vp, _vp, _vp]),
+    "orc_search": (_u64, [_vp, _u64, _u64, _u64]),
+    "orc_draw_target": (_u64, [_u32, _u32, _u64]),
+    "orc_sample": (None, [_vp, _u64, _u64, _u64, _u64, _u32, _u32, _u32, _vp, _vp, _vp]),
+    "orc_amm_product": (None, [_vp, _u64, _u64, _u64, C.c_int, _u32, _vp, _vp, _vp, C.c_int,
+                               _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "orc_exact_product": (None, [_vp, _u64, _u64, _u64, _vp, _vp]),
+    "orc_is_exact_step": (C.c_int, [C.c_int32, _u64, _u64, C.c_double, C.c_double, _u64]),
+    "orc_power_method": (C.c_int, [_vp, _u64, _u64, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "orc_xorshift32": (_u32, [_vp]),
+    "orc_ref_uniform23": (C.c_float, [_vp]),
+    "orc_ref_seed_states": (None, [_u64, _u32, _vp]),
+    "orc_ref_cdf_f32": (None, [_vp, _u64, _vp]),
+    "orc_ref_pick": (_u32, [_vp, _u64, C.c_float]),
+    "orc_ref_power_method": (C.c_int, [_vp, _u64, _u64, _u64, C.c_int32, _u32, _u64, _u64, _vp, _vp]),
+}
+
+KERNELS = {"exact": 0, "previousAMM": 1, "proposedAMM": 2, "proposedAMM_v2": 3}
+
+
+class orc_params(C.Structure):
+    _fields_ = [
+        ("kernel", C.c_int32),
+        ("c", C.c_uint32),
+        ("step", C.c_uint64),
+        ("vr_period", C.c_uint64),
+        ("vr_adaptive_threshold", C.c_double),
+        ("seed", C.c_uint64),
+        ("sanity", C.c_int32),
+        ("trial", C.c_uint32),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            from eigenvaluecomputation_by_amm_amd.build import build_oracle
+            build_oracle()
+        L = C.CDLL(str(LIB_PATH))
+        for k, (r, a) in _SIGS.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def philox4x32_10(ctr, key) -> np.ndarray:
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(_p(c), _p(k), _p(out))
+    return out
+
+
+def uniform53(a: int, b: int) -> float:
+    return lib().orc_uniform53(a, b)
+
+
+def x0(seed: int, trial: int, d: int, normalise: bool = True) -> np.ndarray:
+    x = np.zeros(d, np.float32)
+    (lib().orc_x0 if normalise else lib().orc_x0_raw)(seed, trial, d, _p(x))
+    return x
+
+
+def _as_f32_cols(A: np.ndarray) -> np.ndarray:
+    """A given as (n, ld) rows = columns; returns contiguous float32."""
+    return np.ascontiguousarray(A, np.float32)
+
+
+def colstats(A_cols: np.ndarray, d: int | None = None, bf16: bool = False):
+    """A_cols: (n, ld) array, row j = column j (float32, or uint16 bf16 bits)."""
+    n, ld = A_cols.shape
+    d = ld if d is None else d
+    alpha = np.zeros(n, np.float32)
+    nrm2 = np.zeros(n, np.float64)
+    cnrm2 = np.zeros(n, np.float64)
+    if bf16:
+        a = np.ascontiguousarray(A_cols, np.uint16)
+        lib().orc_colstats_bf16(_p(a), d, n, ld, _p(alpha), _p(nrm2), _p(cnrm2))
+    else:
+        a = _as_f32_cols(A_cols)
+        lib().orc_colstats_f32(_p(a), d, n, ld, _p(alpha), _p(nrm2), _p(cnrm2))
+    return alpha, nrm2, cnrm2
+
+
+def cdf_scale(w: np.ndarray) -> float:
+    w = np.ascontiguousarray(w, np.float64)
+    return float(lib().orc_cdf_scale(_p(w), w.size))
+
+
+def cdf(w: np.ndarray):
+    """Integer CDF: returns (cdf uint64[n], total, last_pos)."""
+    w = np.ascontiguousarray(w, np.float64)
+    out = np.zeros(w.size, np.uint64)
+    tot = C.c_uint64()
+    lp = C.c_uint64()
+    lib().orc_cdf(_p(w), w.size, _p(out), C.byref(tot), C.byref(lp))
+    return out, tot.value, lp.value
+
+
+def search(cdf_: np.ndarray, t: int, last_pos: int) -> int:
+    c = np.ascontiguousarray(cdf_, np.uint64)
+    return int(lib().orc_search(_p(c), c.size, int(t), last_pos))
+
+
+def draw_target(a: int, b: int, total: int) -> int:
+    return int(lib().orc_draw_target(a, b, total))
+
+
+def sample_from_cdf(cd: np.ndarray, total: int, last_pos: int, c: int, seed: int, iteration: int,
+                    trial: int = 0):
+    cd = np.ascontiguousarray(cd, np.uint64)
+    idx = np.zeros(c, np.uint32)
+    wgt = np.zeros(c, np.float32)
+    segc = np.zeros((c + SAMPLE_SEG - 1) // SAMPLE_SEG, np.uint32)
+    lib().orc_sample(_p(cd), cd.size, total, last_pos, seed & 0xFFFFFFFFFFFFFFFF, iteration, trial, c,
+                     _p(idx), _p(wgt), _p(segc))
+    return idx, wgt, segc
+
+
+def sample(w: np.ndarray, c: int, seed: int, iteration: int, trial: int = 0):
+    """Returns (idx, wgt, seg_count) in the library's segment layout."""
+    cd, tot, lp = cdf(w)
+    return sample_from_cdf(cd, tot, lp, c, seed, iteration, trial)
+
+
+def amm_product(A_cols, d, proposed, c, idx, wgt, segc, sanity, x, x_memo=None, y_memo=None,
+                alpha=None, v=None, z=None) -> np.ndarray:
+    a = _as_f32_cols(A_cols)
+    n, ld = a.shape
+    y = np.zeros(d, np.float32)
+    f = lambda t: None if t is None else np.ascontiguousarray(t, np.float32)  # noqa: E731
+    x, x_memo, y_memo, alpha, v, z = map(f, (x, x_memo, y_memo, alpha, v, z))
+    lib().orc_amm_product(_p(a), d, n, ld, 1 if proposed else 0, c,
+                          _p(None if idx is None else np.ascontiguousarray(idx, np.uint32)),
+                          _p(None if wgt is None else np.ascontiguousarray(wgt, np.float32)),
+                          _p(None if segc is None else np.ascontiguousarray(segc, np.uint32)),
+                          1 if sanity else 0, _p(x), _p(x_memo), _p(y_memo), _p(alpha), _p(v), _p(z),
+                          _p(y))
+    return y
+
+
+def exact_product(A_cols, d, x) -> np.ndarray:
+    a = _as_f32_cols(A_cols)
+    n, ld = a.shape
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros(d, np.float32)
+    lib().orc_exact_product(_p(a), d, n, ld, _p(x), _p(y))
+    return y
+
+
+def is_exact_step(kernel: str, i: int, vr_period: int, thr: float, prev_cos: float, last_exact: int) -> bool:
+    return bool(lib().orc_is_exact_step(KERNELS[kernel], i, vr_period, thr, prev_cos, last_exact))
+
+
+def power_method(A_cols, d, kernel="exact", c=0, step=10, vr_period=0, vr_adaptive_threshold=0.0,
+                 seed=0, trial=0, sanity=False, w_plain=None, w_cent=None, eigvec=None):
+    """Returns dict(eig, err, modes, x)."""
+    a = _as_f32_cols(A_cols)
+    n, ld = a.shape
+    P = orc_params(kernel=KERNELS[kernel], c=c, step=step, vr_period=vr_period,
+                   vr_adaptive_threshold=vr_adaptive_threshold, seed=seed & 0xFFFFFFFFFFFFFFFF,
+                   sanity=1 if sanity else 0, trial=trial)
+    eig = np.zeros(step, np.float64)
+    err = np.zeros(step, np.float64)
+    modes = np.zeros(step, np.uint8)
+    x = np.zeros(d, np.float32)
+    f64 = lambda t: None if t is None else np.ascontiguousarray(t, np.float64)  # noqa: E731
+    rc = lib().orc_power_method(_p(a), d, n, ld, C.byref(P), _p(f64(w_plain)), _p(f64(w_cent)),
+                                _p(f64(eigvec)), _p(eig), _p(err), _p(modes), _p(x))
+    if rc != 0:
+        raise ValueError("invalid oracle parameters")
+    return {"eig": eig, "err": err, "modes": modes, "x": x}
+
+
+def xorshift32(state: int) -> int:
+    s = np.array([state], np.uint32)
+    return int(lib().orc_xorshift32(_p(s)))
+
+
+def ref_power_method(A_cols, d, kernel, c, step, seed, x0_):
+    """The reference's own slot sampler (xorshift32, 23-bit, FP32 CDF)."""
+    a = _as_f32_cols(A_cols)
+    n, ld = a.shape
+    eig = np.zeros(step, np.float64)
+    x0_ = np.ascontiguousarray(x0_, np.float32)
+    lib().orc_ref_power_method(_p(a), d, n, ld, KERNELS[kernel], c, step, seed, _p(x0_), _p(eig))
+    return eig
+
+
+# ---------------------------------------------------------------------------
+# FP64 eigenvalue oracle
+# ---------------------------------------------------------------------------
+def lambda_max_eigvalsh(A: np.ndarray) -> float:
+    """A: d x n (any layout); FP64 eigvalsh of A A^T
+    (calculate_largest_eigenvalue.py:9-10)."""
+    A64 = np.asarray(A, np.float64)
+    G = A64 @ A64.T
+    return float(np.linalg.eigvalsh(G)[-1])
+
+
+def lambda_max_svd(A: np.ndarray) -> float:
+    s = np.linalg.svd(np.asarray(A, np.float64), compute_uv=False)
+    return float(s[0] ** 2)
+
+
+def lanczos_lambda_max(matvec, d: int, k: int = 80, seed: int = 0) -> float:
+    """Largest eigenvalue of a symmetric PSD operator by Lanczos with full
+    re-orthogonalisation, FP64 (the eigsh-style oracle for large Gram
+    operators)."""
+    rng = np.random.default_rng(seed)
+    q = rng.standard_normal(d)
+    q /= np.linalg.norm(q)
+    Q = [q]
+    al, be = [], []
+    b = 0.0
+    for j in range(min(k, d)):
+        w = matvec(Q[-1])
+        a = float(Q[-1] @ w)
+        w = w - a * Q[-1] - (b * Q[-2] if j > 0 else 0.0)
+        Qm = np.array(Q)
+        w -= Qm.T @ (Qm @ w)
+        w -= Qm.T @ (Qm @ w)
+        al.append(a)
+        b = float(np.linalg.norm(w))
+        if b < 1e-300:
+            break
+        be.append(b)
+        Q.append(w / b)
+    T = np.diag(al) + np.diag(be[: len(al) - 1], 1) + np.diag(be[: len(al) - 1], -1)
+    return float(np.linalg.eigvalsh(T)[-1])

@@ -1,0 +1,295 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle.
+
+Bars: sampled indices, weights, CDF bit-exact; column statistics and
+products within 1e-5 relative (FP32) of the FP64 oracle over the same
+sample; power-method trajectories within 1e-4 relative of the oracle run on
+the same samples; lambda_max against FP64 eigvalsh.
+"""
+import numpy as np
+import pytest
+
+from helpers import bf16_bits, bf16_to_f32, heavy_columns, lowrank_plus_noise, rel, to_cols
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _amm():
+    from eigenvaluecomputation_by_amm_amd import AmmContext, RunParams, AmmError
+    return AmmContext, RunParams, AmmError
+
+
+def dev_matrix(A: np.ndarray, ld=None, bf16=False):
+    """d x n numpy -> (tensor (n, ld) on cuda, host columns array)."""
+    cols = to_cols(A, ld)
+    if bf16:
+        bits = bf16_bits(cols)
+        t = torch.from_numpy(bits.view(np.int16)).cuda().view(torch.bfloat16)
+        return t, bits
+    return torch.from_numpy(cols).cuda(), cols
+
+
+def banner(msg):
+    print(f"\n[gpu-case] {msg}", flush=True)
+
+
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("d,n,ld,bf16", [(256, 300, 256, False), (1000, 777, 1004, False),
+                                         (37, 5, 40, False), (512, 300, 512, True),
+                                         (1001, 33, 1008, True)])
+def test_column_stats(d, n, ld, bf16):
+    banner(f"column_stats d={d} n={n} ld={ld} bf16={bf16}")
+    AmmContext, *_ = _amm()
+    A = lowrank_plus_noise(d, n, seed=1) + 0.25
+    t, cols = dev_matrix(A, ld, bf16)
+    with AmmContext(t, d=d) as ctx:
+        alpha, nrm2, cnrm2, a2 = ctx.column_stats()
+    ra, rn, rc = O.colstats(cols, d=d, bf16=bf16)
+    assert rel(nrm2, rn) < 1e-12
+    assert rel(cnrm2, rc) < 1e-9
+    assert np.max(np.abs(alpha - ra) / np.maximum(np.abs(ra), 1e-30)) < 2e-7
+    assert abs(a2 - float((alpha.astype(np.float64) ** 2).sum())) / a2 < 1e-12
+
+
+@pytest.mark.parametrize("centred", [False, True])
+def test_cdf_bitwise(centred):
+    banner(f"cdf centred={centred}")
+    AmmContext, *_ = _amm()
+    A = heavy_columns(128, 5000, seed=2)
+    A[:, 17] = 0.0
+    t, _ = dev_matrix(A)
+    with AmmContext(t) as ctx:
+        _, nrm2, cnrm2, _ = ctx.column_stats()
+        cdf, tot, lp = ctx.build_cdf(centred)
+    ocdf, otot, olp = O.cdf(cnrm2 if centred else nrm2)
+    assert np.array_equal(cdf, ocdf) and tot == otot and lp == olp
+
+
+@pytest.mark.parametrize("c,centred,iteration,trial", [(8, False, 0, 0), (1000, False, 3, 1),
+                                                       (8192, True, 7, 0), (20000, False, 11, 5),
+                                                       (4999, True, 2, 3)])
+def test_sample_bitwise(c, centred, iteration, trial):
+    banner(f"sample c={c} centred={centred} it={iteration} trial={trial}")
+    AmmContext, *_ = _amm()
+    A = heavy_columns(64, 5000, seed=3)
+    A[:, [0, 100, 4999]] = 0.0
+    t, _ = dev_matrix(A)
+    seed = 0x1234_5678_9ABC
+    with AmmContext(t) as ctx:
+        _, nrm2, cnrm2, _ = ctx.column_stats()
+        idx, wgt, segc = ctx.sample(c, iteration, seed, trial, centred)
+    oi, ow, os_ = O.sample(cnrm2 if centred else nrm2, c, seed, iteration, trial)
+    assert np.array_equal(segc, os_)
+    for g, u in enumerate(segc):
+        s = slice(g * O.SAMPLE_SEG, g * O.SAMPLE_SEG + int(u))
+        assert np.array_equal(idx[s], oi[s])
+        assert np.array_equal(wgt[s].view(np.uint32), ow[s].view(np.uint32))
+    assert not np.isin([0, 100, 4999], idx[: segc[0]]).any()
+
+
+@pytest.mark.parametrize("d,n,bf16", [(512, 3000, False), (1000, 777, False), (640, 900, True)])
+def test_exact_product(d, n, bf16):
+    banner(f"exact_product d={d} n={n} bf16={bf16}")
+    AmmContext, *_ = _amm()
+    A = lowrank_plus_noise(d, n, seed=4)
+    t, cols = dev_matrix(A, bf16=bf16)
+    Ah = (bf16_to_f32(cols).T if bf16 else A).astype(np.float64)
+    x = np.random.default_rng(5).standard_normal(d).astype(np.float32)
+    with AmmContext(t, d=d) as ctx:
+        y = ctx.exact_product(torch.from_numpy(x).cuda()).cpu().numpy()
+    assert rel(y, Ah @ (Ah.T @ x)) < 1e-5
+
+
+@pytest.mark.parametrize("kernel", ["previousAMM", "proposedAMM"])
+@pytest.mark.parametrize("c,bf16", [(64, False), (1000, False), (9000, False), (256, True)])
+def test_square_of_A(kernel, c, bf16):
+    """One sampled product against the FP64 product over the same sample."""
+    banner(f"square_of_A {kernel} c={c} bf16={bf16}")
+    AmmContext, *_ = _amm()
+    d, n = 520, 9500
+    A = heavy_columns(d, n, seed=6) + 0.3
+    t, cols = dev_matrix(A, bf16=bf16)
+    colsf = bf16_to_f32(cols) if bf16 else cols
+    rng = np.random.default_rng(7)
+    x, xm, ym = (rng.standard_normal(d).astype(np.float32) for _ in range(3))
+    seed, it = 99, 4
+    prop = kernel == "proposedAMM"
+    with AmmContext(t) as ctx:
+        alpha, nrm2, cnrm2, a2 = ctx.column_stats()
+        y = ctx.square_of_A(kernel, torch.from_numpy(x).cuda(), c, it, seed,
+                            x_memo=torch.from_numpy(xm).cuda(), y_memo=torch.from_numpy(ym).cuda())
+        y = y.cpu().numpy()
+    idx, wgt, segc = O.sample(cnrm2 if prop else nrm2, c, seed, it)
+    A64 = colsf.T.astype(np.float64)
+    z = (A64 @ alpha.astype(np.float64)).astype(np.float32)
+    v = (z.astype(np.float64) - a2).astype(np.float32)
+    ref = O.amm_product(colsf, d, prop, c, idx, wgt, segc, False, x, xm, ym, alpha, v, z)
+    assert rel(y, ref) < 1e-5
+
+
+@pytest.mark.parametrize("kernel", ["previousAMM", "proposedAMM"])
+def test_sanity_equals_exact(kernel):
+    banner(f"sanity {kernel}")
+    AmmContext, *_ = _amm()
+    d, n = 300, 700
+    A = lowrank_plus_noise(d, n, seed=8) + 1.0
+    t, _ = dev_matrix(A)
+    x = np.random.default_rng(9).standard_normal(d).astype(np.float32)
+    with AmmContext(t) as ctx:
+        y = ctx.square_of_A(kernel, torch.from_numpy(x).cuda(), n, 0, 1, sanity=True).cpu().numpy()
+    A64 = A.astype(np.float64)
+    assert rel(y, A64 @ (A64.T @ x)) < 1e-5
+
+
+# --------------------------------------------------------------------------
+RUNS = [
+    dict(kernel="exact", step=30),
+    dict(kernel="previousAMM", c=256, step=30),
+    dict(kernel="previousAMM", c=256, step=30, vr_period=5),
+    dict(kernel="proposedAMM", c=256, step=30, vr_period=4),
+    dict(kernel="proposedAMM_v2", c=256, step=30, vr_period=5),
+    dict(kernel="previousAMM", c=2048, step=12, sanity=True),
+    dict(kernel="proposedAMM", c=2048, step=12, sanity=True),
+]
+
+
+@pytest.mark.parametrize("run", RUNS, ids=lambda r: "-".join(f"{k}{v}" for k, v in r.items()))
+def test_power_method_trajectory(run):
+    """Whole runs against the oracle on the device's own column weights
+    (same samples): per-iteration Rayleigh quotient within 1e-4."""
+    banner(f"power_method {run}")
+    AmmContext, RunParams, _ = _amm()
+    d, n = 384, 2048
+    A = lowrank_plus_noise(d, n, seed=10) + 0.05
+    t, cols = dev_matrix(A)
+    with AmmContext(t) as ctx:
+        _, nrm2, cnrm2, _ = ctx.column_stats()
+        res = ctx.power_method(RunParams(seed=77, trial=2, **run))
+        modes = ctx.last_modes(run["step"])
+    ref = O.power_method(cols, d, seed=77, trial=2, w_plain=nrm2, w_cent=cnrm2, **run)
+    omodes = ref["modes"].astype(np.uint32)
+    omodes[omodes == 1] = 1
+    # oracle codes 1 exact(+memo), 2 previous, 3 proposed; device adds 4 = exact without memo
+    dm = modes.copy()
+    dm[dm == 4] = 1
+    assert np.array_equal(dm, omodes)
+    assert np.max(np.abs(res.eig - ref["eig"]) / np.abs(ref["eig"])) < 1e-4
+    assert np.all(np.diff(res.time_ms) >= 0) and res.time_ms[0] > 0
+    lam = O.lambda_max_eigvalsh(A)
+    if run["kernel"] == "exact" or run.get("sanity"):
+        assert abs(res.eig[-1] - lam) / lam < 1e-5
+
+
+def test_adaptive_vr_matches_oracle():
+    banner("adaptive vr")
+    AmmContext, RunParams, _ = _amm()
+    d, n = 256, 1024
+    A = lowrank_plus_noise(d, n, seed=11)
+    t, cols = dev_matrix(A)
+    run = dict(kernel="previousAMM", c=128, step=25, vr_adaptive_threshold=0.02)
+    with AmmContext(t) as ctx:
+        _, nrm2, cnrm2, _ = ctx.column_stats()
+        res = ctx.power_method(RunParams(seed=5, **run))
+        modes = ctx.last_modes(25)
+    ref = O.power_method(cols, d, seed=5, w_plain=nrm2, w_cent=cnrm2, **run)
+    assert np.array_equal(modes, ref["modes"].astype(np.uint32))
+    assert (modes == 1).any() and (modes == 2).any()
+    assert np.max(np.abs(res.eig - ref["eig"]) / np.abs(ref["eig"])) < 1e-4
+
+
+def test_graph_and_stream_launch_agree_bitwise():
+    banner("graph vs eager")
+    AmmContext, RunParams, _ = _amm()
+    A = heavy_columns(256, 4000, seed=12)
+    t, _ = dev_matrix(A)
+    with AmmContext(t) as ctx:
+        p = dict(kernel="proposedAMM_v2", c=512, step=20, vr_period=6, seed=3)
+        a = ctx.power_method(RunParams(**p))
+        b = ctx.power_method(RunParams(use_graph=False, **p))
+        c = ctx.power_method(RunParams(**p))  # graph replay
+    assert np.array_equal(a.eig, b.eig) and np.array_equal(a.eig, c.eig)
+
+
+def test_lambda_small_workload_vs_eigvalsh():
+    """The CPU-runnable workload (1024 x 2048, s_i = 10/i, c = 256, 100
+    iterations): exact reaches the FP64 eigenvalue; previous AMM with VR
+    reaches it within 1e-3."""
+    banner("C1 lambda")
+    AmmContext, RunParams, _ = _amm()
+    A = lowrank_plus_noise(1024, 2048, seed=13)
+    lam = O.lambda_max_eigvalsh(A)
+    t, _ = dev_matrix(A)
+    with AmmContext(t) as ctx:
+        ex = ctx.power_method(RunParams(kernel="exact", step=100, seed=1))
+        pv = ctx.power_method(RunParams(kernel="previousAMM", c=256, step=100, vr_period=10, seed=1))
+    assert abs(ex.eig[-1] - lam) / lam < 1e-6
+    assert abs(pv.eig[-1] - lam) / lam < 1e-3
+
+
+def test_zero_matrix_and_zero_columns():
+    banner("zero matrix")
+    AmmContext, RunParams, _ = _amm()
+    t = torch.zeros((300, 200), device="cuda")
+    with AmmContext(t) as ctx:
+        for k in ["exact", "previousAMM", "proposedAMM", "proposedAMM_v2"]:
+            r = ctx.power_method(RunParams(kernel=k, c=64, step=6, vr_period=3))
+            assert np.all(r.eig == 0.0) and np.all(np.isfinite(r.err))
+
+
+def test_nonfinite_input_rejected():
+    banner("nonfinite")
+    AmmContext, RunParams, AmmError = _amm()
+    A = lowrank_plus_noise(64, 100, seed=14)
+    A[3, 7] = np.nan
+    t, _ = dev_matrix(A)
+    with AmmContext(t) as ctx:
+        with pytest.raises(AmmError):
+            ctx.power_method(RunParams(kernel="previousAMM", c=16, step=3))
+
+
+def test_invalid_arguments():
+    banner("invalid args")
+    AmmContext, RunParams, AmmError = _amm()
+    t = torch.ones((100, 64), device="cuda")
+    with AmmContext(t) as ctx:
+        with pytest.raises(AmmError):
+            ctx.power_method(RunParams(kernel="previousAMM", c=101, step=3))
+        with pytest.raises(AmmError):
+            ctx.power_method(RunParams(kernel="proposedAMM_v2", c=8, step=3, vr_period=0))
+        with pytest.raises(AmmError):
+            ctx.power_method(RunParams(kernel="previousAMM", c=0, step=3))
+
+
+def test_eigenvector_norm_error():
+    banner("norm error")
+    AmmContext, RunParams, _ = _amm()
+    A = lowrank_plus_noise(200, 400, seed=15)
+    w, V = np.linalg.eigh(A.astype(np.float64) @ A.T.astype(np.float64))
+    t, _ = dev_matrix(A)
+    with AmmContext(t) as ctx:
+        ctx.set_exact_eigenvector(V[:, -1])
+        r = ctx.power_method(RunParams(kernel="exact", step=60, seed=2))
+    assert r.norm_error()[-1] < 1e-8
+    assert abs(r.eigenvalue()[-1] - np.sqrt(w[-1])) / np.sqrt(w[-1]) < 1e-6
+
+
+def test_statistical_parity_with_reference_sampler():
+    """100 seeds: the distribution of the AMM Rayleigh quotient (previous,
+    no VR) matches the restatement of the reference's own xorshift/FP32
+    sampler (two-sample KS)."""
+    banner("statistical parity 100 seeds")
+    from scipy import stats
+    AmmContext, RunParams, _ = _amm()
+    d, n, c, step = 128, 1024, 32, 8
+    A = lowrank_plus_noise(d, n, seed=16, decay=0.5)
+    t, cols = dev_matrix(A)
+    x0 = O.x0(1, 0, d)
+    ours, refs = [], []
+    with AmmContext(t) as ctx:
+        for s in range(100):
+            r = ctx.power_method(RunParams(kernel="previousAMM", c=c, step=step, seed=1000 + s))
+            ours.append(r.eig[-1])
+            refs.append(O.ref_power_method(cols, d, "previousAMM", c, step, 5000 + s, x0)[-1])
+    assert stats.ks_2samp(ours, refs).pvalue > 1e-3
