@@ -33,7 +33,7 @@ TOL = 1e-3
 HBM_PEAK_GBS = 8000.0
 
 # headline AMM configuration (chosen from the sweep)
-HEADLINE = dict(kernel="proposedAMM_v2", c=4096, vr_period=10)
+HEADLINE = dict(kernel="previousAMM", c=1024, vr_period=5)
 
 
 def log(*a):
@@ -142,9 +142,11 @@ def main():
     if args.sweep:
         log("sweep kernel x c x vr")
         for kern in ["previousAMM", "proposedAMM", "proposedAMM_v2"]:
-            for c in [1024, 4096, 16384]:
-                for vr in [0, 5, 10, 20]:
+            for c in [256, 512, 1024, 2048, 4096, 16384]:
+                for vr in [0, 2, 3, 4, 5, 10]:
                     if kern == "proposedAMM_v2" and vr == 0:
+                        continue
+                    if c in (256, 512, 2048) and vr in (0, 10):
                         continue
                     rows = run_config(ctx, RunParams, lam, list(range(500, 500 + args.sweep_seeds)), iters,
                                       kernel=kern, c=c, vr_period=vr)
@@ -153,6 +155,14 @@ def main():
                         f"(first-hit {s['median_first_ms']}) iters {s['median_iters']} "
                         f"not_reached {s['not_reached']:.2f} run {s['median_run_ms']:.2f} ms "
                         f"final_rel_med {statistics.median(r['final_rel'] for r in rows):.2e}")
+        for kern in ["previousAMM", "proposedAMM"]:
+            for c in [512, 1024, 4096]:
+                for thr in [0.3, 0.1, 0.03, 0.01]:
+                    rows = run_config(ctx, RunParams, lam, list(range(500, 500 + args.sweep_seeds)), iters,
+                                      kernel=kern, c=c, vr_adaptive_threshold=thr)
+                    s = summarize(rows)
+                    log(f"sweep {kern:15s} c={c:6d} adaptive={thr}: median {s['median_ms']} ms "
+                        f"iters {s['median_iters']} not_reached {s['not_reached']:.2f}")
 
     cfg = dict(HEADLINE)
     log(f"headline {cfg}: warmup {args.warmup}")
@@ -189,8 +199,6 @@ def main():
         if need >= 0:
             log(f"cpu baseline: oracle port, seed {seed0}, {need + 1} iterations")
             Ah = A.cpu().numpy()
-            with torch.cuda.device(0):
-                _, nrm2, cnrm2, _ = ctx.column_stats()
             cpu = cpu_baseline(Ah, d, lam, dict(cfg), seed0, need)
             del Ah
             log(f"cpu: {cpu}")

@@ -38,7 +38,7 @@ HIP_FLAGS = [
     f"-I{CSRC}",
 ]
 
-LIB_SOURCES = ["amm_kernels.hip", "amm_capi.hip"]
+LIB_SOURCES = ["amm_kernels.hip", "amm_generate.hip", "amm_capi.hip"]
 CLI_SOURCES = ["amm_cli.cpp"]
 
 

@@ -429,7 +429,8 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
   const bool sanity = K != AMM_KERNEL_EXACT && p.sanity && p.c == c->n;
   const uint32_t step = (uint32_t)p.step;
   HIPCHK(amm::launch_begin(c->xm, c->ym, c->dpad, c->st, c->mode, adaptive ? modes[0] : 0, c->stream));
-  if (!(p.flags & AMM_FLAG_KEEP_STATS) || !c->stats_valid) {
+  // the exact power method needs no column statistics
+  if (K != AMM_KERNEL_EXACT && (!(p.flags & AMM_FLAG_KEEP_STATS) || !c->stats_valid)) {
     if ((rc = enq_stats(c))) return rc;
   }
   const bool need_plain = K == AMM_KERNEL_PREVIOUS || K == AMM_KERNEL_PROPOSED_V2;
@@ -795,7 +796,7 @@ int amm_power_method(amm_ctx* c, const amm_params* p, double* time_ms, double* e
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
   c->last_step = p->step;
-  c->stats_valid = true;
+  if (p->kernel != AMM_KERNEL_EXACT) c->stats_valid = true;
   if ((rc = check_status(c))) return rc;
   const size_t S = p->step;
   if (eig) HIPCHK(hipMemcpy(eig, c->eig, sizeof(double) * S, hipMemcpyDeviceToHost));
@@ -833,6 +834,34 @@ int amm_get_vector(amm_ctx* c, int32_t which, float* host, uint64_t len) {
 int amm_last_run_ms(amm_ctx* c, double* ms) {
   if (!c || !ms) return fail(AMM_E_INVALID, "NULL argument");
   *ms = c->last_ms;
+  return AMM_OK;
+}
+
+int amm_generate_symmetric(void* A_dev, uint64_t m, uint64_t ld, int32_t dist, float p1, float p2,
+                           uint64_t seed, void* stream, float* delta_out) {
+  if (!A_dev) return fail(AMM_E_INVALID, "A is NULL");
+  if (m < 1 || m >= (1ull << 31) || ld < m) return fail(AMM_E_INVALID, "need 1 <= m < 2^31, ld >= m");
+  if (dist < AMM_DIST_GAUSSIAN || dist > AMM_DIST_UNIFORM)
+    return fail(AMM_E_INVALID, "unknown distribution " + std::to_string(dist));
+  hipStream_t s = (hipStream_t)stream;
+  uint32_t* dbits = nullptr;
+  if (hipMalloc((void**)&dbits, sizeof(uint32_t)) != hipSuccess)
+    return fail(AMM_E_NOMEM, "device allocation failed");
+  uint32_t h = 0;
+  hipError_t e = amm::launch_generate_symmetric((float*)A_dev, m, ld, dist, p1, p2, seed, dbits, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, dbits, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(dbits);
+  if (e != hipSuccess) return fail(AMM_E_HIP, std::string("generate failed: ") + hipGetErrorString(e));
+  float delta;
+  std::memcpy(&delta, &h, sizeof(float));
+  if (delta_out) *delta_out = delta;
+  return AMM_OK;
+}
+
+int amm_f32_to_bf16(const float* src, void* dst, uint64_t count, void* stream) {
+  if (!src || !dst) return fail(AMM_E_INVALID, "NULL argument");
+  HIPCHK(amm::launch_f32_to_bf16(src, (uint16_t*)dst, count, (hipStream_t)stream));
   return AMM_OK;
 }
 

@@ -94,5 +94,8 @@ hipError_t launch_normalize(const float* y, float* x, uint32_t d, uint64_t dpad,
 hipError_t launch_sum_rows(const float* P2, uint32_t S, uint64_t dpad, uint32_t d, float* z,
                            float* v, const DevState* st, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t val, hipStream_t s);
+hipError_t launch_generate_symmetric(float* A, uint64_t m, uint64_t ld, int dist, float p1, float p2,
+                                    uint64_t seed, uint32_t* delta_bits, hipStream_t s);
+hipError_t launch_f32_to_bf16(const float* src, uint16_t* dst, uint64_t count, hipStream_t s);
 
 }  // namespace amm

@@ -40,6 +40,11 @@ extern "C" {
 #define AMM_KERNEL_PROPOSED 2
 #define AMM_KERNEL_PROPOSED_V2 3
 
+/* entry distributions of amm_generate_symmetric (-matrixType, device_memory.h:405-421) */
+#define AMM_DIST_GAUSSIAN 0
+#define AMM_DIST_LOGNORMAL 1
+#define AMM_DIST_UNIFORM 2
+
 /* flags of amm_params.flags */
 #define AMM_FLAG_NO_GRAPH 1u     /* launch kernel by kernel instead of one hipGraph */
 #define AMM_FLAG_KEEP_STATS 2u   /* reuse column statistics of an earlier run */
@@ -144,6 +149,21 @@ int amm_last_run_ms(amm_ctx *ctx, double *ms);
  * of `seed`. */
 int amm_time_product(amm_ctx *ctx, int32_t kernel, uint32_t c, uint64_t seed, uint32_t iters,
                      double *ms_dot, double *ms_axpy, uint64_t *columns);
+
+/* Fill the m x m FP32 device matrix A (leading dimension ld) with a
+ * symmetric random matrix -- entries gaussian(mean p1, std p2),
+ * lognormal(p1, p2) or uniform[p1, p2) -- and add the Gershgorin offset
+ * delta = max_j max(0, sum_i |a_ij| - 2 |a_jj|) to its diagonal; delta is
+ * returned in *diagonal_offset.  Replaces generate_matrix's synthetic branch
+ * (device_memory.h:402-431: curandGenerate*, convert_to_symmetry,
+ * find_offset, cublasIsamax, add_offset).  Synchronises `stream`. */
+int amm_generate_symmetric(void *A_dev, uint64_t m, uint64_t ld, int32_t dist, float p1, float p2,
+                           uint64_t seed, void *stream, float *diagonal_offset);
+
+/* Round count FP32 device values to BF16 (nearest even) into dst_dev; the
+ * storage form of the BF16 path (no reference counterpart: the reference is
+ * FP32 only).  Asynchronous on `stream`. */
+int amm_f32_to_bf16(const float *src_dev, void *dst_dev, uint64_t count, void *stream);
 
 #ifdef __cplusplus
 }

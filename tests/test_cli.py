@@ -1,0 +1,125 @@
+"""The AMM command line (drop-in for the reference's AMM.cu): flags, stdout
+CSV, result/*.binary records, and the synthetic symmetric generator."""
+import ctypes as C
+import struct
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+AMM = ROOT / "eigenvaluecomputation_by_amm_amd" / "AMM"
+
+
+def run_amm(args, cwd, timeout=120):
+    return subprocess.run([str(AMM), *map(str, args)], cwd=cwd, capture_output=True, text=True,
+                          timeout=timeout)
+
+
+# --- argument handling (host_memory.h:65-187); exits before any GPU call ---
+@pytest.mark.parametrize("args,msg", [
+    (["-kernel", "foo"], "[error] '-kernel foo' is not valid."),
+    (["-kernel", "previousAMM", "-c", "7"], "multiple of 8"),
+    (["-kernel", "previousAMM", "-m", "128", "-c", "256"], "1 <= c <= m"),
+    (["-kernel", "proposedAMM_v2", "-c", "64"], "proposedAMM_v2 needs 1 <= vr <= step"),
+    (["-dtype", "fp8"], "'-dtype fp8' is not valid"),
+])
+def test_argument_errors(tmp_path, args, msg):
+    r = run_amm(args, tmp_path)
+    assert r.returncode == 1
+    assert msg in r.stdout + r.stderr
+
+
+def test_missing_file(tmp_path):
+    r = run_amm(["-filepath", tmp_path / "nope.bin"], tmp_path)
+    assert r.returncode == 1 and "Failed to open file." in r.stderr
+
+
+def write_matrix_file(path: Path, A: np.ndarray):
+    """-filepath format: uint32 n, then n rows of n float32 (host_memory.h:143-159)."""
+    with open(path, "wb") as f:
+        f.write(struct.pack("<I", A.shape[0]))
+        f.write(np.ascontiguousarray(A, np.float32).tobytes())
+
+
+def parse_csv(out: str):
+    lines = out.strip().splitlines()
+    assert lines[0] == "Time,Eigenvalue,NormError"
+    rows = [tuple(float(v) for v in ln.split(",")) for ln in lines[1:]]
+    for ln in lines[1:]:
+        for v in ln.split(","):
+            assert len(v.split(".")[1]) == 12  # %.12lf
+    return np.array(rows)
+
+
+@pytest.mark.gpu
+def test_file_input_exact_and_binary(tmp_path):
+    print("\n[gpu-case] cli file input exact", flush=True)
+    rng = np.random.default_rng(0)
+    n = 200  # padded to 256 by the CLI, as the reference does
+    G = rng.standard_normal((n, n))
+    A = ((G + G.T) / 2).astype(np.float32)
+    w, V = np.linalg.eigh(A.astype(np.float64))
+    k = np.argmax(np.abs(w))
+    write_matrix_file(tmp_path / "sym.bin", A)
+    (tmp_path / "result").mkdir()
+    (tmp_path / "data").mkdir()
+    V[:, k].astype(np.float64).tofile(tmp_path / "data" / "exact_eigenvector")
+    r = run_amm(["-filepath", "sym.bin", "-kernel", "exact", "-step", "300", "-r", "2", "-display",
+                 "-seed", "3"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    rows = parse_csv(r.stdout)
+    assert rows.shape == (600, 3)
+    assert abs(rows[-1, 1] - abs(w[k])) / abs(w[k]) < 1e-4
+    assert rows[-1, 2] < 1e-4
+    assert np.all(np.diff(rows[:300, 0]) >= 0)
+    f = tmp_path / "result" / "eigenvalues_by_exact_sym.bin_r=2_step=300.binary"
+    rec = np.fromfile(f, np.float64).reshape(-1, 3)
+    assert rec.shape == (600, 3)
+    assert np.allclose(rec, rows, rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", ["f32", "bf16"])
+def test_synthetic_amm_against_exact(tmp_path, dtype):
+    print(f"\n[gpu-case] cli synthetic {dtype}", flush=True)
+    (tmp_path / "result").mkdir()
+    common = ["-m", "512", "-step", "60", "-r", "1", "-display", "-seed", "11", "-dtype", dtype,
+              "-matrixType", "uniform_-1.0_1.0"]
+    ex = parse_csv(run_amm(["-kernel", "exact", *common], tmp_path).stdout)
+    lam = ex[-1, 1]
+    for kern, extra in [("previousAMM", ["-c", "256", "-vr", "5"]),
+                        ("proposedAMM", ["-c", "256", "-vr", "5"]),
+                        ("proposedAMM_v2", ["-c", "256", "-vr", "5"]),
+                        ("previousAMM", ["-c", "512", "-sanity"])]:
+        r = run_amm(["-kernel", kern, *extra, *common], tmp_path)
+        assert r.returncode == 0, r.stderr
+        rows = parse_csv(r.stdout)
+        assert abs(rows[-1, 1] - lam) / lam < 1e-3, (kern, extra, rows[-1, 1], lam)
+    names = sorted(p.name for p in (tmp_path / "result").iterdir())
+    assert "eigenvalues_by_previousAMM_uniform_-1.0_1.0_r=1_step=60_c=256_vr=5.binary" in names
+
+
+@pytest.mark.gpu
+def test_generator_symmetric_with_gershgorin_offset():
+    print("\n[gpu-case] generator", flush=True)
+    import torch
+    from eigenvaluecomputation_by_amm_amd import _lib
+    lib = _lib.load()
+    m = 300
+    for dist, p1, p2 in [(0, 0.0, 1.0), (1, 0.0, 0.5), (2, -2.0, 3.0)]:
+        A = torch.empty((m, m), device="cuda")
+        delta = C.c_float()
+        assert lib.amm_generate_symmetric(C.c_void_p(A.data_ptr()), m, m, dist, p1, p2, 42, None,
+                                          C.byref(delta)) == 0
+        a = A.cpu().numpy().astype(np.float64)
+        assert np.array_equal(a, a.T)
+        raw = a - np.eye(m) * delta.value
+        off = np.max(np.maximum(0, np.abs(raw).sum(0) - 2 * np.abs(np.diag(raw))))
+        assert abs(off - delta.value) <= 1e-5 * off
+        assert np.linalg.eigvalsh(a)[0] >= -1e-3 * off  # shifted spectrum is non-negative
+        if dist == 2:
+            assert raw.min() >= -2.0 and raw.max() < 3.0
+        if dist == 1:
+            assert raw.min() > 0

@@ -73,8 +73,8 @@ def test_cdf_bitwise(centred):
 def test_sample_bitwise(c, centred, iteration, trial):
     banner(f"sample c={c} centred={centred} it={iteration} trial={trial}")
     AmmContext, *_ = _amm()
-    A = heavy_columns(64, 5000, seed=3)
-    A[:, [0, 100, 4999]] = 0.0
+    A = heavy_columns(64, 25000, seed=3)
+    A[:, [0, 100, 24999]] = 0.0
     t, _ = dev_matrix(A)
     seed = 0x1234_5678_9ABC
     with AmmContext(t) as ctx:
@@ -86,7 +86,7 @@ def test_sample_bitwise(c, centred, iteration, trial):
         s = slice(g * O.SAMPLE_SEG, g * O.SAMPLE_SEG + int(u))
         assert np.array_equal(idx[s], oi[s])
         assert np.array_equal(wgt[s].view(np.uint32), ow[s].view(np.uint32))
-    assert not np.isin([0, 100, 4999], idx[: segc[0]]).any()
+    assert not np.isin([0, 100, 24999], idx[: segc[0]]).any()
 
 
 @pytest.mark.parametrize("d,n,bf16", [(512, 3000, False), (1000, 777, False), (640, 900, True)])
