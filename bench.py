@@ -175,22 +175,48 @@ def main():
     am = summarize(rows)
     log(f"headline: {am}")
 
-    # dominant kernel of the sampled product, HIP events on the context stream
+    # Per-kernel durations (HIP events on the context stream) and each
+    # kernel's share of the headline run up to tolerance; the roofline entry
+    # is the kernel with the largest share.
+    modes = ctx.last_modes(iters)
+    i_set = int(am["median_iters"]) if am["median_iters"] is not None else iters - 1
+    n_exact = int(np.sum((modes[: i_set + 1] == 1) | (modes[: i_set + 1] == 4)))
+    n_amm = i_set + 1 - n_exact
     kname = "proposedAMM" if cfg["kernel"] != "previousAMM" else "previousAMM"
     ms_dot, ms_axpy, ucols = ctx.time_product(kname, cfg["c"], seed=1, iters=args.kernel_iters)
     ex_dot, ex_axpy, _ = ctx.time_product("exact", 0, iters=20)
-    bytes_col = d * A.element_size()
-    dom = ("gather_dot", ms_dot) if ms_dot >= ms_axpy else ("gather_axpy", ms_axpy)
-    achieved = ucols * bytes_col / (dom[1] * 1e-3) / 1e9
-    log(f"kernels: gather_dot {ms_dot*1e3:.1f} us, gather_axpy {ms_axpy*1e3:.1f} us over {ucols} columns; "
-        f"exact dot {ex_dot*1e3:.1f} us axpy {ex_axpy*1e3:.1f} us "
-        f"({2*d*n*4/((ex_dot+ex_axpy)*1e-3)/1e9:.0f} GB/s)")
+    ms_stats = ctx.time_column_stats(iters=10)
+    s_A = A.element_size()
+    kern = {
+        "k_gather_dot<float,4,dense> (exact pass A^T x)": (n_exact * ex_dot, ex_dot, d * n * s_A),
+        "k_gather_axpy<float,dense> (exact pass A t)": (n_exact * ex_axpy, ex_axpy, d * n * s_A),
+        "k_colstats<float> (column statistics)": (ms_stats, ms_stats, d * n * s_A),
+        "k_gather_dot<float,2,list> (sampled pass A_S^T dx)": (n_amm * ms_dot, ms_dot, ucols * d * s_A),
+        "k_gather_axpy<float,list> (sampled pass A_S t)": (n_amm * ms_axpy, ms_axpy, ucols * d * s_A),
+    }
+    for k, (tot, t1, b) in kern.items():
+        log(f"kernel {k}: {t1*1e3:.1f} us/launch, {b/(t1*1e-3)/1e9:.0f} GB/s, {tot*1e3:.0f} us in the run")
+    dom_name, (dom_tot, dom_t, dom_b) = max(kern.items(), key=lambda kv: kv[1][0])
+    achieved = dom_b / (dom_t * 1e-3) / 1e9
+    # a plain device copy for scale (read + write)
+    buf = torch.empty(1 << 29, dtype=torch.float32, device="cuda")
+    out = torch.empty_like(buf)
+    out.copy_(buf)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        out.copy_(buf)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 10 * 2 * buf.numel() * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del buf, out
+    log(f"torch device copy: {copy_gbs:.0f} GB/s (read + write)")
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                kernel=f"k_gather_{'dot' if dom[0]=='gather_dot' else 'axpy'}<float> (list)",
-                bytes_per_launch=ucols * bytes_col, avg_launch_us=round(dom[1] * 1e3, 2),
-                other_kernel_us=round((ms_axpy if dom[0] == "gather_dot" else ms_dot) * 1e3, 2),
-                exact_gemv_pair_GBs=round(2 * d * n * 4 / ((ex_dot + ex_axpy) * 1e-3) / 1e9, 1))
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=dom_name,
+                bytes_per_launch=int(dom_b), avg_launch_us=round(dom_t * 1e3, 2),
+                share_of_run_us=round(dom_tot * 1e3, 1),
+                kernels_us={k: round(v[1] * 1e3, 2) for k, v in kern.items()},
+                sampled_columns=ucols, torch_copy_GBs=round(copy_gbs, 1))
 
     cpu = None
     if not args.no_cpu and am["median_ms"] is not None:

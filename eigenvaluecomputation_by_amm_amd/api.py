@@ -224,6 +224,12 @@ class AmmContext:
                                          C.c_void_p(_ptr(u))))
         return float(t[0]), float(t[1]), int(u[0])
 
+    def time_column_stats(self, iters: int = 20) -> float:
+        """Mean duration (ms) of the column-statistics pass (one read of A)."""
+        t = np.zeros(1, np.float64)
+        check(self._lib.amm_time_column_stats(self._h, iters, C.c_void_p(_ptr(t))))
+        return float(t[0])
+
     def get_vector(self, which: str = "x") -> np.ndarray:
         v = np.empty(self.d, np.float32)
         check(self._lib.amm_get_vector(self._h, 0 if which == "x" else 1, C.c_void_p(_ptr(v)), self.d))

@@ -865,6 +865,24 @@ int amm_f32_to_bf16(const float* src, void* dst, uint64_t count, void* stream) {
   return AMM_OK;
 }
 
+int amm_time_column_stats(amm_ctx* c, uint32_t iters, double* ms) {
+  if (!c || !ms) return fail(AMM_E_INVALID, "NULL argument");
+  if (iters < 1 || iters > 100000) return fail(AMM_E_INVALID, "iters must be in [1, 100000]");
+  int rc;
+  if ((rc = set_device(c))) return rc;
+  if ((rc = enq_stats(c))) return rc;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  for (uint32_t i = 0; i < iters; ++i)
+    if ((rc = enq_stats(c))) return rc;
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float t = 0.0f;
+  HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
+  *ms = t / iters;
+  c->stats_valid = true;
+  return AMM_OK;
+}
+
 int amm_time_product(amm_ctx* c, int32_t kernel, uint32_t cc, uint64_t seed, uint32_t iters,
                      double* ms_dot, double* ms_axpy, uint64_t* columns) {
   if (!c) return fail(AMM_E_INVALID, "ctx is NULL");

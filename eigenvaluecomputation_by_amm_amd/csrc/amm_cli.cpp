@@ -304,9 +304,14 @@ struct device_memory {  // device_memory.h:116-498
 static void power_method(host_memory& p, device_memory& dm, int kernel, bool is_output) {
   uint64_t step = p.step;
   if (!is_output) {
-    // warm-up lengths (AMM.cu:14,30,46,62 with powerMethod_by_previous.cuh:131)
+    // warm-up lengths (AMM.cu:14,30,46,62 with powerMethod_by_previous.cuh:131);
+    // a warm-up of 0 iterations still draws its x0 (powerMethod_by_previous.cuh:128),
+    // so it still consumes one repetition of the random stream
     if (kernel == AMM_KERNEL_PREVIOUS || kernel == AMM_KERNEL_PROPOSED) step = p.vr_period;
-    if (step == 0) return;
+    if (step == 0) {
+      dm.rep++;
+      return;
+    }
   }
   amm_params P{};
   P.kernel = kernel;

@@ -150,6 +150,10 @@ int amm_last_run_ms(amm_ctx *ctx, double *ms);
 int amm_time_product(amm_ctx *ctx, int32_t kernel, uint32_t c, uint64_t seed, uint32_t iters,
                      double *ms_dot, double *ms_axpy, uint64_t *columns);
 
+/* Measurement helper: mean duration (ms) of the column-statistics pass
+ * (one read of A) over `iters` launches, HIP events on the context stream. */
+int amm_time_column_stats(amm_ctx *ctx, uint32_t iters, double *ms);
+
 /* Fill the m x m FP32 device matrix A (leading dimension ld) with a
  * symmetric random matrix -- entries gaussian(mean p1, std p2),
  * lognormal(p1, p2) or uniform[p1, p2) -- and add the Gershgorin offset

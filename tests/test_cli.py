@@ -82,23 +82,47 @@ def test_file_input_exact_and_binary(tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("dtype", ["f32", "bf16"])
-def test_synthetic_amm_against_exact(tmp_path, dtype):
+def test_synthetic_sanity_mode_tracks_exact(tmp_path, dtype):
+    """Shifted synthetic matrices converge slowly (all eigenvalues sit near
+    the Gershgorin offset), so compare trajectories: -sanity with c = m makes
+    every AMM product exact, and the same seed gives the same x0."""
     print(f"\n[gpu-case] cli synthetic {dtype}", flush=True)
     (tmp_path / "result").mkdir()
-    common = ["-m", "512", "-step", "60", "-r", "1", "-display", "-seed", "11", "-dtype", dtype,
+    common = ["-m", "512", "-step", "40", "-r", "2", "-display", "-seed", "11", "-dtype", dtype,
               "-matrixType", "uniform_-1.0_1.0"]
     ex = parse_csv(run_amm(["-kernel", "exact", *common], tmp_path).stdout)
-    lam = ex[-1, 1]
-    for kern, extra in [("previousAMM", ["-c", "256", "-vr", "5"]),
-                        ("proposedAMM", ["-c", "256", "-vr", "5"]),
-                        ("proposedAMM_v2", ["-c", "256", "-vr", "5"]),
-                        ("previousAMM", ["-c", "512", "-sanity"])]:
-        r = run_amm(["-kernel", kern, *extra, *common], tmp_path)
+    for kern in ["previousAMM", "proposedAMM"]:
+        r = run_amm(["-kernel", kern, "-c", "512", "-sanity", *common], tmp_path)
         assert r.returncode == 0, r.stderr
         rows = parse_csv(r.stdout)
-        assert abs(rows[-1, 1] - lam) / lam < 1e-3, (kern, extra, rows[-1, 1], lam)
+        # Eigenvalue = max(0, sqrt(eig) - delta) can be 0 early on: compare on the run's scale
+        assert np.max(np.abs(rows[:, 1] - ex[:, 1])) < 1e-5 * np.max(ex[:, 1])
+    r = run_amm(["-kernel", "proposedAMM_v2", "-c", "256", "-vr", "5", *common], tmp_path)
+    assert r.returncode == 0 and np.all(np.isfinite(parse_csv(r.stdout)))
     names = sorted(p.name for p in (tmp_path / "result").iterdir())
-    assert "eigenvalues_by_previousAMM_uniform_-1.0_1.0_r=1_step=60_c=256_vr=5.binary" in names
+    assert "eigenvalues_by_proposedAMM_v2_uniform_-1.0_1.0_r=2_step=40_c=256_vr=5.binary" in names
+    assert "eigenvalues_by_previousAMM_uniform_-1.0_1.0_r=2_step=40_c=512_vr=0.binary" in names
+
+
+@pytest.mark.gpu
+def test_file_input_amm_reaches_exact(tmp_path):
+    """A symmetric matrix with a spectral gap: every AMM kernel with -vr 5
+    ends within 1e-3 of the exact power method."""
+    print("\n[gpu-case] cli amm vs exact", flush=True)
+    rng = np.random.default_rng(1)
+    n = 384
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    w = 10.0 / np.arange(1, n + 1)
+    A = ((Q * w) @ Q.T + 0.01 * (lambda G: (G + G.T) / 2)(rng.standard_normal((n, n))))
+    write_matrix_file(tmp_path / "gap.bin", A.astype(np.float32))
+    (tmp_path / "result").mkdir()
+    common = ["-filepath", "gap.bin", "-step", "40", "-r", "1", "-display", "-seed", "5"]
+    lam = parse_csv(run_amm(["-kernel", "exact", *common], tmp_path).stdout)[-1, 1]
+    ref = np.max(np.abs(np.linalg.eigvalsh(A.astype(np.float32).astype(np.float64))))
+    assert abs(lam - ref) / ref < 1e-5
+    for kern in ["previousAMM", "proposedAMM", "proposedAMM_v2"]:
+        rows = parse_csv(run_amm(["-kernel", kern, "-c", "64", "-vr", "5", *common], tmp_path).stdout)
+        assert abs(rows[-1, 1] - lam) / lam < 1e-3, (kern, rows[-1, 1], lam)
 
 
 @pytest.mark.gpu
