@@ -90,11 +90,14 @@ struct amm_ctx {
   double* cnrm2 = nullptr;
   uint64_t* cdf[2] = {nullptr, nullptr};
   float *z = nullptr, *v = nullptr, *x = nullptr, *y = nullptr, *xm = nullptr, *ym = nullptr;
-  float* scratch_vec = nullptr;
   double* eigvec = nullptr;
   bool has_eigvec = false;
   DevState* st = nullptr;
   double* BP = nullptr;
+  uint64_t* tile_max = nullptr;
+  uint64_t* tile_sum = nullptr;
+  uint64_t* tile_lp = nullptr;
+  double* tile_a2 = nullptr;
 
   uint32_t* mode = nullptr;
   size_t mode_cap = 0;
@@ -212,15 +215,20 @@ int ensure_lists(amm_ctx* c, uint64_t nit, uint32_t cc) {
 // ---- enqueue helpers (all asynchronous on c->stream) ----------------------
 int enq_stats(amm_ctx* c) {
   HIPCHK(amm::launch_colstats(c->dtype, c->A, c->ld, (uint32_t)c->d, c->n, c->alpha, c->nrm2,
-                              c->cnrm2, c->stream));
+                              c->cnrm2, c->st, c->stream));
   return AMM_OK;
 }
 
-int enq_cdf(amm_ctx* c, int which, bool unit) {
-  HIPCHK(amm::launch_cdf(which ? c->cnrm2 : c->nrm2, unit ? 1 : 0, c->n, c->cdf[which], c->st, which,
-                         which ? c->alpha : nullptr, c->stream));
+int enq_cdf(amm_ctx* c, int which, bool /*unused*/) {
+  HIPCHK(amm::launch_cdf(which ? c->cnrm2 : c->nrm2, c->n, c->cdf[which], c->st, which,
+                         which ? c->alpha : nullptr, c->tile_max, c->tile_sum, c->tile_lp,
+                         c->tile_a2, c->stream));
   return AMM_OK;
 }
+
+// rows above which the normalisation runs as its own kernel instead of in
+// the finish kernel's last block
+constexpr uint64_t kFuseNormRows = 65536;
 
 int enq_sample(amm_ctx* c, int which, uint32_t it0, uint32_t nit, uint32_t buf_it0, uint32_t cc) {
   HIPCHK(amm::launch_sample(c->cdf[which], c->n, c->st, which, it0, nit, buf_it0, cc, c->idx, c->wgt,
@@ -330,8 +338,34 @@ int enq_amm(amm_ctx* c, uint32_t it, uint32_t slot, bool proposed, bool sanity,
   return AMM_OK;
 }
 
+amm::TailArgs tail_args(amm_ctx* c, uint32_t op, uint32_t nb, uint32_t it, bool norm,
+                        const float* zq, bool stamp) {
+  amm::TailArgs t{};
+  t.op = op;
+  t.BP = c->BP;
+  t.nb = nb;
+  t.st = c->st;
+  t.norm = norm ? 1 : 0;
+  t.y = c->y;
+  t.x = c->x;
+  t.xm = c->xm;
+  t.z = zq;
+  t.d = (uint32_t)c->d;
+  t.dpad = c->dpad;
+  t.mode = c->mode;
+  t.it = it;
+  t.eig = c->eig;
+  t.err = c->err;
+  t.ts = c->ts;
+  t.stamp = stamp ? 1 : 0;
+  return t;
+}
+
+// y = the product's partial sums (+ base), then the tail: lambda_i, err_i,
+// scale, step decision; with norm_inline also x = y / ||y|| (+ s, q)
 int enq_finish(amm_ctx* c, uint32_t it, bool sanity, float* y_out, const float* ym_base,
-               uint32_t adapt_kernel, uint64_t period, double thr, uint32_t step) {
+               uint32_t adapt_kernel, uint64_t period, double thr, uint32_t step,
+               bool norm_inline = false, const float* zq = nullptr, bool stamp = false) {
   amm::FinishArgs f{};
   f.P2 = c->P2;
   f.S_exact = c->axpy_dense.S;
@@ -349,14 +383,57 @@ int enq_finish(amm_ctx* c, uint32_t it, bool sanity, float* y_out, const float* 
   f.st = c->st;
   f.mode = c->mode;
   f.it = it;
-  f.eig = c->eig;
-  f.err = c->err;
-  f.ts = c->ts;
-  f.adapt_kernel = adapt_kernel;
-  f.period = period;
-  f.thr = thr;
-  f.step = step;
   HIPCHK(amm::launch_finish(f, c->stream));
+  amm::TailArgs t = tail_args(c, amm::kTailFinish, (uint32_t)amm::finish_blocks(c->dpad), it,
+                              norm_inline, zq, stamp);
+  t.y = y_out;
+  t.adapt_kernel = adapt_kernel;
+  t.period = period;
+  t.thr = thr;
+  t.step = step;
+  HIPCHK(amm::launch_tail(t, c->stream));
+  return AMM_OK;
+}
+
+// x = y / ||y|| with the 1/||y|| already in the state, plus s, q when zq is
+// given; `stamp` closes iteration it.  Small d: one tail block; large d: a
+// grid pass and a tail for s, q.
+int enq_norm(amm_ctx* c, uint32_t it, const float* zq, bool stamp) {
+  if (c->dpad <= kFuseNormRows) {
+    HIPCHK(amm::launch_tail(tail_args(c, amm::kTailNorm, 0, it, true, zq, stamp), c->stream));
+  } else {
+    HIPCHK(amm::launch_normalize(c->y, c->x, (uint32_t)c->d, c->dpad, c->st, c->xm, zq, c->BP,
+                                 c->stream));
+    const uint32_t nb = zq ? (uint32_t)amm::normalize_blocks(c->dpad) : 0u;
+    HIPCHK(amm::launch_tail(tail_args(c, amm::kTailSQ, nb, it, false, nullptr, stamp), c->stream));
+  }
+  return AMM_OK;
+}
+
+// x0 ~ N(0, I), normalised (curandGenerateNormal + normalize<512>)
+int enq_x0(amm_ctx* c, const float* zq) {
+  HIPCHK(amm::launch_x0(c->y, (uint32_t)c->d, c->dpad, c->st, c->BP, c->stream));
+  const bool small = c->dpad <= kFuseNormRows;
+  HIPCHK(amm::launch_tail(tail_args(c, amm::kTailScale, (uint32_t)amm::x0_blocks(c->dpad), 0, small,
+                                    zq, false),
+                          c->stream));
+  if (!small) {
+    HIPCHK(amm::launch_normalize(c->y, c->x, (uint32_t)c->d, c->dpad, c->st, c->xm, zq, c->BP,
+                                 c->stream));
+    if (zq)
+      HIPCHK(amm::launch_tail(tail_args(c, amm::kTailSQ, (uint32_t)amm::normalize_blocks(c->dpad), 0,
+                                        false, nullptr, false),
+                              c->stream));
+  }
+  return AMM_OK;
+}
+
+// s, q of x against x_memo for a stand-alone proposed product
+int enq_dots(amm_ctx* c) {
+  HIPCHK(amm::launch_dots(c->x, c->xm, c->z, c->dpad, c->BP, c->stream));
+  HIPCHK(amm::launch_tail(tail_args(c, amm::kTailSQ, (uint32_t)amm::dots_blocks(c->dpad), 0, false,
+                                    nullptr, false),
+                          c->stream));
   return AMM_OK;
 }
 
@@ -454,9 +531,7 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
     }
   }
   // x0 (curandGenerateNormal + normalize<512>)
-  HIPCHK(amm::launch_x0(c->y, (uint32_t)c->d, c->dpad, c->st, c->BP, c->stream));
-  HIPCHK(amm::launch_normalize(c->y, c->x, (uint32_t)c->d, c->dpad, c->st, c->xm,
-                               K == AMM_KERNEL_PROPOSED ? c->z : nullptr, c->BP, c->stream));
+  if ((rc = enq_x0(c, K == AMM_KERNEL_PROPOSED ? c->z : nullptr))) return rc;
   const uint32_t gate_exact = 1u << amm::kModeExactMemo;
   const uint32_t gate_amm = (1u << amm::kModeAmmPrevious) | (1u << amm::kModeAmmProposed);
   for (uint32_t i = 0; i < step; ++i) {
@@ -473,14 +548,18 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
         if ((rc = enq_amm(c, i, i, md == amm::kModeAmmProposed, sanity, nullptr, 0))) return rc;
       }
     }
+    const bool want_sq = K == AMM_KERNEL_PROPOSED || (K == AMM_KERNEL_PROPOSED_V2 && i + 1 >= P);
+    // the normalisation rides in the finish tail unless d is large or z is
+    // only formed after this iteration's product (the v2 transition)
+    const bool inline_norm = c->dpad <= kFuseNormRows && !z_after;
     if ((rc = enq_finish(c, i, sanity, c->y, c->ym, adaptive ? (uint32_t)K : 0u, P,
-                         p.vr_adaptive_threshold, step)))
+                         p.vr_adaptive_threshold, step, inline_norm, want_sq ? c->z : nullptr,
+                         inline_norm)))
       return rc;
     if (z_after)
       if ((rc = enq_z(c))) return rc;  // [z, .] = A [alpha, .] at the v2 transition
-    const bool want_sq = K == AMM_KERNEL_PROPOSED || (K == AMM_KERNEL_PROPOSED_V2 && i + 1 >= P);
-    HIPCHK(amm::launch_normalize(c->y, c->x, (uint32_t)c->d, c->dpad, c->st, c->xm,
-                                 want_sq ? c->z : nullptr, c->BP, c->stream));
+    if (!inline_norm)
+      if ((rc = enq_norm(c, i, want_sq ? c->z : nullptr, true))) return rc;
   }
   return AMM_OK;
 }
@@ -569,9 +648,14 @@ int amm_ctx_create(const void* A, int32_t dtype, uint64_t d, uint64_t n, uint64_
       (cap = 0, rc = grow(c->cdf[1], cap, n)) || (cap = 0, rc = grow(c->z, cap, dp)) ||
       (cap = 0, rc = grow(c->v, cap, dp)) || (cap = 0, rc = grow(c->x, cap, dp)) ||
       (cap = 0, rc = grow(c->y, cap, dp)) || (cap = 0, rc = grow(c->xm, cap, dp)) ||
-      (cap = 0, rc = grow(c->ym, cap, dp)) || (cap = 0, rc = grow(c->scratch_vec, cap, dp)) ||
-      (cap = 0, rc = grow(c->eigvec, cap, dp)) || (cap = 0, rc = grow(c->st, cap, 1)) ||
-      (cap = 0, rc = grow(c->BP, cap, 4 * (dp / 256 + 8))))
+      (cap = 0, rc = grow(c->ym, cap, dp)) || (cap = 0, rc = grow(c->eigvec, cap, dp)) ||
+      (cap = 0, rc = grow(c->st, cap, 1)) ||
+      // block partials: 3 doubles per finish block bound every reduction kernel
+      (cap = 0, rc = grow(c->BP, cap, 3 * amm::finish_blocks(dp) + 64)) ||
+      (cap = 0, rc = grow(c->tile_max, cap, amm::cdf_tiles(n))) ||
+      (cap = 0, rc = grow(c->tile_sum, cap, amm::cdf_tiles(n))) ||
+      (cap = 0, rc = grow(c->tile_lp, cap, amm::cdf_tiles(n))) ||
+      (cap = 0, rc = grow(c->tile_a2, cap, amm::cdf_tiles(n))))
     return bail(rc);
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
     return bail(fail(AMM_E_HIP, "hipEventCreate failed"));
@@ -585,10 +669,11 @@ int amm_ctx_destroy(amm_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
-  void* bufs[] = {c->alpha, c->nrm2, c->cnrm2, c->cdf[0], c->cdf[1], c->z,    c->v,
-                  c->x,     c->y,    c->xm,    c->ym,     c->scratch_vec, c->eigvec, c->st,
-                  c->BP,    c->mode, c->eig,   c->err,    c->ts,   c->idx, c->wgt,
-                  c->segc,  c->P1,   c->P2,    c->P2s};
+  void* bufs[] = {c->alpha, c->nrm2,  c->cnrm2,    c->cdf[0],  c->cdf[1], c->z,   c->v,
+                  c->x,     c->y,     c->xm,       c->ym,      c->eigvec, c->st,  c->BP,
+                  c->mode,  c->eig,   c->err,      c->ts,      c->idx,    c->wgt, c->segc,
+                  c->P1,    c->P2,    c->P2s,      c->tile_sum, c->tile_lp, c->tile_a2,
+                  c->tile_max};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -709,7 +794,7 @@ int amm_square_of_A(amm_ctx* c, int32_t kernel, int32_t sanity, uint32_t cc, uin
     if ((rc = enq_cdf(c, 0, false))) return rc;
   if (prop) {
     if ((rc = enq_z(c))) return rc;
-    HIPCHK(amm::launch_dots(c->x, c->xm, c->z, c->dpad, c->st, c->BP, c->stream));
+    if ((rc = enq_dots(c))) return rc;
   }
   if ((rc = upload_run_state(c, seed, trial))) return rc;
   if (!san)

@@ -41,8 +41,8 @@ struct alignas(16) DevState {
   uint64_t last_exact;  // last exact iteration (proposed gate)
   uint64_t seed;        // Philox key of the run
   uint32_t trial;       // Philox counter word 2 of the run
-  uint32_t counter[6];  // last-block-done counters (reset by the last block)
   uint32_t status;      // kStatus* bits, sticky until the host clears them
+  uint32_t pad0;
   int64_t t_start;      // wall clock at the run's first kernel
 };
 
@@ -179,28 +179,6 @@ __device__ __forceinline__ void store_vecf(float* p, const float (&v)[V]) {
 #pragma unroll
   for (int k = 0; k < V; k += 4)
     *reinterpret_cast<float4*>(p + k) = make_float4(v[k], v[k + 1], v[k + 2], v[k + 3]);
-}
-
-// Last-block-done: every block calls this once after writing its partials;
-// returns true in exactly one block (the last to arrive), which then owns the
-// final reduction.  The counter is reset by that block for the next launch.
-__device__ __forceinline__ bool last_block_arrive(uint32_t* counter, bool* lds_flag) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t total = gridDim.x * gridDim.y * gridDim.z;
-    const uint32_t prev =
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    *lds_flag = (prev == total - 1);
-  }
-  __syncthreads();
-  const bool last = *lds_flag;
-  if (last) __threadfence();
-  return last;
-}
-
-__device__ __forceinline__ void reset_counter(uint32_t* counter) {
-  __hip_atomic_store(counter, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace amm

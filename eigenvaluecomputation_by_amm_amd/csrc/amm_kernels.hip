@@ -69,93 +69,156 @@ __global__ __launch_bounds__(256) void k_colstats(const T* __restrict__ A, uint6
     const double mean = s / (double)d;
     alpha[j] = (float)mean;
     nrm2[j] = ss;
-    const double c = ss - s * mean;
-    cnrm2[j] = c > 0.0 ? c : 0.0;
+    double c = ss - s * mean;
+    c = c > 0.0 ? c : (c == c ? 0.0 : c);  // clamp rounding below 0, keep NaN
+    cnrm2[j] = c;
   }
 }
 
 // ---------------------------------------------------------------------------
 // Sampling CDF (set_weights + cub::DeviceScan::InclusiveSum,
 // common.h:34-40, powerMethod_by_previous.cuh:26-32) in exact integer
-// arithmetic, one workgroup of 1024 lanes:
+// arithmetic:
 //   scale = 2^(62 - ceil(log2 n)) / max_j w_j,  q_j = floor(w_j * scale),
 //   cdf_j = q_0 + ... + q_j  (uint64).
-// Integer sums are associative, so this parallel scan (lane sums,
-// Hillis-Steele over lanes in LDS, running sums in the lane) equals any
-// sequential replay bit for bit; the CDF is monotone and q_j = 0 columns can
-// never be drawn.  weights: w_j, or 1 for every column when unit != 0.
-// Also ||alpha||^2 when alpha is given (fixed-order FP64 tree).
+// Three launches over tiles of 4096 columns: k_cdf_max (tile maxima of the
+// weights' bit patterns -- non-negative doubles order like their patterns,
+// and NaN/Inf patterns exceed every finite one), k_cdf_tiles (global max
+// from the tile maxima, q_j, in-tile scan), k_cdf_offsets (add the sum of
+// the preceding tiles).  Max and integer sums are order-free, so the result
+// equals any sequential replay bit for bit; the CDF is monotone and q_j = 0
+// columns can never be drawn.  ||alpha||^2 comes along when alpha is given
+// (FP64, fixed order: 16 in a lane, tree over the tile, tiles in order).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_cdf(const double* __restrict__ w, int unit, uint64_t n,
-                                              uint64_t* __restrict__ cdf, DevState* st, int which,
-                                              const float* __restrict__ alpha) {
-  __shared__ double dred[1024];
-  __shared__ uint64_t buf[2][1024];
-  const int t = threadIdx.x;
-  const uint64_t ch = (n + 1023) / 1024;
-  const uint64_t b = (uint64_t)t * ch;
-  const uint64_t e = b + ch < n ? b + ch : n;
-  // 1. max weight, finiteness, ||alpha||^2
-  double mx = 0.0, a2 = 0.0;
-  int bad = 0;
-  for (uint64_t j = b; j < e; ++j) {
-    const double wj = unit ? 1.0 : w[j];
-    if (!(wj >= 0.0) || isinf(wj)) bad = 1;
-    mx = wj > mx ? wj : mx;
-    if (alpha) {
+constexpr uint32_t kCdfPer = 16;
+constexpr uint32_t kCdfTile = 256 * kCdfPer;
+
+__global__ __launch_bounds__(256) void k_cdf_max(const double* __restrict__ w, uint64_t n,
+                                                 uint64_t* __restrict__ tile_max) {
+  __shared__ uint64_t red[256];
+  const uint32_t t = threadIdx.x;
+  uint64_t m = 0;
+  const uint64_t b = (uint64_t)blockIdx.x * kCdfTile;
+  for (uint64_t j = b + t; j < b + kCdfTile && j < n; j += 256) {
+    const uint64_t v = (uint64_t)__double_as_longlong(w[j]);
+    m = v > m ? v : m;
+  }
+  red[t] = m;
+  __syncthreads();
+  for (uint32_t off = 128; off > 0; off >>= 1) {
+    if (t < off) red[t] = red[t] > red[t + off] ? red[t] : red[t + off];
+    __syncthreads();
+  }
+  if (t == 0) tile_max[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_cdf_tiles(const double* __restrict__ w, uint64_t n,
+                                                   uint64_t* __restrict__ cdf, DevState* st,
+                                                   int which, const float* __restrict__ alpha,
+                                                   const uint64_t* __restrict__ tile_max,
+                                                   uint64_t* __restrict__ tile_sum,
+                                                   uint64_t* __restrict__ tile_lp,
+                                                   double* __restrict__ tile_a2) {
+  __shared__ uint64_t sc[2][256];
+  __shared__ double ad[256];
+  const uint32_t t = threadIdx.x;
+  {
+    uint64_t m = 0;
+    for (uint32_t i = t; i < gridDim.x; i += 256) m = tile_max[i] > m ? tile_max[i] : m;
+    sc[0][t] = m;
+    __syncthreads();
+    for (uint32_t off = 128; off > 0; off >>= 1) {
+      if (t < off) sc[0][t] = sc[0][t] > sc[0][t + off] ? sc[0][t] : sc[0][t + off];
+      __syncthreads();
+    }
+  }
+  const double mx = __longlong_as_double((long long)sc[0][0]);
+  const bool bad = !(mx <= 1.79769313486231570e+308);  // NaN or Inf
+  const double scale = (!bad && mx > 0.0) ? ldexp(1.0, 62 - ceil_log2_u64(n)) / mx : 0.0;
+  __syncthreads();
+  if (bad && blockIdx.x == 0 && t == 0) atomicOr(&st->status, kStatusNonFinite);
+  const uint64_t b = (uint64_t)blockIdx.x * kCdfTile + (uint64_t)t * kCdfPer;
+  uint64_t q[kCdfPer];
+  uint64_t run = 0, lp = 0;
+  bool any = false;
+  double a2 = 0.0;
+#pragma unroll
+  for (uint32_t k = 0; k < kCdfPer; ++k) {
+    const uint64_t j = b + k;
+    q[k] = j < n ? (uint64_t)(w[j] * scale) : 0ull;
+    run += q[k];
+    q[k] = run;  // in-lane inclusive
+    if (j < n && q[k] != (k ? q[k - 1] : 0ull)) {
+      lp = j;
+      any = true;
+    }
+    if (alpha && j < n) {
       const double a = alpha[j];
       a2 += a * a;
     }
   }
-  bad = __syncthreads_or(bad);
-  dred[t] = mx;
-  __syncthreads();
-  for (int off = 512; off > 0; off >>= 1) {
-    if (t < off) dred[t] = dred[t] > dred[t + off] ? dred[t] : dred[t + off];
-    __syncthreads();
-  }
-  mx = dred[0];
-  __syncthreads();
-  const double scale = (!bad && mx > 0.0) ? ldexp(1.0, 62 - ceil_log2_u64(n)) / mx : 0.0;
-  if (bad && t == 0) atomicOr(&st->status, kStatusNonFinite);
-  // 2. lane sums of q_j, inclusive scan over lanes
-  uint64_t s = 0;
-  for (uint64_t j = b; j < e; ++j) s += (uint64_t)((unit ? 1.0 : w[j]) * scale);
-  buf[0][t] = s;
+  sc[0][t] = run;
+  ad[t] = a2;
   __syncthreads();
   int cur = 0;
-  for (int off = 1; off < 1024; off <<= 1) {
-    const uint64_t v = t >= off ? buf[cur][t] + buf[cur][t - off] : buf[cur][t];
-    buf[cur ^ 1][t] = v;
+  for (uint32_t off = 1; off < 256; off <<= 1) {
+    const uint64_t v = t >= off ? sc[cur][t] + sc[cur][t - off] : sc[cur][t];
+    sc[cur ^ 1][t] = v;
     cur ^= 1;
     __syncthreads();
   }
-  // 3. running sums in the lane
-  uint64_t run = t == 0 ? 0 : buf[cur][t - 1];
-  uint64_t lp = 0;
-  for (uint64_t j = b; j < e; ++j) {
-    const uint64_t q = (uint64_t)((unit ? 1.0 : w[j]) * scale);
-    run += q;
-    cdf[j] = run;
-    if (q) lp = j;
-  }
-  if (b < n && e == n) st->total[which] = run;
+  const uint64_t pre = t ? sc[cur][t - 1] : 0ull;
+#pragma unroll
+  for (uint32_t k = 0; k < kCdfPer; ++k)
+    if (b + k < n) cdf[b + k] = pre + q[k];
+  // tile-level: total, last positive column, alpha^2 partial
   __syncthreads();
-  buf[0][t] = lp;  // last positive column: max over lanes
+  sc[cur ^ 1][t] = any ? lp + 1 : 0ull;  // 0 = none in this lane
   __syncthreads();
-  for (int off = 512; off > 0; off >>= 1) {
-    if (t < off) buf[0][t] = buf[0][t] > buf[0][t + off] ? buf[0][t] : buf[0][t + off];
-    __syncthreads();
-  }
-  if (t == 0) st->last_pos[which] = buf[0][0];
-  if (alpha) {
-    dred[t] = a2;
-    __syncthreads();
-    for (int off = 512; off > 0; off >>= 1) {
-      if (t < off) dred[t] = dred[t] + dred[t + off];
-      __syncthreads();
+  for (uint32_t off = 128; off > 0; off >>= 1) {
+    if (t < off) {
+      const uint64_t a = sc[cur ^ 1][t], c = sc[cur ^ 1][t + off];
+      sc[cur ^ 1][t] = a > c ? a : c;
+      ad[t] = ad[t] + ad[t + off];
     }
-    if (t == 0) st->alpha2 = dred[0];
+    __syncthreads();
+  }
+  if (t == 0) {
+    tile_sum[blockIdx.x] = sc[cur][255];
+    tile_lp[blockIdx.x] = sc[cur ^ 1][0];
+    tile_a2[blockIdx.x] = ad[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cdf_offsets(uint64_t* __restrict__ cdf, uint64_t n,
+                                                     const uint64_t* __restrict__ tile_sum,
+                                                     const uint64_t* __restrict__ tile_lp,
+                                                     const double* __restrict__ tile_a2,
+                                                     DevState* st, int which, int want_a2) {
+  __shared__ uint64_t red[256];
+  const uint32_t t = threadIdx.x, tile = blockIdx.x;
+  uint64_t s = 0;
+  for (uint32_t i = t; i < tile; i += 256) s += tile_sum[i];
+  red[t] = s;
+  __syncthreads();
+  for (uint32_t off = 128; off > 0; off >>= 1) {
+    if (t < off) red[t] += red[t + off];
+    __syncthreads();
+  }
+  const uint64_t offset = red[0];
+  const uint64_t b = (uint64_t)tile * kCdfTile;
+  if (offset)
+    for (uint64_t j = b + t; j < b + kCdfTile && j < n; j += 256) cdf[j] += offset;
+  if (tile == gridDim.x - 1 && t == 0) {
+    st->total[which] = offset + tile_sum[tile];
+    uint64_t lp = 0;
+    double a2 = 0.0;
+    for (uint32_t i = 0; i <= tile; ++i) {
+      lp = tile_lp[i] > lp ? tile_lp[i] : lp;
+      a2 += tile_a2[i];
+    }
+    st->last_pos[which] = lp ? lp - 1 : 0ull;
+    if (want_a2) st->alpha2 = a2;
   }
 }
 
@@ -283,13 +346,12 @@ __global__ __launch_bounds__(1024) void k_sample(const uint64_t* __restrict__ cd
 // ---------------------------------------------------------------------------
 // x0 ~ N(0, I) (curandGenerateNormal, powerMethod_by_exact.cuh:32): Box-Muller
 // on Philox counter (m, 0, trial, kStreamX0) for rows 2m, 2m+1.  Writes the
-// raw vector into y and 1/||y|| into st->scale; k_normalize then forms x
-// (normalize<512>, common.h:156-183).
+// raw vector into y and per-block sums of squares into BP; k_tail then forms
+// 1/||y|| and x (normalize<512>, common.h:156-183).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_x0(float* __restrict__ y, uint32_t d, uint64_t dpad,
-                                            DevState* st, double* __restrict__ BP) {
+                                            const DevState* st, double* __restrict__ BP) {
   __shared__ double red[4];
-  __shared__ bool last;
   const uint64_t m = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   const uint64_t seed = st->seed;
   double acc = 0.0;
@@ -312,15 +374,6 @@ __global__ __launch_bounds__(256) void k_x0(float* __restrict__ y, uint32_t d, u
   }
   acc = block256_sum(acc, red);
   if (threadIdx.x == 0) BP[blockIdx.x] = acc;
-  if (last_block_arrive(&st->counter[2], &last)) {
-    double s = 0.0;
-    for (uint32_t i = threadIdx.x; i < gridDim.x; i += 256) s += BP[i];
-    s = block256_sum(s, red);
-    if (threadIdx.x == 0) {
-      st->scale = s > 0.0 ? 1.0 / sqrt(s) : 0.0;
-      reset_counter(&st->counter[2]);
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -557,11 +610,9 @@ __global__ __launch_bounds__(256) void k_gather_axpy(
 //   exact:    base = 0
 //   previous: base = y_memo
 //   proposed: base = y_memo + s v + q - sum_s P2s[s]
-// (core_proposedAMM.cuh:198-212), then FP64 y^T x, y^T y, y^T v*.  Exact
-// steps with memo snapshot [x, y] into [x_memo, y_memo]
-// (powerMethod_by_previous.cuh:157).  The last block writes lambda_i = y^T x,
-// err_i = y^T v* / ||y||, cos_i = y^T x / ||y||, 1/||y||, the time stamp, and
-// (adaptive variance reduction) the next iteration's step kind.
+// (core_proposedAMM.cuh:198-212), then per-block FP64 partials of y^T x,
+// y^T y, y^T v* (k_tail finishes them).  Exact steps with memo snapshot
+// [x, y] into [x_memo, y_memo] (powerMethod_by_previous.cuh:157).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool exact_step_due(uint32_t kernel, uint64_t i, uint64_t period,
                                                double thr, double prev_cos, uint64_t last_exact) {
@@ -574,16 +625,17 @@ __device__ __forceinline__ bool exact_step_due(uint32_t kernel, uint64_t i, uint
   return false;
 }
 
+constexpr uint32_t kFinishRows = 64;  // rows per finish block
+
 __global__ __launch_bounds__(256) void k_finish(
     const float* __restrict__ P2, uint32_t S_exact, uint32_t S_amm, uint64_t dpad, uint32_t d,
-    const float* __restrict__ x, float* __restrict__ y, float* __restrict__ xm, float* __restrict__ ym,
-    const float* __restrict__ v, const double* __restrict__ P2s, const double* __restrict__ eigvec,
-    double* __restrict__ BP, DevState* st, uint32_t* __restrict__ mode, uint32_t it,
-    double* __restrict__ eig, double* __restrict__ err, int64_t* __restrict__ ts,
-    uint32_t adapt_kernel, uint64_t period, double thr, uint32_t step) {
+    const float* __restrict__ x, float* __restrict__ y, float* __restrict__ xm,
+    float* __restrict__ ym, const float* __restrict__ v, const double* __restrict__ P2s,
+    const double* __restrict__ eigvec, double* __restrict__ BP, const DevState* st,
+    const uint32_t* __restrict__ mode, uint32_t it) {
+  __shared__ double part[16][kFinishRows];
   __shared__ double red[4];
   __shared__ double corr_l;
-  __shared__ bool last;
   const uint32_t md = mode[it];
   const bool prop = md == kModeAmmProposed;
   const bool memo_base = md == kModeAmmPrevious || md == kModeAmmProposed;
@@ -597,40 +649,43 @@ __global__ __launch_bounds__(256) void k_finish(
   }
   const double sx = prop ? st->s : 0.0, qx = prop ? st->q : 0.0;
   const double corr = prop ? corr_l : 0.0;
+  // 16 groups of 16 lanes; group g sums chunks g, g+16, ... for 64 rows
+  {
+    const uint32_t g = threadIdx.x >> 4, l16 = threadIdx.x & 15;
+    const uint64_t r0 = (uint64_t)blockIdx.x * kFinishRows + l16 * 4;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    if (r0 < dpad) {
+      for (uint32_t s = g; s < S; s += 16) {
+        const float4 p = *reinterpret_cast<const float4*>(P2 + (uint64_t)s * dpad + r0);
+        a0 += p.x;
+        a1 += p.y;
+        a2 += p.z;
+        a3 += p.w;
+      }
+    }
+    part[g][l16 * 4 + 0] = a0;
+    part[g][l16 * 4 + 1] = a1;
+    part[g][l16 * 4 + 2] = a2;
+    part[g][l16 * 4 + 3] = a3;
+  }
+  __syncthreads();
   double yx = 0.0, yy = 0.0, yv = 0.0;
-  const uint64_t r0 = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * 4;
-  if (r0 < dpad) {
-    double acc[4];
+  const uint64_t row = (uint64_t)blockIdx.x * kFinishRows + threadIdx.x;
+  if (threadIdx.x < kFinishRows && row < dpad) {
+    double acc = memo_base ? (double)ym[row] : 0.0;
+    if (prop) acc += sx * (double)v[row] + qx - corr;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) acc[k] = 0.0;
-    if (memo_base) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) acc[k] = ym[r0 + k];
-    }
-    if (prop) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) acc[k] += sx * (double)v[r0 + k] + qx - corr;
-    }
-    for (uint32_t s = 0; s < S; ++s) {
-      const float4 p = *reinterpret_cast<const float4*>(P2 + (uint64_t)s * dpad + r0);
-      acc[0] += p.x;
-      acc[1] += p.y;
-      acc[2] += p.z;
-      acc[3] += p.w;
-    }
-    float yf[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      yf[k] = (r0 + k < d) ? (float)acc[k] : 0.0f;
-      const double yd = yf[k];
-      yx += yd * (double)x[r0 + k];
-      yy += yd * yd;
-      if (eigvec && r0 + k < d) yv += yd * eigvec[r0 + k];
-    }
-    *reinterpret_cast<float4*>(y + r0) = make_float4(yf[0], yf[1], yf[2], yf[3]);
+    for (int g = 0; g < 16; ++g) acc += part[g][threadIdx.x];
+    const float yf = row < d ? (float)acc : 0.0f;
+    const double yd = yf;
+    const float xr = x[row];
+    yx = yd * (double)xr;
+    yy = yd * yd;
+    if (eigvec && row < d) yv = yd * eigvec[row];
+    y[row] = yf;
     if (md == kModeExactMemo) {
-      *reinterpret_cast<float4*>(xm + r0) = *reinterpret_cast<const float4*>(x + r0);
-      *reinterpret_cast<float4*>(ym + r0) = make_float4(yf[0], yf[1], yf[2], yf[3]);
+      xm[row] = xr;
+      ym[row] = yf;
     }
   }
   yx = block256_sum(yx, red);
@@ -641,9 +696,36 @@ __global__ __launch_bounds__(256) void k_finish(
     BP[3 * blockIdx.x + 1] = yy;
     BP[3 * blockIdx.x + 2] = yv;
   }
-  if (last_block_arrive(&st->counter[0], &last)) {
+}
+
+// ---------------------------------------------------------------------------
+// Single-block tail of every grid-wide reduction: sums the block partials in
+// a fixed order and updates the scalar state.  A separate launch (rather
+// than a last-block-done epilogue) keeps every block free of the
+// device-scope fences that cross-XCD visibility would need.
+//   kTailFinish: lambda_i = y^T x, err_i = y^T v* / ||y||, cos_i, 1/||y||,
+//                last exact step, the next step kind (adaptive variance
+//                reduction), the time stamp
+//   kTailScale:  1/||y|| of the raw x0
+//   kTailSQ:     s, q of a stand-alone product or a grid normalisation
+// With norm != 0 the block then writes x = y / ||y|| for all d rows and,
+// with z given, s = 1^T (x - x_memo), q = z^T (x - x_memo).
+// ---------------------------------------------------------------------------
+//   kTailNorm:   nothing to reduce; normalise with the stored 1/||y||
+// (TailOp is declared in amm_launch.hpp.)
+
+__global__ __launch_bounds__(256) void k_tail(
+    uint32_t op, const double* __restrict__ BP, uint32_t nb, DevState* st, int norm,
+    const float* __restrict__ y, float* __restrict__ x, const float* __restrict__ xm,
+    const float* __restrict__ z, uint32_t d, uint64_t dpad, uint32_t* __restrict__ mode, uint32_t it,
+    double* __restrict__ eig, double* __restrict__ err, int64_t* __restrict__ ts, int stamp,
+    uint32_t adapt_kernel, uint64_t period, double thr, uint32_t step) {
+  __shared__ double red[4];
+  __shared__ double scale_l;
+  const uint32_t t = threadIdx.x;
+  if (op == kTailFinish) {
     double a = 0.0, b = 0.0, c = 0.0;
-    for (uint32_t i = threadIdx.x; i < gridDim.x; i += 256) {
+    for (uint32_t i = t; i < nb; i += 256) {
       a += BP[3 * i];
       b += BP[3 * i + 1];
       c += BP[3 * i + 2];
@@ -651,11 +733,13 @@ __global__ __launch_bounds__(256) void k_finish(
     a = block256_sum(a, red);
     b = block256_sum(b, red);
     c = block256_sum(c, red);
-    if (threadIdx.x == 0) {
+    if (t == 0) {
+      const uint32_t md = mode[it];
       const double scale = b > 0.0 ? 1.0 / sqrt(b) : 0.0;
       eig[it] = a;
       err[it] = c * scale;
       st->scale = scale;
+      scale_l = scale;
       const double cs = a * scale;
       st->cos_prev = cs;
       uint64_t le = st->last_exact;
@@ -667,26 +751,77 @@ __global__ __launch_bounds__(256) void k_finish(
         const bool ex = exact_step_due(adapt_kernel, it + 1, period, thr, cs, le);
         mode[it + 1] = ex ? kModeExactMemo : (adapt_kernel == 2 ? kModeAmmProposed : kModeAmmPrevious);
       }
-      ts[it] = wall_clock64();
-      reset_counter(&st->counter[0]);
+    }
+  } else if (op == kTailScale) {
+    double s = 0.0;
+    for (uint32_t i = t; i < nb; i += 256) s += BP[i];
+    s = block256_sum(s, red);
+    if (t == 0) {
+      const double scale = s > 0.0 ? 1.0 / sqrt(s) : 0.0;
+      st->scale = scale;
+      scale_l = scale;
+    }
+  } else if (op == kTailSQ) {
+    double a = 0.0, b = 0.0;
+    for (uint32_t i = t; i < nb; i += 256) {
+      a += BP[2 * i];
+      b += BP[2 * i + 1];
+    }
+    a = block256_sum(a, red);
+    b = block256_sum(b, red);
+    if (t == 0) {
+      st->s = a;
+      st->q = b;
+    }
+  } else if (t == 0) {
+    scale_l = st->scale;
+  }
+  __syncthreads();
+  if (norm) {
+    const double scale = scale_l;
+    double ps = 0.0, pq = 0.0;
+    for (uint64_t r = (uint64_t)t * 4; r < dpad; r += 1024) {
+      const float4 yv4 = *reinterpret_cast<const float4*>(y + r);
+      float xv[4] = {(float)(scale * (double)yv4.x), (float)(scale * (double)yv4.y),
+                     (float)(scale * (double)yv4.z), (float)(scale * (double)yv4.w)};
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (r + k >= d) xv[k] = 0.0f;
+      *reinterpret_cast<float4*>(x + r) = make_float4(xv[0], xv[1], xv[2], xv[3]);
+      if (z) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const double dx = (double)xv[k] - (double)xm[r + k];
+          ps += dx;
+          pq += (double)z[r + k] * dx;
+        }
+      }
+    }
+    if (z) {
+      ps = block256_sum(ps, red);
+      pq = block256_sum(pq, red);
+      if (t == 0) {
+        st->s = ps;
+        st->q = pq;
+      }
     }
   }
+  if (stamp && t == 0) ts[it] = wall_clock64();
 }
 
 // ---------------------------------------------------------------------------
-// x = y / ||y|| (the generate-x half of dotProduct_and_generate_x,
-// common.h:146-153); with z given, also s = 1^T (x - x_memo) and
-// q = z^T (x - x_memo) for the next proposed product (dots<128>,
-// common.h:55-87).
+// x = y / ||y|| over a grid (the generate-x half of
+// dotProduct_and_generate_x, common.h:146-153) for large d; with z given,
+// per-block partials of s = 1^T (x - x_memo) and q = z^T (x - x_memo) for
+// the next proposed product (dots<128>, common.h:55-87), finished by k_tail.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_normalize(const float* __restrict__ y,
                                                    float* __restrict__ x, uint32_t d,
-                                                   uint64_t dpad, DevState* st,
+                                                   uint64_t dpad, const DevState* st,
                                                    const float* __restrict__ xm,
                                                    const float* __restrict__ z,
                                                    double* __restrict__ BP) {
   __shared__ double red[4];
-  __shared__ bool last;
   const double scale = st->scale;
   double ps = 0.0, pq = 0.0;
   const uint64_t r0 = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * 4;
@@ -714,30 +849,15 @@ __global__ __launch_bounds__(256) void k_normalize(const float* __restrict__ y,
     BP[2 * blockIdx.x] = ps;
     BP[2 * blockIdx.x + 1] = pq;
   }
-  if (last_block_arrive(&st->counter[1], &last)) {
-    double a = 0.0, b = 0.0;
-    for (uint32_t i = threadIdx.x; i < gridDim.x; i += 256) {
-      a += BP[2 * i];
-      b += BP[2 * i + 1];
-    }
-    a = block256_sum(a, red);
-    b = block256_sum(b, red);
-    if (threadIdx.x == 0) {
-      st->s = a;
-      st->q = b;
-      reset_counter(&st->counter[1]);
-    }
-  }
 }
 
-// s = 1^T (x - x_memo), q = z^T (x - x_memo) for a stand-alone proposed
-// product (dots<128>, common.h:55-87); FP64, fixed-order reduction.
+// per-block partials of s = 1^T (x - x_memo), q = z^T (x - x_memo) for a
+// stand-alone proposed product (dots<128>, common.h:55-87), finished by k_tail.
 __global__ __launch_bounds__(256) void k_dots(const float* __restrict__ x,
                                               const float* __restrict__ xm,
                                               const float* __restrict__ z, uint64_t dpad,
-                                              DevState* st, double* __restrict__ BP) {
+                                              double* __restrict__ BP) {
   __shared__ double red[4];
-  __shared__ bool last;
   double ps = 0.0, pq = 0.0;
   const uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (r < dpad) {
@@ -750,20 +870,6 @@ __global__ __launch_bounds__(256) void k_dots(const float* __restrict__ x,
   if (threadIdx.x == 0) {
     BP[2 * blockIdx.x] = ps;
     BP[2 * blockIdx.x + 1] = pq;
-  }
-  if (last_block_arrive(&st->counter[3], &last)) {
-    double a = 0.0, b = 0.0;
-    for (uint32_t i = threadIdx.x; i < gridDim.x; i += 256) {
-      a += BP[2 * i];
-      b += BP[2 * i + 1];
-    }
-    a = block256_sum(a, red);
-    b = block256_sum(b, red);
-    if (threadIdx.x == 0) {
-      st->s = a;
-      st->q = b;
-      reset_counter(&st->counter[3]);
-    }
   }
 }
 
@@ -793,8 +899,9 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint
 static inline unsigned grid1(uint64_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
 
 hipError_t launch_colstats(int dtype, const void* A, uint64_t ld, uint32_t d, uint64_t n,
-                           float* alpha, double* nrm2, double* cnrm2, hipStream_t s) {
+                           float* alpha, double* nrm2, double* cnrm2, DevState* st, hipStream_t s) {
   const dim3 g(grid1(n, 4));
+  (void)st;
   if (dtype == 1)
     k_colstats<__hip_bfloat16><<<g, 256, 0, s>>>((const __hip_bfloat16*)A, ld, d, n, alpha, nrm2, cnrm2);
   else
@@ -802,9 +909,15 @@ hipError_t launch_colstats(int dtype, const void* A, uint64_t ld, uint32_t d, ui
   return hipGetLastError();
 }
 
-hipError_t launch_cdf(const double* w, int unit, uint64_t n, uint64_t* cdf, DevState* st, int which,
-                      const float* alpha, hipStream_t s) {
-  k_cdf<<<1, 1024, 0, s>>>(w, unit, n, cdf, st, which, alpha);
+uint64_t cdf_tiles(uint64_t n) { return grid1(n, kCdfTile); }
+
+hipError_t launch_cdf(const double* w, uint64_t n, uint64_t* cdf, DevState* st, int which,
+                      const float* alpha, uint64_t* tile_max, uint64_t* tile_sum, uint64_t* tile_lp,
+                      double* tile_a2, hipStream_t s) {
+  const unsigned nt = (unsigned)cdf_tiles(n);
+  k_cdf_max<<<nt, 256, 0, s>>>(w, n, tile_max);
+  k_cdf_tiles<<<nt, 256, 0, s>>>(w, n, cdf, st, which, alpha, tile_max, tile_sum, tile_lp, tile_a2);
+  k_cdf_offsets<<<nt, 256, 0, s>>>(cdf, n, tile_sum, tile_lp, tile_a2, st, which, alpha ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -817,8 +930,9 @@ hipError_t launch_sample(const uint64_t* cdf, uint64_t n, const DevState* st,
   return hipGetLastError();
 }
 
-hipError_t launch_x0(float* y, uint32_t d, uint64_t dpad, DevState* st, double* BP, hipStream_t s) {
-  k_x0<<<grid1(dpad / 2, 256), 256, 0, s>>>(y, d, dpad, st, BP);
+hipError_t launch_x0(float* y, uint32_t d, uint64_t dpad, const DevState* st, double* BP,
+                     hipStream_t s) {
+  k_x0<<<(unsigned)x0_blocks(dpad), 256, 0, s>>>(y, d, dpad, st, BP);
   return hipGetLastError();
 }
 
@@ -867,15 +981,26 @@ hipError_t launch_gather_axpy(int dtype, const GatherAxpyArgs& a, hipStream_t s)
 }
 
 hipError_t launch_finish(const FinishArgs& a, hipStream_t s) {
-  k_finish<<<grid1(a.dpad, 1024), 256, 0, s>>>(a.P2, a.S_exact, a.S_amm, a.dpad, a.d, a.x, a.y, a.xm, a.ym, a.v,
-                                               a.P2s, a.eigvec, a.BP, a.st, a.mode, a.it, a.eig,
-                                               a.err, a.ts, a.adapt_kernel, a.period, a.thr, a.step);
+  k_finish<<<grid1(a.dpad, kFinishRows), 256, 0, s>>>(a.P2, a.S_exact, a.S_amm, a.dpad, a.d, a.x,
+                                                       a.y, a.xm, a.ym, a.v, a.P2s, a.eigvec, a.BP,
+                                                       a.st, a.mode, a.it);
   return hipGetLastError();
 }
 
-hipError_t launch_normalize(const float* y, float* x, uint32_t d, uint64_t dpad, DevState* st,
+uint64_t finish_blocks(uint64_t dpad) { return grid1(dpad, kFinishRows); }
+uint64_t x0_blocks(uint64_t dpad) { return grid1(dpad / 2, 256); }
+uint64_t normalize_blocks(uint64_t dpad) { return grid1(dpad, 1024); }
+uint64_t dots_blocks(uint64_t dpad) { return grid1(dpad, 256); }
+
+hipError_t launch_tail(const TailArgs& a, hipStream_t s) {
+  k_tail<<<1, 256, 0, s>>>(a.op, a.BP, a.nb, a.st, a.norm, a.y, a.x, a.xm, a.z, a.d, a.dpad, a.mode,
+                           a.it, a.eig, a.err, a.ts, a.stamp, a.adapt_kernel, a.period, a.thr, a.step);
+  return hipGetLastError();
+}
+
+hipError_t launch_normalize(const float* y, float* x, uint32_t d, uint64_t dpad, const DevState* st,
                             const float* xm, const float* z, double* BP, hipStream_t s) {
-  k_normalize<<<grid1(dpad, 1024), 256, 0, s>>>(y, x, d, dpad, st, xm, z, BP);
+  k_normalize<<<(unsigned)normalize_blocks(dpad), 256, 0, s>>>(y, x, d, dpad, st, xm, z, BP);
   return hipGetLastError();
 }
 
@@ -885,9 +1010,9 @@ hipError_t launch_sum_rows(const float* P2, uint32_t S, uint64_t dpad, uint32_t 
   return hipGetLastError();
 }
 
-hipError_t launch_dots(const float* x, const float* xm, const float* z, uint64_t dpad, DevState* st,
-                       double* BP, hipStream_t s) {
-  k_dots<<<grid1(dpad, 256), 256, 0, s>>>(x, xm, z, dpad, st, BP);
+hipError_t launch_dots(const float* x, const float* xm, const float* z, uint64_t dpad, double* BP,
+                       hipStream_t s) {
+  k_dots<<<(unsigned)dots_blocks(dpad), 256, 0, s>>>(x, xm, z, dpad, BP);
   return hipGetLastError();
 }
 

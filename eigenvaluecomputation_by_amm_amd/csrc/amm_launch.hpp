@@ -62,34 +62,66 @@ struct FinishArgs {
   const double* P2s;
   const double* eigvec;
   double* BP;
+  const DevState* st;
+  const uint32_t* mode;
+  uint32_t it;
+};
+
+// single-block tail of a grid reduction (k_tail)
+enum TailOp : uint32_t { kTailFinish = 0, kTailScale = 1, kTailSQ = 2, kTailNorm = 3 };
+
+struct TailArgs {
+  uint32_t op;            // 0 finish, 1 scale of x0, 2 s and q
+  const double* BP;
+  uint32_t nb;            // blocks whose partials are in BP
   DevState* st;
+  int norm;               // also write x = y / ||y|| (and s, q when z is given)
+  const float* y;
+  float* x;
+  const float* xm;
+  const float* z;
+  uint32_t d;
+  uint64_t dpad;
   uint32_t* mode;
   uint32_t it;
   double* eig;
   double* err;
   int64_t* ts;
+  int stamp;              // write the iteration's time stamp
   uint32_t adapt_kernel;  // 0: static schedule; 1 previous / 2 proposed: decide next step on device
   uint64_t period;
   double thr;
   uint32_t step;
 };
 
+// blocks of one k_finish launch (each writes 3 doubles of block partials)
+uint64_t finish_blocks(uint64_t dpad);
+uint64_t x0_blocks(uint64_t dpad);         // 1 double per block
+uint64_t normalize_blocks(uint64_t dpad);  // 2 doubles per block
+uint64_t dots_blocks(uint64_t dpad);       // 2 doubles per block
+hipError_t launch_tail(const TailArgs& a, hipStream_t s);
+// tiles of the CDF scan (tile scratch: one uint64 sum, uint64 last column,
+// double ||alpha||^2 partial per tile)
+uint64_t cdf_tiles(uint64_t n);
+
 hipError_t launch_colstats(int dtype, const void* A, uint64_t ld, uint32_t d, uint64_t n,
-                           float* alpha, double* nrm2, double* cnrm2, hipStream_t s);
-hipError_t launch_cdf(const double* w, int unit, uint64_t n, uint64_t* cdf, DevState* st, int which,
-                      const float* alpha, hipStream_t s);
+                           float* alpha, double* nrm2, double* cnrm2, DevState* st, hipStream_t s);
+hipError_t launch_cdf(const double* w, uint64_t n, uint64_t* cdf, DevState* st, int which,
+                      const float* alpha, uint64_t* tile_max, uint64_t* tile_sum, uint64_t* tile_lp,
+                      double* tile_a2, hipStream_t s);
 hipError_t launch_sample(const uint64_t* cdf, uint64_t n, const DevState* st, int which, uint32_t it0, uint32_t nit,
                          uint32_t buf_it0, uint32_t c, uint32_t* idx, float* wgt,
                          uint32_t* seg_count, hipStream_t s);
-hipError_t launch_dots(const float* x, const float* xm, const float* z, uint64_t dpad, DevState* st,
-                       double* BP, hipStream_t s);
-hipError_t launch_x0(float* y, uint32_t d, uint64_t dpad, DevState* st, double* BP, hipStream_t s);
+hipError_t launch_dots(const float* x, const float* xm, const float* z, uint64_t dpad, double* BP,
+                       hipStream_t s);
+hipError_t launch_x0(float* y, uint32_t d, uint64_t dpad, const DevState* st, double* BP,
+                     hipStream_t s);
 hipError_t launch_begin(float* xm, float* ym, uint64_t dpad, DevState* st, uint32_t* mode,
                         uint32_t mode0, hipStream_t s);
 hipError_t launch_gather_dot(int dtype, const GatherDotArgs& a, hipStream_t s);
 hipError_t launch_gather_axpy(int dtype, const GatherAxpyArgs& a, hipStream_t s);
 hipError_t launch_finish(const FinishArgs& a, hipStream_t s);
-hipError_t launch_normalize(const float* y, float* x, uint32_t d, uint64_t dpad, DevState* st,
+hipError_t launch_normalize(const float* y, float* x, uint32_t d, uint64_t dpad, const DevState* st,
                             const float* xm, const float* z, double* BP, hipStream_t s);
 hipError_t launch_sum_rows(const float* P2, uint32_t S, uint64_t dpad, uint32_t d, float* z,
                            float* v, const DevState* st, hipStream_t s);
