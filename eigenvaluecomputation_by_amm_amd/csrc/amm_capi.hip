@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -140,6 +141,12 @@ struct amm_ctx {
   DotPlan dot_dense{}, dot_list{};
   AxpyPlan axpy_dense{}, axpy_list{};
   uint32_t list_c = 0;
+  // non-temporal loads in the sampled passes: bit 0 dot, bit 1 axpy
+  // (environment AMM_LIST_NT overrides).  Off by default: sampled columns are
+  // read twice per iteration and heavy ones recur across iterations, and on
+  // the 16384 x 65536 workload plain loads were faster for both passes
+  // (dot 7.7 vs 10.7 us, axpy 8.1 vs 9.0 us per launch).
+  uint32_t list_nt = 0;
 };
 
 namespace {
@@ -313,8 +320,10 @@ int enq_amm(amm_ctx* c, uint32_t it, uint32_t slot, bool proposed, bool sanity,
   g.mode = gate_mode;
   g.it = it;
   g.gate_mask = gate_mask;
+  g.nt = (c->list_nt & 1u) != 0;
   HIPCHK(amm::launch_gather_dot(c->dtype, g, c->stream));
   amm::GatherAxpyArgs a{};
+  a.nt = (c->list_nt & 2u) != 0;
   a.A = c->A;
   a.ld = c->ld;
   a.d = (uint32_t)c->d;
@@ -659,6 +668,7 @@ int amm_ctx_create(const void* A, int32_t dtype, uint64_t d, uint64_t n, uint64_
     return bail(rc);
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
     return bail(fail(AMM_E_HIP, "hipEventCreate failed"));
+  if (const char* e = std::getenv("AMM_LIST_NT")) c->list_nt = (uint32_t)std::strtoul(e, nullptr, 0) & 3u;
   if ((rc = ensure_plans(c, 0))) return bail(rc);
   *out = c;
   return AMM_OK;
@@ -1024,6 +1034,8 @@ int amm_time_product(amm_ctx* c, int32_t kernel, uint32_t cc, uint64_t seed, uin
   a.st = c->st;
   a.P2 = c->P2;
   a.P2s = c->P2s;
+  g.nt = (c->list_nt & 1u) != 0;
+  a.nt = (c->list_nt & 2u) != 0;
   HIPCHK(amm::launch_gather_dot(dtype, g, c->stream));
   HIPCHK(amm::launch_gather_axpy(dtype, a, c->stream));
   hipEvent_t e2 = nullptr;

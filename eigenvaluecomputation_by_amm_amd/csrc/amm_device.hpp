@@ -111,6 +111,17 @@ __device__ __forceinline__ float bf16_bits_to_f32(uint32_t h16) {
   return __uint_as_float(h16 << 16);
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// 16-byte load; NT marks it non-temporal (streamed data read once: measured
+// ~7.0 TB/s on MI355X for a 4 GiB read against ~6.2 TB/s for plain loads).
+template <bool NT>
+__device__ __forceinline__ u32x4 load16(const void* p) {
+  if (NT) return __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+  return *reinterpret_cast<const u32x4*>(p);
+}
+
 // Element-type traits: VEC elements per 16-byte load.
 template <typename T>
 struct Elem;
@@ -118,12 +129,13 @@ struct Elem;
 template <>
 struct Elem<float> {
   static constexpr int VEC = 4;
+  template <bool NT = false>
   __device__ __forceinline__ static void load_full(const float* p, float (&v)[4]) {
-    const float4 q = *reinterpret_cast<const float4*>(p);
-    v[0] = q.x;
-    v[1] = q.y;
-    v[2] = q.z;
-    v[3] = q.w;
+    const u32x4 q = load16<NT>(p);
+    v[0] = __uint_as_float(q.x);
+    v[1] = __uint_as_float(q.y);
+    v[2] = __uint_as_float(q.z);
+    v[3] = __uint_as_float(q.w);
   }
   __device__ __forceinline__ static float load_one(const float* p) { return *p; }
 };
@@ -131,8 +143,9 @@ struct Elem<float> {
 template <>
 struct Elem<__hip_bfloat16> {
   static constexpr int VEC = 8;
+  template <bool NT = false>
   __device__ __forceinline__ static void load_full(const __hip_bfloat16* p, float (&v)[8]) {
-    const uint4 q = *reinterpret_cast<const uint4*>(p);
+    const u32x4 q = load16<NT>(p);
     v[0] = bf16_bits_to_f32(q.x & 0xFFFFu);
     v[1] = bf16_bits_to_f32(q.x >> 16);
     v[2] = bf16_bits_to_f32(q.y & 0xFFFFu);

@@ -42,10 +42,10 @@ __global__ __launch_bounds__(256) void k_colstats(const T* __restrict__ A, uint6
   // full steps: every lane of the wave has V in-range rows
   for (; row + 64u * V * 3 + V <= d; row += 64u * V * 4) {
     float a0[V], a1[V], a2[V], a3[V];
-    Elem<T>::load_full(col + row, a0);
-    Elem<T>::load_full(col + row + 64 * V, a1);
-    Elem<T>::load_full(col + row + 128 * V, a2);
-    Elem<T>::load_full(col + row + 192 * V, a3);
+    Elem<T>::template load_full<true>(col + row, a0);
+    Elem<T>::template load_full<true>(col + row + 64 * V, a1);
+    Elem<T>::template load_full<true>(col + row + 128 * V, a2);
+    Elem<T>::template load_full<true>(col + row + 192 * V, a3);
 #pragma unroll
     for (int k = 0; k < V; ++k) {
       const double b0 = a0[k], b1 = a1[k], b2 = a2[k], b3 = a3[k];
@@ -411,7 +411,7 @@ __device__ __forceinline__ bool gate_open(const uint32_t* mode, uint32_t it, uin
 // LIST: col(e) = idx[e], valid when (e % seg) < seg_count[e / seg];
 // dense: col(e) = e (exact step, sanity mode).
 // ---------------------------------------------------------------------------
-template <typename T, int CPW, bool LIST>
+template <typename T, int CPW, bool LIST, bool NT>
 __global__ __launch_bounds__(256) void k_gather_dot(
     const T* __restrict__ A, uint64_t ld, uint32_t d, const uint32_t* __restrict__ idx,
     const uint32_t* __restrict__ seg_count, uint32_t cap, const float* __restrict__ x,
@@ -464,8 +464,8 @@ __global__ __launch_bounds__(256) void k_gather_dot(
 #pragma unroll
     for (int c = 0; c < CPW; ++c) {
       if (valid[c]) {
-        Elem<T>::load_full(col[c] + row, a0[c]);
-        Elem<T>::load_full(col[c] + row + STEP, a1[c]);
+        Elem<T>::template load_full<NT>(col[c] + row, a0[c]);
+        Elem<T>::template load_full<NT>(col[c] + row + STEP, a1[c]);
       }
     }
 #pragma unroll
@@ -518,7 +518,7 @@ __global__ __launch_bounds__(256) void k_gather_dot(
 // ---------------------------------------------------------------------------
 constexpr uint32_t kMaxChunk = 2048;
 
-template <typename T, bool LIST>
+template <typename T, bool LIST, bool NT>
 __global__ __launch_bounds__(256) void k_gather_axpy(
     const T* __restrict__ A, uint64_t ld, uint32_t d, uint64_t dpad,
     const uint32_t* __restrict__ idx, const float* __restrict__ wgt,
@@ -577,7 +577,7 @@ __global__ __launch_bounds__(256) void k_gather_axpy(
     for (; k + 8 <= nv; k += 8) {
       float a[8][V];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) Elem<T>::load_full(Ar + (uint64_t)cl[k + u] * ld, a[u]);
+      for (int u = 0; u < 8; ++u) Elem<T>::template load_full<NT>(Ar + (uint64_t)cl[k + u] * ld, a[u]);
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const float t = tl[k + u];
@@ -587,7 +587,7 @@ __global__ __launch_bounds__(256) void k_gather_axpy(
     }
     for (; k < nv; ++k) {
       float a[V];
-      Elem<T>::load_full(Ar + (uint64_t)cl[k] * ld, a);
+      Elem<T>::template load_full<NT>(Ar + (uint64_t)cl[k] * ld, a);
       const float t = tl[k];
 #pragma unroll
       for (int q = 0; q < V; ++q) acc[q] = fmaf(a[q], t, acc[q]);
@@ -942,41 +942,52 @@ hipError_t launch_begin(float* xm, float* ym, uint64_t dpad, DevState* st, uint3
   return hipGetLastError();
 }
 
-template <typename T, int CPW, bool LIST>
+// Dense passes stream all of A once (non-temporal loads).  A sampled list
+// reads its columns twice in an iteration (dot, then axpy) and heavy
+// columns recur across iterations, so the list passes load non-temporally
+// only where a.nt says so.
+template <typename T, int CPW, bool LIST, bool NT>
 static void gd(const GatherDotArgs& a, hipStream_t s) {
   const dim3 g((a.cap + 4 * CPW - 1) / (4 * CPW), a.R);
-  k_gather_dot<T, CPW, LIST><<<g, 256, 0, s>>>((const T*)a.A, a.ld, a.d, a.idx, a.seg_count, a.cap,
-                                               a.x, a.xm, a.rows_per_split, a.P1, a.mode, a.it,
-                                               a.gate_mask);
+  k_gather_dot<T, CPW, LIST, NT><<<g, 256, 0, s>>>((const T*)a.A, a.ld, a.d, a.idx, a.seg_count,
+                                                   a.cap, a.x, a.xm, a.rows_per_split, a.P1, a.mode,
+                                                   a.it, a.gate_mask);
+}
+
+template <typename T>
+static void gd_any(const GatherDotArgs& a, hipStream_t s) {
+  if (!a.idx) gd<T, 4, false, true>(a, s);
+  else if (a.nt) gd<T, 2, true, true>(a, s);
+  else gd<T, 2, true, false>(a, s);
 }
 
 hipError_t launch_gather_dot(int dtype, const GatherDotArgs& a, hipStream_t s) {
-  const bool list = a.idx != nullptr;
-  if (dtype == 1) {
-    if (list) gd<__hip_bfloat16, 2, true>(a, s); else gd<__hip_bfloat16, 4, false>(a, s);
-  } else {
-    if (list) gd<float, 2, true>(a, s); else gd<float, 4, false>(a, s);
-  }
+  if (dtype == 1) gd_any<__hip_bfloat16>(a, s);
+  else gd_any<float>(a, s);
   return hipGetLastError();
 }
 
-template <typename T, bool LIST>
+template <typename T, bool LIST, bool NT>
 static void ga(const GatherAxpyArgs& a, hipStream_t s) {
   constexpr int V = Elem<T>::VEC;
   const dim3 g(grid1(a.dpad, 256 * V), a.S);
-  k_gather_axpy<T, LIST><<<g, 256, 0, s>>>((const T*)a.A, a.ld, a.d, a.dpad, a.idx, a.wgt,
-                                           a.seg_count, a.cap, a.cs, a.P1, a.R, a.tdirect, a.alpha,
-                                           a.st, a.P2, a.P2s, a.mode, a.it, a.gate_mask);
+  k_gather_axpy<T, LIST, NT><<<g, 256, 0, s>>>((const T*)a.A, a.ld, a.d, a.dpad, a.idx, a.wgt,
+                                               a.seg_count, a.cap, a.cs, a.P1, a.R, a.tdirect,
+                                               a.alpha, a.st, a.P2, a.P2s, a.mode, a.it,
+                                               a.gate_mask);
+}
+
+template <typename T>
+static void ga_any(const GatherAxpyArgs& a, hipStream_t s) {
+  if (!a.idx) ga<T, false, true>(a, s);
+  else if (a.nt) ga<T, true, true>(a, s);
+  else ga<T, true, false>(a, s);
 }
 
 hipError_t launch_gather_axpy(int dtype, const GatherAxpyArgs& a, hipStream_t s) {
   if (a.cs == 0 || a.cs > kMaxChunk) return hipErrorInvalidValue;
-  const bool list = a.idx != nullptr;
-  if (dtype == 1) {
-    if (list) ga<__hip_bfloat16, true>(a, s); else ga<__hip_bfloat16, false>(a, s);
-  } else {
-    if (list) ga<float, true>(a, s); else ga<float, false>(a, s);
-  }
+  if (dtype == 1) ga_any<__hip_bfloat16>(a, s);
+  else ga_any<float>(a, s);
   return hipGetLastError();
 }
 

@@ -23,6 +23,7 @@ struct GatherDotArgs {
   const uint32_t* mode;       // null: always run
   uint32_t it;
   uint32_t gate_mask;
+  bool nt;                    // list mode: non-temporal loads of A
 };
 
 struct GatherAxpyArgs {
@@ -46,6 +47,7 @@ struct GatherAxpyArgs {
   const uint32_t* mode;
   uint32_t it;
   uint32_t gate_mask;
+  bool nt;                    // list mode: non-temporal loads of A
 };
 
 struct FinishArgs {
