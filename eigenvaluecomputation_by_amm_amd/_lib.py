@@ -49,6 +49,9 @@ _SIGS = {
     "amm_time_column_stats": (C.c_int, [_vp, C.c_uint32, _vp]),
     "amm_generate_symmetric": (C.c_int, [_vp, C.c_uint64, C.c_uint64, C.c_int32, C.c_float, C.c_float,
                                          C.c_uint64, _vp, _vp]),
+    "amm_sigma_max": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "amm_rank1_update": (C.c_int, [_vp, C.c_int32, C.c_uint64, C.c_uint64, C.c_uint64, C.c_float,
+                                   _vp, _vp, _vp]),
     "amm_f32_to_bf16": (C.c_int, [_vp, _vp, C.c_uint64, _vp]),
 }
 

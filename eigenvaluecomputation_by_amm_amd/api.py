@@ -230,10 +230,51 @@ class AmmContext:
         check(self._lib.amm_time_column_stats(self._h, iters, C.c_void_p(_ptr(t))))
         return float(t[0])
 
+    def sigma_max(self, params: RunParams, device=None):
+        """sigma_max(A) = sqrt(lambda_max(A A^T)) with unit singular vectors:
+        runs the power method of `params`, u = its final iterate (length d),
+        then one exact pass t = A^T u gives sigma = ||t||, v = t / sigma
+        (length n).  The reference prints sqrt of the Rayleigh quotient
+        (powerMethod_by_exact.cuh:83); this form also returns u and v so that
+        d sigma / d A = u v^T can be applied."""
+        import torch
+        if device is None:
+            device = self._keep.device if hasattr(self._keep, "device") else torch.device("cuda")
+        p = params.to_c()
+        u = torch.empty(self.d, dtype=torch.float32, device=device)
+        v = torch.empty(self.n, dtype=torch.float32, device=device)
+        s = np.zeros(1, np.float64)
+        check(self._lib.amm_sigma_max(self._h, C.byref(p), C.c_void_p(_ptr(s)),
+                                      C.c_void_p(u.data_ptr()), C.c_void_p(v.data_ptr())))
+        return float(s[0]), u, v
+
     def get_vector(self, which: str = "x") -> np.ndarray:
         v = np.empty(self.d, np.float32)
         check(self._lib.amm_get_vector(self._h, 0 if which == "x" else 1, C.c_void_p(_ptr(v)), self.d))
         return v
+
+
+def rank1_update(out, g: float, u, v, d: int, n: int, ld: int) -> None:
+    """out[:, j] = g * u * v[j] for the column-major d x n matrix held by the
+    device tensor `out` (leading dimension ld; float32 or bfloat16): the
+    backward of sigma_max (d sigma / d A = u v^T)."""
+    import torch
+    lib = _lib.load()
+    if out.dtype == torch.float32:
+        dt = _lib.AMM_DTYPE_F32
+    elif out.dtype == torch.bfloat16:
+        dt = _lib.AMM_DTYPE_BF16
+    else:
+        raise ValueError("out must be float32 or bfloat16")
+    for t, m in ((u, d), (v, n)):
+        if t.dtype != torch.float32 or t.numel() != m or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError("u, v must be contiguous float32 device vectors of lengths d, n")
+    if out.numel() < (n - 1) * ld + d:
+        raise ValueError("out is smaller than a d x n matrix of leading dimension ld")
+    stream = torch.cuda.current_stream(out.device).cuda_stream
+    check(lib.amm_rank1_update(C.c_void_p(out.data_ptr()), dt, d, n, ld, float(g),
+                               C.c_void_p(u.data_ptr()), C.c_void_p(v.data_ptr()),
+                               C.c_void_p(stream) if stream else None))
 
 
 def _run(kernel: str, ctx: AmmContext, **kw) -> RunResult:

@@ -1,0 +1,95 @@
+"""Differentiable sigma_max(W) on top of the AMM power method.
+
+sigma_max(W) = sqrt(lambda_max(W W^T)) is what the reference prints as its
+Eigenvalue column (powerMethod_by_exact.cuh:83, the square root of the
+Rayleigh quotient).  `sigma_max` makes it a torch autograd op: the forward
+pass runs the power method in libamm.so (exact or AMM, any variance-reduction
+schedule of `RunParams`) and one exact pass for v = W^T u / sigma; the
+backward pass is the rank-1 adjoint d sigma / d W = u v^T written by
+amm_rank1_update, with no dense SVD anywhere.
+
+Layout: a row-major torch tensor W of shape (rows, cols) is, byte for byte,
+the column-major matrix A = W^T of size cols x rows with leading dimension
+cols.  sigma_max(A) = sigma_max(W); the power method on A A^T = W^T W yields
+the right singular vector of W (length cols) as u_A, and v_A = A^T u_A /
+sigma (length rows) is W's left singular vector.  dW = g * v_A u_A^T as a
+row-major (rows, cols) array is exactly dA = g * u_A v_A^T column-major.
+"""
+from __future__ import annotations
+
+from collections import OrderedDict
+
+import torch
+
+from .api import AmmContext, RunParams, rank1_update
+
+_CACHE_MAX = 4
+_contexts: "OrderedDict[tuple, AmmContext]" = OrderedDict()
+
+
+def _context(Wc: torch.Tensor, cols: int) -> AmmContext:
+    """Contexts are cached per (pointer, shape, dtype, stream): a context
+    holds only the matrix's address, and every run re-reads the matrix."""
+    stream = torch.cuda.current_stream(Wc.device).cuda_stream
+    key = (Wc.data_ptr(), tuple(Wc.shape), Wc.stride(0), cols, Wc.dtype, Wc.device.index, stream)
+    ctx = _contexts.get(key)
+    if ctx is None:
+        ctx = AmmContext(Wc, d=cols)
+        _contexts[key] = ctx
+        while len(_contexts) > _CACHE_MAX:
+            _, old = _contexts.popitem(last=False)
+            old.close()
+    else:
+        _contexts.move_to_end(key)
+        ctx._keep = Wc
+    return ctx
+
+
+def _as_columns(W: torch.Tensor) -> tuple[torch.Tensor, int]:
+    """W (rows, cols) as the (rows, ld) array of a column-major cols x rows
+    matrix: rows contiguous, ld a multiple of 16 bytes (a padded device copy
+    when W's own layout does not qualify)."""
+    rows, cols = W.shape
+    width = 4 if W.dtype == torch.float32 else 8
+    ok = (W.stride(1) == 1 and W.stride(0) >= cols and W.stride(0) % width == 0
+          and W.data_ptr() % 16 == 0)
+    if ok:
+        return W, cols
+    ld = (cols + width - 1) // width * width
+    Wp = torch.zeros(rows, ld, dtype=W.dtype, device=W.device)
+    Wp[:, :cols] = W
+    return Wp, cols
+
+
+class SigmaMax(torch.autograd.Function):
+    """sigma = sigma_max(W) with d sigma / d W = v u^T (row-major view)."""
+
+    @staticmethod
+    def forward(ctx, W: torch.Tensor, params: RunParams):
+        if not W.is_cuda or W.dim() != 2:
+            raise ValueError("W must be a 2-D device tensor")
+        if W.dtype not in (torch.float32, torch.bfloat16):
+            raise ValueError("W must be float32 or bfloat16")
+        Wc, cols = _as_columns(W.detach())
+        amm = _context(Wc, cols)
+        sigma, u, v = amm.sigma_max(params, device=W.device)
+        ctx.save_for_backward(u, v)
+        ctx.shape = tuple(W.shape)
+        ctx.dtype = W.dtype
+        return torch.tensor(sigma, dtype=torch.float32, device=W.device)
+
+    @staticmethod
+    def backward(ctx, g: torch.Tensor):
+        u, v = ctx.saved_tensors
+        rows, cols = ctx.shape
+        dW = torch.empty(rows, cols, dtype=ctx.dtype, device=u.device)
+        rank1_update(dW, float(g.item()), u, v, d=cols, n=rows, ld=cols)
+        return dW, None
+
+
+def sigma_max(W: torch.Tensor, params: RunParams | None = None) -> torch.Tensor:
+    """Largest singular value of W (rows, cols) by the power method on
+    W^T W; differentiable in W.  Default: 200 exact iterations."""
+    if params is None:
+        params = RunParams(kernel="exact", step=200)
+    return SigmaMax.apply(W, params)

@@ -120,6 +120,10 @@ struct amm_ctx {
   size_t P2_cap = 0;
   double* P2s = nullptr;
   size_t P2s_cap = 0;
+  float* tvec = nullptr;  // A^T u of sigma_max
+  size_t tvec_cap = 0;
+  double* tbp = nullptr;  // block partials of ||A^T u||^2
+  size_t tbp_cap = 0;
 
   bool stats_valid = false;
 
@@ -683,7 +687,7 @@ int amm_ctx_destroy(amm_ctx* c) {
                   c->x,     c->y,     c->xm,       c->ym,      c->eigvec, c->st,  c->BP,
                   c->mode,  c->eig,   c->err,      c->ts,      c->idx,    c->wgt, c->segc,
                   c->P1,    c->P2,    c->P2s,      c->tile_sum, c->tile_lp, c->tile_a2,
-                  c->tile_max};
+                  c->tile_max, c->tvec, c->tbp};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -951,6 +955,49 @@ int amm_generate_symmetric(void* A_dev, uint64_t m, uint64_t ld, int32_t dist, f
   float delta;
   std::memcpy(&delta, &h, sizeof(float));
   if (delta_out) *delta_out = delta;
+  return AMM_OK;
+}
+
+int amm_sigma_max(amm_ctx* c, const amm_params* p, double* sigma, float* u_dev, float* v_dev) {
+  if (!c) return fail(AMM_E_INVALID, "ctx is NULL");
+  int rc;
+  if ((rc = amm_power_method(c, p, nullptr, nullptr, nullptr))) return rc;
+  if ((rc = grow(c->tvec, c->tvec_cap, c->n)) ||
+      (rc = grow(c->tbp, c->tbp_cap, amm::reduce_p1_blocks(c->n))))
+    return rc;
+  // t = A^T u for the final iterate u = x (one exact pass)
+  amm::GatherDotArgs g{};
+  g.A = c->A;
+  g.ld = c->ld;
+  g.d = (uint32_t)c->d;
+  g.cap = (uint32_t)c->n;
+  g.x = c->x;
+  g.R = c->dot_dense.R;
+  g.rows_per_split = c->dot_dense.rows_per_split;
+  g.P1 = c->P1;
+  HIPCHK(amm::launch_gather_dot(c->dtype, g, c->stream));
+  HIPCHK(amm::launch_reduce_p1(c->P1, g.R, c->n, c->tvec, c->tbp, c->stream));
+  amm::TailArgs t =
+      tail_args(c, amm::kTailNorm2, (uint32_t)amm::reduce_p1_blocks(c->n), 0, false, nullptr, false);
+  t.BP = c->tbp;
+  HIPCHK(amm::launch_tail(t, c->stream));
+  if (v_dev) HIPCHK(amm::launch_unit(c->tvec, c->n, c->st, v_dev, c->stream));
+  if (u_dev)
+    HIPCHK(hipMemcpyAsync(u_dev, c->x, sizeof(float) * c->d, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (sigma)
+    HIPCHK(hipMemcpy(sigma, (char*)c->st + offsetof(DevState, sigma), sizeof(double),
+                     hipMemcpyDeviceToHost));
+  return AMM_OK;
+}
+
+int amm_rank1_update(void* out, int32_t dtype, uint64_t d, uint64_t n, uint64_t ld, float g,
+                     const float* u_dev, const float* v_dev, void* stream) {
+  if (!out || !u_dev || !v_dev) return fail(AMM_E_INVALID, "NULL argument");
+  if (dtype != AMM_DTYPE_F32 && dtype != AMM_DTYPE_BF16) return fail(AMM_E_INVALID, "unknown dtype");
+  if (d < 1 || n < 1 || ld < d || d >= (1ull << 31) || n >= (1ull << 31))
+    return fail(AMM_E_INVALID, "need 1 <= d, n < 2^31 and ld >= d");
+  HIPCHK(amm::launch_rank1(dtype, out, (uint32_t)d, n, ld, g, u_dev, v_dev, (hipStream_t)stream));
   return AMM_OK;
 }
 

@@ -38,6 +38,7 @@ struct alignas(16) DevState {
   double scale;         // 1 / ||y|| of the latest product
   double s, q;          // proposed: 1^T dx and z^T dx for the next product
   double cos_prev;      // y^T x / ||y|| of the latest iteration
+  double sigma;         // ||A^T u|| of the last sigma_max evaluation
   uint64_t last_exact;  // last exact iteration (proposed gate)
   uint64_t seed;        // Philox key of the run
   uint32_t trial;       // Philox counter word 2 of the run

@@ -773,6 +773,11 @@ __global__ __launch_bounds__(256) void k_tail(
       st->s = a;
       st->q = b;
     }
+  } else if (op == kTailNorm2) {
+    double s = 0.0;
+    for (uint32_t i = t; i < nb; i += 256) s += BP[i];
+    s = block256_sum(s, red);
+    if (t == 0) st->sigma = sqrt(s);
   } else if (t == 0) {
     scale_l = st->scale;
   }
@@ -885,6 +890,75 @@ __global__ __launch_bounds__(256) void k_sum_rows(const float* __restrict__ P2, 
   const float zf = r < d ? (float)acc : 0.0f;
   z[r] = zf;
   if (v) v[r] = r < d ? (float)((double)zf - st->alpha2) : 0.0f;
+}
+
+// t_j = sum_r P1[r][j] (A^T x from a split exact pass) and per-block FP64
+// partials of ||t||^2 (finished by k_tail, op kTailNorm2).
+__global__ __launch_bounds__(256) void k_reduce_p1(const float* __restrict__ P1, uint32_t R,
+                                                   uint64_t n, float* __restrict__ t,
+                                                   double* __restrict__ BP) {
+  __shared__ double red[4];
+  const uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  double acc2 = 0.0;
+  if (j < n) {
+    double a = 0.0;
+    for (uint32_t r = 0; r < R; ++r) a += (double)P1[(uint64_t)r * n + j];
+    const float tf = (float)a;
+    t[j] = tf;
+    acc2 = (double)tf * (double)tf;
+  }
+  acc2 = block256_sum(acc2, red);
+  if (threadIdx.x == 0) BP[blockIdx.x] = acc2;
+}
+
+// v = t / ||t|| with ||t|| from the state (sigma)
+__global__ __launch_bounds__(256) void k_unit(const float* __restrict__ t, uint64_t n,
+                                              const DevState* st, float* __restrict__ v) {
+  const uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const double s = st->sigma;
+  if (j < n) v[j] = s > 0.0 ? (float)((double)t[j] / s) : 0.0f;
+}
+
+// Backward of sigma_max: out[:, j] = g * a * b_j for a column-major d x n
+// matrix (the rank-1 adjoint u v^T of d sigma / d A); 16-byte stores when
+// vec (out 16-byte aligned, ld a multiple of the vector width).
+template <typename T>
+__global__ __launch_bounds__(256) void k_rank1(T* __restrict__ out, uint32_t d, uint64_t ld,
+                                               float g, const float* __restrict__ a,
+                                               const float* __restrict__ b, int vec) {
+  constexpr int V = Elem<T>::VEC;
+  const uint64_t j = blockIdx.x;
+  const uint32_t row = (blockIdx.y * 256 + threadIdx.x) * V;
+  if (row >= d) return;
+  const float s = g * b[j];
+  T* col = out + j * ld + row;
+  float v[V];
+#pragma unroll
+  for (int k = 0; k < V; ++k) v[k] = (row + k < d) ? s * a[row + k] : 0.0f;
+  if (vec && row + V <= d) {
+    if constexpr (V == 4) {
+      *reinterpret_cast<float4*>(col) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+      uint32_t w[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t lo = __float_as_uint(v[2 * k]), hi = __float_as_uint(v[2 * k + 1]);
+        const uint32_t rl = (lo + 0x7FFFu + ((lo >> 16) & 1u)) >> 16;
+        const uint32_t rh = (hi + 0x7FFFu + ((hi >> 16) & 1u)) >> 16;
+        w[k] = rl | (rh << 16);
+      }
+      *reinterpret_cast<uint4*>(col) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+  } else {
+    for (int k = 0; k < V && row + k < d; ++k) {
+      if constexpr (V == 4) {
+        reinterpret_cast<float*>(col)[k] = v[k];
+      } else {
+        const uint32_t u = __float_as_uint(v[k]);
+        reinterpret_cast<uint16_t*>(col)[k] = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+      }
+    }
+  }
 }
 
 // Plain fill kernel (graph-safe replacement for memset nodes).
@@ -1024,6 +1098,34 @@ hipError_t launch_sum_rows(const float* P2, uint32_t S, uint64_t dpad, uint32_t 
 hipError_t launch_dots(const float* x, const float* xm, const float* z, uint64_t dpad, double* BP,
                        hipStream_t s) {
   k_dots<<<(unsigned)dots_blocks(dpad), 256, 0, s>>>(x, xm, z, dpad, BP);
+  return hipGetLastError();
+}
+
+uint64_t reduce_p1_blocks(uint64_t n) { return grid1(n, 256); }
+
+hipError_t launch_reduce_p1(const float* P1, uint32_t R, uint64_t n, float* t, double* BP,
+                            hipStream_t s) {
+  k_reduce_p1<<<(unsigned)reduce_p1_blocks(n), 256, 0, s>>>(P1, R, n, t, BP);
+  return hipGetLastError();
+}
+
+hipError_t launch_unit(const float* t, uint64_t n, const DevState* st, float* v, hipStream_t s) {
+  k_unit<<<(unsigned)grid1(n, 256), 256, 0, s>>>(t, n, st, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_rank1(int dtype, void* out, uint32_t d, uint64_t n, uint64_t ld, float g,
+                        const float* a, const float* b, hipStream_t s) {
+  if (n == 0 || d == 0) return hipSuccess;
+  // one workgroup of 256 work-items per column along x: n * 256 < 2^32
+  if (n >= (1ull << 24)) return hipErrorInvalidValue;
+  const uint64_t V = dtype == 1 ? 8 : 4;
+  const int vec = (ld % V == 0) && (((uintptr_t)out & 15u) == 0);
+  const dim3 grid((unsigned)n, grid1(d, 256 * V));
+  if (dtype == 1)
+    k_rank1<__hip_bfloat16><<<grid, 256, 0, s>>>((__hip_bfloat16*)out, d, ld, g, a, b, vec);
+  else
+    k_rank1<float><<<grid, 256, 0, s>>>((float*)out, d, ld, g, a, b, vec);
   return hipGetLastError();
 }
 

@@ -70,7 +70,13 @@ struct FinishArgs {
 };
 
 // single-block tail of a grid reduction (k_tail)
-enum TailOp : uint32_t { kTailFinish = 0, kTailScale = 1, kTailSQ = 2, kTailNorm = 3 };
+enum TailOp : uint32_t {
+  kTailFinish = 0,
+  kTailScale = 1,
+  kTailSQ = 2,
+  kTailNorm = 3,
+  kTailNorm2 = 4,  // sigma = sqrt(sum of one partial per block)
+};
 
 struct TailArgs {
   uint32_t op;            // 0 finish, 1 scale of x0, 2 s and q
@@ -128,6 +134,12 @@ hipError_t launch_normalize(const float* y, float* x, uint32_t d, uint64_t dpad,
 hipError_t launch_sum_rows(const float* P2, uint32_t S, uint64_t dpad, uint32_t d, float* z,
                            float* v, const DevState* st, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t val, hipStream_t s);
+uint64_t reduce_p1_blocks(uint64_t n);  // 1 double per block
+hipError_t launch_reduce_p1(const float* P1, uint32_t R, uint64_t n, float* t, double* BP,
+                            hipStream_t s);
+hipError_t launch_unit(const float* t, uint64_t n, const DevState* st, float* v, hipStream_t s);
+hipError_t launch_rank1(int dtype, void* out, uint32_t d, uint64_t n, uint64_t ld, float g,
+                        const float* a, const float* b, hipStream_t s);
 hipError_t launch_generate_symmetric(float* A, uint64_t m, uint64_t ld, int dist, float p1, float p2,
                                     uint64_t seed, uint32_t* delta_bits, hipStream_t s);
 hipError_t launch_f32_to_bf16(const float* src, uint16_t* dst, uint64_t count, hipStream_t s);

@@ -164,6 +164,18 @@ int amm_time_column_stats(amm_ctx *ctx, uint32_t iters, double *ms);
 int amm_generate_symmetric(void *A_dev, uint64_t m, uint64_t ld, int32_t dist, float p1, float p2,
                            uint64_t seed, void *stream, float *diagonal_offset);
 
+/* sigma_max(A) with its singular vectors (no reference counterpart: the
+ * reference reports sqrt(lambda) only, powerMethod_by_exact.cuh:83): runs
+ * the power method of p, takes its final iterate u (device, length d), and
+ * one exact pass t = A^T u gives sigma = ||t|| (FP64 norm, host) and
+ * v = t / sigma (device, length n).  At convergence d sigma / d A = u v^T. */
+int amm_sigma_max(amm_ctx *ctx, const amm_params *p, double *sigma, float *u_dev, float *v_dev);
+
+/* out = g * u v^T for a column-major d x n device matrix (leading dimension
+ * ld, FP32 or BF16): the backward of sigma_max.  Asynchronous on stream. */
+int amm_rank1_update(void *out_dev, int32_t dtype, uint64_t d, uint64_t n, uint64_t ld, float g,
+                     const float *u_dev, const float *v_dev, void *stream);
+
 /* Round count FP32 device values to BF16 (nearest even) into dst_dev; the
  * storage form of the BF16 path (no reference counterpart: the reference is
  * FP32 only).  Asynchronous on `stream`. */
