@@ -5,7 +5,9 @@ Takuya-Okuyama/eigenvalueComputation_by_AMM, rebuilt as hand-written HIP
 kernels for gfx950 behind the C ABI in include/amm.h (libamm.so).
 """
 from .api import (  # noqa: F401
+    AmmBatch,
     AmmContext,
+    BatchResult,
     AmmError,
     RunParams,
     RunResult,

@@ -52,8 +52,33 @@ _SIGS = {
     "amm_sigma_max": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "amm_rank1_update": (C.c_int, [_vp, C.c_int32, C.c_uint64, C.c_uint64, C.c_uint64, C.c_float,
                                    _vp, _vp, _vp]),
+    "amm_batch_create": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _vp, C.POINTER(_vp)]),
+    "amm_batch_destroy": (C.c_int, [_vp]),
+    "amm_batch_set_exact_eigenvector": (C.c_int, [_vp, C.c_uint32, _vp, C.c_uint64]),
+    "amm_batch_power_method": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "amm_batch_last_modes": (C.c_int, [_vp, _vp, C.c_uint64]),
+    "amm_batch_get_vector": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int32, _vp, C.c_uint64]),
+    "amm_batch_last_run_ms": (C.c_int, [_vp, _vp]),
+    "amm_batch_set_tuning": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "amm_batch_get_tuning": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp]),
+    "amm_batch_autotune": (C.c_int, [_vp, C.c_uint32, _vp]),
+    "amm_batch_sample": (C.c_int, [_vp, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, _vp, _vp, _vp]),
     "amm_f32_to_bf16": (C.c_int, [_vp, _vp, C.c_uint64, _vp]),
 }
+
+
+class amm_problem(C.Structure):
+    """struct amm_problem of include/amm.h."""
+
+    _fields_ = [
+        ("A", C.c_void_p),
+        ("dtype", C.c_int32),
+        ("c", C.c_uint32),
+        ("d", C.c_uint64),
+        ("n", C.c_uint64),
+        ("ld", C.c_uint64),
+        ("seed", C.c_uint64),
+    ]
 
 
 class amm_params(C.Structure):

@@ -299,3 +299,131 @@ def powerMethod_by_proposed(ctx: AmmContext, **kw) -> RunResult:
 def powerMethod_by_proposed_v2(ctx: AmmContext, **kw) -> RunResult:
     """powerMethod_by_proposed_v2.cuh:199-377"""
     return _run("proposedAMM_v2", ctx, **kw)
+
+
+@dataclass
+class BatchResult:
+    time_ms: np.ndarray  # [step] device time from the run's first kernel (all problems together)
+    eig: np.ndarray      # [problems][trials][step] raw Rayleigh quotients
+    err: np.ndarray      # [problems][trials][step] y^T v* / ||y||
+    wall_ms: float
+
+    def eigenvalue(self) -> np.ndarray:
+        return np.sqrt(np.maximum(0.0, self.eig))
+
+
+class AmmBatch:
+    """Batched power method over many device matrices, `trials` (<= 16)
+    sampling seeds each, stacked as one block of right-hand sides.
+
+    `problems` is a list of dicts with keys A (a device tensor (n, ld) whose
+    row j is column j of A, float32 or bfloat16), c (sampled columns per
+    product), seed, and optionally d (rows, default ld).  Trial k of problem
+    i reproduces the single-matrix run with seed = problem seed and trial =
+    params.trial + k.
+    """
+
+    def __init__(self, problems, trials: int = 16, stream: int = 0):
+        import torch
+        self._lib = _lib.load()
+        self._keep = [p["A"] for p in problems]
+        arr = (_lib.amm_problem * len(problems))()
+        dev = None
+        for i, p in enumerate(problems):
+            A = p["A"]
+            if A.dim() != 2 or not A.is_cuda or A.stride(1) != 1:
+                raise ValueError("A must be a 2-D device tensor with contiguous rows")
+            if A.dtype == torch.float32:
+                dt = _lib.AMM_DTYPE_F32
+            elif A.dtype == torch.bfloat16:
+                dt = _lib.AMM_DTYPE_BF16
+            else:
+                raise ValueError("A must be float32 or bfloat16")
+            d = int(p.get("d", A.shape[1]))
+            arr[i] = _lib.amm_problem(A=A.data_ptr(), dtype=dt, c=int(p.get("c", 0)), d=d,
+                                      n=int(A.shape[0]), ld=int(A.stride(0)),
+                                      seed=int(p.get("seed", 0)) & 0xFFFFFFFFFFFFFFFF)
+            dev = A.device
+        if stream == 0:
+            stream = torch.cuda.current_stream(dev).cuda_stream
+        self.count = len(problems)
+        self.trials = int(trials)
+        self.dims = [(int(arr[i].d), int(arr[i].n)) for i in range(self.count)]
+        h = C.c_void_p()
+        check(self._lib.amm_batch_create(arr, self.count, self.trials,
+                                         C.c_void_p(stream) if stream else None, C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.amm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_exact_eigenvector(self, problem: int, v) -> None:
+        if v is None:
+            check(self._lib.amm_batch_set_exact_eigenvector(self._h, problem, None, 0))
+            return
+        a = np.ascontiguousarray(v, np.float64)
+        check(self._lib.amm_batch_set_exact_eigenvector(self._h, problem, C.c_void_p(_ptr(a)), a.size))
+
+    def power_method(self, params: RunParams) -> BatchResult:
+        p = params.to_c()
+        S = int(params.step)
+        t = np.zeros(S, np.float64)
+        e = np.zeros((self.count, self.trials, S), np.float64)
+        r = np.zeros((self.count, self.trials, S), np.float64)
+        check(self._lib.amm_batch_power_method(self._h, C.byref(p), C.c_void_p(_ptr(t)),
+                                               C.c_void_p(_ptr(e)), C.c_void_p(_ptr(r))))
+        ms = np.zeros(1, np.float64)
+        check(self._lib.amm_batch_last_run_ms(self._h, C.c_void_p(_ptr(ms))))
+        return BatchResult(t, e, r, float(ms[0]))
+
+    def last_modes(self, step: int) -> np.ndarray:
+        m = np.zeros((self.count, self.trials, step), np.uint32)
+        check(self._lib.amm_batch_last_modes(self._h, C.c_void_p(_ptr(m)), m.size))
+        return m
+
+    def get_vector(self, problem: int, trial: int, which: str = "x") -> np.ndarray:
+        v = np.empty(self.dims[problem][0], np.float32)
+        check(self._lib.amm_batch_get_vector(self._h, problem, trial, 0 if which == "x" else 1,
+                                             C.c_void_p(_ptr(v)), v.size))
+        return v
+
+    def set_tuning(self, problem: int, kslice_rows: int, passes: int, chunks: int) -> None:
+        check(self._lib.amm_batch_set_tuning(self._h, problem, kslice_rows, passes, chunks))
+
+    def get_tuning(self, problem: int):
+        a = np.zeros(3, np.uint32)
+        check(self._lib.amm_batch_get_tuning(self._h, problem, C.c_void_p(_ptr(a)),
+                                             C.c_void_p(_ptr(a) + 4), C.c_void_p(_ptr(a) + 8)))
+        return int(a[0]), int(a[1]), int(a[2])
+
+    def autotune(self, iters: int = 5) -> float:
+        ms = np.zeros(1, np.float64)
+        check(self._lib.amm_batch_autotune(self._h, iters, C.c_void_p(_ptr(ms))))
+        return float(ms[0])
+
+    def sample(self, kernel: str, iteration: int, trial0: int, problem: int):
+        """Union of the trials' samples of one iteration: (columns ascending,
+        weights [columns][trials])."""
+        n = self.dims[problem][1]
+        cols = np.zeros(n, np.uint32)
+        w = np.zeros((n, self.trials), np.float32)
+        cnt = np.zeros(1, np.uint64)
+        check(self._lib.amm_batch_sample(self._h, _lib.KERNEL_NAMES[kernel], iteration, trial0, problem,
+                                         C.c_void_p(_ptr(cols)), C.c_void_p(_ptr(w)),
+                                         C.c_void_p(_ptr(cnt))))
+        m = int(cnt[0])
+        return cols[:m].copy(), w[:m].copy()

@@ -1,0 +1,120 @@
+// amm_batch.hpp -- batched AMM power method: many matrices, and up to 16
+// trials (sampling seeds) per matrix stacked as one right-hand-side block, so
+// that both contractions of a product are skinny GEMMs on the matrix cores.
+//
+// Per iteration and problem, the 16 trials' samples are merged into one
+// union of columns U (tiles of 64 columns compacted in place); then
+//   P = A_U^T DX            (contraction 1, MFMA, DX = [x_k - x_memo_k])
+//   T = W o (P - alpha s^T) (per-trial weights; proposed mean shift)
+//   Y = A_U T               (contraction 2, MFMA, split over column chunks)
+// Every launch covers all problems ("grouped" launches: a workgroup finds
+// its problem by binary search over per-problem item offsets).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "amm_device.hpp"
+
+namespace amm {
+
+constexpr uint32_t kBT = 16;      // trials per right-hand-side block
+constexpr uint32_t kBTile = 64;   // columns per compaction tile
+constexpr uint32_t kBVecRows = 64;  // rows per block of the vector kernels
+constexpr uint32_t kBStageAlign = 128;  // staged vectors are padded to this many rows
+
+// Per (problem, trial) scalar state.
+struct alignas(16) BTrial {
+  double scale;         // 1 / ||y|| of the latest product
+  double cos_prev;      // y^T x / ||y|| of the latest iteration
+  double s, q;          // proposed: 1^T dx, z^T dx for the current product
+  uint64_t last_exact;  // last exact iteration (proposed gate)
+  uint32_t mode;        // StepMode of the current iteration
+  uint32_t pad;
+};
+
+// Device-resident problem descriptor (all pointers are device addresses).
+struct alignas(16) BProb {
+  const void* A;        // d x n column-major, leading dimension ld
+  uint64_t ld;
+  uint64_t seed;        // Philox key
+  uint32_t d, n;
+  uint32_t dstage;      // rows of each staged vector (multiple of 128, >= d)
+  uint32_t c;           // samples per trial and iteration
+  int32_t dtype;        // 0 FP32, 1 BF16
+  uint32_t ntiles;      // ceil(n / 64)
+  uint32_t kslices;     // contraction-1 row slices
+  uint32_t kslice_rows; // rows per slice (multiple of 256)
+  uint32_t passes;      // contraction-2 passes per wave (64 FP32 / 128 BF16 rows each)
+  uint32_t rblocks;     // contraction-2 row blocks (4 waves x passes each)
+  uint32_t chunks;      // contraction-2 column chunks
+  uint32_t tiles_per_chunk;
+  uint32_t p2pitch;     // rows of one P2 partial (rblocks * rows per block)
+  uint32_t nvblk;       // vector-kernel blocks = dstage / 64
+  float* X;             // [16][dstage] iterate
+  float* Y;             // [16][dstage] product
+  float* XM;            // [16][dstage] x_memo
+  float* YM;            // [16][dstage] y_memo
+  float* DX;            // [16][dstage] FP32 staging of x - x_memo (FP32 A)
+  uint16_t* DXh;        // [16][dstage] BF16 high part (BF16 A)
+  uint16_t* DXl;        // [16][dstage] BF16 low part
+  uint32_t* cnt;        // [ntiles*64][16] draw counts (zero between iterations)
+  float* W;             // [ntiles*64][16] weights of the compacted columns
+  uint32_t* list;       // [ntiles*64] compacted column indices, per tile
+  uint32_t* tcount;     // [ntiles] columns present per tile
+  float* P1;            // [kslices][ntiles*64][16]
+  float* P2;            // [chunks][16][p2pitch]
+  double* P2s;          // [chunks][16] proposed: sum alpha_j t_j per chunk
+  double* BP;           // [nvblk][16][3] block partials of y^T x, y^T y, y^T v*
+  double* BQ;           // [nvblk][16][2] block partials of 1^T dx, z^T dx
+  DevState* st;         // column statistics: totals, last positive column, ||alpha||^2
+  const uint64_t* cdf0; // plain CDF
+  const uint64_t* cdf1; // centred CDF
+  const float* alpha;   // column means
+  const float* z;       // A alpha
+  const float* v;       // z - ||alpha||^2
+  const double* eigvec; // reference eigenvector (may be null)
+  BTrial* bt;           // [16]
+  double* eig;          // [16][step]
+  double* err;          // [16][step]
+  uint32_t* modes;      // [16][step] step kind used at each iteration
+  uint32_t* pad_ptr;
+};
+
+// Scalars shared by every launch of a run.
+struct BRun {
+  uint32_t np;          // problems
+  uint32_t ntr;         // trials per problem (<= 16)
+  uint32_t trial0;      // Philox counter word 2 of trial 0
+  uint32_t kernel;      // AMM_KERNEL_* (1 previous / 2 proposed drive adaptive steps)
+  uint32_t step;
+  uint32_t adaptive;
+  uint64_t period;
+  double thr;
+  const uint32_t* modes_static;  // [step] static schedule
+  int64_t* ts;          // [step] device time stamps
+  int64_t* t_start;     // wall clock at the run's first kernel
+};
+
+enum BTailOp : uint32_t { kBTailFinish = 0, kBTailScale = 1 };
+
+// grouped launch of one kernel: per-problem item offsets [np + 1]
+struct BGroup {
+  const uint32_t* off;
+  uint32_t total;
+};
+
+hipError_t launch_bbegin(const BProb* P, BGroup g, BRun r, hipStream_t s);
+hipError_t launch_bx0(const BProb* P, BGroup g, BRun r, hipStream_t s);
+hipError_t launch_btail(const BProb* P, BRun r, uint32_t op, uint32_t it, hipStream_t s);
+hipError_t launch_bnorm(const BProb* P, BGroup g, BRun r, hipStream_t s);
+hipError_t launch_bsample(const BProb* P, BGroup g, BRun r, uint32_t it, hipStream_t s);
+hipError_t launch_bprep(const BProb* P, BGroup g, BRun r, hipStream_t s);
+// dtype-uniform groups: all problems of a group share the element type
+hipError_t launch_bdot(int dtype, const BProb* P, BGroup g, uint32_t np, hipStream_t s);
+// passes: 1 or 2 (a group shares it)
+hipError_t launch_baxpy(int dtype, uint32_t passes, const BProb* P, BGroup g, uint32_t np,
+                        uint32_t proposed_any, hipStream_t s);
+hipError_t launch_bfinish(const BProb* P, BGroup g, BRun r, hipStream_t s);
+
+}  // namespace amm

@@ -1,0 +1,880 @@
+// amm_batch_capi.hip -- C ABI of the batched AMM power method
+// (include/amm.h, amm_batch_*): many matrices of uneven sizes, up to 16
+// trials each, every launch grouped over all problems and the whole run --
+// column statistics, CDFs, z = A alpha, x0 and all iterations -- captured
+// in one hipGraph.  Tile shapes come from a launch-planning heuristic or
+// from amm_batch_autotune, which times candidate shapes on the live
+// problems.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "amm.h"
+#include "amm_batch.hpp"
+#include "amm_device.hpp"
+#include "amm_launch.hpp"
+
+namespace amm_host {
+int fail(int code, const std::string& msg);
+}
+
+using amm::BProb;
+using amm::BTrial;
+using amm::DevState;
+using amm_host::fail;
+
+#define BHIPCHK(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(AMM_E_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+
+constexpr uint32_t kTargetItems = 2048;  // workgroups a contraction launch aims for
+constexpr uint32_t kMaxSlices = 16;      // contraction-1 row slices per problem
+constexpr uint32_t kZChunk = 2048;       // columns per chunk of z = A alpha
+
+uint64_t rup(uint64_t a, uint64_t b) { return (a + b - 1) / b * b; }
+uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// One device allocation carved into 256-byte aligned pieces.
+struct Pool {
+  size_t bytes = 0;
+  template <typename T>
+  size_t take(size_t count) {
+    const size_t o = bytes;
+    bytes = rup(bytes + std::max<size_t>(count, 1) * sizeof(T), 256);
+    return o;
+  }
+};
+
+struct Tuning {
+  uint32_t kslice_rows, passes, chunks;
+};
+
+// per-problem buffers of the single-matrix setup kernels
+struct SetupBufs {
+  float* alpha;
+  double* nrm2;
+  double* cnrm2;
+  uint64_t* cdf0;
+  uint64_t* cdf1;
+  float* z;
+  float* v;
+  DevState* st;
+  double* eigvec;
+  uint64_t dpad8;
+};
+
+uint32_t pass_rows(int dtype) { return dtype == AMM_DTYPE_BF16 ? 128u : 64u; }
+
+}  // namespace
+
+struct amm_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  uint32_t np = 0, ntr = 0;
+  std::vector<amm_problem> probs;
+  std::vector<BProb> hp;
+  std::vector<SetupBufs> sb;
+  std::vector<Tuning> tune;
+  BProb* dP = nullptr;
+  char* spool = nullptr;  // static per-problem buffers
+  char* tpool = nullptr;  // tuning-dependent partials (P1, P2, P2s)
+  size_t tpool_bytes = 0;
+  char* rpool = nullptr;  // per-run outputs (eig, err, modes, ts, schedule)
+  uint64_t rstep = 0;
+  // shared setup scratch
+  uint64_t* tile_max = nullptr;
+  uint64_t* tile_sum = nullptr;
+  uint64_t* tile_lp = nullptr;
+  double* tile_a2 = nullptr;
+  float* zP2 = nullptr;
+  double* zP2s = nullptr;
+  // grouped-launch item offsets
+  uint32_t* offs = nullptr;
+  size_t offs_cap = 0;
+  amm::BGroup g_vec{}, g_sample{}, g_prep{}, g_dot[2]{}, g_axpy[2][2]{};
+  bool plan_valid = false;
+  int plan_kernel = -1;
+  // run state
+  uint32_t* modes_static = nullptr;
+  int64_t* ts = nullptr;
+  int64_t* t_start = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool graph_valid = false;
+  bool graph_setup = false;
+  amm_params key{};
+  bool stats_valid = false;
+  int stats_kernel = -1;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  int wall_khz = 100000;
+  uint64_t last_step = 0;
+  bool ran = false;
+};
+
+namespace {
+
+void drop_graph(amm_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_valid = false;
+}
+
+// Default shapes: each problem gets a share of ~kTargetItems workgroups in
+// both contractions proportional to its column tiles x rows.
+Tuning default_tuning_one(const BProb& p, double share) {
+  const double want = std::max(1.0, kTargetItems * share);
+  const uint32_t max_slices = (uint32_t)std::min<uint64_t>(kMaxSlices, cdiv(p.dstage, 256));
+  const uint32_t ks = (uint32_t)std::min<double>(max_slices, std::max(1.0, std::ceil(want / p.ntiles)));
+  const uint32_t ksr = (uint32_t)rup(cdiv(p.dstage, ks), 256);
+  const uint32_t passes = 2;
+  const uint32_t rbrows = 4 * passes * pass_rows(p.dtype);
+  const uint32_t rblocks = (uint32_t)cdiv(p.d, rbrows);
+  const uint32_t chunks = (uint32_t)std::min<double>(p.ntiles, std::max(1.0, std::round(want / rblocks)));
+  return Tuning{ksr, passes, chunks};
+}
+
+void default_tuning(amm_batch* b) {
+  double tot = 0.0;
+  for (const BProb& p : b->hp) tot += (double)p.ntiles * (double)p.d;
+  for (uint32_t i = 0; i < b->np; ++i) {
+    const BProb& p = b->hp[i];
+    b->tune[i] = default_tuning_one(p, tot > 0 ? (double)p.ntiles * (double)p.d / tot : 1.0);
+  }
+}
+
+int check_tuning(const BProb& p, const Tuning& t) {
+  if (t.passes < 1 || t.passes > 2) return fail(AMM_E_INVALID, "passes must be 1 or 2");
+  if (t.kslice_rows < 256 || t.kslice_rows % 256)
+    return fail(AMM_E_INVALID, "kslice_rows must be a positive multiple of 256");
+  if (cdiv(p.dstage, t.kslice_rows) > kMaxSlices)
+    return fail(AMM_E_INVALID, "kslice_rows gives more than 16 row slices");
+  if (t.chunks < 1) return fail(AMM_E_INVALID, "chunks must be >= 1");
+  return AMM_OK;
+}
+
+// Descriptors from the tuning, (re)allocation of the partial buffers and
+// the grouped-launch offsets.
+int upload_plan(amm_batch* b, int kernel) {
+  const uint32_t np = b->np;
+  BHIPCHK(hipStreamSynchronize(b->stream));
+  Pool pl;
+  std::vector<size_t> o1(np), o2(np), o2s(np);
+  for (uint32_t i = 0; i < np; ++i) {
+    BProb& p = b->hp[i];
+    const Tuning& t = b->tune[i];
+    p.kslice_rows = t.kslice_rows;
+    p.kslices = (uint32_t)cdiv(p.dstage, t.kslice_rows);
+    p.passes = t.passes;
+    const uint32_t rbrows = 4 * t.passes * pass_rows(p.dtype);
+    p.rblocks = (uint32_t)cdiv(p.d, rbrows);
+    p.p2pitch = p.rblocks * rbrows;
+    p.tiles_per_chunk = (uint32_t)cdiv(p.ntiles, std::max<uint32_t>(1, std::min(t.chunks, p.ntiles)));
+    p.chunks = (uint32_t)cdiv(p.ntiles, p.tiles_per_chunk);
+    o1[i] = pl.take<float>((size_t)p.kslices * p.ntiles * amm::kBTile * amm::kBT);
+    o2[i] = pl.take<float>((size_t)p.chunks * amm::kBT * p.p2pitch);
+    o2s[i] = pl.take<double>((size_t)p.chunks * amm::kBT);
+  }
+  if (pl.bytes > b->tpool_bytes) {
+    if (b->tpool) (void)hipFree(b->tpool);
+    b->tpool = nullptr;
+    b->tpool_bytes = 0;
+    if (hipMalloc((void**)&b->tpool, pl.bytes) != hipSuccess)
+      return fail(AMM_E_NOMEM, "device allocation of " + std::to_string(pl.bytes) + " bytes failed");
+    BHIPCHK(hipMemset(b->tpool, 0, pl.bytes));
+    b->tpool_bytes = pl.bytes;
+  }
+  for (uint32_t i = 0; i < np; ++i) {
+    b->hp[i].P1 = reinterpret_cast<float*>(b->tpool + o1[i]);
+    b->hp[i].P2 = reinterpret_cast<float*>(b->tpool + o2[i]);
+    b->hp[i].P2s = reinterpret_cast<double*>(b->tpool + o2s[i]);
+  }
+  std::vector<uint32_t> all;
+  std::vector<std::pair<amm::BGroup*, size_t>> fix;
+  auto add_group = [&](amm::BGroup& g, auto count_fn) {
+    const size_t base = all.size();
+    uint64_t acc = 0;
+    all.push_back(0);
+    for (uint32_t i = 0; i < np; ++i) {
+      acc += count_fn(b->hp[i]);
+      all.push_back((uint32_t)acc);
+    }
+    g.total = (uint32_t)acc;
+    fix.emplace_back(&g, base);
+    return acc < (1ull << 31);
+  };
+  bool ok = true;
+  ok &= add_group(b->g_vec, [](const BProb& p) { return p.nvblk; });
+  ok &= add_group(b->g_sample, [&](const BProb& p) {
+    return kernel == AMM_KERNEL_EXACT ? 0u : b->ntr * (uint32_t)cdiv(p.c, 2048);
+  });
+  ok &= add_group(b->g_prep, [](const BProb& p) { return (uint32_t)cdiv(p.ntiles, 4); });
+  for (int dt = 0; dt < 2; ++dt)
+    ok &= add_group(b->g_dot[dt], [dt](const BProb& p) { return p.dtype == dt ? p.ntiles * p.kslices : 0u; });
+  for (int dt = 0; dt < 2; ++dt)
+    for (int pa = 0; pa < 2; ++pa)
+      ok &= add_group(b->g_axpy[dt][pa], [dt, pa](const BProb& p) {
+        return (p.dtype == dt && p.passes == (uint32_t)pa + 1) ? p.rblocks * p.chunks : 0u;
+      });
+  if (!ok) return fail(AMM_E_INVALID, "a grouped launch exceeds 2^31 workgroups");
+  if (all.size() > b->offs_cap) {
+    if (b->offs) (void)hipFree(b->offs);
+    b->offs = nullptr;
+    b->offs_cap = 0;
+    BHIPCHK(hipMalloc((void**)&b->offs, all.size() * sizeof(uint32_t)));
+    b->offs_cap = all.size();
+  }
+  BHIPCHK(hipMemcpy(b->offs, all.data(), all.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  for (auto& f : fix) f.first->off = b->offs + f.second;
+  BHIPCHK(hipMemcpy(b->dP, b->hp.data(), np * sizeof(BProb), hipMemcpyHostToDevice));
+  b->plan_valid = true;
+  b->plan_kernel = kernel;
+  drop_graph(b);
+  return AMM_OK;
+}
+
+// per-run outputs: eig, err, modes [16][step] per problem; ts; schedule
+int layout_run(amm_batch* b, uint64_t step) {
+  if (b->rpool && b->rstep == step) return AMM_OK;
+  Pool pl;
+  std::vector<size_t> oe(b->np), oerr(b->np), om(b->np);
+  for (uint32_t i = 0; i < b->np; ++i) {
+    oe[i] = pl.take<double>(amm::kBT * step);
+    oerr[i] = pl.take<double>(amm::kBT * step);
+    om[i] = pl.take<uint32_t>(amm::kBT * step);
+  }
+  const size_t ots = pl.take<int64_t>(step);
+  const size_t ot0 = pl.take<int64_t>(1);
+  const size_t oms = pl.take<uint32_t>(step);
+  if (b->rpool) (void)hipFree(b->rpool);
+  b->rpool = nullptr;
+  b->rstep = 0;
+  if (hipMalloc((void**)&b->rpool, pl.bytes) != hipSuccess)
+    return fail(AMM_E_NOMEM, "device allocation of " + std::to_string(pl.bytes) + " bytes failed");
+  BHIPCHK(hipMemset(b->rpool, 0, pl.bytes));
+  for (uint32_t i = 0; i < b->np; ++i) {
+    b->hp[i].eig = reinterpret_cast<double*>(b->rpool + oe[i]);
+    b->hp[i].err = reinterpret_cast<double*>(b->rpool + oerr[i]);
+    b->hp[i].modes = reinterpret_cast<uint32_t*>(b->rpool + om[i]);
+  }
+  b->ts = reinterpret_cast<int64_t*>(b->rpool + ots);
+  b->t_start = reinterpret_cast<int64_t*>(b->rpool + ot0);
+  b->modes_static = reinterpret_cast<uint32_t*>(b->rpool + oms);
+  b->rstep = step;
+  b->plan_valid = false;  // descriptors changed
+  drop_graph(b);
+  return AMM_OK;
+}
+
+std::vector<uint32_t> static_modes(const amm_params& p) {
+  std::vector<uint32_t> m(p.step);
+  uint64_t last_exact = 0;
+  const uint64_t P = p.vr_period;
+  for (uint64_t i = 0; i < p.step; ++i) {
+    uint32_t md;
+    const bool due = P > 0 && (i + 1) % P == 0;
+    if (p.kernel == AMM_KERNEL_EXACT) {
+      md = amm::kModeExactNoMemo;
+    } else if (p.kernel == AMM_KERNEL_PREVIOUS) {
+      md = due ? amm::kModeExactMemo : amm::kModeAmmPrevious;
+    } else if (p.kernel == AMM_KERNEL_PROPOSED) {
+      md = (i > last_exact + 1 && due) ? amm::kModeExactMemo : amm::kModeAmmProposed;
+    } else {
+      if (i + 1 < P) md = amm::kModeAmmPrevious;
+      else if (i + 1 == P) md = amm::kModeExactNoMemo;
+      else md = due ? amm::kModeExactMemo : amm::kModeAmmProposed;
+    }
+    if (md == amm::kModeExactMemo || md == amm::kModeExactNoMemo) last_exact = i;
+    m[i] = md;
+  }
+  return m;
+}
+
+bool is_adaptive(const amm_params& p) {
+  return p.vr_adaptive_threshold > 0.0 &&
+         (p.kernel == AMM_KERNEL_PREVIOUS || p.kernel == AMM_KERNEL_PROPOSED);
+}
+
+amm::BRun run_args(amm_batch* b, const amm_params& p) {
+  amm::BRun r{};
+  r.np = b->np;
+  r.ntr = b->ntr;
+  r.trial0 = p.trial;
+  r.kernel = (uint32_t)p.kernel;
+  r.step = (uint32_t)p.step;
+  r.adaptive = is_adaptive(p) ? 1u : 0u;
+  r.period = p.vr_period;
+  r.thr = p.vr_adaptive_threshold;
+  r.modes_static = b->modes_static;
+  r.ts = b->ts;
+  r.t_start = b->t_start;
+  return r;
+}
+
+// Column statistics, CDFs and z = A alpha of every problem (the
+// single-matrix kernels, one problem after another on the stream).
+int enq_setup(amm_batch* b, int K) {
+  const bool need_plain = K == AMM_KERNEL_PREVIOUS || K == AMM_KERNEL_PROPOSED_V2;
+  const bool need_cent = K == AMM_KERNEL_PROPOSED || K == AMM_KERNEL_PROPOSED_V2;
+  for (uint32_t i = 0; i < b->np; ++i) {
+    const BProb& p = b->hp[i];
+    const SetupBufs& s = b->sb[i];
+    BHIPCHK(amm::launch_colstats(p.dtype, p.A, p.ld, p.d, p.n, s.alpha, s.nrm2, s.cnrm2, s.st,
+                                 b->stream));
+    if (need_plain)
+      BHIPCHK(amm::launch_cdf(s.nrm2, p.n, s.cdf0, s.st, 0, nullptr, b->tile_max, b->tile_sum,
+                              b->tile_lp, b->tile_a2, b->stream));
+    if (need_cent) {
+      BHIPCHK(amm::launch_cdf(s.cnrm2, p.n, s.cdf1, s.st, 1, s.alpha, b->tile_max, b->tile_sum,
+                              b->tile_lp, b->tile_a2, b->stream));
+      amm::GatherAxpyArgs a{};
+      a.A = p.A;
+      a.ld = p.ld;
+      a.d = p.d;
+      a.dpad = s.dpad8;
+      a.cap = p.n;
+      a.cs = kZChunk;
+      a.S = (uint32_t)cdiv(p.n, kZChunk);
+      a.tdirect = s.alpha;
+      a.st = s.st;
+      a.P2 = b->zP2;
+      a.P2s = b->zP2s;
+      BHIPCHK(amm::launch_gather_axpy(p.dtype, a, b->stream));
+      BHIPCHK(amm::launch_sum_rows(b->zP2, a.S, s.dpad8, p.d, s.z, s.v, s.st, b->stream));
+    }
+  }
+  return AMM_OK;
+}
+
+// the iterations' kernels (after x0)
+int enq_iteration(amm_batch* b, const amm::BRun& r, int K, uint32_t it) {
+  const BProb* P = b->dP;
+  hipStream_t s = b->stream;
+  if (K != AMM_KERNEL_EXACT) BHIPCHK(amm::launch_bsample(P, b->g_sample, r, it, s));
+  BHIPCHK(amm::launch_bprep(P, b->g_prep, r, s));
+  for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bdot(dt, P, b->g_dot[dt], b->np, s));
+  const uint32_t prop = (K == AMM_KERNEL_PROPOSED || K == AMM_KERNEL_PROPOSED_V2) ? 1u : 0u;
+  for (int dt = 0; dt < 2; ++dt)
+    for (int pa = 0; pa < 2; ++pa)
+      BHIPCHK(amm::launch_baxpy(dt, (uint32_t)pa + 1, P, b->g_axpy[dt][pa], b->np, prop, s));
+  BHIPCHK(amm::launch_bfinish(P, b->g_vec, r, s));
+  BHIPCHK(amm::launch_btail(P, r, amm::kBTailFinish, it, s));
+  BHIPCHK(amm::launch_bnorm(P, b->g_vec, r, s));
+  return AMM_OK;
+}
+
+int enqueue_run(amm_batch* b, const amm_params& p, bool setup) {
+  int rc;
+  const int K = p.kernel;
+  const amm::BRun r = run_args(b, p);
+  BHIPCHK(amm::launch_bbegin(b->dP, b->g_vec, r, b->stream));
+  if (setup)
+    if ((rc = enq_setup(b, K))) return rc;
+  BHIPCHK(amm::launch_bx0(b->dP, b->g_vec, r, b->stream));
+  BHIPCHK(amm::launch_btail(b->dP, r, amm::kBTailScale, 0, b->stream));
+  BHIPCHK(amm::launch_bnorm(b->dP, b->g_vec, r, b->stream));
+  for (uint32_t it = 0; it < (uint32_t)p.step; ++it)
+    if ((rc = enq_iteration(b, r, K, it))) return rc;
+  return AMM_OK;
+}
+
+bool same_key(const amm_params& a, const amm_params& b) {
+  return a.kernel == b.kernel && a.step == b.step && a.vr_period == b.vr_period &&
+         a.vr_adaptive_threshold == b.vr_adaptive_threshold && a.trial == b.trial &&
+         a.flags == b.flags;
+}
+
+int validate(const amm_batch* b, const amm_params* p) {
+  if (!p) return fail(AMM_E_INVALID, "params is NULL");
+  if (p->kernel < AMM_KERNEL_EXACT || p->kernel > AMM_KERNEL_PROPOSED_V2)
+    return fail(AMM_E_INVALID, "unknown kernel " + std::to_string(p->kernel));
+  if (p->step < 1 || p->step > (1ull << 24)) return fail(AMM_E_INVALID, "step must be in [1, 2^24]");
+  if (p->sanity) return fail(AMM_E_INVALID, "sanity mode is a single-matrix debug mode");
+  if (p->kernel == AMM_KERNEL_PROPOSED_V2 && (p->vr_period < 1 || p->vr_period > p->step))
+    return fail(AMM_E_INVALID,
+                "proposedAMM_v2 needs 1 <= vr_period <= step (powerMethod_by_proposed_v2.cuh:234)");
+  if (!(p->vr_adaptive_threshold >= 0.0)) return fail(AMM_E_INVALID, "vr_adaptive_threshold < 0");
+  if (p->kernel != AMM_KERNEL_EXACT)
+    for (const BProb& q : b->hp)
+      if (q.c < 1 || q.c > q.n)
+        return fail(AMM_E_INVALID, "every problem needs 1 <= c <= n (host_memory.h:166-167)");
+  return AMM_OK;
+}
+
+int check_status(amm_batch* b) {
+  for (uint32_t i = 0; i < b->np; ++i) {
+    uint32_t stt = 0;
+    BHIPCHK(hipMemcpy(&stt, (char*)b->sb[i].st + offsetof(DevState, status), sizeof(uint32_t),
+                      hipMemcpyDeviceToHost));
+    if (stt & amm::kStatusNonFinite)
+      return fail(AMM_E_INVALID, "problem " + std::to_string(i) +
+                                     ": the matrix has non-finite entries (NaN or Inf column norms)");
+  }
+  return AMM_OK;
+}
+
+// Prepare descriptors, schedule and graph for parameters p (no launch).
+int prepare(amm_batch* b, const amm_params* p) {
+  int rc;
+  if ((rc = layout_run(b, p->step))) return rc;
+  if (!b->plan_valid || b->plan_kernel != p->kernel)
+    if ((rc = upload_plan(b, p->kernel))) return rc;
+  const std::vector<uint32_t> modes = static_modes(*p);
+  BHIPCHK(hipMemcpyAsync(b->modes_static, modes.data(), modes.size() * sizeof(uint32_t),
+                         hipMemcpyHostToDevice, b->stream));
+  return AMM_OK;
+}
+
+int run(amm_batch* b, const amm_params* p) {
+  int rc;
+  BHIPCHK(hipSetDevice(b->device));
+  if ((rc = validate(b, p))) return rc;
+  if ((rc = prepare(b, p))) return rc;
+  const bool use_graph = !(p->flags & AMM_FLAG_NO_GRAPH);
+  // the setup kernels are part of the run unless statistics are kept
+  const bool setup_included =
+      p->kernel != AMM_KERNEL_EXACT &&
+      (!(p->flags & AMM_FLAG_KEEP_STATS) || !b->stats_valid || b->stats_kernel != p->kernel);
+  if (use_graph) {
+    if (!b->graph_valid || !same_key(b->key, *p) || b->graph_setup != setup_included) {
+      drop_graph(b);
+      BHIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_run(b, *p, setup_included);
+      hipGraph_t g = nullptr;
+      const hipError_t ce = hipStreamEndCapture(b->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (ce != hipSuccess) return fail(AMM_E_HIP, std::string("graph capture failed: ") + hipGetErrorString(ce));
+      b->graph = g;
+      BHIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->graph_valid = true;
+      b->graph_setup = setup_included;
+      b->key = *p;
+    }
+    BHIPCHK(hipEventRecord(b->ev0, b->stream));
+    BHIPCHK(hipGraphLaunch(b->exec, b->stream));
+    BHIPCHK(hipEventRecord(b->ev1, b->stream));
+  } else {
+    BHIPCHK(hipEventRecord(b->ev0, b->stream));
+    if ((rc = enqueue_run(b, *p, setup_included))) return rc;
+    BHIPCHK(hipEventRecord(b->ev1, b->stream));
+  }
+  BHIPCHK(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  BHIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->last_ms = ms;
+  if (p->kernel != AMM_KERNEL_EXACT) {
+    if ((rc = check_status(b))) {
+      b->stats_valid = false;
+      return rc;
+    }
+    if (setup_included) {
+      b->stats_valid = true;
+      b->stats_kernel = p->kernel;
+    }
+  }
+  b->last_step = p->step;
+  b->ran = true;
+  return AMM_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, void* stream,
+                     amm_batch** out) {
+  if (!out) return fail(AMM_E_INVALID, "out is NULL");
+  *out = nullptr;
+  if (!probs || count < 1) return fail(AMM_E_INVALID, "need at least one problem");
+  if (trials < 1 || trials > amm::kBT) return fail(AMM_E_INVALID, "trials must be in [1, 16]");
+  for (uint32_t i = 0; i < count; ++i) {
+    const amm_problem& q = probs[i];
+    const std::string w = "problem " + std::to_string(i) + ": ";
+    if (!q.A) return fail(AMM_E_INVALID, w + "A is NULL");
+    if (q.dtype != AMM_DTYPE_F32 && q.dtype != AMM_DTYPE_BF16) return fail(AMM_E_INVALID, w + "unknown dtype");
+    if (q.d < 1 || q.n < 1 || q.d >= (1ull << 31) || q.n >= (1ull << 26))
+      return fail(AMM_E_INVALID, w + "need 1 <= d < 2^31 and 1 <= n < 2^26");
+    if (q.ld < q.d) return fail(AMM_E_INVALID, w + "ld must be >= d");
+    const uint64_t V = q.dtype == AMM_DTYPE_BF16 ? 8 : 4;
+    if (q.ld % V) return fail(AMM_E_INVALID, w + "ld must be a multiple of 16 bytes of elements");
+    if (((uintptr_t)q.A) & 15u) return fail(AMM_E_INVALID, w + "A must be 16-byte aligned");
+    if (q.c > q.n) return fail(AMM_E_INVALID, w + "c must be <= n");
+  }
+  amm_batch* b = new amm_batch();
+  auto bail = [&](int r) {
+    amm_batch_destroy(b);
+    return r;
+  };
+  b->np = count;
+  b->ntr = trials;
+  b->probs.assign(probs, probs + count);
+  if (hipGetDevice(&b->device) != hipSuccess) return bail(fail(AMM_E_HIP, "hipGetDevice failed"));
+  if (stream) {
+    b->stream = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamDefault) != hipSuccess)
+      return bail(fail(AMM_E_HIP, "hipStreamCreate failed"));
+    b->own_stream = true;
+  }
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device) == hipSuccess && khz > 0)
+    b->wall_khz = khz;
+  // static per-problem layout
+  Pool pl;
+  struct Off {
+    size_t X, Y, XM, YM, DX, DXh, DXl, cnt, W, list, tcount, BP, BQ, bt, alpha, nrm2, cnrm2, cdf0,
+        cdf1, z, v, st, eigvec;
+  };
+  std::vector<Off> o(count);
+  b->hp.resize(count);
+  b->sb.resize(count);
+  b->tune.resize(count);
+  uint64_t max_n = 0, max_zs = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const amm_problem& q = probs[i];
+    BProb& p = b->hp[i];
+    p = BProb{};
+    p.A = q.A;
+    p.ld = q.ld;
+    p.seed = q.seed;
+    p.d = (uint32_t)q.d;
+    p.n = (uint32_t)q.n;
+    p.dstage = (uint32_t)rup(q.d, amm::kBStageAlign);
+    p.c = q.c;
+    p.dtype = q.dtype;
+    p.ntiles = (uint32_t)cdiv(q.n, amm::kBTile);
+    p.nvblk = p.dstage / amm::kBVecRows;
+    const size_t vec = (size_t)amm::kBT * p.dstage;
+    const size_t cols = (size_t)p.ntiles * amm::kBTile;
+    const uint64_t dpad8 = rup(q.d, 8);
+    Off& f = o[i];
+    f.X = pl.take<float>(vec);
+    f.Y = pl.take<float>(vec);
+    f.XM = pl.take<float>(vec);
+    f.YM = pl.take<float>(vec);
+    f.DX = pl.take<float>(q.dtype == AMM_DTYPE_F32 ? vec : 1);
+    f.DXh = pl.take<uint16_t>(q.dtype == AMM_DTYPE_BF16 ? vec : 1);
+    f.DXl = pl.take<uint16_t>(q.dtype == AMM_DTYPE_BF16 ? vec : 1);
+    f.cnt = pl.take<uint32_t>(cols * amm::kBT);
+    f.W = pl.take<float>(cols * amm::kBT);
+    f.list = pl.take<uint32_t>(cols);
+    f.tcount = pl.take<uint32_t>(p.ntiles);
+    f.BP = pl.take<double>((size_t)p.nvblk * amm::kBT * 3);
+    f.BQ = pl.take<double>((size_t)p.nvblk * amm::kBT * 2);
+    f.bt = pl.take<BTrial>(amm::kBT);
+    f.alpha = pl.take<float>(q.n);
+    f.nrm2 = pl.take<double>(q.n);
+    f.cnrm2 = pl.take<double>(q.n);
+    f.cdf0 = pl.take<uint64_t>(q.n);
+    f.cdf1 = pl.take<uint64_t>(q.n);
+    f.z = pl.take<float>(dpad8);
+    f.v = pl.take<float>(dpad8);
+    f.st = pl.take<DevState>(1);
+    f.eigvec = pl.take<double>(dpad8);
+    b->sb[i].dpad8 = dpad8;
+    max_n = std::max<uint64_t>(max_n, q.n);
+    max_zs = std::max<uint64_t>(max_zs, cdiv(q.n, kZChunk) * dpad8);
+  }
+  if (hipMalloc((void**)&b->spool, pl.bytes) != hipSuccess)
+    return bail(fail(AMM_E_NOMEM, "device allocation of " + std::to_string(pl.bytes) + " bytes failed"));
+  if (hipMemset(b->spool, 0, pl.bytes) != hipSuccess) return bail(fail(AMM_E_HIP, "hipMemset failed"));
+  char* s0 = b->spool;
+  for (uint32_t i = 0; i < count; ++i) {
+    BProb& p = b->hp[i];
+    SetupBufs& s = b->sb[i];
+    const Off& f = o[i];
+    p.X = reinterpret_cast<float*>(s0 + f.X);
+    p.Y = reinterpret_cast<float*>(s0 + f.Y);
+    p.XM = reinterpret_cast<float*>(s0 + f.XM);
+    p.YM = reinterpret_cast<float*>(s0 + f.YM);
+    p.DX = reinterpret_cast<float*>(s0 + f.DX);
+    p.DXh = reinterpret_cast<uint16_t*>(s0 + f.DXh);
+    p.DXl = reinterpret_cast<uint16_t*>(s0 + f.DXl);
+    p.cnt = reinterpret_cast<uint32_t*>(s0 + f.cnt);
+    p.W = reinterpret_cast<float*>(s0 + f.W);
+    p.list = reinterpret_cast<uint32_t*>(s0 + f.list);
+    p.tcount = reinterpret_cast<uint32_t*>(s0 + f.tcount);
+    p.BP = reinterpret_cast<double*>(s0 + f.BP);
+    p.BQ = reinterpret_cast<double*>(s0 + f.BQ);
+    p.bt = reinterpret_cast<BTrial*>(s0 + f.bt);
+    s.alpha = reinterpret_cast<float*>(s0 + f.alpha);
+    s.nrm2 = reinterpret_cast<double*>(s0 + f.nrm2);
+    s.cnrm2 = reinterpret_cast<double*>(s0 + f.cnrm2);
+    s.cdf0 = reinterpret_cast<uint64_t*>(s0 + f.cdf0);
+    s.cdf1 = reinterpret_cast<uint64_t*>(s0 + f.cdf1);
+    s.z = reinterpret_cast<float*>(s0 + f.z);
+    s.v = reinterpret_cast<float*>(s0 + f.v);
+    s.st = reinterpret_cast<DevState*>(s0 + f.st);
+    s.eigvec = reinterpret_cast<double*>(s0 + f.eigvec);
+    p.st = s.st;
+    p.cdf0 = s.cdf0;
+    p.cdf1 = s.cdf1;
+    p.alpha = s.alpha;
+    p.z = s.z;
+    p.v = s.v;
+    p.eigvec = nullptr;
+  }
+  const uint64_t tiles = amm::cdf_tiles(max_n);
+  if (hipMalloc((void**)&b->tile_max, tiles * 8) != hipSuccess ||
+      hipMalloc((void**)&b->tile_sum, tiles * 8) != hipSuccess ||
+      hipMalloc((void**)&b->tile_lp, tiles * 8) != hipSuccess ||
+      hipMalloc((void**)&b->tile_a2, tiles * 8) != hipSuccess ||
+      hipMalloc((void**)&b->zP2, std::max<uint64_t>(max_zs, 1) * 4) != hipSuccess ||
+      hipMalloc((void**)&b->zP2s, std::max<uint64_t>(cdiv(max_n, kZChunk), 1) * 8) != hipSuccess ||
+      hipMalloc((void**)&b->dP, count * sizeof(BProb)) != hipSuccess)
+    return bail(fail(AMM_E_NOMEM, "device allocation failed"));
+  if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+    return bail(fail(AMM_E_HIP, "hipEventCreate failed"));
+  default_tuning(b);
+  *out = b;
+  return AMM_OK;
+}
+
+int amm_batch_destroy(amm_batch* b) {
+  if (!b) return AMM_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  void* bufs[] = {b->spool, b->tpool,  b->rpool, b->tile_max, b->tile_sum, b->tile_lp,
+                  b->tile_a2, b->zP2, b->zP2s, b->offs,     b->dP};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return AMM_OK;
+}
+
+int amm_batch_set_exact_eigenvector(amm_batch* b, uint32_t problem, const double* v, uint64_t len) {
+  if (!b) return fail(AMM_E_INVALID, "batch is NULL");
+  if (problem >= b->np) return fail(AMM_E_INVALID, "problem index out of range");
+  BProb& p = b->hp[problem];
+  if (!v) {
+    p.eigvec = nullptr;
+  } else {
+    if (len != p.d) return fail(AMM_E_INVALID, "eigenvector length must equal d");
+    BHIPCHK(hipMemcpy(b->sb[problem].eigvec, v, len * sizeof(double), hipMemcpyHostToDevice));
+    p.eigvec = b->sb[problem].eigvec;
+  }
+  b->plan_valid = false;
+  drop_graph(b);
+  return AMM_OK;
+}
+
+int amm_batch_power_method(amm_batch* b, const amm_params* p, double* time_ms, double* eig, double* err) {
+  if (!b) return fail(AMM_E_INVALID, "batch is NULL");
+  int rc;
+  if ((rc = run(b, p))) return rc;
+  const uint64_t S = p->step;
+  if (time_ms) {
+    std::vector<int64_t> t(S);
+    BHIPCHK(hipMemcpy(t.data(), b->ts, S * sizeof(int64_t), hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < S; ++i) time_ms[i] = (double)t[i] / (double)b->wall_khz;
+  }
+  std::vector<double> tmp(amm::kBT * S);
+  for (int which = 0; which < 2; ++which) {
+    double* dst = which ? err : eig;
+    if (!dst) continue;
+    for (uint32_t i = 0; i < b->np; ++i) {
+      BHIPCHK(hipMemcpy(tmp.data(), which ? b->hp[i].err : b->hp[i].eig, tmp.size() * sizeof(double),
+                        hipMemcpyDeviceToHost));
+      std::memcpy(dst + (uint64_t)i * b->ntr * S, tmp.data(), (size_t)b->ntr * S * sizeof(double));
+    }
+  }
+  return AMM_OK;
+}
+
+int amm_batch_last_modes(amm_batch* b, uint32_t* modes, uint64_t len) {
+  if (!b || !modes) return fail(AMM_E_INVALID, "NULL argument");
+  if (!b->ran) return fail(AMM_E_STATE, "no run yet");
+  const uint64_t S = b->last_step;
+  if (len < (uint64_t)b->np * b->ntr * S) return fail(AMM_E_INVALID, "modes buffer too small");
+  std::vector<uint32_t> tmp(amm::kBT * S);
+  for (uint32_t i = 0; i < b->np; ++i) {
+    BHIPCHK(hipMemcpy(tmp.data(), b->hp[i].modes, tmp.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::memcpy(modes + (uint64_t)i * b->ntr * S, tmp.data(), (size_t)b->ntr * S * sizeof(uint32_t));
+  }
+  return AMM_OK;
+}
+
+int amm_batch_get_vector(amm_batch* b, uint32_t problem, uint32_t trial, int32_t which, float* host,
+                         uint64_t len) {
+  if (!b || !host) return fail(AMM_E_INVALID, "NULL argument");
+  if (problem >= b->np || trial >= b->ntr) return fail(AMM_E_INVALID, "problem or trial out of range");
+  const BProb& p = b->hp[problem];
+  if (len < p.d) return fail(AMM_E_INVALID, "buffer shorter than d");
+  const float* src = (which == 1 ? p.Y : p.X) + (uint64_t)trial * p.dstage;
+  BHIPCHK(hipStreamSynchronize(b->stream));
+  BHIPCHK(hipMemcpy(host, src, p.d * sizeof(float), hipMemcpyDeviceToHost));
+  return AMM_OK;
+}
+
+int amm_batch_last_run_ms(amm_batch* b, double* ms) {
+  if (!b || !ms) return fail(AMM_E_INVALID, "NULL argument");
+  *ms = b->last_ms;
+  return AMM_OK;
+}
+
+int amm_batch_set_tuning(amm_batch* b, uint32_t problem, uint32_t kslice_rows, uint32_t passes,
+                         uint32_t chunks) {
+  if (!b) return fail(AMM_E_INVALID, "batch is NULL");
+  if (problem >= b->np) return fail(AMM_E_INVALID, "problem index out of range");
+  const Tuning t{kslice_rows, passes, chunks};
+  int rc;
+  if ((rc = check_tuning(b->hp[problem], t))) return rc;
+  b->tune[problem] = t;
+  b->plan_valid = false;
+  drop_graph(b);
+  return AMM_OK;
+}
+
+int amm_batch_get_tuning(amm_batch* b, uint32_t problem, uint32_t* kslice_rows, uint32_t* passes,
+                         uint32_t* chunks) {
+  if (!b) return fail(AMM_E_INVALID, "batch is NULL");
+  if (problem >= b->np) return fail(AMM_E_INVALID, "problem index out of range");
+  const Tuning& t = b->tune[problem];
+  if (kslice_rows) *kslice_rows = t.kslice_rows;
+  if (passes) *passes = t.passes;
+  if (chunks) *chunks = std::min(t.chunks, b->hp[problem].ntiles);
+  return AMM_OK;
+}
+
+// Union of the trials' samples of iteration `iter` for one problem: runs
+// the setup and the draw + compaction kernels only (every trial in the step
+// kind of `kernel`: previous -> plain CDF, proposed -> centred CDF) and
+// copies the compacted columns (ascending) with their 16 weights.
+int amm_batch_sample(amm_batch* b, int32_t kernel, uint32_t iter, uint32_t trial0, uint32_t problem,
+                     uint32_t* cols_host, float* w_host, uint64_t* count) {
+  if (!b || !count) return fail(AMM_E_INVALID, "NULL argument");
+  if (kernel != AMM_KERNEL_PREVIOUS && kernel != AMM_KERNEL_PROPOSED)
+    return fail(AMM_E_INVALID, "kernel must be previousAMM or proposedAMM");
+  if (problem >= b->np) return fail(AMM_E_INVALID, "problem index out of range");
+  BHIPCHK(hipSetDevice(b->device));
+  amm_params p{};
+  p.kernel = kernel;
+  p.step = 1;
+  p.vr_period = 0;
+  p.trial = trial0;
+  int rc;
+  if ((rc = validate(b, &p))) return rc;
+  if ((rc = prepare(b, &p))) return rc;
+  drop_graph(b);
+  const amm::BRun r = run_args(b, p);
+  BHIPCHK(amm::launch_bbegin(b->dP, b->g_vec, r, b->stream));
+  if ((rc = enq_setup(b, kernel))) return rc;
+  b->stats_valid = false;
+  BHIPCHK(amm::launch_bsample(b->dP, b->g_sample, r, iter, b->stream));
+  BHIPCHK(amm::launch_bprep(b->dP, b->g_prep, r, b->stream));
+  BHIPCHK(hipStreamSynchronize(b->stream));
+  const BProb& q = b->hp[problem];
+  std::vector<uint32_t> tc(q.ntiles), list((size_t)q.ntiles * amm::kBTile);
+  std::vector<float> W((size_t)q.ntiles * amm::kBTile * amm::kBT);
+  BHIPCHK(hipMemcpy(tc.data(), q.tcount, tc.size() * 4, hipMemcpyDeviceToHost));
+  BHIPCHK(hipMemcpy(list.data(), q.list, list.size() * 4, hipMemcpyDeviceToHost));
+  BHIPCHK(hipMemcpy(W.data(), q.W, W.size() * 4, hipMemcpyDeviceToHost));
+  uint64_t m = 0;
+  for (uint32_t t = 0; t < q.ntiles; ++t) {
+    for (uint32_t s = 0; s < tc[t]; ++s) {
+      const size_t slot = (size_t)t * amm::kBTile + s;
+      if (cols_host) cols_host[m] = list[slot];
+      if (w_host)
+        for (uint32_t k = 0; k < b->ntr; ++k) w_host[m * b->ntr + k] = W[slot * amm::kBT + k];
+      ++m;
+    }
+  }
+  *count = m;
+  return AMM_OK;
+}
+
+// Time candidate shapes of the two contractions on the live problems (one
+// exact-step iteration: every column present) and keep the fastest per
+// problem size class.  best_ms: time of one iteration's contractions with
+// the chosen shapes.
+int amm_batch_autotune(amm_batch* b, uint32_t iters, double* best_ms) {
+  if (!b) return fail(AMM_E_INVALID, "batch is NULL");
+  if (iters < 1) iters = 1;
+  BHIPCHK(hipSetDevice(b->device));
+  amm_params p{};
+  p.kernel = AMM_KERNEL_EXACT;
+  p.step = 1;
+  int rc;
+  if ((rc = prepare(b, &p))) return rc;
+  drop_graph(b);
+  // candidate rules: global multipliers of the default workgroup targets
+  struct Cand {
+    double ks_mult, ch_mult;
+    uint32_t passes;
+  };
+  const Cand cands[] = {{1, 1, 2}, {0.5, 0.5, 2}, {2, 2, 2}, {4, 4, 2}, {1, 1, 1}, {2, 2, 1}, {0.25, 0.25, 2}};
+  const std::vector<Tuning> base = b->tune;
+  double tot = 0.0;
+  for (const BProb& q : b->hp) tot += (double)q.ntiles * (double)q.d;
+  double best = 1e30;
+  std::vector<Tuning> best_t = base;
+  for (const Cand& cd : cands) {
+    for (uint32_t i = 0; i < b->np; ++i) {
+      const BProb& q = b->hp[i];
+      const double share = tot > 0 ? (double)q.ntiles * (double)q.d / tot : 1.0;
+      Tuning t = default_tuning_one(q, share);
+      const uint32_t max_slices = (uint32_t)std::min<uint64_t>(kMaxSlices, cdiv(q.dstage, 256));
+      uint32_t ks = (uint32_t)cdiv(q.dstage, t.kslice_rows);
+      ks = (uint32_t)std::min<double>(max_slices, std::max(1.0, std::round(ks * cd.ks_mult)));
+      t.kslice_rows = (uint32_t)rup(cdiv(q.dstage, ks), 256);
+      t.passes = cd.passes;
+      const uint32_t rbrows = 4 * t.passes * pass_rows(q.dtype);
+      const double want = std::max(1.0, kTargetItems * share);
+      const double ch = std::max(1.0, std::round(want / cdiv(q.d, rbrows) * cd.ch_mult));
+      t.chunks = (uint32_t)std::min<double>(q.ntiles, ch);
+      b->tune[i] = t;
+    }
+    if ((rc = upload_plan(b, AMM_KERNEL_EXACT))) return rc;
+    const amm::BRun r = run_args(b, p);
+    // all trials exact: every column present
+    BHIPCHK(amm::launch_bbegin(b->dP, b->g_vec, r, b->stream));
+    BHIPCHK(amm::launch_bprep(b->dP, b->g_prep, r, b->stream));
+    for (uint32_t w = 0; w < 2; ++w) {  // warm-up, then timed
+      BHIPCHK(hipEventRecord(b->ev0, b->stream));
+      for (uint32_t k = 0; k < (w ? iters : 1); ++k) {
+        for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bdot(dt, b->dP, b->g_dot[dt], b->np, b->stream));
+        for (int dt = 0; dt < 2; ++dt)
+          for (int pa = 0; pa < 2; ++pa)
+            BHIPCHK(amm::launch_baxpy(dt, (uint32_t)pa + 1, b->dP, b->g_axpy[dt][pa], b->np, 0u, b->stream));
+      }
+      BHIPCHK(hipEventRecord(b->ev1, b->stream));
+    }
+    BHIPCHK(hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    BHIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    const double per = (double)ms / iters;
+    if (per < best) {
+      best = per;
+      best_t = b->tune;
+    }
+  }
+  b->tune = best_t;
+  b->plan_valid = false;
+  if ((rc = upload_plan(b, AMM_KERNEL_EXACT))) return rc;
+  if (best_ms) *best_ms = best;
+  return AMM_OK;
+}
+
+}  // extern "C"

@@ -76,6 +76,11 @@ struct AxpyPlan {
 
 }  // namespace
 
+// error reporting shared with the batched engine (amm_batch_capi.hip)
+namespace amm_host {
+int fail(int code, const std::string& msg) { return ::fail(code, msg); }
+}  // namespace amm_host
+
 struct amm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;

@@ -28,6 +28,18 @@ enum StepMode : uint32_t {
   kModeExactNoMemo = 4,  // exact product, memo untouched
 };
 
+// Adaptive variance reduction: is iteration i an exact step?  Mirrors
+// powerMethod_by_previous.cuh:135-138 (kernel 1) and
+// powerMethod_by_proposed.cuh:159-161 (kernel 2, at most every other step).
+__device__ __forceinline__ bool exact_step_due(uint32_t kernel, uint64_t i, uint64_t period,
+                                               double thr, double prev_cos, uint64_t last_exact) {
+  const bool vr = period > 0 || thr > 0.0;
+  const bool due = (period > 0 && (i + 1) % period == 0) || (thr > 0.0 && prev_cos > 1.0 - thr);
+  if (kernel == 1) return vr && due;
+  if (kernel == 2) return (i > last_exact + 1) && vr && due;
+  return false;
+}
+
 // Status bits of DevState::status.
 constexpr uint32_t kStatusNonFinite = 1u;  // a column weight was NaN or Inf
 

@@ -614,17 +614,6 @@ __global__ __launch_bounds__(256) void k_gather_axpy(
 // y^T y, y^T v* (k_tail finishes them).  Exact steps with memo snapshot
 // [x, y] into [x_memo, y_memo] (powerMethod_by_previous.cuh:157).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool exact_step_due(uint32_t kernel, uint64_t i, uint64_t period,
-                                               double thr, double prev_cos, uint64_t last_exact) {
-  // mirrors orc_is_exact_step: powerMethod_by_previous.cuh:135-138,
-  // powerMethod_by_proposed.cuh:159-161
-  const bool vr = period > 0 || thr > 0.0;
-  const bool due = (period > 0 && (i + 1) % period == 0) || (thr > 0.0 && prev_cos > 1.0 - thr);
-  if (kernel == 1) return vr && due;
-  if (kernel == 2) return (i > last_exact + 1) && vr && due;
-  return false;
-}
-
 constexpr uint32_t kFinishRows = 64;  // rows per finish block
 
 __global__ __launch_bounds__(256) void k_finish(

@@ -176,6 +176,75 @@ int amm_sigma_max(amm_ctx *ctx, const amm_params *p, double *sigma, float *u_dev
 int amm_rank1_update(void *out_dev, int32_t dtype, uint64_t d, uint64_t n, uint64_t ld, float g,
                      const float *u_dev, const float *v_dev, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * Batched form (no reference counterpart: the reference solves one matrix
+ * with one right-hand side per run, AMM.cu:7-72).  Many matrices of uneven
+ * sizes, each with `trials` (<= 16) sampling seeds stacked as one block of
+ * right-hand sides, so that both contractions of a product run as skinny
+ * GEMMs on the matrix cores over the union of the trials' samples.  Trial k
+ * of problem i reproduces amm_power_method on that matrix with p->seed =
+ * problem.seed and p->trial = p->trial + k: the same sampled columns,
+ * multiplicities and weights in every iteration (the summation order of
+ * the products differs).  Every launch covers all problems; the whole run
+ * is one hipGraph.
+ * ------------------------------------------------------------------------- */
+typedef struct amm_problem {
+  const void *A;   /* device matrix, d x n column-major, leading dimension ld */
+  int32_t dtype;   /* AMM_DTYPE_F32 or AMM_DTYPE_BF16 */
+  uint32_t c;      /* sampled columns per trial and product (0: exact runs only) */
+  uint64_t d, n, ld;
+  uint64_t seed;   /* Philox key of this problem */
+} amm_problem;
+
+typedef struct amm_batch amm_batch;
+
+int amm_batch_create(const amm_problem *problems, uint32_t count, uint32_t trials, void *stream,
+                     amm_batch **out);
+int amm_batch_destroy(amm_batch *b);
+
+/* Reference eigenvector of one problem for the err output (NULL clears). */
+int amm_batch_set_exact_eigenvector(amm_batch *b, uint32_t problem, const double *v_host,
+                                    uint64_t len);
+
+/* One batched run (kernel, step, vr_period, vr_adaptive_threshold, trial,
+ * flags of p; c and seed per problem; sanity mode is not supported).
+ * time_ms: host, length step (device time from the run's first kernel to
+ * the end of each iteration, all problems advance together); eig, err:
+ * host, [count][trials][step] as in amm_power_method.  Any may be NULL. */
+int amm_batch_power_method(amm_batch *b, const amm_params *p, double *time_ms, double *eig,
+                           double *err);
+
+/* Step kinds of the last run, host [count][trials][step]. */
+int amm_batch_last_modes(amm_batch *b, uint32_t *modes_host, uint64_t len);
+
+/* Current iterate (which = 0) or last product (1) of one (problem, trial). */
+int amm_batch_get_vector(amm_batch *b, uint32_t problem, uint32_t trial, int32_t which,
+                         float *host, uint64_t len);
+
+/* Wall time of the last batched run from HIP events around it (ms). */
+int amm_batch_last_run_ms(amm_batch *b, double *ms);
+
+/* Launch shapes of one problem: rows per contraction-1 slice (multiple of
+ * 256, at most 16 slices), contraction-2 passes per wave (1 or 2) and
+ * column chunks (split of the second contraction). */
+int amm_batch_set_tuning(amm_batch *b, uint32_t problem, uint32_t kslice_rows, uint32_t passes,
+                         uint32_t chunks);
+int amm_batch_get_tuning(amm_batch *b, uint32_t problem, uint32_t *kslice_rows, uint32_t *passes,
+                         uint32_t *chunks);
+
+/* Autotuner: times candidate launch shapes of both contractions on the live
+ * problems (all columns present, `iters` repetitions each) and keeps the
+ * fastest; *best_ms = one pass of both contractions with it. */
+int amm_batch_autotune(amm_batch *b, uint32_t iters, double *best_ms);
+
+/* Union sample of iteration `iter` for one problem with every trial in the
+ * step kind of `kernel` (previousAMM: plain CDF, proposedAMM: centred),
+ * trial words trial0 + k: the distinct columns (ascending, host, up to n)
+ * and their per-trial weights mult * total / (c q_j) (host [count][trials],
+ * 0 where a trial did not draw the column); *count = number of columns. */
+int amm_batch_sample(amm_batch *b, int32_t kernel, uint32_t iter, uint32_t trial0,
+                     uint32_t problem, uint32_t *cols_host, float *w_host, uint64_t *count);
+
 /* Round count FP32 device values to BF16 (nearest even) into dst_dev; the
  * storage form of the BF16 path (no reference counterpart: the reference is
  * FP32 only).  Asynchronous on `stream`. */

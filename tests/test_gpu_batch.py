@@ -1,0 +1,182 @@
+"""GPU parity of the batched power method (many matrices x 16 trials).
+
+Pins:
+* the union sample of an iteration is bit-exact against the oracle's
+  per-trial samples (columns, and each trial's weight bit for bit);
+* trial k of a problem follows the single-matrix run with trial word
+  trial0 + k (same samples): Rayleigh-quotient trajectories within 1e-5
+  (exact) / 1e-4 (sampled) relative, the final iterate within 1e-4;
+* a grouped launch over uneven problems gives bit-identical results to the
+  same problems launched one by one with the same launch shapes.
+"""
+import numpy as np
+import pytest
+
+from helpers import bf16_bits, heavy_columns, lowrank_plus_noise, to_cols
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def banner(msg):
+    print(f"\n[gpu-case] {msg}", flush=True)
+
+
+def _api():
+    from eigenvaluecomputation_by_amm_amd import AmmBatch, AmmContext, RunParams, AmmError
+    return AmmBatch, AmmContext, RunParams, AmmError
+
+
+def dev(A, bf16=False, ld=None):
+    cols = to_cols(A, ld)
+    if bf16:
+        return torch.from_numpy(bf16_bits(cols).view(np.int16)).cuda().view(torch.bfloat16)
+    return torch.from_numpy(cols).cuda()
+
+
+def relmax(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+@pytest.mark.parametrize("kernel", ["previousAMM", "proposedAMM"])
+@pytest.mark.parametrize("c,trials", [(300, 16), (2000, 5), (8192, 3)])
+def test_union_sample_bitexact(kernel, c, trials):
+    banner(f"batch union sample {kernel} c={c} trials={trials}")
+    AmmBatch, AmmContext, *_ = _api()
+    A = heavy_columns(96, 20000, seed=11) + (0.2 if kernel == "proposedAMM" else 0.0)
+    A[:, [5, 19999]] = 0.0
+    seed, it, trial0 = 0xABCDEF12345, 7, 3
+    A_t = dev(A)
+    with AmmBatch([dict(A=A_t, c=c, seed=seed)], trials=trials) as b:
+        cols, W = b.sample(kernel, it, trial0, 0)
+    # column weights from the same statistics kernel (pinned against the
+    # FP64 closed form by test_column_stats)
+    with AmmContext(A_t) as ctx:
+        _, nrm2, cnrm2, _ = ctx.column_stats()
+    w = cnrm2 if kernel == "proposedAMM" else nrm2
+    ref = {}
+    for k in range(trials):
+        oi, ow, os_ = O.sample(w, c, seed, it, trial0 + k)
+        assert len(os_) == 1
+        for j, wt in zip(oi[: os_[0]], ow[: os_[0]]):
+            ref.setdefault(int(j), np.zeros(trials, np.float32))[k] = wt
+    keys = np.array(sorted(ref), np.uint32)
+    assert np.array_equal(cols, keys)
+    Wref = np.stack([ref[int(j)] for j in keys])
+    assert np.array_equal(W.view(np.uint32), Wref.view(np.uint32))
+    assert not np.isin([5, 19999], cols).any()
+
+
+def _single(A_t, d, params, trial):
+    _, AmmContext, RunParams, _ = _api()
+    with AmmContext(A_t, d=d) as ctx:
+        p = RunParams(**{**params.__dict__, "trial": trial})
+        r = ctx.power_method(p)
+        return r.eig.copy(), ctx.get_vector("x"), ctx.last_modes(p.step)
+
+
+@pytest.mark.parametrize("kernel,extra,tol", [
+    ("exact", {}, 1e-5),
+    ("previousAMM", {"c": 700, "vr_period": 5}, 1e-4),
+    ("proposedAMM", {"c": 700, "vr_period": 4}, 1e-4),
+    ("proposedAMM_v2", {"c": 700, "vr_period": 6}, 1e-4),
+    ("previousAMM", {"c": 500, "vr_adaptive_threshold": 1e-3}, 1e-4),
+])
+@pytest.mark.parametrize("bf16", [False, True])
+def test_trials_follow_single_path(kernel, extra, tol, bf16):
+    banner(f"batch vs single {kernel} {extra} bf16={bf16}")
+    AmmBatch, _, RunParams, _ = _api()
+    d, n = 600, 2800
+    A = lowrank_plus_noise(d, n, seed=21, noise=0.05) + 0.1
+    A_t = dev(A, bf16)
+    seed = 4242
+    c = extra.get("c", 0)
+    params = RunParams(kernel=kernel, step=24, seed=seed, trial=5,
+                       **{k: v for k, v in extra.items() if k != "c"}, c=c)
+    ntr = 4
+    with AmmBatch([dict(A=A_t, c=c, seed=seed)], trials=ntr) as b:
+        res = b.power_method(params)
+        modes = b.last_modes(params.step)
+        xs = [b.get_vector(0, k) for k in range(ntr)]
+    # BF16 stores dx as hi + lo BF16 pairs: products agree to ~2^-16
+    tol_k = tol if not bf16 else max(tol, 1e-4)
+    for k in range(ntr):
+        e, x, m = _single(A_t, d, params, params.trial + k)
+        assert np.array_equal(modes[0, k], m), (k, modes[0, k], m)
+        assert relmax(res.eig[0, k], e) < tol_k, (k, relmax(res.eig[0, k], e))
+        assert relmax(xs[k], x) < 50 * tol_k or np.max(np.abs(xs[k] - x)) < 50 * tol_k
+    assert np.all(np.diff(res.time_ms) >= 0) and res.time_ms[-1] > 0
+
+
+def test_grouped_equals_per_problem():
+    """Uneven problems (FP32 and BF16, several shapes) in one grouped run are
+    bit-identical to each problem run alone with the same launch shapes."""
+    banner("batch grouped vs per-problem")
+    AmmBatch, _, RunParams, _ = _api()
+    rng = np.random.default_rng(3)
+    probs = []
+    for i, (d, n, bf) in enumerate([(256, 1024, False), (700, 3000, True), (520, 777, False),
+                                    (1100, 4100, True), (90, 2500, False)]):
+        A = lowrank_plus_noise(d, n, seed=100 + i, noise=0.05) + 0.05
+        ld = (d + 7) // 8 * 8  # padded leading dimension (16-byte columns)
+        probs.append(dict(A=dev(A, bf, ld), d=d, c=max(1, n // 8), seed=int(rng.integers(1, 2**40))))
+    params = RunParams(kernel="previousAMM", step=12, vr_period=4, trial=1)
+    with AmmBatch(probs, trials=16) as b:
+        tunes = [b.get_tuning(i) for i in range(len(probs))]
+        allr = b.power_method(params)
+    for i, p in enumerate(probs):
+        with AmmBatch([p], trials=16) as b1:
+            b1.set_tuning(0, *tunes[i])
+            r1 = b1.power_method(params)
+        assert np.array_equal(r1.eig[0].view(np.uint64), allr.eig[i].view(np.uint64)), i
+
+
+def test_multi_problem_lambda():
+    """Each problem's trials converge to its own lambda_max (FP64 eigvalsh)."""
+    banner("batch lambda of uneven problems")
+    AmmBatch, _, RunParams, _ = _api()
+    shapes = [(300, 900), (512, 2048), (128, 3000), (800, 1200)]
+    probs, lam = [], []
+    for i, (d, n) in enumerate(shapes):
+        A = lowrank_plus_noise(d, n, seed=200 + i, noise=0.01)
+        probs.append(dict(A=dev(A), c=n // 8, seed=77 + i))
+        lam.append(O.lambda_max_eigvalsh(A.astype(np.float64)))
+    params = RunParams(kernel="previousAMM", step=60, vr_period=5, trial=0)
+    with AmmBatch(probs, trials=16) as b:
+        r = b.power_method(params)
+    for i, l in enumerate(lam):
+        final = r.eig[i, :, -1]
+        assert np.all(np.abs(final - l) / l < 1e-3), (i, final, l)
+
+
+def test_autotune_keeps_results():
+    banner("batch autotune")
+    AmmBatch, _, RunParams, _ = _api()
+    A = lowrank_plus_noise(1000, 6000, seed=9, noise=0.02)
+    A_t = dev(A, True)
+    params = RunParams(kernel="previousAMM", c=750, step=16, vr_period=4, seed=5)
+    with AmmBatch([dict(A=A_t, c=750, seed=5)], trials=16) as b:
+        r0 = b.power_method(params)
+        ms = b.autotune(3)
+        r1 = b.power_method(params)
+    assert ms > 0
+    assert relmax(r1.eig, r0.eig) < 1e-4
+
+
+def test_batch_invalid_arguments():
+    banner("batch invalid arguments")
+    AmmBatch, _, RunParams, AmmError = _api()
+    A_t = dev(lowrank_plus_noise(64, 128, seed=1))
+    with pytest.raises(AmmError):
+        AmmBatch([dict(A=A_t, c=10, seed=1)], trials=17)
+    with pytest.raises(AmmError):
+        AmmBatch([dict(A=A_t, c=129, seed=1)], trials=2)
+    with AmmBatch([dict(A=A_t, c=10, seed=1)], trials=2) as b:
+        with pytest.raises(AmmError):
+            b.power_method(RunParams(kernel="previousAMM", c=10, step=3, sanity=True))
+        with pytest.raises(AmmError):
+            b.set_tuning(0, 100, 2, 1)
