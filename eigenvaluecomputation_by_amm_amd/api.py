@@ -401,8 +401,8 @@ class AmmBatch:
                                              C.c_void_p(_ptr(v)), v.size))
         return v
 
-    def set_tuning(self, problem: int, kslice_rows: int, passes: int, chunks: int) -> None:
-        check(self._lib.amm_batch_set_tuning(self._h, problem, kslice_rows, passes, chunks))
+    def set_tuning(self, problem: int, dot_blocks: int, passes: int, chunks: int) -> None:
+        check(self._lib.amm_batch_set_tuning(self._h, problem, dot_blocks, passes, chunks))
 
     def get_tuning(self, problem: int):
         a = np.zeros(3, np.uint32)

@@ -224,19 +224,58 @@ __global__ __launch_bounds__(256) void k_bnorm(const BProb* __restrict__ P, cons
 // counts the draw in cnt[j][k].  Items: trials x ceil(c / 2048) per problem.
 // The first block of a proposed trial also finishes s, q of its product.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t bcdf_search(const uint64_t* __restrict__ cdf, uint32_t n,
-                                                uint64_t t, uint64_t last_pos) {
-  uint32_t lo = 0, hi = n;
+// Coarse table: kCoarse bucket ends e_m = floor((m + 1) n / kCoarse) - 1 and
+// their CDF values in LDS; a target is first located among the buckets in
+// LDS, then within its bucket (about n / kCoarse entries) in global memory.
+// Same result as the plain search: the smallest j with cdf[j] > t.
+constexpr uint32_t kCoarse = 1024;
+
+__device__ __forceinline__ uint32_t bucket_end(uint32_t m, uint32_t n) {
+  return (uint32_t)(((uint64_t)(m + 1) * n) / kCoarse) - 1u;  // may wrap for empty leading buckets
+}
+
+__device__ __forceinline__ uint32_t bcdf_search(const uint64_t* __restrict__ cdf, const uint64_t* cs,
+                                                uint32_t n, uint64_t t, uint64_t last_pos) {
+  // smallest bucket m with cs[m] > t (cs[m] = cdf at its end; empty buckets
+  // repeat the previous end's value)
+  uint32_t lo = 0, hi = kCoarse;
   while (lo < hi) {
-    const uint32_t mid = lo + ((hi - lo) >> 1);
-    if (cdf[mid] > t) hi = mid; else lo = mid + 1;
+    const uint32_t mid = (lo + hi) >> 1;
+    if (cs[mid] > t) hi = mid; else lo = mid + 1;
   }
-  return lo < n ? lo : (uint32_t)last_pos;
+  if (lo >= kCoarse) return (uint32_t)last_pos;
+  uint32_t jlo = lo ? bucket_end(lo - 1, n) + 1u : 0u;
+  uint32_t jhi = bucket_end(lo, n) + 1u;
+  while (jlo < jhi) {
+    const uint32_t mid = jlo + ((jhi - jlo) >> 1);
+    if (cdf[mid] > t) jhi = mid; else jlo = mid + 1;
+  }
+  return jlo < n ? jlo : (uint32_t)last_pos;
+}
+
+// Open-addressing count table in LDS: key = column + 1 (0 = empty).  A block
+// draws at most 2048 columns, so 4096 entries stay at most half full.
+constexpr uint32_t kHash = 4096;
+
+__device__ __forceinline__ void hash_count(uint32_t* hkey, uint32_t* hcnt, uint32_t j) {
+  const uint32_t key = j + 1u;
+  uint32_t h = (j * 2654435761u) >> 20;  // top 12 bits: [0, 4096)
+  while (true) {
+    const uint32_t old = atomicCAS(&hkey[h], 0u, key);
+    if (old == 0u || old == key) {
+      atomicAdd(&hcnt[h], 1u);
+      return;
+    }
+    h = (h + 1u) & (kHash - 1u);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_bsample(const BProb* __restrict__ P, const uint32_t* off,
                                                  BRun r, uint32_t it) {
   __shared__ double red[4];
+  __shared__ uint64_t cs[kCoarse];
+  __shared__ uint32_t hkey[kHash];
+  __shared__ uint32_t hcnt[kHash];
   const uint32_t p = find_problem(off, r.np, blockIdx.x);
   const BProb& pr = P[p];
   const uint32_t local = blockIdx.x - off[p];
@@ -263,6 +302,16 @@ __global__ __launch_bounds__(256) void k_bsample(const BProb* __restrict__ P, co
   const uint64_t total = pr.st->total[which];
   const uint64_t last_pos = pr.st->last_pos[which];
   const uint64_t* cdf = which ? pr.cdf1 : pr.cdf0;
+  for (uint32_t m = threadIdx.x; m < kCoarse; m += 256) {
+    const uint32_t e = bucket_end(m, pr.n);
+    // an empty bucket before the first column ends at "-1": cdf value 0
+    cs[m] = ((uint64_t)(m + 1) * pr.n) / kCoarse == 0 ? 0ull : cdf[e];
+  }
+  for (uint32_t e = threadIdx.x; e < kHash; e += 256) {
+    hkey[e] = 0u;
+    hcnt[e] = 0u;
+  }
+  __syncthreads();
   const uint32_t k0 = (uint32_t)pr.seed, k1 = (uint32_t)(pr.seed >> 32);
 #pragma unroll
   for (int h = 0; h < 4; ++h) {
@@ -271,13 +320,17 @@ __global__ __launch_bounds__(256) void k_bsample(const BProb* __restrict__ P, co
     if (slot < pr.c) {
       uint32_t q[4];
       philox4x32_10(pair, it, r.trial0 + k, kStreamSample, k0, k1, q);
-      const uint32_t j0 = bcdf_search(cdf, pr.n, draw_target(q[0], q[1], total), last_pos);
-      atomicAdd(pr.cnt + (uint64_t)j0 * kBT + k, 1u);
-      if (slot + 1 < pr.c) {
-        const uint32_t j1 = bcdf_search(cdf, pr.n, draw_target(q[2], q[3], total), last_pos);
-        atomicAdd(pr.cnt + (uint64_t)j1 * kBT + k, 1u);
-      }
+      hash_count(hkey, hcnt, bcdf_search(cdf, cs, pr.n, draw_target(q[0], q[1], total), last_pos));
+      if (slot + 1 < pr.c)
+        hash_count(hkey, hcnt, bcdf_search(cdf, cs, pr.n, draw_target(q[2], q[3], total), last_pos));
     }
+  }
+  __syncthreads();
+  // one global update per distinct column of this block (heavy columns
+  // would otherwise serialise on one address)
+  for (uint32_t e = threadIdx.x; e < kHash; e += 256) {
+    const uint32_t key = hkey[e];
+    if (key) atomicAdd(pr.cnt + (uint64_t)(key - 1u) * kBT + k, hcnt[e]);
   }
 }
 
@@ -376,220 +429,352 @@ __device__ __forceinline__ u32x4 load_a16(const T* col, uint32_t row, uint32_t d
 }
 
 // ---------------------------------------------------------------------------
+// Union prefix (one 1024-thread workgroup per problem): tpre[t] = number of
+// present columns in tiles < t; tpre[ntiles] = |U|.  Compacted slot s of
+// the union is slot s - tpre[t] of tile t = max{t : tpre[t] <= s}.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_bscan(const BProb* __restrict__ P) {
+  __shared__ uint32_t sc[2][1024];
+  const BProb& pr = P[blockIdx.x];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t per = (pr.ntiles + 1023) / 1024;
+  const uint32_t b = tid * per;
+  uint32_t sum = 0;
+  for (uint32_t i = 0; i < per; ++i)
+    if (b + i < pr.ntiles) sum += pr.tcount[b + i];
+  sc[0][tid] = sum;
+  __syncthreads();
+  int cur = 0;
+  for (uint32_t o = 1; o < 1024; o <<= 1) {
+    const uint32_t v = tid >= o ? sc[cur][tid] + sc[cur][tid - o] : sc[cur][tid];
+    sc[cur ^ 1][tid] = v;
+    cur ^= 1;
+    __syncthreads();
+  }
+  uint32_t run = sc[cur][tid] - sum;  // exclusive prefix
+  for (uint32_t i = 0; i < per; ++i) {
+    if (b + i < pr.ntiles) {
+      pr.tpre[b + i] = run;
+      run += pr.tcount[b + i];
+    }
+  }
+  if (tid == 1023) pr.tpre[pr.ntiles] = sc[cur][1023];
+}
+
+// tile of compacted slot s: largest t with tpre[t] <= s (s < tpre[ntiles])
+__device__ __forceinline__ uint32_t slot_tile(const uint32_t* __restrict__ tpre, uint32_t ntiles,
+                                              uint32_t s) {
+  uint32_t lo = 0, hi = ntiles;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (tpre[mid] <= s) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------
+// LDS tile of C columns x RB rows staged from HBM with long contiguous runs:
+// each column's RB rows are read by RB*sizeof(T)/16 consecutive lanes (512
+// or 1024 contiguous bytes), prefetched into registers one round ahead and
+// stored to LDS at column pitch RB*sizeof(T) + 16 bytes.
+// ---------------------------------------------------------------------------
+template <typename T, uint32_t C, uint32_t RB>
+struct TileStage {
+  static constexpr uint32_t kColBytes = RB * (uint32_t)sizeof(T);
+  static constexpr uint32_t kLanesPerCol = kColBytes / 16;
+  static constexpr uint32_t kColsPerPass = 256 / kLanesPerCol;
+  static constexpr uint32_t kLoads = C / kColsPerPass;
+  static constexpr uint32_t kPitch = kColBytes + 16;
+  static constexpr uint32_t kBytes = C * kPitch;
+  static constexpr uint32_t kElems16 = 16 / (uint32_t)sizeof(T);
+  u32x4 r[kLoads];
+
+  // cols: C column indices in LDS (0xFFFFFFFF: absent column, reads 0)
+  __device__ __forceinline__ void load(const T* __restrict__ A, uint64_t ld, const uint32_t* cols,
+                                       uint32_t row0, uint32_t d) {
+    const uint32_t sub = threadIdx.x % kLanesPerCol, c0 = threadIdx.x / kLanesPerCol;
+    const uint32_t row = row0 + sub * kElems16;
+#pragma unroll
+    for (uint32_t q = 0; q < kLoads; ++q) {
+      const uint32_t j = cols[q * kColsPerPass + c0];
+      r[q] = j != 0xFFFFFFFFu ? load_a16<T>(A + (uint64_t)j * ld, row, d) : u32x4{0u, 0u, 0u, 0u};
+    }
+  }
+  __device__ __forceinline__ void store(char* lds) const {
+    const uint32_t sub = threadIdx.x % kLanesPerCol, c0 = threadIdx.x / kLanesPerCol;
+#pragma unroll
+    for (uint32_t q = 0; q < kLoads; ++q)
+      *reinterpret_cast<u32x4*>(lds + (q * kColsPerPass + c0) * kPitch + sub * 16) = r[q];
+  }
+};
+
+// ---------------------------------------------------------------------------
 // Contraction 1 (previousAMM_kernel1 / proposedAMM_kernel1 for 16 trials):
-// P1[ks][slot][k] = a_{list[slot]}[rows of slice ks]^T dx_k.  Workgroup =
-// (tile, row slice); wave w takes a quarter of the slice's rows and all
-// present columns of the tile in groups of 16 (one MFMA tile 16 columns x
-// 16 trials).  BF16: v_mfma_f32_16x16x32_bf16 with dx = hi + lo (two MFMAs
-// per 32 rows); FP32: v_mfma_f32_16x16x4_f32, four per 16-byte load.
-// A-operand lane (i, g) holds column slot 16G + i, rows 8g.. (BF16) or
-// 4g.. (FP32) of the step; the B operand holds the same rows of dx for trial
-// i; accumulator lane (i, g) element e is slot 16G + 4g + e, trial i.
+// P1[ks][s][k] = a_{col(s)}[rows of slice ks]^T dx_k for the 64 compacted
+// union slots s of virtual tile vt.  Per round the workgroup stages the 64
+// columns x RB rows in LDS (RB = 256 BF16 / 128 FP32, 512 contiguous bytes
+// per column); wave w multiplies its quarter of the round's rows for all
+// four groups of 16 columns: BF16 v_mfma_f32_16x16x32_bf16 with dx = hi +
+// lo (two MFMAs per 32 rows), FP32 v_mfma_f32_16x16x4_f32 (four per 16
+// rows).  A-operand lane (i, g): column 16G + i, rows 8g.. (BF16) / 4g..
+// (FP32) of the step; B-operand lane (i, g): the same rows of dx_i;
+// accumulator lane (i, g) element e: slot 16G + 4g + e, trial i.  The
+// slice-0 workgroup also writes the compacted columns and weights.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const uint32_t* off,
                                               uint32_t np) {
-  __shared__ float red[4][kBTile][kBT + 1];
+  constexpr bool BF = Elem<T>::VEC == 8;
+  constexpr uint32_t RB = BF ? 256 : 128;
+  using Stage = TileStage<T, 64, RB>;
+  __shared__ __align__(16) char lds[Stage::kBytes];
+  __shared__ uint32_t cols[64];
   const uint32_t p = find_problem(off, np, blockIdx.x);
   const BProb& pr = P[p];
   const uint32_t local = blockIdx.x - off[p];
-  const uint32_t tile = local / pr.kslices, ks = local % pr.kslices;
-  const uint32_t cnt = pr.tcount[tile];
-  if (cnt == 0) return;
+  // slice-major order: the active (tile, slice) items of a sparse union are
+  // one contiguous range of blocks, spread evenly over the XCDs (blocks are
+  // dealt to the XCDs round-robin)
+  const uint32_t ks = local / pr.ntiles, vt = local % pr.ntiles;
+  const UPlan up = union_plan(pr, pr.tpre[pr.ntiles]);
+  const uint32_t U = up.U;
+  const uint32_t s0 = vt * kBTile;
+  if (s0 >= U || ks >= up.kslices) return;
+  const uint32_t cnt = min(kBTile, U - s0);
+  if (threadIdx.x < kBTile) {
+    uint32_t j = 0xFFFFFFFFu;
+    if (threadIdx.x < cnt) {
+      const uint32_t s = s0 + threadIdx.x;
+      const uint32_t t = slot_tile(pr.tpre, pr.ntiles, s);
+      const uint64_t ts = (uint64_t)t * kBTile + (s - pr.tpre[t]);
+      j = pr.list[ts];
+      if (ks == 0) {
+        pr.colc[s] = j;
+        const float4* w = reinterpret_cast<const float4*>(pr.W + ts * kBT);
+        float4* wc = reinterpret_cast<float4*>(pr.Wc + (uint64_t)s * kBT);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) wc[v] = w[v];
+      }
+    }
+    cols[threadIdx.x] = j;
+  }
+  __syncthreads();
   const uint32_t ngroups = (cnt + 15) / 16;
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t li = lane & 15, lg = lane >> 4;
-  const uint32_t r0 = ks * pr.kslice_rows;
-  const uint32_t r1 = min(r0 + pr.kslice_rows, pr.dstage);
-  const uint32_t Q = pr.kslice_rows / 4;
-  const uint32_t w0 = r0 + wave * Q;
-  const uint32_t w1 = min(w0 + Q, r1);
+  const uint32_t r0 = ks * up.kslice_rows;
+  const uint32_t r1 = min(r0 + up.kslice_rows, pr.dstage);
   const T* A = reinterpret_cast<const T*>(pr.A);
-  const T* col[4];
-  bool val[4];
-#pragma unroll
-  for (int G = 0; G < 4; ++G) {
-    const uint32_t slot = G * 16 + li;
-    val[G] = slot < cnt;
-    col[G] = A + (uint64_t)(val[G] ? pr.list[(uint64_t)tile * kBTile + slot] : 0u) * pr.ld;
-  }
   f32x4_t acc[4];
 #pragma unroll
   for (int G = 0; G < 4; ++G) acc[G] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-  const uint32_t d = pr.d;
-  if constexpr (Elem<T>::VEC == 8) {
-    const uint16_t* xh = pr.DXh + (uint64_t)li * pr.dstage;
-    const uint16_t* xl = pr.DXl + (uint64_t)li * pr.dstage;
-    // two 32-row steps per trip: 64 rows (lengths are multiples of 64)
-    for (uint32_t kb = w0; kb < w1; kb += 64) {
-      const uint32_t ra = kb + 8 * lg, rb = kb + 32 + 8 * lg;
-      const u32x4 bh0 = *reinterpret_cast<const u32x4*>(xh + ra);
-      const u32x4 bl0 = *reinterpret_cast<const u32x4*>(xl + ra);
-      const u32x4 bh1 = *reinterpret_cast<const u32x4*>(xh + rb);
-      const u32x4 bl1 = *reinterpret_cast<const u32x4*>(xl + rb);
-      u32x4 a0[4], a1[4];
+  Stage st;
+  if (r0 < r1) st.load(A, pr.ld, cols, r0, pr.d);
+  for (uint32_t row0 = r0; row0 < r1; row0 += RB) {
+    st.store(lds);
+    __syncthreads();
+    if (row0 + RB < r1) st.load(A, pr.ld, cols, row0 + RB, pr.d);
+    if constexpr (BF) {
+      const uint16_t* xh = pr.DXh + (uint64_t)li * pr.dstage + row0;
+      const uint16_t* xl = pr.DXl + (uint64_t)li * pr.dstage + row0;
 #pragma unroll
-      for (int G = 0; G < 4; ++G) {
-        if ((uint32_t)G < ngroups) {
-          a0[G] = val[G] ? load_a16<T>(col[G], ra, d) : u32x4{0u, 0u, 0u, 0u};
-          a1[G] = val[G] ? load_a16<T>(col[G], rb, d) : u32x4{0u, 0u, 0u, 0u};
+      for (uint32_t s = 0; s < 2; ++s) {
+        const uint32_t rr = wave * 64 + s * 32 + 8 * lg;  // row within the round
+        const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4*>(xh + rr));
+        const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4*>(xl + rr));
+#pragma unroll
+        for (uint32_t G = 0; G < 4; ++G) {
+          if (G < ngroups) {
+            const bf16x8_t a = __builtin_bit_cast(
+                bf16x8_t, *reinterpret_cast<const u32x4*>(lds + (G * 16 + li) * Stage::kPitch + rr * 2));
+            acc[G] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh, acc[G], 0, 0, 0);
+            acc[G] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl, acc[G], 0, 0, 0);
+          }
         }
       }
+    } else {
+      const float* xs = pr.DX + (uint64_t)li * pr.dstage + row0;
 #pragma unroll
-      for (int G = 0; G < 4; ++G) {
-        if ((uint32_t)G < ngroups) {
-          acc[G] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a0[G]),
-                                                           __builtin_bit_cast(bf16x8_t, bh0), acc[G], 0, 0, 0);
-          acc[G] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a0[G]),
-                                                           __builtin_bit_cast(bf16x8_t, bl0), acc[G], 0, 0, 0);
-          acc[G] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a1[G]),
-                                                           __builtin_bit_cast(bf16x8_t, bh1), acc[G], 0, 0, 0);
-          acc[G] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a1[G]),
-                                                           __builtin_bit_cast(bf16x8_t, bl1), acc[G], 0, 0, 0);
-        }
-      }
-    }
-  } else {
-    const float* xs = pr.DX + (uint64_t)li * pr.dstage;
-    // four 16-row steps per trip: 64 rows
-    for (uint32_t kb = w0; kb < w1; kb += 64) {
-      u32x4 b[4];
+      for (uint32_t s = 0; s < 2; ++s) {
+        const uint32_t rr = wave * 32 + s * 16 + 4 * lg;
+        const u32x4 b = *reinterpret_cast<const u32x4*>(xs + rr);
 #pragma unroll
-      for (int s = 0; s < 4; ++s) b[s] = *reinterpret_cast<const u32x4*>(xs + kb + 16 * s + 4 * lg);
-      u32x4 a[4][4];
-#pragma unroll
-      for (int G = 0; G < 4; ++G) {
-        if ((uint32_t)G < ngroups) {
-#pragma unroll
-          for (int s = 0; s < 4; ++s)
-            a[G][s] = val[G] ? load_a16<T>(col[G], kb + 16 * s + 4 * lg, d) : u32x4{0u, 0u, 0u, 0u};
-        }
-      }
-#pragma unroll
-      for (int G = 0; G < 4; ++G) {
-        if ((uint32_t)G < ngroups) {
-#pragma unroll
-          for (int s = 0; s < 4; ++s) {
+        for (uint32_t G = 0; G < 4; ++G) {
+          if (G < ngroups) {
+            const u32x4 a = *reinterpret_cast<const u32x4*>(lds + (G * 16 + li) * Stage::kPitch + rr * 4);
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-              acc[G] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a[G][s][u]),
-                                                            __uint_as_float(b[s][u]), acc[G], 0, 0, 0);
+              acc[G] = __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a[u]), __uint_as_float(b[u]),
+                                                            acc[G], 0, 0, 0);
           }
         }
       }
     }
+    __syncthreads();
   }
-  // cross-wave sum in LDS, then one coalesced write of the tile's partials
+  // cross-wave sum in LDS (reusing the tile buffer), one coalesced write
+  float* red = reinterpret_cast<float*>(lds);  // [4][64][17]
+  static_assert(4 * 64 * 17 * 4 <= Stage::kBytes, "reduction buffer");
 #pragma unroll
   for (int G = 0; G < 4; ++G) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) red[wave][G * 16 + 4 * lg + e][li] = acc[G][e];
+    for (int e = 0; e < 4; ++e) red[(wave * 64 + G * 16 + 4 * lg + e) * 17 + li] = acc[G][e];
   }
   __syncthreads();
-  float* out = pr.P1 + ((uint64_t)ks * pr.ntiles * kBTile + (uint64_t)tile * kBTile) * kBT;
+  float* out = pr.P1 + ((uint64_t)ks * pr.ntiles * kBTile + s0) * kBT;
   for (uint32_t i = threadIdx.x; i < cnt * kBT; i += 256) {
     const uint32_t slot = i / kBT, k = i % kBT;
-    out[i] = (red[0][slot][k] + red[1][slot][k]) + (red[2][slot][k] + red[3][slot][k]);
+    out[i] = (red[slot * 17 + k] + red[(64 + slot) * 17 + k]) +
+             (red[(128 + slot) * 17 + k] + red[(192 + slot) * 17 + k]);
   }
 }
 
 // ---------------------------------------------------------------------------
 // Contraction 2 (previousAMM_kernel2 / proposedAMM_kernel2 for 16 trials):
-// P2[ch][k][rows] = sum_{slots of chunk ch} a_slot[rows] t_slot,k with
+// P2[ch][k][rows] = sum_{s in chunk ch} a_{col(s)}[rows] t_{s,k} with
 //   t = W (sum_ks P1[ks] - alpha_j s_k)       (s_k = 0 unless proposed)
-// staged per tile in LDS.  Workgroup = (row block, column chunk); wave w
-// owns `passes` runs of 64 (FP32) or 128 (BF16) rows; MFMA
-// v_mfma_f32_16x16x4_f32 with K = 4 columns: A-operand lane (i, g) holds
-// column 4q + g at row base + V i + u (u-th element of its 16-byte load),
-// B-operand lane (i, g) holds t of column 4q + g, trial i; accumulator u
-// lane (i, g) element e is row base + V (4g + e) + u, trial i.  The proposed
-// scalar sum_j alpha_j t_j of the chunk goes to P2s (row block 0).
+// Chunk ch = compacted slots [ch cpc, (ch + 1) cpc) of the union (chunks
+// beyond |U| exit).  Workgroup = (row block of 256 PASSES rows, chunk).  Per
+// round: C columns of the chunk staged in LDS (contiguous runs of 512 / 1024
+// bytes per column) and their t in LDS.
+//   BF16: C = 32 = one v_mfma_f32_16x16x32_bf16 K step; A-operand lane
+//     (i, g) = row i, columns 8g..8g+7 read transposed from LDS; t = hi + lo
+//     (two MFMAs); accumulator lane (i, g) element e = row 4g + e, trial i.
+//   FP32: C = 16, v_mfma_f32_16x16x4_f32 on 4 columns: A-operand lane (i, g)
+//     = column 4q + g, rows 4i + u (element u of a 16-byte LDS read);
+//     accumulator u lane (i, g) element e = row 16g + 4e + u, trial i.
+// The proposed scalar sum_j alpha_j t_j of the chunk goes to P2s (row block
+// 0 only).
 // ---------------------------------------------------------------------------
-constexpr uint32_t kMaxPasses = 2;
-
 template <typename T, uint32_t PASSES>
 __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, const uint32_t* off,
                                                uint32_t np, uint32_t proposed_any) {
-  constexpr uint32_t V = Elem<T>::VEC;
-  constexpr uint32_t PR = 16 * V;  // rows per pass
-  __shared__ float tl[kBTile][kBT];
-  __shared__ uint32_t cl[kBTile];
+  constexpr bool BF = Elem<T>::VEC == 8;
+  constexpr uint32_t C = BF ? 32 : 16;
+  constexpr uint32_t RB = 256 * PASSES;
+  constexpr uint32_t WR = 64 * PASSES;  // rows per wave
+  using Stage = TileStage<T, C, RB>;
+  __shared__ __align__(16) char lds[Stage::kBytes];
+  __shared__ uint32_t cols[C];
+  __shared__ __align__(16) uint16_t th[kBT][C];  // BF16 t, high part [trial][column]
+  __shared__ __align__(16) uint16_t tlo[kBT][C];
+  __shared__ float tf[C][kBT];                   // FP32 t [column][trial]
   __shared__ double cred[256];
   const uint32_t p = find_problem(off, np, blockIdx.x);
   const BProb& pr = P[p];
   const uint32_t local = blockIdx.x - off[p];
-  const uint32_t rb = local / pr.chunks, ch = local % pr.chunks;
+  // chunk-major order: active chunks form one contiguous range of blocks
+  const uint32_t ch = local / pr.rblocks, rb = local % pr.rblocks;
+  const UPlan up = union_plan(pr, pr.tpre[pr.ntiles]);
+  const uint32_t U = up.U;
+  if (ch >= up.active) return;
+  const uint32_t sb = ch * up.cpc;
+  const uint32_t se = min(sb + up.cpc, U);
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t li = lane & 15, lg = lane >> 4;
-  const uint32_t t0 = ch * pr.tiles_per_chunk;
-  const uint32_t t1 = min(t0 + pr.tiles_per_chunk, pr.ntiles);
-  const uint32_t wrow = rb * 4 * PASSES * PR + wave * PASSES * PR;
+  const uint32_t R0 = rb * RB;
   const T* A = reinterpret_cast<const T*>(pr.A);
   const uint32_t d = pr.d;
-  // per-trial shift s_k (proposed trials only)
-  const uint32_t kk = threadIdx.x & 15;
+  const uint32_t kk = threadIdx.x & 15;  // trial of this thread's t entries
   double sk = 0.0;
   if (proposed_any && pr.bt[kk].mode == kModeAmmProposed) sk = pr.bt[kk].s;
   double corr = 0.0;
-  f32x4_t acc[PASSES][V];
-#pragma unroll
-  for (uint32_t m = 0; m < PASSES; ++m)
-#pragma unroll
-    for (uint32_t u = 0; u < V; ++u) acc[m][u] = f32x4_t{0.f, 0.f, 0.f, 0.f};
   const uint64_t slab = (uint64_t)pr.ntiles * kBTile * kBT;
-  for (uint32_t tile = t0; tile < t1; ++tile) {
-    const uint32_t cnt = pr.tcount[tile];
-    if (cnt == 0) continue;
-    __syncthreads();  // the previous tile's readers are done with tl / cl
-    for (uint32_t i = threadIdx.x; i < cnt * kBT; i += 256) {
-      const uint32_t slot = i >> 4;
-      const uint64_t e = ((uint64_t)tile * kBTile + slot) * kBT + (i & 15);
-      double acc1 = 0.0;
-      for (uint32_t ks = 0; ks < pr.kslices; ++ks) acc1 += (double)pr.P1[ks * slab + e];
-      const uint32_t j = pr.list[(uint64_t)tile * kBTile + slot];
-      const double aj = sk != 0.0 ? (double)pr.alpha[j] : 0.0;
-      const float tt = (float)((double)pr.W[e] * (acc1 - aj * sk));
-      tl[slot][i & 15] = tt;
-      if (proposed_any && pr.alpha) corr += (double)pr.alpha[j] * (double)tt;
-      if ((i & 15) == 0) cl[slot] = j;
-    }
+  constexpr uint32_t NACC = BF ? 4 * PASSES : 4 * PASSES;
+  f32x4_t acc[NACC];
+#pragma unroll
+  for (uint32_t m = 0; m < NACC; ++m) acc[m] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  Stage st;
+  auto set_cols = [&](uint32_t s) {
+    __syncthreads();  // readers of the previous cols are done
+    if (threadIdx.x < C) cols[threadIdx.x] = (s + threadIdx.x < se) ? pr.colc[s + threadIdx.x] : 0xFFFFFFFFu;
     __syncthreads();
-    for (uint32_t q = 0; q < cnt; q += 4) {
-      const uint32_t slot = q + lg;
-      const bool v = slot < cnt;
-      const float b = v ? tl[slot][li] : 0.0f;
-      const T* col = A + (uint64_t)(v ? cl[slot] : 0u) * pr.ld;
-      u32x4 a[PASSES];
+  };
+  set_cols(sb);
+  st.load(A, pr.ld, cols, R0, d);
+  for (uint32_t s = sb; s < se; s += C) {
+    // t of this round's columns
+    for (uint32_t i = threadIdx.x; i < C * kBT; i += 256) {
+      const uint32_t c = i >> 4, k = i & 15;
+      float tt = 0.0f;
+      if (s + c < se) {
+        const uint64_t e = (uint64_t)(s + c) * kBT + k;
+        double a1 = 0.0;
+        for (uint32_t q = 0; q < up.kslices; ++q) a1 += (double)pr.P1[q * slab + e];
+        const uint32_t j = cols[c];
+        const double aj = (sk != 0.0) ? (double)pr.alpha[j] : 0.0;
+        tt = (float)((double)pr.Wc[e] * (a1 - aj * sk));
+        if (proposed_any && pr.alpha) corr += (double)pr.alpha[j] * (double)tt;
+      }
+      if constexpr (BF) {
+        const uint16_t h = bf16_rne(tt);
+        th[k][c] = h;
+        tlo[k][c] = bf16_rne(tt - bf16_bits_to_f32(h));
+      } else {
+        tf[c][k] = tt;
+      }
+    }
+    st.store(lds);
+    __syncthreads();
+    if (s + C < se) {
+      set_cols(s + C);
+      st.load(A, pr.ld, cols, R0, d);
+    }
+    if constexpr (BF) {
+      const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4*>(&th[li][8 * lg]));
+      const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4*>(&tlo[li][8 * lg]));
+      const uint16_t* l16 = reinterpret_cast<const uint16_t*>(lds);
 #pragma unroll
-      for (uint32_t m = 0; m < PASSES; ++m)
-        a[m] = v ? load_a16<T>(col, wrow + m * PR + V * li, d) : u32x4{0u, 0u, 0u, 0u};
+      for (uint32_t b = 0; b < 4 * PASSES; ++b) {
+        const uint32_t row = wave * WR + b * 16 + li;
+        uint32_t w4[4];
 #pragma unroll
-      for (uint32_t m = 0; m < PASSES; ++m) {
+        for (uint32_t e = 0; e < 4; ++e) {
+          const uint32_t c0 = 8 * lg + 2 * e;
+          const uint32_t lo16 = l16[(c0 * Stage::kPitch) / 2 + row];
+          const uint32_t hi16 = l16[((c0 + 1) * Stage::kPitch) / 2 + row];
+          w4[e] = lo16 | (hi16 << 16);
+        }
+        const bf16x8_t a = __builtin_bit_cast(bf16x8_t, u32x4{w4[0], w4[1], w4[2], w4[3]});
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh, acc[b], 0, 0, 0);
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl, acc[b], 0, 0, 0);
+      }
+    } else {
 #pragma unroll
-        for (uint32_t u = 0; u < V; ++u) {
-          float av;
-          if constexpr (V == 4) av = __uint_as_float(a[m][u]);
-          else av = bf16_bits_to_f32((a[m][u >> 1] >> (16 * (u & 1))) & 0xFFFFu);
-          acc[m][u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b, acc[m][u], 0, 0, 0);
+      for (uint32_t q = 0; q < C / 4; ++q) {
+        const float bv = tf[4 * q + lg][li];
+#pragma unroll
+        for (uint32_t m = 0; m < PASSES; ++m) {
+          const u32x4 a = *reinterpret_cast<const u32x4*>(lds + (4 * q + lg) * Stage::kPitch +
+                                                           (wave * WR + m * 64 + 4 * li) * 4);
+#pragma unroll
+          for (uint32_t u = 0; u < 4; ++u)
+            acc[m * 4 + u] =
+                __builtin_amdgcn_mfma_f32_16x16x4f32(__uint_as_float(a[u]), bv, acc[m * 4 + u], 0, 0, 0);
         }
       }
     }
+    __syncthreads();
   }
-  // partial rows of trial li: row base + V (4 lg + e) + u
   float* out = pr.P2 + ((uint64_t)ch * kBT + li) * pr.p2pitch;
+  if constexpr (BF) {
 #pragma unroll
-  for (uint32_t m = 0; m < PASSES; ++m) {
+    for (uint32_t b = 0; b < 4 * PASSES; ++b) {
+      const uint32_t row = R0 + wave * WR + b * 16 + 4 * lg;
+      *reinterpret_cast<float4*>(out + row) = make_float4(acc[b][0], acc[b][1], acc[b][2], acc[b][3]);
+    }
+  } else {
 #pragma unroll
-    for (uint32_t e = 0; e < 4; ++e) {
-      const uint32_t rbase = wrow + m * PR + V * (4 * lg + e);
+    for (uint32_t m = 0; m < PASSES; ++m) {
 #pragma unroll
-      for (uint32_t u = 0; u < V; u += 4)
-        *reinterpret_cast<float4*>(out + rbase + u) =
-            make_float4(acc[m][u][e], acc[m][u + 1][e], acc[m][u + 2][e], acc[m][u + 3][e]);
+      for (uint32_t e = 0; e < 4; ++e) {
+        const uint32_t row = R0 + wave * WR + m * 64 + 16 * lg + 4 * e;
+        *reinterpret_cast<float4*>(out + row) =
+            make_float4(acc[m * 4 + 0][e], acc[m * 4 + 1][e], acc[m * 4 + 2][e], acc[m * 4 + 3][e]);
+      }
     }
   }
   if (proposed_any && rb == 0) {
-    // thread t accumulated trial t % 16: sum the 16 threads of each trial
     cred[threadIdx.x] = corr;
     __syncthreads();
     if (threadIdx.x < kBT) {
@@ -619,8 +804,9 @@ __global__ __launch_bounds__(256) void k_bfinish(const BProb* __restrict__ P, co
   const uint32_t md = k < r.ntr ? pr.bt[k].mode : kModeExactNoMemo;
   const bool memo_base = is_amm(md);
   const bool prop = md == kModeAmmProposed;
+  const uint32_t active = union_plan(pr, pr.tpre[pr.ntiles]).active;
   double a[4] = {0.0, 0.0, 0.0, 0.0};
-  for (uint32_t ch = 0; ch < pr.chunks; ++ch) {
+  for (uint32_t ch = 0; ch < active; ++ch) {
     const float4 q = *reinterpret_cast<const float4*>(pr.P2 + ((uint64_t)ch * kBT + k) * pr.p2pitch + row);
     a[0] += q.x;
     a[1] += q.y;
@@ -631,7 +817,7 @@ __global__ __launch_bounds__(256) void k_bfinish(const BProb* __restrict__ P, co
   if (prop) {
     sx = pr.bt[k].s;
     qx = pr.bt[k].q;
-    for (uint32_t ch = 0; ch < pr.chunks; ++ch) corr += pr.P2s[(uint64_t)ch * kBT + k];
+    for (uint32_t ch = 0; ch < active; ++ch) corr += pr.P2s[(uint64_t)ch * kBT + k];
   }
   float4 ym4 = make_float4(0.f, 0.f, 0.f, 0.f);
   if (memo_base) ym4 = *reinterpret_cast<const float4*>(pr.YM + o);
@@ -697,6 +883,11 @@ hipError_t launch_bsample(const BProb* P, BGroup g, BRun r, uint32_t it, hipStre
 
 hipError_t launch_bprep(const BProb* P, BGroup g, BRun r, hipStream_t s) {
   if (g.total) k_bprep<<<g.total, 256, 0, s>>>(P, g.off, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_bscan(const BProb* P, uint32_t np, hipStream_t s) {
+  k_bscan<<<np, 1024, 0, s>>>(P);
   return hipGetLastError();
 }
 

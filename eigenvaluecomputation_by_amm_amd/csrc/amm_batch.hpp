@@ -21,7 +21,7 @@ namespace amm {
 constexpr uint32_t kBT = 16;      // trials per right-hand-side block
 constexpr uint32_t kBTile = 64;   // columns per compaction tile
 constexpr uint32_t kBVecRows = 64;  // rows per block of the vector kernels
-constexpr uint32_t kBStageAlign = 128;  // staged vectors are padded to this many rows
+constexpr uint32_t kBStageAlign = 256;  // staged vectors are padded to this many rows
 
 // Per (problem, trial) scalar state.
 struct alignas(16) BTrial {
@@ -39,16 +39,15 @@ struct alignas(16) BProb {
   uint64_t ld;
   uint64_t seed;        // Philox key
   uint32_t d, n;
-  uint32_t dstage;      // rows of each staged vector (multiple of 128, >= d)
+  uint32_t dstage;      // rows of each staged vector (multiple of 256, >= d)
   uint32_t c;           // samples per trial and iteration
   int32_t dtype;        // 0 FP32, 1 BF16
   uint32_t ntiles;      // ceil(n / 64)
-  uint32_t kslices;     // contraction-1 row slices
-  uint32_t kslice_rows; // rows per slice (multiple of 256)
-  uint32_t passes;      // contraction-2 passes per wave (64 FP32 / 128 BF16 rows each)
-  uint32_t rblocks;     // contraction-2 row blocks (4 waves x passes each)
-  uint32_t chunks;      // contraction-2 column chunks
-  uint32_t tiles_per_chunk;
+  uint32_t kslices_max; // contraction-1 row slices the partial buffer holds (<= 16)
+  uint32_t dot_blocks;  // contraction-1 workgroups to aim for (sets the slices per |U|)
+  uint32_t passes;      // contraction-2 row block = 256 passes rows
+  uint32_t rblocks;     // contraction-2 row blocks
+  uint32_t chunks;      // contraction-2 column chunks of the union (at most)
   uint32_t p2pitch;     // rows of one P2 partial (rblocks * rows per block)
   uint32_t nvblk;       // vector-kernel blocks = dstage / 64
   float* X;             // [16][dstage] iterate
@@ -62,7 +61,10 @@ struct alignas(16) BProb {
   float* W;             // [ntiles*64][16] weights of the compacted columns
   uint32_t* list;       // [ntiles*64] compacted column indices, per tile
   uint32_t* tcount;     // [ntiles] columns present per tile
-  float* P1;            // [kslices][ntiles*64][16]
+  uint32_t* tpre;       // [ntiles + 1] prefix of tcount; tpre[ntiles] = |U|
+  uint32_t* colc;       // [n] union columns in compacted order
+  float* Wc;            // [n][16] their weights
+  float* P1;            // [kslices_max][ntiles*64][16], compacted slots
   float* P2;            // [chunks][16][p2pitch]
   double* P2s;          // [chunks][16] proposed: sum alpha_j t_j per chunk
   double* BP;           // [nvblk][16][3] block partials of y^T x, y^T y, y^T v*
@@ -98,6 +100,31 @@ struct BRun {
 
 enum BTailOp : uint32_t { kBTailFinish = 0, kBTailScale = 1 };
 
+// Launch plan of one product from the size of its union (every kernel
+// derives it identically from |U| = tpre[ntiles]):
+//   contraction 1: ceil(|U| / 64) virtual tiles x kslices row slices of
+//     kslice_rows rows, kslices ~ dot_blocks / tiles (at most kslices_max);
+//   contraction 2: the union in `active` chunks of cpc slots (multiple of
+//     64), at most `chunks` of them.
+struct UPlan {
+  uint32_t U, vtiles, kslices, kslice_rows, cpc, active;
+};
+
+__host__ __device__ inline UPlan union_plan(const BProb& p, uint32_t U) {
+  UPlan q;
+  q.U = U;
+  q.vtiles = (U + 63) / 64;
+  uint32_t ks = q.vtiles ? (p.dot_blocks + q.vtiles - 1) / q.vtiles : 1;
+  ks = ks < 1 ? 1 : (ks > p.kslices_max ? p.kslices_max : ks);
+  const uint32_t rows = (p.dstage + ks - 1) / ks;
+  q.kslice_rows = (rows + 255) / 256 * 256;
+  q.kslices = (p.dstage + q.kslice_rows - 1) / q.kslice_rows;
+  const uint32_t per = (U + p.chunks - 1) / p.chunks;
+  q.cpc = per < 64 ? 64 : (per + 63) / 64 * 64;
+  q.active = (U + q.cpc - 1) / q.cpc;
+  return q;
+}
+
 // grouped launch of one kernel: per-problem item offsets [np + 1]
 struct BGroup {
   const uint32_t* off;
@@ -110,6 +137,7 @@ hipError_t launch_btail(const BProb* P, BRun r, uint32_t op, uint32_t it, hipStr
 hipError_t launch_bnorm(const BProb* P, BGroup g, BRun r, hipStream_t s);
 hipError_t launch_bsample(const BProb* P, BGroup g, BRun r, uint32_t it, hipStream_t s);
 hipError_t launch_bprep(const BProb* P, BGroup g, BRun r, hipStream_t s);
+hipError_t launch_bscan(const BProb* P, uint32_t np, hipStream_t s);
 // dtype-uniform groups: all problems of a group share the element type
 hipError_t launch_bdot(int dtype, const BProb* P, BGroup g, uint32_t np, hipStream_t s);
 // passes: 1 or 2 (a group shares it)

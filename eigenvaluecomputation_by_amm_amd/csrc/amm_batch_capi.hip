@@ -57,7 +57,7 @@ struct Pool {
 };
 
 struct Tuning {
-  uint32_t kslice_rows, passes, chunks;
+  uint32_t dot_blocks, passes, chunks;
 };
 
 // per-problem buffers of the single-matrix setup kernels
@@ -74,7 +74,8 @@ struct SetupBufs {
   uint64_t dpad8;
 };
 
-uint32_t pass_rows(int dtype) { return dtype == AMM_DTYPE_BF16 ? 128u : 64u; }
+// rows of one contraction-2 row block
+uint32_t block_rows(uint32_t passes) { return 256u * passes; }
 
 }  // namespace
 
@@ -135,17 +136,18 @@ void drop_graph(amm_batch* b) {
 }
 
 // Default shapes: each problem gets a share of ~kTargetItems workgroups in
-// both contractions proportional to its column tiles x rows.
-Tuning default_tuning_one(const BProb& p, double share) {
+// both contractions proportional to its columns x rows; the slices of
+// contraction 1 and the chunk width of contraction 2 then follow each
+// product's union size on the device (union_plan).
+Tuning default_tuning_one(const BProb& p, double share, double ks_mult = 1.0, double ch_mult = 1.0,
+                          uint32_t passes = 0) {
   const double want = std::max(1.0, kTargetItems * share);
-  const uint32_t max_slices = (uint32_t)std::min<uint64_t>(kMaxSlices, cdiv(p.dstage, 256));
-  const uint32_t ks = (uint32_t)std::min<double>(max_slices, std::max(1.0, std::ceil(want / p.ntiles)));
-  const uint32_t ksr = (uint32_t)rup(cdiv(p.dstage, ks), 256);
-  const uint32_t passes = 2;
-  const uint32_t rbrows = 4 * passes * pass_rows(p.dtype);
-  const uint32_t rblocks = (uint32_t)cdiv(p.d, rbrows);
-  const uint32_t chunks = (uint32_t)std::min<double>(p.ntiles, std::max(1.0, std::round(want / rblocks)));
-  return Tuning{ksr, passes, chunks};
+  if (!passes) passes = p.d >= 4096 ? 2u : 1u;
+  const uint32_t rblocks = (uint32_t)cdiv(p.d, block_rows(passes));
+  const uint32_t dot_blocks = (uint32_t)std::max(1.0, std::round(want * ks_mult));
+  const double ch = std::max(1.0, std::round(want / rblocks * ch_mult));
+  const uint32_t chunks = (uint32_t)std::min<double>((double)cdiv(p.n, 64), ch);
+  return Tuning{dot_blocks, passes, chunks};
 }
 
 void default_tuning(amm_batch* b) {
@@ -159,11 +161,9 @@ void default_tuning(amm_batch* b) {
 
 int check_tuning(const BProb& p, const Tuning& t) {
   if (t.passes < 1 || t.passes > 2) return fail(AMM_E_INVALID, "passes must be 1 or 2");
-  if (t.kslice_rows < 256 || t.kslice_rows % 256)
-    return fail(AMM_E_INVALID, "kslice_rows must be a positive multiple of 256");
-  if (cdiv(p.dstage, t.kslice_rows) > kMaxSlices)
-    return fail(AMM_E_INVALID, "kslice_rows gives more than 16 row slices");
-  if (t.chunks < 1) return fail(AMM_E_INVALID, "chunks must be >= 1");
+  if (t.dot_blocks < 1) return fail(AMM_E_INVALID, "dot_blocks must be >= 1");
+  if (t.chunks < 1 || t.chunks > cdiv(p.n, 64))
+    return fail(AMM_E_INVALID, "chunks must be in [1, ceil(n / 64)]");
   return AMM_OK;
 }
 
@@ -177,15 +177,13 @@ int upload_plan(amm_batch* b, int kernel) {
   for (uint32_t i = 0; i < np; ++i) {
     BProb& p = b->hp[i];
     const Tuning& t = b->tune[i];
-    p.kslice_rows = t.kslice_rows;
-    p.kslices = (uint32_t)cdiv(p.dstage, t.kslice_rows);
+    p.kslices_max = (uint32_t)std::min<uint64_t>(kMaxSlices, p.dstage / 256);
+    p.dot_blocks = t.dot_blocks;
     p.passes = t.passes;
-    const uint32_t rbrows = 4 * t.passes * pass_rows(p.dtype);
-    p.rblocks = (uint32_t)cdiv(p.d, rbrows);
-    p.p2pitch = p.rblocks * rbrows;
-    p.tiles_per_chunk = (uint32_t)cdiv(p.ntiles, std::max<uint32_t>(1, std::min(t.chunks, p.ntiles)));
-    p.chunks = (uint32_t)cdiv(p.ntiles, p.tiles_per_chunk);
-    o1[i] = pl.take<float>((size_t)p.kslices * p.ntiles * amm::kBTile * amm::kBT);
+    p.rblocks = (uint32_t)cdiv(p.d, block_rows(t.passes));
+    p.p2pitch = p.rblocks * block_rows(t.passes);
+    p.chunks = t.chunks;
+    o1[i] = pl.take<float>((size_t)p.kslices_max * p.ntiles * amm::kBTile * amm::kBT);
     o2[i] = pl.take<float>((size_t)p.chunks * amm::kBT * p.p2pitch);
     o2s[i] = pl.take<double>((size_t)p.chunks * amm::kBT);
   }
@@ -224,7 +222,7 @@ int upload_plan(amm_batch* b, int kernel) {
   });
   ok &= add_group(b->g_prep, [](const BProb& p) { return (uint32_t)cdiv(p.ntiles, 4); });
   for (int dt = 0; dt < 2; ++dt)
-    ok &= add_group(b->g_dot[dt], [dt](const BProb& p) { return p.dtype == dt ? p.ntiles * p.kslices : 0u; });
+    ok &= add_group(b->g_dot[dt], [dt](const BProb& p) { return p.dtype == dt ? p.ntiles * p.kslices_max : 0u; });
   for (int dt = 0; dt < 2; ++dt)
     for (int pa = 0; pa < 2; ++pa)
       ok &= add_group(b->g_axpy[dt][pa], [dt, pa](const BProb& p) {
@@ -366,6 +364,7 @@ int enq_iteration(amm_batch* b, const amm::BRun& r, int K, uint32_t it) {
   hipStream_t s = b->stream;
   if (K != AMM_KERNEL_EXACT) BHIPCHK(amm::launch_bsample(P, b->g_sample, r, it, s));
   BHIPCHK(amm::launch_bprep(P, b->g_prep, r, s));
+  BHIPCHK(amm::launch_bscan(P, b->np, s));
   for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bdot(dt, P, b->g_dot[dt], b->np, s));
   const uint32_t prop = (K == AMM_KERNEL_PROPOSED || K == AMM_KERNEL_PROPOSED_V2) ? 1u : 0u;
   for (int dt = 0; dt < 2; ++dt)
@@ -540,8 +539,8 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
   // static per-problem layout
   Pool pl;
   struct Off {
-    size_t X, Y, XM, YM, DX, DXh, DXl, cnt, W, list, tcount, BP, BQ, bt, alpha, nrm2, cnrm2, cdf0,
-        cdf1, z, v, st, eigvec;
+    size_t X, Y, XM, YM, DX, DXh, DXl, cnt, W, list, tcount, tpre, colc, Wc, BP, BQ, bt, alpha, nrm2,
+        cnrm2, cdf0, cdf1, z, v, st, eigvec;
   };
   std::vector<Off> o(count);
   b->hp.resize(count);
@@ -577,6 +576,9 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     f.W = pl.take<float>(cols * amm::kBT);
     f.list = pl.take<uint32_t>(cols);
     f.tcount = pl.take<uint32_t>(p.ntiles);
+    f.tpre = pl.take<uint32_t>(p.ntiles + 1);
+    f.colc = pl.take<uint32_t>(q.n);
+    f.Wc = pl.take<float>(q.n * amm::kBT);
     f.BP = pl.take<double>((size_t)p.nvblk * amm::kBT * 3);
     f.BQ = pl.take<double>((size_t)p.nvblk * amm::kBT * 2);
     f.bt = pl.take<BTrial>(amm::kBT);
@@ -612,6 +614,9 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     p.W = reinterpret_cast<float*>(s0 + f.W);
     p.list = reinterpret_cast<uint32_t*>(s0 + f.list);
     p.tcount = reinterpret_cast<uint32_t*>(s0 + f.tcount);
+    p.tpre = reinterpret_cast<uint32_t*>(s0 + f.tpre);
+    p.colc = reinterpret_cast<uint32_t*>(s0 + f.colc);
+    p.Wc = reinterpret_cast<float*>(s0 + f.Wc);
     p.BP = reinterpret_cast<double*>(s0 + f.BP);
     p.BQ = reinterpret_cast<double*>(s0 + f.BQ);
     p.bt = reinterpret_cast<BTrial*>(s0 + f.bt);
@@ -734,11 +739,11 @@ int amm_batch_last_run_ms(amm_batch* b, double* ms) {
   return AMM_OK;
 }
 
-int amm_batch_set_tuning(amm_batch* b, uint32_t problem, uint32_t kslice_rows, uint32_t passes,
+int amm_batch_set_tuning(amm_batch* b, uint32_t problem, uint32_t dot_blocks, uint32_t passes,
                          uint32_t chunks) {
   if (!b) return fail(AMM_E_INVALID, "batch is NULL");
   if (problem >= b->np) return fail(AMM_E_INVALID, "problem index out of range");
-  const Tuning t{kslice_rows, passes, chunks};
+  const Tuning t{dot_blocks, passes, chunks};
   int rc;
   if ((rc = check_tuning(b->hp[problem], t))) return rc;
   b->tune[problem] = t;
@@ -747,14 +752,14 @@ int amm_batch_set_tuning(amm_batch* b, uint32_t problem, uint32_t kslice_rows, u
   return AMM_OK;
 }
 
-int amm_batch_get_tuning(amm_batch* b, uint32_t problem, uint32_t* kslice_rows, uint32_t* passes,
+int amm_batch_get_tuning(amm_batch* b, uint32_t problem, uint32_t* dot_blocks, uint32_t* passes,
                          uint32_t* chunks) {
   if (!b) return fail(AMM_E_INVALID, "batch is NULL");
   if (problem >= b->np) return fail(AMM_E_INVALID, "problem index out of range");
   const Tuning& t = b->tune[problem];
-  if (kslice_rows) *kslice_rows = t.kslice_rows;
+  if (dot_blocks) *dot_blocks = t.dot_blocks;
   if (passes) *passes = t.passes;
-  if (chunks) *chunks = std::min(t.chunks, b->hp[problem].ntiles);
+  if (chunks) *chunks = t.chunks;
   return AMM_OK;
 }
 
@@ -819,38 +824,29 @@ int amm_batch_autotune(amm_batch* b, uint32_t iters, double* best_ms) {
   int rc;
   if ((rc = prepare(b, &p))) return rc;
   drop_graph(b);
-  // candidate rules: global multipliers of the default workgroup targets
+  // candidate rules: multipliers of the default workgroup targets, passes
   struct Cand {
     double ks_mult, ch_mult;
     uint32_t passes;
   };
-  const Cand cands[] = {{1, 1, 2}, {0.5, 0.5, 2}, {2, 2, 2}, {4, 4, 2}, {1, 1, 1}, {2, 2, 1}, {0.25, 0.25, 2}};
-  const std::vector<Tuning> base = b->tune;
+  const Cand cands[] = {{1, 1, 0},   {0.5, 0.5, 0}, {2, 2, 0},    {1, 1, 1},  {1, 1, 2},
+                        {2, 0.5, 0}, {0.5, 2, 0},   {0.25, 0.25, 0}, {4, 4, 0}};
   double tot = 0.0;
   for (const BProb& q : b->hp) tot += (double)q.ntiles * (double)q.d;
   double best = 1e30;
-  std::vector<Tuning> best_t = base;
+  std::vector<Tuning> best_t = b->tune;
   for (const Cand& cd : cands) {
     for (uint32_t i = 0; i < b->np; ++i) {
       const BProb& q = b->hp[i];
       const double share = tot > 0 ? (double)q.ntiles * (double)q.d / tot : 1.0;
-      Tuning t = default_tuning_one(q, share);
-      const uint32_t max_slices = (uint32_t)std::min<uint64_t>(kMaxSlices, cdiv(q.dstage, 256));
-      uint32_t ks = (uint32_t)cdiv(q.dstage, t.kslice_rows);
-      ks = (uint32_t)std::min<double>(max_slices, std::max(1.0, std::round(ks * cd.ks_mult)));
-      t.kslice_rows = (uint32_t)rup(cdiv(q.dstage, ks), 256);
-      t.passes = cd.passes;
-      const uint32_t rbrows = 4 * t.passes * pass_rows(q.dtype);
-      const double want = std::max(1.0, kTargetItems * share);
-      const double ch = std::max(1.0, std::round(want / cdiv(q.d, rbrows) * cd.ch_mult));
-      t.chunks = (uint32_t)std::min<double>(q.ntiles, ch);
-      b->tune[i] = t;
+      b->tune[i] = default_tuning_one(q, share, cd.ks_mult, cd.ch_mult, cd.passes);
     }
     if ((rc = upload_plan(b, AMM_KERNEL_EXACT))) return rc;
     const amm::BRun r = run_args(b, p);
     // all trials exact: every column present
     BHIPCHK(amm::launch_bbegin(b->dP, b->g_vec, r, b->stream));
     BHIPCHK(amm::launch_bprep(b->dP, b->g_prep, r, b->stream));
+    BHIPCHK(amm::launch_bscan(b->dP, b->np, b->stream));
     for (uint32_t w = 0; w < 2; ++w) {  // warm-up, then timed
       BHIPCHK(hipEventRecord(b->ev0, b->stream));
       for (uint32_t k = 0; k < (w ? iters : 1); ++k) {

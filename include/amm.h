@@ -224,12 +224,14 @@ int amm_batch_get_vector(amm_batch *b, uint32_t problem, uint32_t trial, int32_t
 /* Wall time of the last batched run from HIP events around it (ms). */
 int amm_batch_last_run_ms(amm_batch *b, double *ms);
 
-/* Launch shapes of one problem: rows per contraction-1 slice (multiple of
- * 256, at most 16 slices), contraction-2 passes per wave (1 or 2) and
- * column chunks (split of the second contraction). */
-int amm_batch_set_tuning(amm_batch *b, uint32_t problem, uint32_t kslice_rows, uint32_t passes,
+/* Launch shapes of one problem: workgroups the first contraction aims for
+ * (its row slices per product follow from the union size), row block of
+ * the second contraction = 256 * passes rows (passes 1 or 2), and the
+ * number of chunks the union is split into for the second contraction
+ * (1 .. ceil(n / 64)). */
+int amm_batch_set_tuning(amm_batch *b, uint32_t problem, uint32_t dot_blocks, uint32_t passes,
                          uint32_t chunks);
-int amm_batch_get_tuning(amm_batch *b, uint32_t problem, uint32_t *kslice_rows, uint32_t *passes,
+int amm_batch_get_tuning(amm_batch *b, uint32_t problem, uint32_t *dot_blocks, uint32_t *passes,
                          uint32_t *chunks);
 
 /* Autotuner: times candidate launch shapes of both contractions on the live

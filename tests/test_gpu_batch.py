@@ -179,4 +179,4 @@ def test_batch_invalid_arguments():
         with pytest.raises(AmmError):
             b.power_method(RunParams(kernel="previousAMM", c=10, step=3, sanity=True))
         with pytest.raises(AmmError):
-            b.set_tuning(0, 100, 2, 1)
+            b.set_tuning(0, 100, 3, 1)
