@@ -11,6 +11,8 @@ namespace amm {
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
+constexpr uint32_t kMaxSlicesDev = 16;  // contraction-1 row slices at most
+
 // largest p with off[p] <= b (off is non-decreasing, off[0] = 0, b < off[np])
 __device__ __forceinline__ uint32_t find_problem(const uint32_t* __restrict__ off, uint32_t np,
                                                  uint32_t b) {
@@ -49,10 +51,12 @@ __global__ __launch_bounds__(256) void k_bbegin(const BProb* __restrict__ P, con
   const BProb& pr = P[p];
   const uint32_t blk = blockIdx.x - off[p];
   const uint32_t k = threadIdx.x >> 4;
-  const uint64_t row = (uint64_t)blk * kBVecRows + (threadIdx.x & 15) * 4;
-  const uint64_t o = (uint64_t)k * pr.dstage + row;
-  *reinterpret_cast<float4*>(pr.XM + o) = make_float4(0.f, 0.f, 0.f, 0.f);
-  *reinterpret_cast<float4*>(pr.YM + o) = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (uint32_t sub = 0; sub < kBVecRows / 64; ++sub) {
+    const uint64_t row = (uint64_t)blk * kBVecRows + sub * 64 + (threadIdx.x & 15) * 4;
+    const uint64_t o = (uint64_t)k * pr.dstage + row;
+    *reinterpret_cast<float4*>(pr.XM + o) = make_float4(0.f, 0.f, 0.f, 0.f);
+    *reinterpret_cast<float4*>(pr.YM + o) = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
   if (blk == 0 && threadIdx.x < kBT) {
     BTrial t{};
     t.mode = r.modes_static[0];
@@ -73,29 +77,31 @@ __global__ __launch_bounds__(256) void k_bx0(const BProb* __restrict__ P, const 
   const BProb& pr = P[p];
   const uint32_t blk = blockIdx.x - off[p];
   const uint32_t k = threadIdx.x >> 4;
-  const uint32_t row = blk * kBVecRows + (threadIdx.x & 15) * 4;
-  float g[4] = {0.f, 0.f, 0.f, 0.f};
-  if (k < r.ntr) {
-    const uint32_t k0 = (uint32_t)pr.seed, k1 = (uint32_t)(pr.seed >> 32);
+  const uint32_t k0 = (uint32_t)pr.seed, k1 = (uint32_t)(pr.seed >> 32);
+  double yy = 0.0;
+  for (uint32_t sub = 0; sub < kBVecRows / 64; ++sub) {
+    const uint32_t row = blk * kBVecRows + sub * 64 + (threadIdx.x & 15) * 4;
+    float g[4] = {0.f, 0.f, 0.f, 0.f};
+    if (k < r.ntr) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const uint32_t r0 = row + 2 * h;
-      if (r0 < pr.d) {
-        uint32_t q[4];
-        philox4x32_10(r0 >> 1, 0u, r.trial0 + k, kStreamX0, k0, k1, q);
-        const double u1 = uniform53(q[0], q[1]);
-        const double u2 = uniform53(q[2], q[3]);
-        const double rad = sqrt(-2.0 * log(1.0 - u1));
-        const double th = 6.283185307179586 * u2;
-        g[2 * h] = (float)(rad * cos(th));
-        if (r0 + 1 < pr.d) g[2 * h + 1] = (float)(rad * sin(th));
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t r0 = row + 2 * h;
+        if (r0 < pr.d) {
+          uint32_t q[4];
+          philox4x32_10(r0 >> 1, 0u, r.trial0 + k, kStreamX0, k0, k1, q);
+          const double u1 = uniform53(q[0], q[1]);
+          const double u2 = uniform53(q[2], q[3]);
+          const double rad = sqrt(-2.0 * log(1.0 - u1));
+          const double th = 6.283185307179586 * u2;
+          g[2 * h] = (float)(rad * cos(th));
+          if (r0 + 1 < pr.d) g[2 * h + 1] = (float)(rad * sin(th));
+        }
       }
     }
-  }
-  *reinterpret_cast<float4*>(pr.Y + (uint64_t)k * pr.dstage + row) = make_float4(g[0], g[1], g[2], g[3]);
-  double yy = 0.0;
+    *reinterpret_cast<float4*>(pr.Y + (uint64_t)k * pr.dstage + row) = make_float4(g[0], g[1], g[2], g[3]);
 #pragma unroll
-  for (int e = 0; e < 4; ++e) yy += (double)g[e] * (double)g[e];
+    for (int e = 0; e < 4; ++e) yy += (double)g[e] * (double)g[e];
+  }
   yy = sum16(yy);
   if ((threadIdx.x & 15) == 0) {
     double* b = pr.BP + ((uint64_t)blk * kBT + k) * 3;
@@ -117,6 +123,7 @@ __global__ __launch_bounds__(256) void k_btail(const BProb* __restrict__ P, BRun
   const BProb& pr = P[blockIdx.x];
   const uint32_t k = threadIdx.x >> 4, sub = threadIdx.x & 15;
   double a = 0.0, b = 0.0, c = 0.0;
+#pragma unroll 8
   for (uint32_t blk = sub; blk < pr.nvblk; blk += 16) {
     const double* q = pr.BP + ((uint64_t)blk * kBT + k) * 3;
     a += q[0];
@@ -167,43 +174,45 @@ __global__ __launch_bounds__(256) void k_bnorm(const BProb* __restrict__ P, cons
   const BProb& pr = P[p];
   const uint32_t blk = blockIdx.x - off[p];
   const uint32_t k = threadIdx.x >> 4;
-  const uint32_t row = blk * kBVecRows + (threadIdx.x & 15) * 4;
-  const uint64_t o = (uint64_t)k * pr.dstage + row;
   const BTrial& t = pr.bt[k];
   const double scale = k < r.ntr ? t.scale : 0.0;
   const uint32_t md = k < r.ntr ? t.mode : kModeExactNoMemo;
   const bool memo = is_amm(md);
-  const float4 y4 = *reinterpret_cast<const float4*>(pr.Y + o);
-  float4 m4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (memo) m4 = *reinterpret_cast<const float4*>(pr.XM + o);
-  const float yv[4] = {y4.x, y4.y, y4.z, y4.w};
-  const float mv[4] = {m4.x, m4.y, m4.z, m4.w};
-  float xv[4], dv[4];
   double s = 0.0, q = 0.0;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    xv[e] = (row + e < pr.d) ? (float)(scale * (double)yv[e]) : 0.0f;
-    dv[e] = xv[e] - mv[e];
-    if (md == kModeAmmProposed && row + e < pr.d) {
-      const double dx = (double)xv[e] - (double)mv[e];
-      s += dx;
-      q += (double)pr.z[row + e] * dx;
-    }
-  }
-  *reinterpret_cast<float4*>(pr.X + o) = make_float4(xv[0], xv[1], xv[2], xv[3]);
-  if (pr.dtype == 1) {
-    uint16_t h[4], l[4];
+  for (uint32_t sub = 0; sub < kBVecRows / 64; ++sub) {
+    const uint32_t row = blk * kBVecRows + sub * 64 + (threadIdx.x & 15) * 4;
+    const uint64_t o = (uint64_t)k * pr.dstage + row;
+    const float4 y4 = *reinterpret_cast<const float4*>(pr.Y + o);
+    float4 m4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (memo) m4 = *reinterpret_cast<const float4*>(pr.XM + o);
+    const float yv[4] = {y4.x, y4.y, y4.z, y4.w};
+    const float mv[4] = {m4.x, m4.y, m4.z, m4.w};
+    float xv[4], dv[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      h[e] = bf16_rne(dv[e]);
-      l[e] = bf16_rne(dv[e] - bf16_bits_to_f32(h[e]));
+      xv[e] = (row + e < pr.d) ? (float)(scale * (double)yv[e]) : 0.0f;
+      dv[e] = xv[e] - mv[e];
+      if (md == kModeAmmProposed && row + e < pr.d) {
+        const double dx = (double)xv[e] - (double)mv[e];
+        s += dx;
+        q += (double)pr.z[row + e] * dx;
+      }
     }
-    *reinterpret_cast<uint2*>(pr.DXh + o) =
-        make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
-    *reinterpret_cast<uint2*>(pr.DXl + o) =
-        make_uint2((uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16));
-  } else {
-    *reinterpret_cast<float4*>(pr.DX + o) = make_float4(dv[0], dv[1], dv[2], dv[3]);
+    *reinterpret_cast<float4*>(pr.X + o) = make_float4(xv[0], xv[1], xv[2], xv[3]);
+    if (pr.dtype == 1) {
+      uint16_t h[4], l[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        h[e] = bf16_rne(dv[e]);
+        l[e] = bf16_rne(dv[e] - bf16_bits_to_f32(h[e]));
+      }
+      *reinterpret_cast<uint2*>(pr.DXh + o) =
+          make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+      *reinterpret_cast<uint2*>(pr.DXl + o) =
+          make_uint2((uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16));
+    } else {
+      *reinterpret_cast<float4*>(pr.DX + o) = make_float4(dv[0], dv[1], dv[2], dv[3]);
+    }
   }
   if (md == kModeAmmProposed) {
     s = sum16(s);
@@ -254,12 +263,13 @@ __device__ __forceinline__ uint32_t bcdf_search(const uint64_t* __restrict__ cdf
 }
 
 // Open-addressing count table in LDS: key = column + 1 (0 = empty).  A block
-// draws at most 2048 columns, so 4096 entries stay at most half full.
-constexpr uint32_t kHash = 4096;
+// draws at most 512 columns, so 1024 entries stay at most half full.
+constexpr uint32_t kSlotsPerBlock = 512;
+constexpr uint32_t kHash = 1024;
 
 __device__ __forceinline__ void hash_count(uint32_t* hkey, uint32_t* hcnt, uint32_t j) {
   const uint32_t key = j + 1u;
-  uint32_t h = (j * 2654435761u) >> 20;  // top 12 bits: [0, 4096)
+  uint32_t h = (j * 2654435761u) >> 22;  // top 10 bits: [0, 1024)
   while (true) {
     const uint32_t old = atomicCAS(&hkey[h], 0u, key);
     if (old == 0u || old == key) {
@@ -279,7 +289,7 @@ __global__ __launch_bounds__(256) void k_bsample(const BProb* __restrict__ P, co
   const uint32_t p = find_problem(off, r.np, blockIdx.x);
   const BProb& pr = P[p];
   const uint32_t local = blockIdx.x - off[p];
-  const uint32_t per = (pr.c + 2047) / 2048;
+  const uint32_t per = (pr.c + kSlotsPerBlock - 1) / kSlotsPerBlock;
   const uint32_t k = local / per, part = local % per;
   BTrial& t = pr.bt[k];
   const uint32_t md = t.mode;
@@ -313,9 +323,8 @@ __global__ __launch_bounds__(256) void k_bsample(const BProb* __restrict__ P, co
   }
   __syncthreads();
   const uint32_t k0 = (uint32_t)pr.seed, k1 = (uint32_t)(pr.seed >> 32);
-#pragma unroll
-  for (int h = 0; h < 4; ++h) {
-    const uint32_t pair = part * 1024 + h * 256 + threadIdx.x;
+  {
+    const uint32_t pair = part * (kSlotsPerBlock / 2) + threadIdx.x;
     const uint32_t slot = 2 * pair;
     if (slot < pr.c) {
       uint32_t q[4];
@@ -345,8 +354,13 @@ __global__ __launch_bounds__(256) void k_bsample(const BProb* __restrict__ P, co
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bprep(const BProb* __restrict__ P, const uint32_t* off,
                                                BRun r) {
+  __shared__ uint32_t mode_s[kBT];
+  __shared__ uint64_t total_s[2];
   const uint32_t p = find_problem(off, r.np, blockIdx.x);
   const BProb& pr = P[p];
+  if (threadIdx.x < kBT) mode_s[threadIdx.x] = threadIdx.x < r.ntr ? pr.bt[threadIdx.x].mode : 0u;
+  if (threadIdx.x < 2) total_s[threadIdx.x] = pr.st->total[threadIdx.x];
+  __syncthreads();
   const uint32_t tile = (blockIdx.x - off[p]) * 4 + (threadIdx.x >> 6);
   if (tile >= pr.ntiles) return;
   const uint32_t lane = threadIdx.x & 63;
@@ -375,7 +389,7 @@ __global__ __launch_bounds__(256) void k_bprep(const BProb* __restrict__ P, cons
   for (int k = 0; k < (int)kBT; ++k) {
     w[k] = 0.0f;
     if ((uint32_t)k >= r.ntr || !inb) continue;
-    const uint32_t md = pr.bt[k].mode;
+    const uint32_t md = mode_s[k];
     if (md == kModeExactMemo || md == kModeExactNoMemo) {
       w[k] = 1.0f;
       present = true;
@@ -387,7 +401,7 @@ __global__ __launch_bounds__(256) void k_bprep(const BProb* __restrict__ P, cons
       }
       const int which = md == kModeAmmProposed ? 1 : 0;
       const uint64_t qj = which ? q1 : q0;
-      const double num = (double)cnt[k] * (double)pr.st->total[which];
+      const double num = (double)cnt[k] * (double)total_s[which];
       const double den = (double)pr.c * (double)qj;
       w[k] = den > 0.0 ? (float)(num / den) : 0.0f;
       present = true;
@@ -461,15 +475,25 @@ __global__ __launch_bounds__(1024) void k_bscan(const BProb* __restrict__ P) {
   if (tid == 1023) pr.tpre[pr.ntiles] = sc[cur][1023];
 }
 
-// tile of compacted slot s: largest t with tpre[t] <= s (s < tpre[ntiles])
-__device__ __forceinline__ uint32_t slot_tile(const uint32_t* __restrict__ tpre, uint32_t ntiles,
-                                              uint32_t s) {
-  uint32_t lo = 0, hi = ntiles;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (tpre[mid] <= s) lo = mid; else hi = mid;
+// ---------------------------------------------------------------------------
+// Compacted union: tile t's present columns and weights move to union slots
+// tpre[t] ... (one wave per tile, 4 tiles per workgroup).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bcompact(const BProb* __restrict__ P, const uint32_t* off,
+                                                  uint32_t np) {
+  const uint32_t p = find_problem(off, np, blockIdx.x);
+  const BProb& pr = P[p];
+  const uint32_t tile = (blockIdx.x - off[p]) * 4 + (threadIdx.x >> 6);
+  if (tile >= pr.ntiles) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t cnt = pr.tcount[tile];
+  const uint32_t base = pr.tpre[tile];
+  // 64 slots x 16 weights: lane moves slot (i >> 4)'s weight (i & 15)
+  for (uint32_t i = lane; i < cnt * kBT; i += 64) {
+    const uint32_t slot = i >> 4;
+    pr.Wc[(uint64_t)(base + slot) * kBT + (i & 15)] = pr.W[((uint64_t)tile * kBTile + slot) * kBT + (i & 15)];
   }
-  return lo;
+  if (lane < cnt) pr.colc[base + lane] = pr.list[(uint64_t)tile * kBTile + lane];
 }
 
 // ---------------------------------------------------------------------------
@@ -518,8 +542,7 @@ struct TileStage {
 // lo (two MFMAs per 32 rows), FP32 v_mfma_f32_16x16x4_f32 (four per 16
 // rows).  A-operand lane (i, g): column 16G + i, rows 8g.. (BF16) / 4g..
 // (FP32) of the step; B-operand lane (i, g): the same rows of dx_i;
-// accumulator lane (i, g) element e: slot 16G + 4g + e, trial i.  The
-// slice-0 workgroup also writes the compacted columns and weights.
+// accumulator lane (i, g) element e: slot 16G + 4g + e, trial i.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const uint32_t* off,
@@ -532,32 +555,17 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
   const uint32_t p = find_problem(off, np, blockIdx.x);
   const BProb& pr = P[p];
   const uint32_t local = blockIdx.x - off[p];
-  // slice-major order: the active (tile, slice) items of a sparse union are
-  // one contiguous range of blocks, spread evenly over the XCDs (blocks are
-  // dealt to the XCDs round-robin)
-  const uint32_t ks = local / pr.ntiles, vt = local % pr.ntiles;
+  // the grid covers the largest vtiles x kslices any union size can need
+  // (< dot_blocks + ntiles); item = slice-major over this product's
+  // virtual tiles, so the active items are the first blocks, spread evenly
+  // over the XCDs (blocks are dealt to the XCDs round-robin)
   const UPlan up = union_plan(pr, pr.tpre[pr.ntiles]);
+  if (local >= up.vtiles * up.kslices) return;
+  const uint32_t ks = local / up.vtiles, vt = local % up.vtiles;
   const uint32_t U = up.U;
   const uint32_t s0 = vt * kBTile;
-  if (s0 >= U || ks >= up.kslices) return;
   const uint32_t cnt = min(kBTile, U - s0);
-  if (threadIdx.x < kBTile) {
-    uint32_t j = 0xFFFFFFFFu;
-    if (threadIdx.x < cnt) {
-      const uint32_t s = s0 + threadIdx.x;
-      const uint32_t t = slot_tile(pr.tpre, pr.ntiles, s);
-      const uint64_t ts = (uint64_t)t * kBTile + (s - pr.tpre[t]);
-      j = pr.list[ts];
-      if (ks == 0) {
-        pr.colc[s] = j;
-        const float4* w = reinterpret_cast<const float4*>(pr.W + ts * kBT);
-        float4* wc = reinterpret_cast<float4*>(pr.Wc + (uint64_t)s * kBT);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) wc[v] = w[v];
-      }
-    }
-    cols[threadIdx.x] = j;
-  }
+  if (threadIdx.x < kBTile) cols[threadIdx.x] = threadIdx.x < cnt ? pr.colc[s0 + threadIdx.x] : 0xFFFFFFFFu;
   __syncthreads();
   const uint32_t ngroups = (cnt + 15) / 16;
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -630,6 +638,37 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
 }
 
 // ---------------------------------------------------------------------------
+// Weights of the current product, once per union slot (previousAMM_kernel1 /
+// proposedAMM_kernel1 epilogue, core_previousAMM.cuh:4-175,
+// core_proposedAMM.cuh:17-213): Tc[s][k] = W[s][k] (sum_ks P1[ks][s][k] -
+// alpha_j s_k), s_k = 1^T dx_k for a proposed trial, else 0.  Thread
+// (slot of 16, trial); the problem's blocks stride over the union.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bweight(const BProb* __restrict__ P, const uint32_t* off,
+                                                 BRun r, uint32_t proposed_any) {
+  const uint32_t p = find_problem(off, r.np, blockIdx.x);
+  const BProb& pr = P[p];
+  const uint32_t local = blockIdx.x - off[p];
+  const uint32_t nblk = off[p + 1] - off[p];
+  const uint32_t k = threadIdx.x & 15;
+  const UPlan up = union_plan(pr, pr.tpre[pr.ntiles]);
+  double sk = 0.0;
+  if (proposed_any && pr.bt[k].mode == kModeAmmProposed) sk = pr.bt[k].s;
+  const uint64_t slab = (uint64_t)pr.ntiles * kBTile * kBT;
+  for (uint32_t s = local * 16 + (threadIdx.x >> 4); s < up.U; s += nblk * 16) {
+    const uint64_t e = (uint64_t)s * kBT + k;
+    float part[kMaxSlicesDev];
+#pragma unroll
+    for (uint32_t q = 0; q < kMaxSlicesDev; ++q) part[q] = q < up.kslices ? pr.P1[q * slab + e] : 0.0f;
+    double a1 = 0.0;
+#pragma unroll
+    for (uint32_t q = 0; q < kMaxSlicesDev; ++q) a1 += (double)part[q];
+    const double aj = sk != 0.0 ? (double)pr.alpha[pr.colc[s]] : 0.0;
+    pr.Tc[e] = (float)((double)pr.Wc[e] * (a1 - aj * sk));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Contraction 2 (previousAMM_kernel2 / proposedAMM_kernel2 for 16 trials):
 // P2[ch][k][rows] = sum_{s in chunk ch} a_{col(s)}[rows] t_{s,k} with
 //   t = W (sum_ks P1[ks] - alpha_j s_k)       (s_k = 0 unless proposed)
@@ -675,11 +714,7 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
   const uint32_t R0 = rb * RB;
   const T* A = reinterpret_cast<const T*>(pr.A);
   const uint32_t d = pr.d;
-  const uint32_t kk = threadIdx.x & 15;  // trial of this thread's t entries
-  double sk = 0.0;
-  if (proposed_any && pr.bt[kk].mode == kModeAmmProposed) sk = pr.bt[kk].s;
-  double corr = 0.0;
-  const uint64_t slab = (uint64_t)pr.ntiles * kBTile * kBT;
+  double corr = 0.0;  // proposed: sum_j alpha_j t_j of this thread's trial (threadIdx.x & 15)
   constexpr uint32_t NACC = BF ? 4 * PASSES : 4 * PASSES;
   f32x4_t acc[NACC];
 #pragma unroll
@@ -698,13 +733,8 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
       const uint32_t c = i >> 4, k = i & 15;
       float tt = 0.0f;
       if (s + c < se) {
-        const uint64_t e = (uint64_t)(s + c) * kBT + k;
-        double a1 = 0.0;
-        for (uint32_t q = 0; q < up.kslices; ++q) a1 += (double)pr.P1[q * slab + e];
-        const uint32_t j = cols[c];
-        const double aj = (sk != 0.0) ? (double)pr.alpha[j] : 0.0;
-        tt = (float)((double)pr.Wc[e] * (a1 - aj * sk));
-        if (proposed_any && pr.alpha) corr += (double)pr.alpha[j] * (double)tt;
+        tt = pr.Tc[(uint64_t)(s + c) * kBT + k];
+        if (proposed_any && rb == 0) corr += (double)pr.alpha[cols[c]] * (double)tt;
       }
       if constexpr (BF) {
         const uint16_t h = bf16_rne(tt);
@@ -799,48 +829,50 @@ __global__ __launch_bounds__(256) void k_bfinish(const BProb* __restrict__ P, co
   const BProb& pr = P[p];
   const uint32_t blk = blockIdx.x - off[p];
   const uint32_t k = threadIdx.x >> 4;
-  const uint32_t row = blk * kBVecRows + (threadIdx.x & 15) * 4;
-  const uint64_t o = (uint64_t)k * pr.dstage + row;
   const uint32_t md = k < r.ntr ? pr.bt[k].mode : kModeExactNoMemo;
   const bool memo_base = is_amm(md);
   const bool prop = md == kModeAmmProposed;
   const uint32_t active = union_plan(pr, pr.tpre[pr.ntiles]).active;
-  double a[4] = {0.0, 0.0, 0.0, 0.0};
-  for (uint32_t ch = 0; ch < active; ++ch) {
-    const float4 q = *reinterpret_cast<const float4*>(pr.P2 + ((uint64_t)ch * kBT + k) * pr.p2pitch + row);
-    a[0] += q.x;
-    a[1] += q.y;
-    a[2] += q.z;
-    a[3] += q.w;
-  }
   double sx = 0.0, qx = 0.0, corr = 0.0;
   if (prop) {
     sx = pr.bt[k].s;
     qx = pr.bt[k].q;
     for (uint32_t ch = 0; ch < active; ++ch) corr += pr.P2s[(uint64_t)ch * kBT + k];
   }
-  float4 ym4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (memo_base) ym4 = *reinterpret_cast<const float4*>(pr.YM + o);
-  const float ymv[4] = {ym4.x, ym4.y, ym4.z, ym4.w};
-  const float4 x4 = *reinterpret_cast<const float4*>(pr.X + o);
-  const float xv[4] = {x4.x, x4.y, x4.z, x4.w};
-  float yv[4];
   double yx = 0.0, yy = 0.0, yw = 0.0;
+  for (uint32_t sub = 0; sub < kBVecRows / 64; ++sub) {
+    const uint32_t row = blk * kBVecRows + sub * 64 + (threadIdx.x & 15) * 4;
+    const uint64_t o = (uint64_t)k * pr.dstage + row;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t ch = 0; ch < active; ++ch) {
+      const float4 q = *reinterpret_cast<const float4*>(pr.P2 + ((uint64_t)ch * kBT + k) * pr.p2pitch + row);
+      a[0] += q.x;
+      a[1] += q.y;
+      a[2] += q.z;
+      a[3] += q.w;
+    }
+    float4 ym4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (memo_base) ym4 = *reinterpret_cast<const float4*>(pr.YM + o);
+    const float ymv[4] = {ym4.x, ym4.y, ym4.z, ym4.w};
+    const float4 x4 = *reinterpret_cast<const float4*>(pr.X + o);
+    const float xv[4] = {x4.x, x4.y, x4.z, x4.w};
+    float yv[4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    double acc = memo_base ? (double)ymv[e] : 0.0;
-    if (prop && row + e < pr.d) acc += sx * (double)pr.v[row + e] + qx - corr;
-    acc += a[e];
-    yv[e] = (row + e < pr.d && k < r.ntr) ? (float)acc : 0.0f;
-    const double yd = yv[e];
-    yx += yd * (double)xv[e];
-    yy += yd * yd;
-    if (pr.eigvec && row + e < pr.d) yw += yd * pr.eigvec[row + e];
-  }
-  *reinterpret_cast<float4*>(pr.Y + o) = make_float4(yv[0], yv[1], yv[2], yv[3]);
-  if (md == kModeExactMemo && k < r.ntr) {
-    *reinterpret_cast<float4*>(pr.XM + o) = x4;
-    *reinterpret_cast<float4*>(pr.YM + o) = make_float4(yv[0], yv[1], yv[2], yv[3]);
+    for (int e = 0; e < 4; ++e) {
+      double acc = memo_base ? (double)ymv[e] : 0.0;
+      if (prop && row + e < pr.d) acc += sx * (double)pr.v[row + e] + qx - corr;
+      acc += a[e];
+      yv[e] = (row + e < pr.d && k < r.ntr) ? (float)acc : 0.0f;
+      const double yd = yv[e];
+      yx += yd * (double)xv[e];
+      yy += yd * yd;
+      if (pr.eigvec && row + e < pr.d) yw += yd * pr.eigvec[row + e];
+    }
+    *reinterpret_cast<float4*>(pr.Y + o) = make_float4(yv[0], yv[1], yv[2], yv[3]);
+    if (md == kModeExactMemo && k < r.ntr) {
+      *reinterpret_cast<float4*>(pr.XM + o) = x4;
+      *reinterpret_cast<float4*>(pr.YM + o) = make_float4(yv[0], yv[1], yv[2], yv[3]);
+    }
   }
   yx = sum16(yx);
   yy = sum16(yy);
@@ -888,6 +920,16 @@ hipError_t launch_bprep(const BProb* P, BGroup g, BRun r, hipStream_t s) {
 
 hipError_t launch_bscan(const BProb* P, uint32_t np, hipStream_t s) {
   k_bscan<<<np, 1024, 0, s>>>(P);
+  return hipGetLastError();
+}
+
+hipError_t launch_bcompact(const BProb* P, BGroup g, uint32_t np, hipStream_t s) {
+  if (g.total) k_bcompact<<<g.total, 256, 0, s>>>(P, g.off, np);
+  return hipGetLastError();
+}
+
+hipError_t launch_bweight(const BProb* P, BGroup g, BRun r, uint32_t proposed_any, hipStream_t s) {
+  if (g.total) k_bweight<<<g.total, 256, 0, s>>>(P, g.off, r, proposed_any);
   return hipGetLastError();
 }
 

@@ -64,6 +64,7 @@ struct alignas(16) BProb {
   uint32_t* tpre;       // [ntiles + 1] prefix of tcount; tpre[ntiles] = |U|
   uint32_t* colc;       // [n] union columns in compacted order
   float* Wc;            // [n][16] their weights
+  float* Tc;            // [n][16] t = W (A_U^T dx - alpha s) of the current product
   float* P1;            // [kslices_max][ntiles*64][16], compacted slots
   float* P2;            // [chunks][16][p2pitch]
   double* P2s;          // [chunks][16] proposed: sum alpha_j t_j per chunk
@@ -106,6 +107,8 @@ enum BTailOp : uint32_t { kBTailFinish = 0, kBTailScale = 1 };
 //     kslice_rows rows, kslices ~ dot_blocks / tiles (at most kslices_max);
 //   contraction 2: the union in `active` chunks of cpc slots (multiple of
 //     64), at most `chunks` of them.
+constexpr uint32_t kMinChunk = 256;
+
 struct UPlan {
   uint32_t U, vtiles, kslices, kslice_rows, cpc, active;
 };
@@ -119,8 +122,10 @@ __host__ __device__ inline UPlan union_plan(const BProb& p, uint32_t U) {
   const uint32_t rows = (p.dstage + ks - 1) / ks;
   q.kslice_rows = (rows + 255) / 256 * 256;
   q.kslices = (p.dstage + q.kslice_rows - 1) / q.kslice_rows;
+  // at least kMinChunk columns per chunk: a chunk's partial (16 trials x
+  // 4 bytes per row) stays within 1/8 of the bytes it reads per row
   const uint32_t per = (U + p.chunks - 1) / p.chunks;
-  q.cpc = per < 64 ? 64 : (per + 63) / 64 * 64;
+  q.cpc = per < kMinChunk ? kMinChunk : (per + 63) / 64 * 64;
   q.active = (U + q.cpc - 1) / q.cpc;
   return q;
 }
@@ -138,6 +143,10 @@ hipError_t launch_bnorm(const BProb* P, BGroup g, BRun r, hipStream_t s);
 hipError_t launch_bsample(const BProb* P, BGroup g, BRun r, uint32_t it, hipStream_t s);
 hipError_t launch_bprep(const BProb* P, BGroup g, BRun r, hipStream_t s);
 hipError_t launch_bscan(const BProb* P, uint32_t np, hipStream_t s);
+// items: ceil(ntiles / 4) per problem (the k_bprep group)
+hipError_t launch_bcompact(const BProb* P, BGroup g, uint32_t np, hipStream_t s);
+// items: any count per problem (the blocks stride over the union slots)
+hipError_t launch_bweight(const BProb* P, BGroup g, BRun r, uint32_t proposed_any, hipStream_t s);
 // dtype-uniform groups: all problems of a group share the element type
 hipError_t launch_bdot(int dtype, const BProb* P, BGroup g, uint32_t np, hipStream_t s);
 // passes: 1 or 2 (a group shares it)

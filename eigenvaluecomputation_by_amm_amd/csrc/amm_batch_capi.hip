@@ -104,7 +104,7 @@ struct amm_batch {
   // grouped-launch item offsets
   uint32_t* offs = nullptr;
   size_t offs_cap = 0;
-  amm::BGroup g_vec{}, g_sample{}, g_prep{}, g_dot[2]{}, g_axpy[2][2]{};
+  amm::BGroup g_vec{}, g_sample{}, g_prep{}, g_weight{}, g_dot[2]{}, g_axpy[2][2]{};
   bool plan_valid = false;
   int plan_kernel = -1;
   // run state
@@ -218,11 +218,15 @@ int upload_plan(amm_batch* b, int kernel) {
   bool ok = true;
   ok &= add_group(b->g_vec, [](const BProb& p) { return p.nvblk; });
   ok &= add_group(b->g_sample, [&](const BProb& p) {
-    return kernel == AMM_KERNEL_EXACT ? 0u : b->ntr * (uint32_t)cdiv(p.c, 2048);
+    return kernel == AMM_KERNEL_EXACT ? 0u : b->ntr * (uint32_t)cdiv(p.c, 512);
   });
   ok &= add_group(b->g_prep, [](const BProb& p) { return (uint32_t)cdiv(p.ntiles, 4); });
+  ok &= add_group(b->g_weight, [](const BProb& p) { return std::min<uint32_t>(p.ntiles * 4u, 1024u); });
   for (int dt = 0; dt < 2; ++dt)
-    ok &= add_group(b->g_dot[dt], [dt](const BProb& p) { return p.dtype == dt ? p.ntiles * p.kslices_max : 0u; });
+    ok &= add_group(b->g_dot[dt], [dt](const BProb& p) {
+      // vtiles * kslices(|U|) < dot_blocks + vtiles for every union size
+      return p.dtype == dt ? std::min(p.ntiles * p.kslices_max, p.dot_blocks + p.ntiles) : 0u;
+    });
   for (int dt = 0; dt < 2; ++dt)
     for (int pa = 0; pa < 2; ++pa)
       ok &= add_group(b->g_axpy[dt][pa], [dt, pa](const BProb& p) {
@@ -365,8 +369,10 @@ int enq_iteration(amm_batch* b, const amm::BRun& r, int K, uint32_t it) {
   if (K != AMM_KERNEL_EXACT) BHIPCHK(amm::launch_bsample(P, b->g_sample, r, it, s));
   BHIPCHK(amm::launch_bprep(P, b->g_prep, r, s));
   BHIPCHK(amm::launch_bscan(P, b->np, s));
+  BHIPCHK(amm::launch_bcompact(P, b->g_prep, b->np, s));
   for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bdot(dt, P, b->g_dot[dt], b->np, s));
   const uint32_t prop = (K == AMM_KERNEL_PROPOSED || K == AMM_KERNEL_PROPOSED_V2) ? 1u : 0u;
+  BHIPCHK(amm::launch_bweight(P, b->g_weight, r, prop, s));
   for (int dt = 0; dt < 2; ++dt)
     for (int pa = 0; pa < 2; ++pa)
       BHIPCHK(amm::launch_baxpy(dt, (uint32_t)pa + 1, P, b->g_axpy[dt][pa], b->np, prop, s));
@@ -539,7 +545,7 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
   // static per-problem layout
   Pool pl;
   struct Off {
-    size_t X, Y, XM, YM, DX, DXh, DXl, cnt, W, list, tcount, tpre, colc, Wc, BP, BQ, bt, alpha, nrm2,
+    size_t X, Y, XM, YM, DX, DXh, DXl, cnt, W, list, tcount, tpre, colc, Wc, Tc, BP, BQ, bt, alpha, nrm2,
         cnrm2, cdf0, cdf1, z, v, st, eigvec;
   };
   std::vector<Off> o(count);
@@ -579,6 +585,7 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     f.tpre = pl.take<uint32_t>(p.ntiles + 1);
     f.colc = pl.take<uint32_t>(q.n);
     f.Wc = pl.take<float>(q.n * amm::kBT);
+    f.Tc = pl.take<float>(q.n * amm::kBT);
     f.BP = pl.take<double>((size_t)p.nvblk * amm::kBT * 3);
     f.BQ = pl.take<double>((size_t)p.nvblk * amm::kBT * 2);
     f.bt = pl.take<BTrial>(amm::kBT);
@@ -617,6 +624,7 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     p.tpre = reinterpret_cast<uint32_t*>(s0 + f.tpre);
     p.colc = reinterpret_cast<uint32_t*>(s0 + f.colc);
     p.Wc = reinterpret_cast<float*>(s0 + f.Wc);
+    p.Tc = reinterpret_cast<float*>(s0 + f.Tc);
     p.BP = reinterpret_cast<double*>(s0 + f.BP);
     p.BQ = reinterpret_cast<double*>(s0 + f.BQ);
     p.bt = reinterpret_cast<BTrial*>(s0 + f.bt);
@@ -847,10 +855,12 @@ int amm_batch_autotune(amm_batch* b, uint32_t iters, double* best_ms) {
     BHIPCHK(amm::launch_bbegin(b->dP, b->g_vec, r, b->stream));
     BHIPCHK(amm::launch_bprep(b->dP, b->g_prep, r, b->stream));
     BHIPCHK(amm::launch_bscan(b->dP, b->np, b->stream));
+    BHIPCHK(amm::launch_bcompact(b->dP, b->g_prep, b->np, b->stream));
     for (uint32_t w = 0; w < 2; ++w) {  // warm-up, then timed
       BHIPCHK(hipEventRecord(b->ev0, b->stream));
       for (uint32_t k = 0; k < (w ? iters : 1); ++k) {
         for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bdot(dt, b->dP, b->g_dot[dt], b->np, b->stream));
+        BHIPCHK(amm::launch_bweight(b->dP, b->g_weight, r, 0u, b->stream));
         for (int dt = 0; dt < 2; ++dt)
           for (int pa = 0; pa < 2; ++pa)
             BHIPCHK(amm::launch_baxpy(dt, (uint32_t)pa + 1, b->dP, b->g_axpy[dt][pa], b->np, 0u, b->stream));
