@@ -96,6 +96,108 @@ def cpu_baseline(A_cols_host, d, lam, cfg, seed, iters_needed, budget_s=30.0):
                         f"16384x65536 matrix incl. column statistics; final rel err {rel_err:.2e}"))
 
 
+def batched_trials(A, d, n, lam_f32, iters, cfg, reps, trials=16):
+    """The batched engine on the same matrix: `trials` seeds per run stacked
+    as one right-hand-side block (FP32 and BF16 storage).  Per run, the time
+    until every trial has settled within tolerance (device clock, setup
+    included) and the per-solve share of it."""
+    import torch
+    from eigenvaluecomputation_by_amm_amd import AmmBatch, RunParams
+    from eigenvaluecomputation_by_amm_amd.workloads import lambda_max_lanczos
+    out = {"trials": trials}
+    Ab = A.to(torch.bfloat16)
+    lam_b, _ = lambda_max_lanczos(Ab, d, k=64)
+    for name, M, lam, s_A in (("bf16", Ab, lam_b, 2), ("f32", A, lam_f32, 4)):
+        with AmmBatch([dict(A=M, c=cfg["c"], seed=1)], trials=trials) as b:
+            dense_ms = b.autotune(3)
+            b.power_method(RunParams(step=iters, trial=10_000, **cfg))  # graph build + warm-up
+            alls, per, nr = [], [], 0
+            for rep in range(reps):
+                r = b.power_method(RunParams(step=iters, trial=trials * rep, **cfg))
+                idx = [settled_index(r.eig[0, k], lam) for k in range(trials)]
+                if min(idx) < 0:
+                    nr += sum(1 for i in idx if i < 0)
+                    continue
+                t_all = float(r.time_ms[max(idx)])
+                alls.append(t_all)
+                per.append(t_all / trials)
+            rx = b.power_method(RunParams(kernel="exact", step=min(iters, 40), trial=0))
+            idx = [settled_index(rx.eig[0, k], lam) for k in range(trials)]
+            ex_all = float(rx.time_ms[max(idx)]) if min(idx) >= 0 else None
+        out[name] = dict(
+            lambda_star=lam, time_all_ms=(statistics.median(alls) if alls else None),
+            per_solve_ms=(statistics.median(per) if per else None), not_reached=nr / (reps * trials),
+            exact16_time_all_ms=ex_all, exact16_per_solve_ms=(ex_all / trials if ex_all else None),
+            dense_pass_ms=round(dense_ms, 4),
+            dense_pass_GBs=round(2 * d * n * s_A / (dense_ms * 1e-3) / 1e9, 1))
+        log(f"batched {name}: {out[name]}")
+    del Ab
+    return out
+
+
+def batch_config(args):
+    """Many uneven matrices x 16 seeds in one batched run (BF16 storage)
+    against the FP32 exact power method, one matrix at a time."""
+    import torch
+    from eigenvaluecomputation_by_amm_amd import AmmBatch, AmmContext, RunParams
+    from eigenvaluecomputation_by_amm_amd.workloads import (lambda_max_lanczos, lowrank_plus_noise_fast,
+                                                            uneven_problem_shapes)
+    shapes = uneven_problem_shapes(args.count)
+    log(f"generating {len(shapes)} matrices, {sum(d * n for d, n in shapes) / 1e9:.2f} G elements")
+    m32, m16, lams = [], [], []
+    for i, (d, n) in enumerate(shapes):
+        M = lowrank_plus_noise_fast(d, n, seed=1000 + i)
+        lams.append(lambda_max_lanczos(M, d, k=40)[0])
+        m32.append(M)
+        m16.append(M.to(torch.bfloat16))
+    torch.cuda.synchronize()
+    iters = args.batch_iters
+    cfg = dict(kernel="previousAMM", vr_period=5)
+    probs = [dict(A=m16[i], c=max(1, n // 8), seed=77 + i) for i, (d, n) in enumerate(shapes)]
+    with AmmBatch(probs, trials=16) as b:
+        log("autotune")
+        dense_ms = b.autotune(2)
+        b.power_method(RunParams(step=iters, trial=10_000, **cfg))  # graph build + warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = b.power_method(RunParams(step=iters, trial=0, **cfg))
+        wall = time.perf_counter() - t0
+    idx = np.array([[settled_index(r.eig[i, k], lams[i]) for k in range(16)] for i in range(len(shapes))])
+    reached = idx >= 0
+    t_all = float(r.time_ms[idx.max()]) if reached.all() else None
+    log(f"batched run: {r.wall_ms:.1f} ms wall, settled by iteration {idx.max()} "
+        f"({reached.mean():.4f} reached)")
+    # comparator: FP32 exact power method, one matrix at a time
+    t_exact = []
+    for i, (d, n) in enumerate(shapes):
+        with AmmContext(m32[i]) as ctx:
+            rx = ctx.power_method(RunParams(kernel="exact", step=args.exact_iters, seed=5))
+            j = settled_index(rx.eig, lams[i])
+            t_exact.append(float(rx.time_ms[j]) if j >= 0 else float("nan"))
+    tot_exact = float(np.nansum(t_exact))
+    solves = 16 * len(shapes)
+    thr_amm = solves / (t_all * 1e-3) if t_all else None
+    thr_ex = len(shapes) / (tot_exact * 1e-3)
+    elems = sum(d * n for d, n in shapes)
+    line = {
+        "metric": "lambda_max solves per second within 1e-3 rel err (batched BF16 AMM vs FP32 exact), 1 MI355X",
+        "value": (round(thr_amm, 1) if thr_amm else None), "unit": "solves/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": f"{len(shapes)} matrices, d~U[512,8192], n~U[1024,32768], "
+                               "s_i=10/i rank-64 + 0.01 N(0,1), BF16 storage; 16 seeds each",
+                   "elements": elems, "kernel": cfg["kernel"], "c": "n/8", "vr_period": cfg["vr_period"],
+                   "iterations": iters, "tolerance": TOL,
+                   "time_all_ms": t_all, "settled_iteration_max": int(idx.max()),
+                   "reached_fraction": float(reached.mean()),
+                   "exact_f32_total_ms": round(tot_exact, 3), "exact_f32_solves_per_s": round(thr_ex, 2),
+                   "throughput_vs_exact": (round(thr_amm / thr_ex, 2) if thr_amm else None),
+                   "dense_pass_ms": round(dense_ms, 3),
+                   "dense_pass_GBs": round(2 * elems * 2 / (dense_ms * 1e-3) / 1e9, 1)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,12 +211,22 @@ def main():
     ap.add_argument("--d", type=int, default=16384)
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--kernel-iters", type=int, default=200)
+    ap.add_argument("--batch-reps", type=int, default=3, help="batched runs (16 seeds each)")
+    ap.add_argument("--config", choices=["headline", "batch"], default="headline",
+                    help="batch: many uneven matrices x 16 seeds (separate JSON line)")
+    ap.add_argument("--count", type=int, default=256, help="matrices of --config batch")
+    ap.add_argument("--batch-iters", type=int, default=150)
+    ap.add_argument("--exact-iters", type=int, default=40)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(see DESIGN.md for how the work would shard)", file=sys.stderr)
         sys.exit(2)
+
+    if args.config == "batch":
+        batch_config(args)
+        return
 
     import torch
     from eigenvaluecomputation_by_amm_amd import AmmContext, RunParams
@@ -229,6 +341,11 @@ def main():
             del Ah
             log(f"cpu: {cpu}")
 
+    log("batched engine on the same matrix (16 seeds per run)")
+    batched = batched_trials(A, d, n, lam, iters, dict(cfg), args.batch_reps)
+    if batched.get("bf16", {}).get("per_solve_ms") and ex["median_ms"]:
+        batched["throughput_bf16_vs_exact_f32"] = round(ex["median_ms"] / batched["bf16"]["per_solve_ms"], 2)
+
     value = am["median_ms"]
     speed = (ex["median_ms"] / value) if (value and ex["median_ms"]) else None
     line = {
@@ -247,6 +364,7 @@ def main():
                    "speedup_vs_exact": (round(speed, 3) if speed else None)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

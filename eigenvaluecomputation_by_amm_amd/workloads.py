@@ -82,3 +82,29 @@ def lambda_max_lanczos(A_cols: torch.Tensor, d: int | None = None, k: int = 64, 
     change = abs(hist[-1] - hist[-9]) / lam if len(hist) > 8 else float("inf")
     del A64
     return lam, change
+
+
+def uneven_problem_shapes(count: int = 256, seed: int = 7):
+    """(d, n) of the many-matrix workload: d ~ U[512, 8192], n ~ U[1024,
+    32768], rounded up to multiples of 8 (16-byte BF16 columns)."""
+    g = torch.Generator().manual_seed(seed)
+    d = torch.randint(512, 8193, (count,), generator=g)
+    n = torch.randint(1024, 32769, (count,), generator=g)
+    return [(int((a + 7) // 8 * 8), int((b + 7) // 8 * 8)) for a, b in zip(d.tolist(), n.tolist())]
+
+
+def lowrank_plus_noise_fast(d: int, n: int, seed: int, rank: int = 64, noise: float = 0.01,
+                            device="cuda") -> torch.Tensor:
+    """The 1024 x 2048 construction (s_i = 10/i, Haar factors, 0.01 N(0,1)
+    noise) truncated to its top `rank` singular pairs: s_65 = 0.15 lies far
+    below the noise edge 0.01 (sqrt d + sqrt n) >= 0.55, so the top of the
+    spectrum is that of the full construction while generation costs
+    O((d + n) rank^2 + d n rank) instead of a full QR.  FP32, (n, d) layout."""
+    gen = torch.Generator(device=device).manual_seed(seed)
+    r = min(rank, d, n)
+    U = _haar(d, r, gen, device).float()
+    V = _haar(n, r, gen, device).float()
+    s = (10.0 / torch.arange(1, r + 1, device=device, dtype=torch.float32))
+    A = torch.randn((n, d), generator=gen, device=device, dtype=torch.float32).mul_(noise)
+    A.addmm_(V * s, U.T.contiguous())
+    return A
