@@ -32,7 +32,17 @@ METRIC = "time to λ_max within 1e-3 rel err; AMM vs exact power method, 1 MI355
 TOL = 1e-3
 HBM_PEAK_GBS = 8000.0
 
-# headline AMM configuration (chosen from the sweep)
+# HBM bytes per launch on the 16384 x 65536 FP32 workload from two rocprofv3
+# PMC passes (FETCH_SIZE, then WRITE_SIZE, over
+# `python3 -m eigenvaluecomputation_by_amm_amd.kernel_probe`), each scaled by
+# what the counter reads for a 1 GiB device copy; used for roofline.traffic.
+PMC_TRAFFIC_16384x65536 = {
+    "k_colstats<float> (column statistics)": 4.2964e9,
+    "k_gather_dot<float,4,dense> (exact pass A^T x)": 4.2956e9,
+    "k_gather_axpy<float,dense> (exact pass A t)": 4.3011e9,
+}
+
+# headline AMM configuration: the fastest settled time of the kernel x c x vr sweep (--sweep)
 HEADLINE = dict(kernel="previousAMM", c=1024, vr_period=5)
 
 
@@ -221,7 +231,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the work would shard)", file=sys.stderr)
+              "(the batched form would shard by matrix, a single matrix by column blocks)", file=sys.stderr)
         sys.exit(2)
 
     if args.config == "batch":
@@ -323,8 +333,9 @@ def main():
     copy_gbs = 10 * 2 * buf.numel() * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9
     del buf, out
     log(f"torch device copy: {copy_gbs:.0f} GB/s (read + write)")
+    traffic = PMC_TRAFFIC_16384x65536.get(dom_name) if (d, n) == (16384, 65536) else None
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=dom_name,
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=dom_name,
                 bytes_per_launch=int(dom_b), avg_launch_us=round(dom_t * 1e3, 2),
                 share_of_run_us=round(dom_tot * 1e3, 1),
                 kernels_us={k: round(v[1] * 1e3, 2) for k, v in kern.items()},
