@@ -208,6 +208,67 @@ def batch_config(args):
     print(json.dumps(line), flush=True)
 
 
+def bf16_config(args):
+    """BF16 storage on image-like data: 40000 Gaussian-random-field images of
+    64 x 64 pixels (d = 4096), proposed AMM c = 2048 on the batched MFMA
+    engine (16 seeds per run) against the FP64 eigen-solve of the stored
+    BF16 matrix and against the FP32 exact power method on the FP32 data."""
+    import torch
+    from eigenvaluecomputation_by_amm_amd import AmmBatch, AmmContext, RunParams
+    from eigenvaluecomputation_by_amm_amd.workloads import grf_images
+    A32 = grf_images(side=64, n=40000)
+    Ab = A32.to(torch.bfloat16)
+    d, n = A32.shape[1], A32.shape[0]
+
+    def lam64(M):
+        M64 = M.double()
+        G = M64.T @ M64
+        del M64
+        return float(torch.linalg.eigvalsh(G)[-1])
+
+    lam_b = lam64(Ab)
+    lam_32 = lam64(A32)
+    log(f"lambda* (FP64, BF16 matrix) {lam_b:.12e}; (FP64, FP32 matrix) {lam_32:.12e}")
+    iters = args.iterations
+    with AmmContext(A32) as ctx:
+        rx = [ctx.power_method(RunParams(kernel="exact", step=iters, seed=s)) for s in range(3)]
+    ex_idx = [settled_index(r.eig, lam_32) for r in rx]
+    ex_ms = statistics.median(float(r.time_ms[i]) for r, i in zip(rx, ex_idx) if i >= 0)
+    lam_ex32 = float(rx[0].eig[-1])
+    cfg = dict(kernel="proposedAMM", c=2048, vr_period=5)
+    with AmmBatch([dict(A=Ab, c=cfg["c"], seed=3)], trials=16) as b:
+        b.autotune(3)
+        b.power_method(RunParams(step=iters, trial=10_000, **cfg))
+        alls, fin = [], []
+        for rep in range(args.batch_reps):
+            r = b.power_method(RunParams(step=iters, trial=16 * rep, **cfg))
+            idx = [settled_index(r.eig[0, k], lam_b) for k in range(16)]
+            if min(idx) >= 0:
+                alls.append(float(r.time_ms[max(idx)]))
+            fin.extend(float(e) for e in r.eig[0, :, -1])
+    fin = np.array(fin)
+    t_all = statistics.median(alls) if alls else None
+    line = {
+        "metric": "time to lambda_max within 1e-3 rel err, BF16 storage (16 seeds per run), 1 MI355X",
+        "value": (round(t_all / 16, 4) if t_all else None), "unit": "ms per solve", "n_gpus": 1,
+        "steps": args.batch_reps, "warmup": 1, "ms_per_step": (round(t_all, 4) if t_all else None),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": f"{n} Gaussian-random-field images 64x64 (d={d}), SE length 4 px, "
+                               "5% uniform noise, rows centred; BF16 storage",
+                   "kernel": cfg["kernel"], "c": cfg["c"], "vr_period": cfg["vr_period"],
+                   "iterations": iters, "tolerance": TOL,
+                   "time_all_16_ms": t_all,
+                   "final_rel_err_vs_fp64_of_bf16_max": float(np.max(np.abs(fin - lam_b) / lam_b)),
+                   "final_rel_err_vs_fp32_exact_max": float(np.max(np.abs(fin - lam_ex32) / lam_ex32)),
+                   "fp32_exact_rel_err_vs_fp64": abs(lam_ex32 - lam_32) / lam_32,
+                   "bf16_vs_fp32_matrix_lambda_rel": abs(lam_b - lam_32) / lam_32,
+                   "exact_f32_ms_per_solve": ex_ms,
+                   "throughput_vs_exact_f32": (round(ex_ms / (t_all / 16), 2) if t_all else None)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,8 +283,9 @@ def main():
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--kernel-iters", type=int, default=200)
     ap.add_argument("--batch-reps", type=int, default=3, help="batched runs (16 seeds each)")
-    ap.add_argument("--config", choices=["headline", "batch"], default="headline",
-                    help="batch: many uneven matrices x 16 seeds (separate JSON line)")
+    ap.add_argument("--config", choices=["headline", "batch", "bf16"], default="headline",
+                    help="batch: many uneven matrices x 16 seeds; bf16: image-like BF16 workload "
+                         "(separate JSON lines)")
     ap.add_argument("--count", type=int, default=256, help="matrices of --config batch")
     ap.add_argument("--batch-iters", type=int, default=150)
     ap.add_argument("--exact-iters", type=int, default=40)
@@ -236,6 +298,9 @@ def main():
 
     if args.config == "batch":
         batch_config(args)
+        return
+    if args.config == "bf16":
+        bf16_config(args)
         return
 
     import torch

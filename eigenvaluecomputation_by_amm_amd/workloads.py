@@ -108,3 +108,32 @@ def lowrank_plus_noise_fast(d: int, n: int, seed: int, rank: int = 64, noise: fl
     A = torch.randn((n, d), generator=gen, device=device, dtype=torch.float32).mul_(noise)
     A.addmm_(V * s, U.T.contiguous())
     return A
+
+
+@torch.no_grad()
+def grf_images(side: int = 64, n: int = 40000, length_scale: float = 4.0, noise: float = 0.05,
+               seed: int = 0, device="cuda", chunk: int = 4096) -> torch.Tensor:
+    """Image-like columns: n images of side x side pixels (d = side^2 rows).
+    Each image is a Gaussian random field with squared-exponential
+    covariance exp(-r^2 / (2 l^2)) (spectral synthesis: white noise filtered
+    by exp(-pi^2 l^2 |k|^2), the square root of that kernel's spectrum, on
+    the periodic grid), min-max scaled to [0, 1], mixed with `noise` of
+    U[0, 1] noise (x = (1 - noise) f + noise u), and finally every row
+    (pixel) is centred over the images.  FP32, (n, d) layout."""
+    gen = torch.Generator(device=device).manual_seed(seed)
+    ky = torch.fft.fftfreq(side, device=device, dtype=torch.float64)
+    kx = torch.fft.rfftfreq(side, device=device, dtype=torch.float64)
+    k2 = ky[:, None] ** 2 + kx[None, :] ** 2
+    amp = torch.exp(-(math.pi ** 2) * length_scale ** 2 * k2).float()
+    out = torch.empty((n, side * side), device=device, dtype=torch.float32)
+    for b in range(0, n, chunk):
+        m = min(chunk, n - b)
+        w = torch.randn((m, side, side), generator=gen, device=device, dtype=torch.float32)
+        f = torch.fft.irfft2(torch.fft.rfft2(w) * amp, s=(side, side))
+        lo = f.amin(dim=(1, 2), keepdim=True)
+        hi = f.amax(dim=(1, 2), keepdim=True)
+        f = (f - lo) / (hi - lo)
+        u = torch.rand((m, side, side), generator=gen, device=device, dtype=torch.float32)
+        out[b:b + m] = ((1.0 - noise) * f + noise * u).reshape(m, -1)
+    out -= out.mean(dim=0, keepdim=True)
+    return out
