@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import statistics
 import sys
@@ -269,6 +270,86 @@ def bf16_config(args):
     print(json.dumps(line), flush=True)
 
 
+def sigma_config(args):
+    """Differentiable sigma_max of a stack of 64 weight-like matrices
+    W_k = G / sqrt(4096) + sum_{i<4} s_i u_i v_i^T (s = 5, 4, 3, 2; Haar u,
+    v), 4096 x 4096 BF16: forward = batched AMM power method (previousAMM
+    c = 1024, 50 iterations, exact step every 5) + one exact pass for v,
+    backward = the rank-1 adjoint v u^T per matrix.  Checked against an FP64
+    power iteration (300 steps) on the stored BF16 values of two matrices."""
+    import torch
+    from eigenvaluecomputation_by_amm_amd import RunParams
+    from eigenvaluecomputation_by_amm_amd.autograd import sigma_max, sigma_max_stack
+    from eigenvaluecomputation_by_amm_amd.workloads import _haar
+    B, m = args.stack, 4096
+    gen = torch.Generator(device="cuda").manual_seed(17)
+    W = torch.empty(B, m, m, dtype=torch.bfloat16, device="cuda")
+    sv = torch.tensor([5.0, 4.0, 3.0, 2.0], device="cuda", dtype=torch.float64)
+    for k in range(B):
+        G = torch.randn(m, m, generator=gen, device="cuda", dtype=torch.float32) / math.sqrt(m)
+        U = _haar(m, 4, gen, "cuda")
+        V = _haar(m, 4, gen, "cuda")
+        W[k] = (G + ((U * sv) @ V.T).float()).to(torch.bfloat16)
+    torch.cuda.synchronize()
+    params = RunParams(kernel="previousAMM", c=1024, step=50, vr_period=5, seed=1)
+    Wg = W.requires_grad_(True)
+    s = sigma_max_stack(Wg, params)  # warm-up: context + graph
+    s.sum().backward()
+    Wg.grad = None
+    torch.cuda.synchronize()
+    fw, bw = [], []
+    for _ in range(args.sigma_reps):
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        s = sigma_max_stack(Wg, params)
+        e1.record()
+        s.sum().backward()
+        e2.record()
+        torch.cuda.synchronize()
+        fw.append(e0.elapsed_time(e1))
+        bw.append(e1.elapsed_time(e2))
+        Wg.grad = None
+    s = sigma_max_stack(Wg, params)
+    s.sum().backward()
+    # FP64 power iteration on the stored values (independent check)
+    errs_s, errs_g = [], []
+    for k in (0, B - 1):
+        M = W[k].detach().double()
+        x = torch.randn(m, generator=gen, device="cuda", dtype=torch.float64)
+        for _ in range(300):
+            x = M.T @ (M @ x)
+            x /= torch.linalg.norm(x)
+        t = M @ x
+        sg = float(torch.linalg.norm(t))
+        errs_s.append(abs(float(s[k].detach()) - sg) / sg)
+        Gref = torch.outer(t / sg, x)
+        errs_g.append(float((Wg.grad[k].double() - Gref).abs().max() / Gref.abs().max()))
+    # comparator: FP32 exact power method (single matrix, 50 iterations)
+    pe = RunParams(kernel="exact", step=50)
+    W32 = W[0].detach().float().contiguous()
+    sigma_max(W32, pe)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(4):
+        sigma_max(W32, pe)
+    e1.record()
+    torch.cuda.synchronize()
+    ex_one = e0.elapsed_time(e1) / 4
+    fwd = statistics.median(fw)
+    line = {
+        "metric": "sigma_max forward + backward of a weight stack, 1 MI355X", "value": round(fwd + statistics.median(bw), 3),
+        "unit": "ms", "n_gpus": 1, "steps": args.sigma_reps, "warmup": 1, "ms_per_step": round(fwd + statistics.median(bw), 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": f"{B} x 4096x4096 BF16, G/64 + rank-4 spike (5,4,3,2)",
+                   "kernel": "previousAMM", "c": 1024, "vr_period": 5, "iterations": 50,
+                   "forward_ms": round(fwd, 3), "backward_ms": round(statistics.median(bw), 3),
+                   "sigma_rel_err_vs_fp64_max": max(errs_s), "grad_rel_err_vs_fp64_max": max(errs_g),
+                   "exact_f32_single_forward_ms": round(ex_one, 3),
+                   "exact_f32_stack_forward_ms_estimate": round(ex_one * B, 3)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,12 +364,14 @@ def main():
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--kernel-iters", type=int, default=200)
     ap.add_argument("--batch-reps", type=int, default=3, help="batched runs (16 seeds each)")
-    ap.add_argument("--config", choices=["headline", "batch", "bf16"], default="headline",
+    ap.add_argument("--config", choices=["headline", "batch", "bf16", "sigma"], default="headline",
                     help="batch: many uneven matrices x 16 seeds; bf16: image-like BF16 workload "
                          "(separate JSON lines)")
     ap.add_argument("--count", type=int, default=256, help="matrices of --config batch")
     ap.add_argument("--batch-iters", type=int, default=150)
     ap.add_argument("--exact-iters", type=int, default=40)
+    ap.add_argument("--stack", type=int, default=64, help="matrices of --config sigma")
+    ap.add_argument("--sigma-reps", type=int, default=3)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -301,6 +384,9 @@ def main():
         return
     if args.config == "bf16":
         bf16_config(args)
+        return
+    if args.config == "sigma":
+        sigma_config(args)
         return
 
     import torch

@@ -415,6 +415,24 @@ class AmmBatch:
         check(self._lib.amm_batch_autotune(self._h, iters, C.c_void_p(_ptr(ms))))
         return float(ms[0])
 
+    def sigma_max(self, params: RunParams, vectors: bool = True):
+        """sigma_max of every (problem, trial) ([problems][trials] float64)
+        and, for trial 0 of each problem, the unit singular vectors u
+        (length d) and v (length n) as lists of device tensors."""
+        import torch
+        p = params.to_c()
+        sig = np.zeros((self.count, self.trials), np.float64)
+        U = V = None
+        up = vp = None
+        if vectors:
+            dev = self._keep[0].device
+            U = [torch.empty(d, dtype=torch.float32, device=dev) for d, _ in self.dims]
+            V = [torch.empty(n, dtype=torch.float32, device=dev) for _, n in self.dims]
+            up = (C.c_void_p * self.count)(*[u.data_ptr() for u in U])
+            vp = (C.c_void_p * self.count)(*[v.data_ptr() for v in V])
+        check(self._lib.amm_batch_sigma_max(self._h, C.byref(p), C.c_void_p(_ptr(sig)), up, vp))
+        return sig, U, V
+
     def sample(self, kernel: str, iteration: int, trial0: int, problem: int):
         """Union of the trials' samples of one iteration: (columns ascending,
         weights [columns][trials])."""

@@ -21,7 +21,7 @@ from collections import OrderedDict
 
 import torch
 
-from .api import AmmContext, RunParams, rank1_update
+from .api import AmmBatch, AmmContext, RunParams, rank1_update
 
 _CACHE_MAX = 4
 _contexts: "OrderedDict[tuple, AmmContext]" = OrderedDict()
@@ -93,3 +93,65 @@ def sigma_max(W: torch.Tensor, params: RunParams | None = None) -> torch.Tensor:
     if params is None:
         params = RunParams(kernel="exact", step=200)
     return SigmaMax.apply(W, params)
+
+
+_batches: "OrderedDict[tuple, AmmBatch]" = OrderedDict()
+
+
+def _batch(mats, cols: int, params: RunParams) -> AmmBatch:
+    """Batched contexts cached per (pointers, shapes, dtype, c, seed, stream)."""
+    stream = torch.cuda.current_stream(mats[0].device).cuda_stream
+    key = (tuple(m.data_ptr() for m in mats), tuple(mats[0].shape), mats[0].stride(0), cols,
+           mats[0].dtype, int(params.c), int(params.seed), stream)
+    b = _batches.get(key)
+    if b is None:
+        probs = [dict(A=m, d=cols, c=int(params.c), seed=int(params.seed) + i) for i, m in enumerate(mats)]
+        b = AmmBatch(probs, trials=1)
+        _batches[key] = b
+        while len(_batches) > _CACHE_MAX:
+            _, old = _batches.popitem(last=False)
+            old.close()
+    else:
+        _batches.move_to_end(key)
+        b._keep = list(mats)
+    return b
+
+
+class SigmaMaxStack(torch.autograd.Function):
+    """sigma_k = sigma_max(W[k]) for a stack W (B, rows, cols), all matrices
+    in one batched run (one grouped launch per kernel); d sigma_k / d W[k] =
+    v_k u_k^T (row-major view).  Matrix k uses the Philox key params.seed + k."""
+
+    @staticmethod
+    def forward(ctx, W: torch.Tensor, params: RunParams):
+        if not W.is_cuda or W.dim() != 3:
+            raise ValueError("W must be a 3-D device tensor (stack, rows, cols)")
+        if W.dtype not in (torch.float32, torch.bfloat16):
+            raise ValueError("W must be float32 or bfloat16")
+        B, rows, cols = W.shape
+        Wd = W.detach()
+        mats, cols_ = zip(*[_as_columns(Wd[k]) for k in range(B)])
+        b = _batch(list(mats), cols_[0], params)
+        sig, U, V = b.sigma_max(params)
+        ctx.save_for_backward(torch.stack(U), torch.stack(V))
+        ctx.shape = (B, rows, cols)
+        ctx.dtype = W.dtype
+        return torch.tensor(sig[:, 0], dtype=torch.float32, device=W.device)
+
+    @staticmethod
+    def backward(ctx, g: torch.Tensor):
+        U, V = ctx.saved_tensors
+        B, rows, cols = ctx.shape
+        dW = torch.empty(B, rows, cols, dtype=ctx.dtype, device=U.device)
+        gh = g.detach().float().cpu().tolist()
+        for k in range(B):
+            rank1_update(dW[k], gh[k], U[k], V[k], d=cols, n=rows, ld=cols)
+        return dW, None
+
+
+def sigma_max_stack(W: torch.Tensor, params: RunParams | None = None) -> torch.Tensor:
+    """Largest singular value of every matrix of W (B, rows, cols),
+    differentiable in W.  Default: 200 exact iterations."""
+    if params is None:
+        params = RunParams(kernel="exact", step=200)
+    return SigmaMaxStack.apply(W, params)

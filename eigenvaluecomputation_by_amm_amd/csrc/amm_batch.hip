@@ -886,6 +886,46 @@ __global__ __launch_bounds__(256) void k_bfinish(const BProb* __restrict__ P, co
 }
 
 // ---------------------------------------------------------------------------
+// sigma_max of every (problem, trial) after a run: set every trial to an
+// exact step (k_bset_mode) so that k_bnorm stages dx = x and the dense union
+// gives Tc[j][k] = a_j^T x_k; then sigma_k = ||Tc[:, k]|| (FP64, fixed
+// order), and for trial 0 u = x_0, v = Tc[:, 0] / sigma_0 (k_bsigma, one
+// workgroup per problem; thread (k, sub) sums columns sub, sub + 16, ...).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_bset_mode(const BProb* __restrict__ P, uint32_t mode) {
+  if (threadIdx.x < kBT) P[blockIdx.x].bt[threadIdx.x].mode = mode;
+}
+
+__global__ __launch_bounds__(256) void k_bsigma(const BProb* __restrict__ P, BRun r,
+                                                double* __restrict__ sigma,
+                                                float* const* __restrict__ u_out,
+                                                float* const* __restrict__ v_out) {
+  __shared__ double s0;
+  const BProb& pr = P[blockIdx.x];
+  const uint32_t k = threadIdx.x >> 4, sub = threadIdx.x & 15;
+  double acc = 0.0;
+  for (uint32_t j = sub; j < pr.n; j += 16) {
+    const double t = pr.Tc[(uint64_t)j * kBT + k];
+    acc += t * t;
+  }
+  acc = sum16(acc);
+  if (sub == 0 && k < r.ntr) {
+    const double sg = sqrt(acc);
+    sigma[(uint64_t)blockIdx.x * r.ntr + k] = sg;
+    if (k == 0) s0 = sg;
+  }
+  __syncthreads();
+  float* u = u_out ? u_out[blockIdx.x] : nullptr;
+  float* v = v_out ? v_out[blockIdx.x] : nullptr;
+  const double sg = s0;
+  if (v)
+    for (uint32_t j = threadIdx.x; j < pr.n; j += 256)
+      v[j] = sg > 0.0 ? (float)((double)pr.Tc[(uint64_t)j * kBT] / sg) : 0.0f;
+  if (u)
+    for (uint32_t i = threadIdx.x; i < pr.d; i += 256) u[i] = pr.X[i];
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_bbegin(const BProb* P, BGroup g, BRun r, hipStream_t s) {
@@ -956,6 +996,17 @@ hipError_t launch_baxpy(int dtype, uint32_t passes, const BProb* P, BGroup g, ui
     else
       k_baxpy<float, 2><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_bset_mode(const BProb* P, uint32_t np, uint32_t mode, hipStream_t s) {
+  k_bset_mode<<<np, 64, 0, s>>>(P, mode);
+  return hipGetLastError();
+}
+
+hipError_t launch_bsigma(const BProb* P, BRun r, double* sigma, float* const* u, float* const* v,
+                         hipStream_t s) {
+  k_bsigma<<<r.np, 256, 0, s>>>(P, r, sigma, u, v);
   return hipGetLastError();
 }
 

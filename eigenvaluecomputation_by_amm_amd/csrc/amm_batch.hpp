@@ -153,5 +153,9 @@ hipError_t launch_bdot(int dtype, const BProb* P, BGroup g, uint32_t np, hipStre
 hipError_t launch_baxpy(int dtype, uint32_t passes, const BProb* P, BGroup g, uint32_t np,
                         uint32_t proposed_any, hipStream_t s);
 hipError_t launch_bfinish(const BProb* P, BGroup g, BRun r, hipStream_t s);
+hipError_t launch_bset_mode(const BProb* P, uint32_t np, uint32_t mode, hipStream_t s);
+// sigma: device [np][ntr]; u, v: device arrays of np device pointers (or null)
+hipError_t launch_bsigma(const BProb* P, BRun r, double* sigma, float* const* u, float* const* v,
+                         hipStream_t s);
 
 }  // namespace amm

@@ -818,6 +818,55 @@ int amm_batch_sample(amm_batch* b, int32_t kernel, uint32_t iter, uint32_t trial
   return AMM_OK;
 }
 
+// sigma_max of every (problem, trial) after a run with parameters p, and
+// the singular vectors of trial 0 (see include/amm.h).
+int amm_batch_sigma_max(amm_batch* b, const amm_params* p, double* sigma_host, float* const* u_dev,
+                        float* const* v_dev) {
+  if (!b) return fail(AMM_E_INVALID, "batch is NULL");
+  int rc;
+  if ((rc = run(b, p))) return rc;
+  const amm::BRun r = run_args(b, *p);
+  const size_t ns = (size_t)b->np * b->ntr;
+  double* dsig = nullptr;
+  float** dptr = nullptr;
+  BHIPCHK(hipMalloc((void**)&dsig, ns * sizeof(double)));
+  auto cleanup = [&]() {
+    (void)hipFree(dsig);
+    if (dptr) (void)hipFree(dptr);
+  };
+  if (u_dev || v_dev) {
+    std::vector<float*> hp(2 * (size_t)b->np, nullptr);
+    for (uint32_t i = 0; i < b->np; ++i) {
+      if (u_dev) hp[i] = u_dev[i];
+      if (v_dev) hp[b->np + i] = v_dev[i];
+    }
+    if (hipMalloc((void**)&dptr, hp.size() * sizeof(float*)) != hipSuccess) {
+      cleanup();
+      return fail(AMM_E_NOMEM, "device allocation failed");
+    }
+    if (hipMemcpy(dptr, hp.data(), hp.size() * sizeof(float*), hipMemcpyHostToDevice) != hipSuccess) {
+      cleanup();
+      return fail(AMM_E_HIP, "hipMemcpy failed");
+    }
+  }
+  // t = A^T x for every trial: one exact half-step over the dense union
+  hipError_t e = amm::launch_bset_mode(b->dP, b->np, amm::kModeExactNoMemo, b->stream);
+  if (e == hipSuccess) e = amm::launch_bnorm(b->dP, b->g_vec, r, b->stream);
+  if (e == hipSuccess) e = amm::launch_bprep(b->dP, b->g_prep, r, b->stream);
+  if (e == hipSuccess) e = amm::launch_bscan(b->dP, b->np, b->stream);
+  if (e == hipSuccess) e = amm::launch_bcompact(b->dP, b->g_prep, b->np, b->stream);
+  for (int dt = 0; dt < 2 && e == hipSuccess; ++dt) e = amm::launch_bdot(dt, b->dP, b->g_dot[dt], b->np, b->stream);
+  if (e == hipSuccess) e = amm::launch_bweight(b->dP, b->g_weight, r, 0u, b->stream);
+  if (e == hipSuccess)
+    e = amm::launch_bsigma(b->dP, r, dsig, u_dev && dptr ? dptr : nullptr,
+                           v_dev && dptr ? dptr + b->np : nullptr, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e == hipSuccess && sigma_host) e = hipMemcpy(sigma_host, dsig, ns * sizeof(double), hipMemcpyDeviceToHost);
+  cleanup();
+  if (e != hipSuccess) return fail(AMM_E_HIP, std::string("batched sigma_max failed: ") + hipGetErrorString(e));
+  return AMM_OK;
+}
+
 // Time candidate shapes of the two contractions on the live problems (one
 // exact-step iteration: every column present) and keep the fastest per
 // problem size class.  best_ms: time of one iteration's contractions with

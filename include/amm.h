@@ -247,6 +247,15 @@ int amm_batch_autotune(amm_batch *b, uint32_t iters, double *best_ms);
 int amm_batch_sample(amm_batch *b, int32_t kernel, uint32_t iter, uint32_t trial0,
                      uint32_t problem, uint32_t *cols_host, float *w_host, uint64_t *count);
 
+/* sigma_max of every (problem, trial) of a batched run with parameters p
+ * (no reference counterpart; see amm_sigma_max): runs the power method,
+ * then one exact pass t = A^T x per trial.  sigma_host: host
+ * [count][trials], sigma = ||t||; u_dev[i] (length d_i) and v_dev[i]
+ * (length n_i): host arrays of device pointers receiving trial 0's unit
+ * singular vectors u = x, v = t / sigma (either array may be NULL). */
+int amm_batch_sigma_max(amm_batch *b, const amm_params *p, double *sigma_host, float *const *u_dev,
+                        float *const *v_dev);
+
 /* Round count FP32 device values to BF16 (nearest even) into dst_dev; the
  * storage form of the BF16 path (no reference counterpart: the reference is
  * FP32 only).  Asynchronous on `stream`. */

@@ -148,3 +148,38 @@ def test_rank1_update_bitwise(d, n, ld, dtype):
     assert torch.equal(got[:, :d].contiguous().view(bits), ref.contiguous().view(bits))
     if ld > d:
         assert torch.all(got[:, d:] == 7.0)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
+def test_sigma_stack(dtype):
+    """A stack of planted matrices in one batched run: sigma and the
+    gradient of every matrix against FP64 svdvals (of the stored values)."""
+    banner(f"sigma_max stack {dtype}")
+    from eigenvaluecomputation_by_amm_amd import RunParams
+    from eigenvaluecomputation_by_amm_amd.autograd import sigma_max_stack
+    Ws = torch.stack([planted(320, 256, seed=40 + k, top=10.0 + k, second=6.0) for k in range(6)])
+    W = Ws.to(dtype).cuda().requires_grad_(True)
+    s = sigma_max_stack(W, RunParams(kernel="previousAMM", c=128, step=80, vr_period=5, seed=3))
+    (s * torch.arange(1, 7, device="cuda", dtype=torch.float32)).sum().backward()
+    Wq = W.detach().double().cpu()
+    tol_g = 2e-5 if dtype == torch.float32 else 2 ** -7
+    for k in range(6):
+        ref = torch.linalg.svdvals(Wq[k])[0].item()
+        assert abs(s[k].item() - ref) <= 5e-6 * ref, (k, s[k].item(), ref)
+        Gref = (k + 1) * _svd_grad(Wq[k])
+        G = W.grad[k].double().cpu()
+        assert (G - Gref).abs().max().item() <= tol_g * Gref.abs().max().item() + 1e-6, k
+
+
+def test_sigma_stack_equals_single_exact():
+    """With the exact kernel the batched forward gives the single-matrix
+    sigma of every matrix (same iteration, different summation order)."""
+    banner("sigma_max stack vs single")
+    from eigenvaluecomputation_by_amm_amd import RunParams
+    from eigenvaluecomputation_by_amm_amd.autograd import sigma_max, sigma_max_stack
+    Ws = torch.stack([planted(200, 264, seed=60 + k) for k in range(4)]).float().cuda()
+    p = RunParams(kernel="exact", step=60)
+    sb = sigma_max_stack(Ws, p)
+    for k in range(4):
+        ss = sigma_max(Ws[k].contiguous(), p)
+        assert abs(sb[k].item() - ss.item()) <= 1e-6 * ss.item()
