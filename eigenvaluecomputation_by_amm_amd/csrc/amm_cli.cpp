@@ -71,6 +71,7 @@ class host_memory {  // host_memory.h:23-192
   bool sanity_check = false;
   bool variance_reduction = false;
   bool display = false;
+  bool dump_matrix = false;
 
   host_memory() { seed = (uint32_t)time(NULL); }
 
@@ -116,6 +117,8 @@ class host_memory {  // host_memory.h:23-192
         verbose = std::atoi(next("-verbose"));
       } else if (!strcmp(argv[i], "-dtype")) {
         dtype = next("-dtype");
+      } else if (!strcmp(argv[i], "-dump_matrix")) {
+        dump_matrix = true;
       } else {
         std::printf("[warning] unknown parameter: %s\n", argv[i]);
       }
@@ -240,6 +243,20 @@ struct device_memory {  // device_memory.h:116-498
     if (stream) (void)hipStreamDestroy(stream);
   }
 
+  // matrix.dat: the m x m matrix as stored (padded), float32 in memory order,
+  // with the diagonal offset taken off the diagonal again -- the file the
+  // reference's output_matrix writes (device_memory.h:436-473) for an FP64
+  // eigen-solve (calculate_largest_eigenvalue.py).
+  void output_matrix(float diagonal_offset) const {
+    std::vector<float> h((size_t)m * m);
+    if (hipMemcpy(h.data(), dA32, sizeof(float) * h.size(), hipMemcpyDeviceToHost) != hipSuccess)
+      die("hipMemcpy");
+    for (uint64_t i = 0; i < m; ++i) h[i * m + i] -= diagonal_offset;
+    std::ofstream fs("matrix.dat", std::ios::out | std::ios::binary);
+    fs.write((const char*)h.data(), sizeof(float) * h.size());
+    if (!fs) std::fprintf(stderr, "[warning] cannot write matrix.dat\n");
+  }
+
   // device_memory.h:316-337: generate or read A, then the column statistics
   void write_data(host_memory& p) {
     const size_t elems = (size_t)m * m;
@@ -275,6 +292,7 @@ struct device_memory {  // device_memory.h:116-498
       if (amm_generate_symmetric(dA32, m, m, dist, p1, p2, p.seed, stream, &p.diagonal_offset))
         die("amm_generate_symmetric");
     }
+    if (p.dump_matrix) output_matrix(p.diagonal_offset);
     int dt = AMM_DTYPE_F32;
     if (dA != dA32) {
       if (amm_f32_to_bf16(dA32, dA, elems, stream)) die("amm_f32_to_bf16");

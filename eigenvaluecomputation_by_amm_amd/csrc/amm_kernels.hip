@@ -20,6 +20,8 @@
 #include "amm_device.hpp"
 #include "amm_launch.hpp"
 
+#include <algorithm>
+
 namespace amm {
 
 // ---------------------------------------------------------------------------
@@ -910,47 +912,50 @@ __global__ __launch_bounds__(256) void k_unit(const float* __restrict__ t, uint6
 
 // Backward of sigma_max: out[:, j] = g * a * b_j for a column-major d x n
 // matrix (the rank-1 adjoint u v^T of d sigma / d A); 16-byte stores when
-// vec (out 16-byte aligned, ld a multiple of the vector width).
+// vec (out 16-byte aligned, ld a multiple of the vector width).  Grid-stride
+// over the 16-byte chunks of the matrix, column by column.
 template <typename T>
 __global__ __launch_bounds__(256) void k_rank1(T* __restrict__ out, uint32_t d, uint64_t ld,
-                                               float g, const float* __restrict__ a,
+                                               uint64_t n, float g, const float* __restrict__ a,
                                                const float* __restrict__ b, int vec) {
-  constexpr int V = Elem<T>::VEC;
-  const uint64_t j = blockIdx.x;
-  const uint32_t row = (blockIdx.y * 256 + threadIdx.x) * V;
-  if (row >= d) return;
-  const float s = g * b[j];
-  T* col = out + j * ld + row;
-  float v[V];
+  constexpr uint32_t V = Elem<T>::VEC;
+  const uint32_t cpc = (d + V - 1) / V;  // chunks per column
+  const uint64_t total = (uint64_t)cpc * n;
+  for (uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x; q < total; q += (uint64_t)gridDim.x * 256) {
+    const uint64_t j = q / cpc;
+    const uint32_t row = (uint32_t)(q - j * cpc) * V;
+    const float s = g * b[j];
+    T* col = out + j * ld + row;
+    float v[V];
 #pragma unroll
-  for (int k = 0; k < V; ++k) v[k] = (row + k < d) ? s * a[row + k] : 0.0f;
-  if (vec && row + V <= d) {
-    if constexpr (V == 4) {
-      *reinterpret_cast<float4*>(col) = make_float4(v[0], v[1], v[2], v[3]);
-    } else {
-      uint32_t w[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const uint32_t lo = __float_as_uint(v[2 * k]), hi = __float_as_uint(v[2 * k + 1]);
-        const uint32_t rl = (lo + 0x7FFFu + ((lo >> 16) & 1u)) >> 16;
-        const uint32_t rh = (hi + 0x7FFFu + ((hi >> 16) & 1u)) >> 16;
-        w[k] = rl | (rh << 16);
-      }
-      *reinterpret_cast<uint4*>(col) = make_uint4(w[0], w[1], w[2], w[3]);
-    }
-  } else {
-    for (int k = 0; k < V && row + k < d; ++k) {
+    for (uint32_t k = 0; k < V; ++k) v[k] = (row + k < d) ? s * a[row + k] : 0.0f;
+    if (vec && row + V <= d) {
       if constexpr (V == 4) {
-        reinterpret_cast<float*>(col)[k] = v[k];
+        *reinterpret_cast<float4*>(col) = make_float4(v[0], v[1], v[2], v[3]);
       } else {
-        const uint32_t u = __float_as_uint(v[k]);
-        reinterpret_cast<uint16_t*>(col)[k] = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+        uint32_t w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t lo = __float_as_uint(v[2 * k]), hi = __float_as_uint(v[2 * k + 1]);
+          const uint32_t rl = (lo + 0x7FFFu + ((lo >> 16) & 1u)) >> 16;
+          const uint32_t rh = (hi + 0x7FFFu + ((hi >> 16) & 1u)) >> 16;
+          w[k] = rl | (rh << 16);
+        }
+        *reinterpret_cast<uint4*>(col) = make_uint4(w[0], w[1], w[2], w[3]);
+      }
+    } else {
+      for (uint32_t k = 0; k < V && row + k < d; ++k) {
+        if constexpr (V == 4) {
+          reinterpret_cast<float*>(col)[k] = v[k];
+        } else {
+          const uint32_t u = __float_as_uint(v[k]);
+          reinterpret_cast<uint16_t*>(col)[k] = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+        }
       }
     }
   }
 }
 
-// Plain fill kernel (graph-safe replacement for memset nodes).
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t val) {
   const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < n) p[i] = val;
@@ -1106,15 +1111,14 @@ hipError_t launch_unit(const float* t, uint64_t n, const DevState* st, float* v,
 hipError_t launch_rank1(int dtype, void* out, uint32_t d, uint64_t n, uint64_t ld, float g,
                         const float* a, const float* b, hipStream_t s) {
   if (n == 0 || d == 0) return hipSuccess;
-  // one workgroup of 256 work-items per column along x: n * 256 < 2^32
-  if (n >= (1ull << 24)) return hipErrorInvalidValue;
   const uint64_t V = dtype == 1 ? 8 : 4;
   const int vec = (ld % V == 0) && (((uintptr_t)out & 15u) == 0);
-  const dim3 grid((unsigned)n, grid1(d, 256 * V));
+  const uint64_t chunks = (d + V - 1) / V * n;
+  const unsigned blocks = (unsigned)std::min<uint64_t>((chunks + 255) / 256, 8192);
   if (dtype == 1)
-    k_rank1<__hip_bfloat16><<<grid, 256, 0, s>>>((__hip_bfloat16*)out, d, ld, g, a, b, vec);
+    k_rank1<__hip_bfloat16><<<blocks, 256, 0, s>>>((__hip_bfloat16*)out, d, ld, n, g, a, b, vec);
   else
-    k_rank1<float><<<grid, 256, 0, s>>>((float*)out, d, ld, g, a, b, vec);
+    k_rank1<float><<<blocks, 256, 0, s>>>((float*)out, d, ld, n, g, a, b, vec);
   return hipGetLastError();
 }
 
