@@ -576,20 +576,45 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
   f32x4_t acc[4];
 #pragma unroll
   for (int G = 0; G < 4; ++G) acc[G] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  // the next round's A tile and this wave's dx rows are prefetched together
   Stage st;
-  if (r0 < r1) st.load(A, pr.ld, cols, r0, pr.d);
+  u32x4 xa[2], xb[2];  // BF16: dx hi, lo of K-steps 0, 1; FP32: dx of K-steps 0, 1 (xa)
+  auto load_dx = [&](uint32_t row0) {
+#pragma unroll
+    for (uint32_t s = 0; s < 2; ++s) {
+      if constexpr (BF) {
+        const uint32_t rr = row0 + wave * 64 + s * 32 + 8 * lg;
+        xa[s] = *reinterpret_cast<const u32x4*>(pr.DXh + (uint64_t)li * pr.dstage + rr);
+        xb[s] = *reinterpret_cast<const u32x4*>(pr.DXl + (uint64_t)li * pr.dstage + rr);
+      } else {
+        const uint32_t rr = row0 + wave * 32 + s * 16 + 4 * lg;
+        xa[s] = *reinterpret_cast<const u32x4*>(pr.DX + (uint64_t)li * pr.dstage + rr);
+      }
+    }
+  };
+  if (r0 < r1) {
+    st.load(A, pr.ld, cols, r0, pr.d);
+    load_dx(r0);
+  }
   for (uint32_t row0 = r0; row0 < r1; row0 += RB) {
     st.store(lds);
     __syncthreads();
-    if (row0 + RB < r1) st.load(A, pr.ld, cols, row0 + RB, pr.d);
+    u32x4 ca[2], cb[2];
+#pragma unroll
+    for (uint32_t s = 0; s < 2; ++s) {
+      ca[s] = xa[s];
+      if constexpr (BF) cb[s] = xb[s];
+    }
+    if (row0 + RB < r1) {
+      st.load(A, pr.ld, cols, row0 + RB, pr.d);
+      load_dx(row0 + RB);
+    }
     if constexpr (BF) {
-      const uint16_t* xh = pr.DXh + (uint64_t)li * pr.dstage + row0;
-      const uint16_t* xl = pr.DXl + (uint64_t)li * pr.dstage + row0;
 #pragma unroll
       for (uint32_t s = 0; s < 2; ++s) {
         const uint32_t rr = wave * 64 + s * 32 + 8 * lg;  // row within the round
-        const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4*>(xh + rr));
-        const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4*>(xl + rr));
+        const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, ca[s]);
+        const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, cb[s]);
 #pragma unroll
         for (uint32_t G = 0; G < 4; ++G) {
           if (G < ngroups) {
@@ -601,11 +626,10 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
         }
       }
     } else {
-      const float* xs = pr.DX + (uint64_t)li * pr.dstage + row0;
 #pragma unroll
       for (uint32_t s = 0; s < 2; ++s) {
         const uint32_t rr = wave * 32 + s * 16 + 4 * lg;
-        const u32x4 b = *reinterpret_cast<const u32x4*>(xs + rr);
+        const u32x4 b = ca[s];
 #pragma unroll
         for (uint32_t G = 0; G < 4; ++G) {
           if (G < ngroups) {
@@ -694,7 +718,7 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
   constexpr uint32_t WR = 64 * PASSES;  // rows per wave
   using Stage = TileStage<T, C, RB>;
   __shared__ __align__(16) char lds[Stage::kBytes];
-  __shared__ uint32_t cols[C];
+  __shared__ uint32_t cols[2][C];                // column indices, double-buffered by round
   __shared__ __align__(16) uint16_t th[kBT][C];  // BF16 t, high part [trial][column]
   __shared__ __align__(16) uint16_t tlo[kBT][C];
   __shared__ float tf[C][kBT];                   // FP32 t [column][trial]
@@ -719,23 +743,30 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
   f32x4_t acc[NACC];
 #pragma unroll
   for (uint32_t m = 0; m < NACC; ++m) acc[m] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  // Every load of round r + 1 (A tile, t, and the column indices of round
+  // r + 2) is issued right after round r's first barrier, so it overlaps
+  // round r's MFMAs; one barrier pair per round.
+  constexpr uint32_t NT = C * kBT / 256;  // t entries per thread
+  const float* Tc = pr.Tc;
+  if (threadIdx.x < C) cols[0][threadIdx.x] = (sb + threadIdx.x < se) ? pr.colc[sb + threadIdx.x] : 0xFFFFFFFFu;
+  uint32_t colnext = (threadIdx.x < C && sb + C + threadIdx.x < se) ? pr.colc[sb + C + threadIdx.x] : 0xFFFFFFFFu;
+  __syncthreads();
   Stage st;
-  auto set_cols = [&](uint32_t s) {
-    __syncthreads();  // readers of the previous cols are done
-    if (threadIdx.x < C) cols[threadIdx.x] = (s + threadIdx.x < se) ? pr.colc[s + threadIdx.x] : 0xFFFFFFFFu;
-    __syncthreads();
-  };
-  set_cols(sb);
-  st.load(A, pr.ld, cols, R0, d);
-  for (uint32_t s = sb; s < se; s += C) {
-    // t of this round's columns
-    for (uint32_t i = threadIdx.x; i < C * kBT; i += 256) {
-      const uint32_t c = i >> 4, k = i & 15;
-      float tt = 0.0f;
-      if (s + c < se) {
-        tt = pr.Tc[(uint64_t)(s + c) * kBT + k];
-        if (proposed_any && rb == 0) corr += (double)pr.alpha[cols[c]] * (double)tt;
-      }
+  st.load(A, pr.ld, cols[0], R0, d);
+  float tr[NT];
+#pragma unroll
+  for (uint32_t q = 0; q < NT; ++q) {
+    const uint32_t i = threadIdx.x + 256 * q, c = i >> 4;
+    tr[q] = (sb + c < se) ? Tc[(uint64_t)(sb + c) * kBT + (i & 15)] : 0.0f;
+  }
+  uint32_t buf = 0;
+  for (uint32_t s = sb; s < se; s += C, buf ^= 1u) {
+    // this round's t into LDS (BF16: hi + lo), its A tile, next round's columns
+#pragma unroll
+    for (uint32_t q = 0; q < NT; ++q) {
+      const uint32_t i = threadIdx.x + 256 * q, c = i >> 4, k = i & 15;
+      const float tt = tr[q];
+      if (proposed_any && rb == 0 && s + c < se) corr += (double)pr.alpha[cols[buf][c]] * (double)tt;
       if constexpr (BF) {
         const uint16_t h = bf16_rne(tt);
         th[k][c] = h;
@@ -745,10 +776,17 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
       }
     }
     st.store(lds);
+    if (threadIdx.x < C) cols[buf ^ 1u][threadIdx.x] = colnext;
     __syncthreads();
-    if (s + C < se) {
-      set_cols(s + C);
-      st.load(A, pr.ld, cols, R0, d);
+    const uint32_t sn = s + C;
+    if (sn < se) {
+      st.load(A, pr.ld, cols[buf ^ 1u], R0, d);
+#pragma unroll
+      for (uint32_t q = 0; q < NT; ++q) {
+        const uint32_t i = threadIdx.x + 256 * q, c = i >> 4;
+        tr[q] = (sn + c < se) ? Tc[(uint64_t)(sn + c) * kBT + (i & 15)] : 0.0f;
+      }
+      colnext = (threadIdx.x < C && sn + C + threadIdx.x < se) ? pr.colc[sn + C + threadIdx.x] : 0xFFFFFFFFu;
     }
     if constexpr (BF) {
       const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4*>(&th[li][8 * lg]));
