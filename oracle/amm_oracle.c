@@ -29,6 +29,9 @@
  */
 #include <math.h>
 #include <stdint.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -116,6 +119,7 @@ static inline float bf16_to_f32(uint16_t h)
 static void colstats_core(const void *A, int bf16, uint64_t d, uint64_t n, uint64_t ld,
                           float *alpha, double *nrm2, double *cnrm2)
 {
+#pragma omp parallel for schedule(static)
   for (uint64_t j = 0; j < n; ++j) {
     double s = 0.0, ss = 0.0;
     for (uint64_t i = 0; i < d; ++i) {
@@ -283,6 +287,35 @@ static double col_dot(const orc_mat *M, uint64_t j, const double *v)
   return s;
 }
 
+/* acc[i] += sum_e a_{cols[e]}[i] * t[e], every element summed in entry
+ * order (the serial order); row blocks run in parallel (OpenMP), so the
+ * result does not depend on the thread count. */
+#define ORC_ROW_BLOCK 1024
+static void axpy_rows(const float *A, uint64_t ld, uint64_t d, uint64_t m, const uint64_t *cols,
+                      const double *t, double *acc)
+{
+  const int64_t nblk = (int64_t)((d + ORC_ROW_BLOCK - 1) / ORC_ROW_BLOCK);
+#pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b < nblk; ++b) {
+    const uint64_t i0 = (uint64_t)b * ORC_ROW_BLOCK;
+    const uint64_t i1 = i0 + ORC_ROW_BLOCK < d ? i0 + ORC_ROW_BLOCK : d;
+    for (uint64_t e = 0; e < m; ++e) {
+      const float *a = A + (cols ? cols[e] : e) * ld;
+      const double te = t[e];
+      for (uint64_t i = i0; i < i1; ++i) acc[i] += (double)a[i] * te;
+    }
+  }
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 void orc_amm_product(const float *A, uint64_t d, uint64_t n, uint64_t ld, int proposed,
                      uint32_t c, const uint32_t *idx, const float *wgt, const uint32_t *seg_count,
                      int sanity, const float *x, const float *x_memo, const float *y_memo,
@@ -298,26 +331,38 @@ void orc_amm_product(const float *A, uint64_t d, uint64_t n, uint64_t ld, int pr
   if (proposed) {
     for (uint64_t i = 0; i < d; ++i) { s += dx[i]; q += (double)z[i] * dx[i]; }
   }
-  double corr = 0.0;
+  /* the list in order: segment by segment (sampled) or all n columns (sanity) */
   const uint32_t nseg = sanity ? 1 : (c + SAMPLE_SEG - 1) / SAMPLE_SEG;
+  uint64_t m = 0;
+  for (uint32_t g = 0; g < nseg; ++g) m += sanity ? n : seg_count[g];
+  uint64_t *cols = (uint64_t *)malloc(sizeof(uint64_t) * (m ? m : 1));
+  double *w = (double *)malloc(sizeof(double) * (m ? m : 1));
+  double *t = (double *)malloc(sizeof(double) * (m ? m : 1));
+  uint64_t e = 0;
   for (uint32_t g = 0; g < nseg; ++g) {
     uint32_t cnt = sanity ? (uint32_t)n : seg_count[g];
-    for (uint32_t p = 0; p < cnt; ++p) {
-      uint64_t j = sanity ? p : idx[g * SAMPLE_SEG + p];
-      double wj = sanity ? 1.0 : (double)wgt[g * SAMPLE_SEG + p];
-      double r = col_dot(&M, j, dx);
-      double aj = proposed ? (double)alpha[j] : 0.0;
-      double t = wj * (r - aj * s);
-      const float *a = A + j * ld;
-      for (uint64_t i = 0; i < d; ++i) acc[i] += (double)a[i] * t;
-      corr += aj * t;
+    for (uint32_t p = 0; p < cnt; ++p, ++e) {
+      cols[e] = sanity ? p : idx[g * SAMPLE_SEG + p];
+      w[e] = sanity ? 1.0 : (double)wgt[g * SAMPLE_SEG + p];
     }
   }
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < (int64_t)m; ++k) {
+    const double r = col_dot(&M, cols[k], dx);
+    const double aj = proposed ? (double)alpha[cols[k]] : 0.0;
+    t[k] = w[k] * (r - aj * s);
+  }
+  double corr = 0.0;
+  for (uint64_t k = 0; k < m; ++k) corr += (proposed ? (double)alpha[cols[k]] : 0.0) * t[k];
+  axpy_rows(A, ld, d, m, cols, t, acc);
   for (uint64_t i = 0; i < d; ++i) {
     double b = y_memo ? (double)y_memo[i] : 0.0;
     if (proposed) b += s * (double)v[i] + q - corr;
     y[i] = (float)(b + acc[i]);
   }
+  free(cols);
+  free(w);
+  free(t);
   free(dx);
   free(acc);
 }
@@ -327,14 +372,14 @@ void orc_exact_product(const float *A, uint64_t d, uint64_t n, uint64_t ld, cons
 {
   double *xv = (double *)malloc(sizeof(double) * d);
   double *acc = (double *)calloc(d, sizeof(double));
+  double *t = (double *)malloc(sizeof(double) * (n ? n : 1));
   orc_mat M = {A, d, n, ld};
   for (uint64_t i = 0; i < d; ++i) xv[i] = x[i];
-  for (uint64_t j = 0; j < n; ++j) {
-    double t = col_dot(&M, j, xv);
-    const float *a = A + j * ld;
-    for (uint64_t i = 0; i < d; ++i) acc[i] += (double)a[i] * t;
-  }
+#pragma omp parallel for schedule(static)
+  for (int64_t j = 0; j < (int64_t)n; ++j) t[j] = col_dot(&M, (uint64_t)j, xv);
+  axpy_rows(A, ld, d, n, NULL, t, acc);
   for (uint64_t i = 0; i < d; ++i) y[i] = (float)acc[i];
+  free(t);
   free(xv);
   free(acc);
 }
@@ -413,10 +458,10 @@ int orc_power_method(const float *A, uint64_t d, uint64_t n, uint64_t ld, const 
   for (uint64_t j = 0; j < n; ++j) a2 += (double)alpha[j] * (double)alpha[j];
   if (kern >= 2) {
     double *acc = (double *)calloc(d, sizeof(double));
-    for (uint64_t j = 0; j < n; ++j) {
-      const float *a = A + j * ld;
-      for (uint64_t i = 0; i < d; ++i) acc[i] += (double)a[i] * (double)alpha[j];
-    }
+    double *al = (double *)malloc(sizeof(double) * (n ? n : 1));
+    for (uint64_t j = 0; j < n; ++j) al[j] = (double)alpha[j];
+    axpy_rows(A, ld, d, n, NULL, al, acc);
+    free(al);
     for (uint64_t i = 0; i < d; ++i) {
       z[i] = (float)acc[i];
       v[i] = (float)((double)z[i] - a2);

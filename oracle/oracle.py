@@ -48,6 +48,7 @@ _SIGS = {
     "orc_ref_cdf_f32": (None, [_vp, _u64, _vp]),
     "orc_ref_pick": (_u32, [_vp, _u64, C.c_float]),
     "orc_ref_power_method": (C.c_int, [_vp, _u64, _u64, _u64, C.c_int32, _u32, _u64, _u64, _vp, _vp]),
+    "orc_num_threads": (C.c_int, []),
 }
 
 KERNELS = {"exact": 0, "previousAMM": 1, "proposedAMM": 2, "proposedAMM_v2": 3}
@@ -82,6 +83,11 @@ def lib() -> C.CDLL:
             f.argtypes = a
         _lib = L
     return _lib
+
+
+def num_threads() -> int:
+    """OpenMP threads the oracle's column and row-block loops use."""
+    return int(lib().orc_num_threads())
 
 
 def _p(a):
