@@ -201,3 +201,34 @@ def test_bf16_image_workload():
     assert np.all(rel[:, -1] < 1e-5), rel[:, -1]
     settled = [int(np.nonzero(rel[k] > 1e-3)[0][-1]) + 1 if np.any(rel[k] > 1e-3) else 0 for k in range(16)]
     assert max(settled) < 120
+
+
+@pytest.mark.parametrize("shapes,trials,c", [
+    ([(5, 3), (9, 70), (1, 40)], 3, 1),                       # tiny and ragged, one draw per product
+    ([(7, 129), (300, 65), (64, 64)], 1, 8),                 # one trial; n just past a 64-column tile
+    ([(257, 513), (8, 8), (130, 1000)], 16, 100),            # d just past a 256-row stage
+])
+@pytest.mark.parametrize("bf16", [False, True])
+def test_batch_edge_shapes(shapes, trials, c, bf16):
+    """Ragged d, n around the tile sizes, one-column and one-row matrices,
+    c = 1: every trial still follows the single-matrix run."""
+    banner(f"batch edge shapes {shapes} trials={trials} c={c} bf16={bf16}")
+    AmmBatch, _, RunParams, _ = _api()
+    probs, mats = [], []
+    for i, (d, n) in enumerate(shapes):
+        A = lowrank_plus_noise(d, n, seed=300 + i, noise=0.1) + 0.05
+        ld = (d + 7) // 8 * 8
+        t = dev(A, bf16, ld)
+        mats.append((t, d))
+        probs.append(dict(A=t, d=d, c=min(c, n), seed=900 + i))
+    params = RunParams(kernel="previousAMM", step=9, vr_period=3, trial=2)
+    with AmmBatch(probs, trials=trials) as b:
+        r = b.power_method(params)
+        modes = b.last_modes(params.step)
+    for i, (t, d) in enumerate(mats):
+        for k in range(trials):
+            p = RunParams(kernel="previousAMM", c=probs[i]["c"], step=9, vr_period=3, trial=2 + k,
+                          seed=probs[i]["seed"])
+            e, _, m = _single(t, d, p, 2 + k)
+            assert np.array_equal(modes[i, k], m)
+            assert relmax(r.eig[i, k], e) < 1e-4, (i, k, r.eig[i, k], e)
