@@ -328,21 +328,33 @@ amm::BRun run_args(amm_batch* b, const amm_params& p) {
 }
 
 // Column statistics, CDFs and z = A alpha of every problem (the
-// single-matrix kernels, one problem after another on the stream).
+// single-matrix kernels, one problem after another on the stream).  z rides
+// in the statistics pass where the columns fit it (the partials of one
+// problem are summed before the next problem's pass reuses the buffer).
 int enq_setup(amm_batch* b, int K) {
   const bool need_plain = K == AMM_KERNEL_PREVIOUS || K == AMM_KERNEL_PROPOSED_V2;
   const bool need_cent = K == AMM_KERNEL_PROPOSED || K == AMM_KERNEL_PROPOSED_V2;
   for (uint32_t i = 0; i < b->np; ++i) {
     const BProb& p = b->hp[i];
     const SetupBufs& s = b->sb[i];
-    BHIPCHK(amm::launch_colstats(p.dtype, p.A, p.ld, p.d, p.n, s.alpha, s.nrm2, s.cnrm2, s.st,
-                                 b->stream));
+    const bool fused = need_cent && amm::colstats_fuses_z(p.dtype, p.d);
+    if (fused)
+      BHIPCHK(amm::launch_colstats_z(p.dtype, p.A, p.ld, p.d, p.n, s.dpad8, s.alpha, s.nrm2, s.cnrm2,
+                                     b->zP2, s.st, b->stream));
+    else
+      BHIPCHK(amm::launch_colstats(p.dtype, p.A, p.ld, p.d, p.n, s.alpha, s.nrm2, s.cnrm2, s.st,
+                                   b->stream));
     if (need_plain)
       BHIPCHK(amm::launch_cdf(s.nrm2, p.n, s.cdf0, s.st, 0, nullptr, b->tile_max, b->tile_sum,
                               b->tile_lp, b->tile_a2, b->stream));
     if (need_cent) {
       BHIPCHK(amm::launch_cdf(s.cnrm2, p.n, s.cdf1, s.st, 1, s.alpha, b->tile_max, b->tile_sum,
                               b->tile_lp, b->tile_a2, b->stream));
+      if (fused) {
+        BHIPCHK(amm::launch_sum_rows(b->zP2, amm::colstats_z_blocks(p.dtype, p.d, p.n), s.dpad8, p.d, s.z,
+                                     s.v, s.st, b->stream));
+        continue;
+      }
       amm::GatherAxpyArgs a{};
       a.A = p.A;
       a.ld = p.ld;
@@ -600,7 +612,10 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     f.eigvec = pl.take<double>(dpad8);
     b->sb[i].dpad8 = dpad8;
     max_n = std::max<uint64_t>(max_n, q.n);
-    max_zs = std::max<uint64_t>(max_zs, cdiv(q.n, kZChunk) * dpad8);
+    uint64_t zs = cdiv(q.n, kZChunk);
+    if (amm::colstats_fuses_z(q.dtype, (uint32_t)q.d))
+      zs = std::max<uint64_t>(zs, amm::colstats_z_blocks(q.dtype, (uint32_t)q.d, q.n));
+    max_zs = std::max<uint64_t>(max_zs, zs * dpad8);
   }
   if (hipMalloc((void**)&b->spool, pl.bytes) != hipSuccess)
     return bail(fail(AMM_E_NOMEM, "device allocation of " + std::to_string(pl.bytes) + " bytes failed"));

@@ -129,6 +129,8 @@ struct amm_ctx {
   size_t tvec_cap = 0;
   double* tbp = nullptr;  // block partials of ||A^T u||^2
   size_t tbp_cap = 0;
+  float* ZP = nullptr;  // per-workgroup partials of z = A alpha from the statistics pass
+  size_t ZP_cap = 0;
 
   bool stats_valid = false;
 
@@ -232,6 +234,27 @@ int ensure_lists(amm_ctx* c, uint64_t nit, uint32_t cc) {
 int enq_stats(amm_ctx* c) {
   HIPCHK(amm::launch_colstats(c->dtype, c->A, c->ld, (uint32_t)c->d, c->n, c->alpha, c->nrm2,
                               c->cnrm2, c->st, c->stream));
+  return AMM_OK;
+}
+
+// z = A alpha rides in the statistics pass when the columns fit the
+// register-held form (d <= 16384 FP32, 32768 BF16) and the partial buffer
+// is there; the run then finishes it with enq_z_partials after the
+// centred CDF has set ||alpha||^2.
+bool z_fusable(const amm_ctx* c) {
+  return amm::colstats_fuses_z(c->dtype, (uint32_t)c->d) &&
+         c->ZP_cap >= (size_t)amm::colstats_z_blocks(c->dtype, (uint32_t)c->d, c->n) * c->dpad;
+}
+
+int enq_stats_z(amm_ctx* c) {
+  HIPCHK(amm::launch_colstats_z(c->dtype, c->A, c->ld, (uint32_t)c->d, c->n, c->dpad, c->alpha, c->nrm2,
+                                c->cnrm2, c->ZP, c->st, c->stream));
+  return AMM_OK;
+}
+
+int enq_z_partials(amm_ctx* c) {
+  const uint32_t S = amm::colstats_z_blocks(c->dtype, (uint32_t)c->d, c->n);
+  HIPCHK(amm::launch_sum_rows(c->ZP, S, c->dpad, (uint32_t)c->d, c->z, c->v, c->st, c->stream));
   return AMM_OK;
 }
 
@@ -524,18 +547,24 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
   const bool sanity = K != AMM_KERNEL_EXACT && p.sanity && p.c == c->n;
   const uint32_t step = (uint32_t)p.step;
   HIPCHK(amm::launch_begin(c->xm, c->ym, c->dpad, c->st, c->mode, adaptive ? modes[0] : 0, c->stream));
-  // the exact power method needs no column statistics
-  if (K != AMM_KERNEL_EXACT && (!(p.flags & AMM_FLAG_KEEP_STATS) || !c->stats_valid)) {
-    if ((rc = enq_stats(c))) return rc;
-  }
   const bool need_plain = K == AMM_KERNEL_PREVIOUS || K == AMM_KERNEL_PROPOSED_V2;
   const bool need_cent = K == AMM_KERNEL_PROPOSED || K == AMM_KERNEL_PROPOSED_V2;
+  // the exact power method needs no column statistics; the centred kernels
+  // take z = A alpha from the same read of A when it fits
+  bool fused_z = false;
+  if (K != AMM_KERNEL_EXACT && (!(p.flags & AMM_FLAG_KEEP_STATS) || !c->stats_valid)) {
+    fused_z = need_cent && z_fusable(c);
+    if ((rc = fused_z ? enq_stats_z(c) : enq_stats(c))) return rc;
+  }
   if (need_plain && !sanity)
     if ((rc = enq_cdf(c, 0, false))) return rc;
   if (need_cent)
     if ((rc = enq_cdf(c, 1, false))) return rc;  // also ||alpha||^2
-  if (K == AMM_KERNEL_PROPOSED)
+  if (fused_z) {
+    if ((rc = enq_z_partials(c))) return rc;
+  } else if (K == AMM_KERNEL_PROPOSED) {
     if ((rc = enq_z(c))) return rc;
+  }
   if (!sanity && K != AMM_KERNEL_EXACT) {
     if (K == AMM_KERNEL_PREVIOUS) {
       if ((rc = enq_sample(c, 0, 0, step, 0, p.c))) return rc;
@@ -561,7 +590,7 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
       const uint32_t md = modes[i];
       if (md == amm::kModeExactMemo || md == amm::kModeExactNoMemo) {
         if ((rc = enq_exact(c, i, nullptr, 0))) return rc;
-        z_after = K == AMM_KERNEL_PROPOSED_V2 && i + 1 == P;
+        z_after = K == AMM_KERNEL_PROPOSED_V2 && i + 1 == P && !fused_z;
       } else {
         if ((rc = enq_amm(c, i, i, md == amm::kModeAmmProposed, sanity, nullptr, 0))) return rc;
       }
@@ -692,7 +721,7 @@ int amm_ctx_destroy(amm_ctx* c) {
                   c->x,     c->y,     c->xm,       c->ym,      c->eigvec, c->st,  c->BP,
                   c->mode,  c->eig,   c->err,      c->ts,      c->idx,    c->wgt, c->segc,
                   c->P1,    c->P2,    c->P2s,      c->tile_sum, c->tile_lp, c->tile_a2,
-                  c->tile_max, c->tvec, c->tbp};
+                  c->tile_max, c->tvec, c->tbp, c->ZP};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -859,6 +888,11 @@ int amm_power_method(amm_ctx* c, const amm_params* p, double* time_ms, double* e
     if ((rc = ensure_step(c, p->step))) return rc;
     if (cc)
       if ((rc = ensure_lists(c, p->step, cc))) return rc;
+    if ((p->kernel == AMM_KERNEL_PROPOSED || p->kernel == AMM_KERNEL_PROPOSED_V2) &&
+        amm::colstats_fuses_z(c->dtype, (uint32_t)c->d))
+      if ((rc = grow(c->ZP, c->ZP_cap,
+                     (size_t)amm::colstats_z_blocks(c->dtype, (uint32_t)c->d, c->n) * c->dpad)))
+        return rc;
     c->host_modes = static_modes(*p);
     if (is_adaptive(*p)) {
       // iteration 0: prev_cos = 0, last_exact = 0

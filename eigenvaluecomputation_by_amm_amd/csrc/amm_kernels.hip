@@ -93,6 +93,132 @@ __global__ __launch_bounds__(256) void k_colstats(const T* __restrict__ A, uint6
 }
 
 // ---------------------------------------------------------------------------
+// Column statistics with z = A alpha in the same read of A (get_norms,
+// device_memory.h:34-59, and the z = A alpha product of
+// powerMethod_by_proposed.cuh:132-139).  Persistent workgroups walk the
+// columns j = blockIdx.x, blockIdx.x + gridDim.x, ...; wave w holds quarter w
+// of the column in registers (L non-temporal 16-byte loads per lane, the
+// next column prefetched), the workgroup forms alpha_j, and every lane adds
+// alpha_j a_j to its rows of a private FP32 partial of z, written to
+// ZP[blockIdx.x][:] at the end (k_sum_rows adds the partials in FP64).
+// Rows per wave q = L x 64 lanes x VEC (d <= 4 L VEC 64).
+// ---------------------------------------------------------------------------
+template <typename T, int L>
+__global__ __launch_bounds__(256) void k_colstats_z(const T* __restrict__ A, uint64_t ld, uint32_t d,
+                                                    uint64_t n, uint64_t dpad, float* __restrict__ alpha,
+                                                    double* __restrict__ nrm2, double* __restrict__ cnrm2,
+                                                    float* __restrict__ ZP) {
+  constexpr int V = Elem<T>::VEC;
+  constexpr uint32_t step = 64u * V;
+  __shared__ double part[2][2][4];  // [column parity][sum | sum of squares][wave]
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t q = (((d + 3) / 4) + step - 1) / step * step;  // <= L * step (host)
+  const uint32_t rb = w * q;
+  const uint32_t re = rb + q < d ? rb + q : d;
+  u32x4 raw[L];
+  auto load = [&](uint64_t j) {
+    const T* col = A + j * ld;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const uint32_t row = rb + lane * V + l * step;
+      if (row + V <= re) {
+        raw[l] = load16<true>(col + row);
+      } else {
+        float v[V];
+        load_col<T>(col, row, re, v);  // zeros at and beyond re
+        if constexpr (V == 4) {
+          raw[l] = u32x4{__float_as_uint(v[0]), __float_as_uint(v[1]), __float_as_uint(v[2]),
+                         __float_as_uint(v[3])};
+        } else {
+          uint32_t h[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            h[k] = (__float_as_uint(v[2 * k]) >> 16) | (__float_as_uint(v[2 * k + 1]) & 0xFFFF0000u);
+          raw[l] = u32x4{h[0], h[1], h[2], h[3]};
+        }
+      }
+    }
+  };
+  auto unpack = [&](const u32x4& r, float (&v)[V]) {
+    if constexpr (V == 4) {
+      v[0] = __uint_as_float(r.x);
+      v[1] = __uint_as_float(r.y);
+      v[2] = __uint_as_float(r.z);
+      v[3] = __uint_as_float(r.w);
+    } else {
+      v[0] = bf16_bits_to_f32(r.x & 0xFFFFu);
+      v[1] = bf16_bits_to_f32(r.x >> 16);
+      v[2] = bf16_bits_to_f32(r.y & 0xFFFFu);
+      v[3] = bf16_bits_to_f32(r.y >> 16);
+      v[4] = bf16_bits_to_f32(r.z & 0xFFFFu);
+      v[5] = bf16_bits_to_f32(r.z >> 16);
+      v[6] = bf16_bits_to_f32(r.w & 0xFFFFu);
+      v[7] = bf16_bits_to_f32(r.w >> 16);
+    }
+  };
+  float zp[L * V];
+#pragma unroll
+  for (int e = 0; e < L * V; ++e) zp[e] = 0.0f;
+  uint64_t j = blockIdx.x;
+  if (j < n) load(j);
+  for (int par = 0; j < n; j += gridDim.x, par ^= 1) {
+    double s = 0.0, ss = 0.0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      float v[V];
+      unpack(raw[l], v);
+#pragma unroll
+      for (int k = 0; k < V; ++k) {
+        const double b = v[k];
+        s += b;
+        ss += b * b;
+      }
+    }
+    s = wave_sum(s);
+    ss = wave_sum(ss);
+    if (lane == 0) {
+      part[par][0][w] = s;
+      part[par][1][w] = ss;
+    }
+    u32x4 cur[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) cur[l] = raw[l];
+    if (j + gridDim.x < n) load(j + gridDim.x);  // the next column streams in meanwhile
+    __syncthreads();
+    const double S = (part[par][0][0] + part[par][0][1]) + (part[par][0][2] + part[par][0][3]);
+    const double SS = (part[par][1][0] + part[par][1][1]) + (part[par][1][2] + part[par][1][3]);
+    const double mean = S / (double)d;
+    if (threadIdx.x == 0) {
+      alpha[j] = (float)mean;
+      nrm2[j] = SS;
+      double c = SS - S * mean;
+      c = c > 0.0 ? c : (c == c ? 0.0 : c);  // clamp rounding below 0, keep NaN
+      cnrm2[j] = c;
+    }
+    const float af = (float)mean;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      float v[V];
+      unpack(cur[l], v);
+#pragma unroll
+      for (int k = 0; k < V; ++k) zp[l * V + k] = fmaf(af, v[k], zp[l * V + k]);
+    }
+  }
+  // rows [0, dpad) are covered: 4 q is a multiple of 1024 >= d, dpad <= 4 q
+  float* out = ZP + (uint64_t)blockIdx.x * dpad;
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const uint32_t row = rb + lane * V + l * step;
+    if (row < dpad) {
+      float v[V];
+#pragma unroll
+      for (int k = 0; k < V; ++k) v[k] = zp[l * V + k];
+      store_vecf<V>(out + row, v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Sampling CDF (set_weights + cub::DeviceScan::InclusiveSum,
 // common.h:34-40, powerMethod_by_previous.cuh:26-32) in exact integer
 // arithmetic:
@@ -885,15 +1011,38 @@ __global__ __launch_bounds__(256) void k_dots(const float* __restrict__ x,
 }
 
 // z = sum_s P2[s] (z = A alpha, powerMethod_by_proposed.cuh:132-139) and
-// v = z - ||alpha||^2 (generate_v, core_proposedAMM.cuh:4-15).
-__global__ __launch_bounds__(256) void k_sum_rows(const float* __restrict__ P2, uint32_t S,
-                                                  uint64_t dpad, uint32_t d, float* __restrict__ z,
-                                                  float* __restrict__ v, const DevState* st) {
+// v = z - ||alpha||^2 (generate_v, core_proposedAMM.cuh:4-15).  A workgroup
+// owns 256 rows (a float4 per lane); its 16 waves take the partials
+// s = w, w + 16, ... in FP64 and are added in wave order, so the result
+// depends only on S and the partials.
+constexpr uint32_t kSumRowsWaves = 16;
+__global__ __launch_bounds__(1024) void k_sum_rows(const float* __restrict__ P2, uint32_t S,
+                                                   uint64_t dpad, uint32_t d, float* __restrict__ z,
+                                                   float* __restrict__ v, const DevState* st) {
+  __shared__ double red[kSumRowsWaves][256];
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t r0 = (uint64_t)blockIdx.x * 256 + lane * 4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (r0 < dpad) {  // dpad is a multiple of 8, so r0 + 4 <= dpad
+#pragma unroll 4
+    for (uint32_t s = w; s < S; s += kSumRowsWaves) {
+      const float4 q = *reinterpret_cast<const float4*>(P2 + (uint64_t)s * dpad + r0);
+      acc[0] += (double)q.x;
+      acc[1] += (double)q.y;
+      acc[2] += (double)q.z;
+      acc[3] += (double)q.w;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) red[w][lane * 4 + k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x >= 256) return;
   const uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (r >= dpad) return;
-  double acc = 0.0;
-  for (uint32_t s = 0; s < S; ++s) acc += (double)P2[(uint64_t)s * dpad + r];
-  const float zf = r < d ? (float)acc : 0.0f;
+  double t = 0.0;
+#pragma unroll
+  for (uint32_t k = 0; k < kSumRowsWaves; ++k) t += red[k][threadIdx.x];
+  const float zf = r < d ? (float)t : 0.0f;
   z[r] = zf;
   if (v) v[r] = r < d ? (float)((double)zf - st->alpha2) : 0.0f;
 }
@@ -981,10 +1130,65 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint
 // ===========================================================================
 static inline unsigned grid1(uint64_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
 
+// The fused statistics hold q = L x 64 x VEC rows per wave in registers,
+// L <= 16: d <= 16384 (FP32) / 32768 (BF16).
+static int colstats_L(int dtype, uint32_t d) {
+  const uint32_t V = dtype == 1 ? 8u : 4u, step = 64u * V;
+  const uint32_t q = ((d + 3) / 4 + step - 1) / step * step;
+  const uint32_t L = q / step;
+  if (L <= 1) return 1;
+  if (L <= 2) return 2;
+  if (L <= 4) return 4;
+  if (L <= 8) return 8;
+  if (L <= 16) return 16;
+  return 0;
+}
+
+// Persistent workgroups: as many as stay resident (two per CU at 16 loads
+// per lane, four below), at most one per column.
+static uint32_t colstats_reg_blocks(int L, uint64_t n) {
+  return (uint32_t)std::min<uint64_t>(n, L >= 16 ? 512 : 1024);
+}
+
+uint32_t colstats_z_blocks(int dtype, uint32_t d, uint64_t n) {
+  return colstats_reg_blocks(colstats_L(dtype, d), n);
+}
+
+bool colstats_fuses_z(int dtype, uint32_t d) { return colstats_L(dtype, d) != 0; }
+
+template <typename T>
+static void cs_reg(int L, const void* A, uint64_t ld, uint32_t d, uint64_t n, uint64_t dpad, float* alpha,
+                   double* nrm2, double* cnrm2, float* ZP, hipStream_t s) {
+  const unsigned g = colstats_reg_blocks(L, n);
+  const T* a = (const T*)A;
+  switch (L) {
+    case 1: k_colstats_z<T, 1><<<g, 256, 0, s>>>(a, ld, d, n, dpad, alpha, nrm2, cnrm2, ZP); break;
+    case 2: k_colstats_z<T, 2><<<g, 256, 0, s>>>(a, ld, d, n, dpad, alpha, nrm2, cnrm2, ZP); break;
+    case 4: k_colstats_z<T, 4><<<g, 256, 0, s>>>(a, ld, d, n, dpad, alpha, nrm2, cnrm2, ZP); break;
+    case 8: k_colstats_z<T, 8><<<g, 256, 0, s>>>(a, ld, d, n, dpad, alpha, nrm2, cnrm2, ZP); break;
+    default: k_colstats_z<T, 16><<<g, 256, 0, s>>>(a, ld, d, n, dpad, alpha, nrm2, cnrm2, ZP); break;
+  }
+}
+
+hipError_t launch_colstats_z(int dtype, const void* A, uint64_t ld, uint32_t d, uint64_t n, uint64_t dpad,
+                             float* alpha, double* nrm2, double* cnrm2, float* ZP, DevState* st,
+                             hipStream_t s) {
+  (void)st;
+  const int L = colstats_L(dtype, d);
+  if (!L || !ZP) return hipErrorInvalidValue;  // z cannot be fused at this d
+  if (dtype == 1)
+    cs_reg<__hip_bfloat16>(L, A, ld, d, n, dpad, alpha, nrm2, cnrm2, ZP, s);
+  else
+    cs_reg<float>(L, A, ld, d, n, dpad, alpha, nrm2, cnrm2, ZP, s);
+  return hipGetLastError();
+}
+
+// Without z the streamed form (one workgroup per column) is as fast or
+// faster at every shape measured.
 hipError_t launch_colstats(int dtype, const void* A, uint64_t ld, uint32_t d, uint64_t n,
                            float* alpha, double* nrm2, double* cnrm2, DevState* st, hipStream_t s) {
-  const dim3 g((unsigned)n);  // one workgroup per column
   (void)st;
+  const dim3 g((unsigned)n);  // one workgroup per column, streamed
   if (dtype == 1)
     k_colstats<__hip_bfloat16><<<g, 256, 0, s>>>((const __hip_bfloat16*)A, ld, d, n, alpha, nrm2, cnrm2);
   else
@@ -1102,7 +1306,7 @@ hipError_t launch_normalize(const float* y, float* x, uint32_t d, uint64_t dpad,
 
 hipError_t launch_sum_rows(const float* P2, uint32_t S, uint64_t dpad, uint32_t d, float* z, float* v,
                            const DevState* st, hipStream_t s) {
-  k_sum_rows<<<grid1(dpad, 256), 256, 0, s>>>(P2, S, dpad, d, z, v, st);
+  k_sum_rows<<<grid1(dpad, 256), 1024, 0, s>>>(P2, S, dpad, d, z, v, st);
   return hipGetLastError();
 }
 

@@ -114,6 +114,14 @@ uint64_t cdf_tiles(uint64_t n);
 
 hipError_t launch_colstats(int dtype, const void* A, uint64_t ld, uint32_t d, uint64_t n,
                            float* alpha, double* nrm2, double* cnrm2, DevState* st, hipStream_t s);
+// Column statistics and the z = A alpha partials
+// ZP[colstats_z_blocks(dtype, d, n)][dpad] in the same read of A (finish
+// with launch_sum_rows).  Needs colstats_fuses_z(dtype, d).
+hipError_t launch_colstats_z(int dtype, const void* A, uint64_t ld, uint32_t d, uint64_t n, uint64_t dpad,
+                             float* alpha, double* nrm2, double* cnrm2, float* ZP, DevState* st,
+                             hipStream_t s);
+uint32_t colstats_z_blocks(int dtype, uint32_t d, uint64_t n);
+bool colstats_fuses_z(int dtype, uint32_t d);
 hipError_t launch_cdf(const double* w, uint64_t n, uint64_t* cdf, DevState* st, int which,
                       const float* alpha, uint64_t* tile_max, uint64_t* tile_sum, uint64_t* tile_lp,
                       double* tile_a2, hipStream_t s);

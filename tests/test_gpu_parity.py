@@ -212,6 +212,43 @@ def test_graph_and_stream_launch_agree_bitwise():
     assert np.array_equal(a.eig, b.eig) and np.array_equal(a.eig, c.eig)
 
 
+@pytest.mark.parametrize("kernel", ["proposedAMM", "proposedAMM_v2"])
+@pytest.mark.parametrize("bf16", [False, True])
+def test_fused_z_equals_separate_pass(kernel, bf16):
+    """z = A alpha formed inside the column-statistics pass (a fresh run)
+    against z from its own gather pass (a keep_stats run reuses the
+    statistics, so only z's summation order differs): same samples, same
+    modes, trajectories within 1e-5."""
+    banner(f"fused z {kernel} bf16={bf16}")
+    AmmContext, RunParams, _ = _amm()
+    A = lowrank_plus_noise(1000, 3000, seed=13) + 0.2
+    t, _ = dev_matrix(A, bf16=bf16)
+    p = dict(kernel=kernel, c=400, step=20, vr_period=5, seed=9)
+    with AmmContext(t) as ctx:
+        a = ctx.power_method(RunParams(**p))
+        ma = ctx.last_modes(20)
+        b = ctx.power_method(RunParams(keep_stats=True, **p))
+        mb = ctx.last_modes(20)
+    assert np.array_equal(ma, mb)
+    assert np.max(np.abs(a.eig - b.eig) / np.abs(b.eig)) < 1e-5
+
+
+def test_proposed_beyond_register_rows():
+    """d above the register-held statistics form (16384 FP32 rows): the
+    streamed statistics and a separate z pass, against the oracle."""
+    banner("proposed d=16392")
+    AmmContext, RunParams, _ = _amm()
+    d, n = 16392, 300
+    A = lowrank_plus_noise(d, n, seed=14) + 0.1
+    t, cols = dev_matrix(A)
+    run = dict(kernel="proposedAMM", c=64, step=8, vr_period=3)
+    with AmmContext(t) as ctx:
+        _, nrm2, cnrm2, _ = ctx.column_stats()
+        res = ctx.power_method(RunParams(seed=21, **run))
+    ref = O.power_method(cols, d, seed=21, w_plain=nrm2, w_cent=cnrm2, **run)
+    assert np.max(np.abs(res.eig - ref["eig"]) / np.abs(ref["eig"])) < 1e-4
+
+
 def test_lambda_small_workload_vs_eigvalsh():
     """The CPU-runnable workload (1024 x 2048, s_i = 10/i, c = 256, 100
     iterations): exact reaches the FP64 eigenvalue; previous AMM with VR
