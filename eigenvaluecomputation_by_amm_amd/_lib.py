@@ -47,6 +47,7 @@ _SIGS = {
     "amm_last_run_ms": (C.c_int, [_vp, _vp]),
     "amm_time_product": (C.c_int, [_vp, C.c_int32, C.c_uint32, C.c_uint64, C.c_uint32, _vp, _vp, _vp]),
     "amm_time_column_stats": (C.c_int, [_vp, C.c_uint32, _vp]),
+    "amm_time_pass": (C.c_int, [_vp, C.c_int32, C.c_uint32, _vp]),
     "amm_generate_symmetric": (C.c_int, [_vp, C.c_uint64, C.c_uint64, C.c_int32, C.c_float, C.c_float,
                                          C.c_uint64, _vp, _vp]),
     "amm_sigma_max": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),

@@ -230,6 +230,15 @@ class AmmContext:
         check(self._lib.amm_time_column_stats(self._h, iters, C.c_void_p(_ptr(t))))
         return float(t[0])
 
+    def time_pass(self, pass_: str, iters: int = 20) -> float:
+        """Mean duration (ms) of one pass as a run issues it: "stats_z"
+        (column statistics with z = A alpha, kernel and partial sum) or
+        "exact" (the exact product without its finish)."""
+        code = {"stats_z": 1, "exact": 2}[pass_]
+        t = np.zeros(1, np.float64)
+        check(self._lib.amm_time_pass(self._h, code, iters, C.c_void_p(_ptr(t))))
+        return float(t[0])
+
     def sigma_max(self, params: RunParams, device=None):
         """sigma_max(A) = sqrt(lambda_max(A A^T)) with unit singular vectors:
         runs the power method of `params`, u = its final iterate (length d),

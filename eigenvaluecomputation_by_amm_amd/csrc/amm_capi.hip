@@ -150,6 +150,7 @@ struct amm_ctx {
 
   // launch plans
   DotPlan dot_dense{}, dot_list{};
+  uint32_t gram_S = 0;  // partials of the one-pass exact product (0: two-pass form)
   AxpyPlan axpy_dense{}, axpy_list{};
   uint32_t list_c = 0;
   // non-temporal loads in the sampled passes: bit 0 dot, bit 1 axpy
@@ -204,7 +205,8 @@ int ensure_plans(amm_ctx* c, uint32_t list_c) {
   }
   size_t p1 = (size_t)c->dot_dense.R * c->n;
   if (list_c) p1 = std::max(p1, (size_t)c->dot_list.R * list_c);
-  const uint32_t S = std::max(c->axpy_dense.S, c->axpy_list.S);
+  c->gram_S = amm::gram_fits(c->dtype, (uint32_t)c->d) ? amm::gram_blocks(c->dtype, (uint32_t)c->d, c->n) : 0;
+  const uint32_t S = std::max({c->axpy_dense.S, c->axpy_list.S, c->gram_S});
   int rc;
   if ((rc = grow(c->P1, c->P1_cap, p1))) return rc;
   if ((rc = grow(c->P2, c->P2_cap, (size_t)S * c->dpad))) return rc;
@@ -294,8 +296,14 @@ int enq_z(amm_ctx* c) {
   return AMM_OK;
 }
 
-// exact product: P1 = A^T x (dense), P2 = A P1
+// exact product y = A (A^T x): one read of A where the columns fit the
+// register-held form, else P1 = A^T x (dense) and P2 = A P1
 int enq_exact(amm_ctx* c, uint32_t it, const uint32_t* gate_mode, uint32_t gate_mask) {
+  if (c->gram_S) {
+    HIPCHK(amm::launch_gram(c->dtype, c->A, c->ld, (uint32_t)c->d, c->n, c->dpad, c->x, c->P2, gate_mode, it,
+                            gate_mask, c->stream));
+    return AMM_OK;
+  }
   amm::GatherDotArgs g{};
   g.A = c->A;
   g.ld = c->ld;
@@ -409,7 +417,7 @@ int enq_finish(amm_ctx* c, uint32_t it, bool sanity, float* y_out, const float* 
                bool norm_inline = false, const float* zq = nullptr, bool stamp = false) {
   amm::FinishArgs f{};
   f.P2 = c->P2;
-  f.S_exact = c->axpy_dense.S;
+  f.S_exact = c->gram_S ? c->gram_S : c->axpy_dense.S;
   f.S_amm = sanity ? c->axpy_dense.S : c->axpy_list.S;
   f.dpad = c->dpad;
   f.d = (uint32_t)c->d;
@@ -1061,6 +1069,39 @@ int amm_time_column_stats(amm_ctx* c, uint32_t iters, double* ms) {
   HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
   *ms = t / iters;
   c->stats_valid = true;
+  return AMM_OK;
+}
+
+int amm_time_pass(amm_ctx* c, int32_t pass, uint32_t iters, double* ms) {
+  if (!c || !ms) return fail(AMM_E_INVALID, "NULL argument");
+  if (iters < 1 || iters > 100000) return fail(AMM_E_INVALID, "iters must be in [1, 100000]");
+  if (pass != AMM_PASS_STATS_Z && pass != AMM_PASS_EXACT) return fail(AMM_E_INVALID, "unknown pass");
+  int rc;
+  if ((rc = set_device(c))) return rc;
+  drop_graph(c);
+  if ((rc = ensure_plans(c, c->list_c))) return rc;
+  if (pass == AMM_PASS_STATS_Z) {
+    if (!amm::colstats_fuses_z(c->dtype, (uint32_t)c->d))
+      return fail(AMM_E_INVALID, "z = A alpha is not fused with the statistics at this d");
+    if ((rc = grow(c->ZP, c->ZP_cap,
+                   (size_t)amm::colstats_z_blocks(c->dtype, (uint32_t)c->d, c->n) * c->dpad)))
+      return rc;
+  }
+  auto one = [&]() -> int {
+    if (pass == AMM_PASS_EXACT) return enq_exact(c, 0, nullptr, 0);
+    int r = enq_stats_z(c);
+    return r ? r : enq_z_partials(c);
+  };
+  if ((rc = one())) return rc;  // warm-up (and ||alpha||^2 = 0 is harmless for v)
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  for (uint32_t i = 0; i < iters; ++i)
+    if ((rc = one())) return rc;
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float t = 0.0f;
+  HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
+  *ms = t / iters;
+  if (pass == AMM_PASS_STATS_Z) c->stats_valid = true;
   return AMM_OK;
 }
 

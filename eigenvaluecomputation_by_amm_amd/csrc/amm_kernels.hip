@@ -21,6 +21,8 @@
 #include "amm_launch.hpp"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 namespace amm {
 
@@ -93,31 +95,23 @@ __global__ __launch_bounds__(256) void k_colstats(const T* __restrict__ A, uint6
 }
 
 // ---------------------------------------------------------------------------
-// Column statistics with z = A alpha in the same read of A (get_norms,
-// device_memory.h:34-59, and the z = A alpha product of
-// powerMethod_by_proposed.cuh:132-139).  Persistent workgroups walk the
-// columns j = blockIdx.x, blockIdx.x + gridDim.x, ...; wave w holds quarter w
-// of the column in registers (L non-temporal 16-byte loads per lane, the
-// next column prefetched), the workgroup forms alpha_j, and every lane adds
-// alpha_j a_j to its rows of a private FP32 partial of z, written to
-// ZP[blockIdx.x][:] at the end (k_sum_rows adds the partials in FP64).
-// Rows per wave q = L x 64 lanes x VEC (d <= 4 L VEC 64).
+// Column-resident passes.  A 256-thread workgroup holds one column of A in
+// registers: wave w owns rows [w q, (w + 1) q) with q = L x 64 lanes x VEC
+// (d <= 4 q), each lane L non-temporal 16-byte loads (raw bits; rows at and
+// beyond d read as zeros).  Persistent workgroups walk the columns
+// j = blockIdx.x, blockIdx.x + gridDim.x, ... and prefetch the next column
+// while the workgroup reduces the current one, so A is read once and a
+// per-column scalar c_j (a workgroup reduction over the column) can be
+// applied back to the same column: every lane adds c_j a_j to its rows of a
+// private FP32 partial, written to OUT[blockIdx.x][0, dpad) at the end.
 // ---------------------------------------------------------------------------
 template <typename T, int L>
-__global__ __launch_bounds__(256) void k_colstats_z(const T* __restrict__ A, uint64_t ld, uint32_t d,
-                                                    uint64_t n, uint64_t dpad, float* __restrict__ alpha,
-                                                    double* __restrict__ nrm2, double* __restrict__ cnrm2,
-                                                    float* __restrict__ ZP) {
-  constexpr int V = Elem<T>::VEC;
-  constexpr uint32_t step = 64u * V;
-  __shared__ double part[2][2][4];  // [column parity][sum | sum of squares][wave]
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint32_t q = (((d + 3) / 4) + step - 1) / step * step;  // <= L * step (host)
-  const uint32_t rb = w * q;
-  const uint32_t re = rb + q < d ? rb + q : d;
+struct ColRegs {
+  static constexpr int V = Elem<T>::VEC;
+  static constexpr uint32_t step = 64u * V;
   u32x4 raw[L];
-  auto load = [&](uint64_t j) {
-    const T* col = A + j * ld;
+
+  __device__ __forceinline__ void load(const T* col, uint32_t rb, uint32_t re, int lane) {
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       const uint32_t row = rb + lane * V + l * step;
@@ -138,8 +132,9 @@ __global__ __launch_bounds__(256) void k_colstats_z(const T* __restrict__ A, uin
         }
       }
     }
-  };
-  auto unpack = [&](const u32x4& r, float (&v)[V]) {
+  }
+
+  static __device__ __forceinline__ void unpack(const u32x4& r, float (&v)[V]) {
     if constexpr (V == 4) {
       v[0] = __uint_as_float(r.x);
       v[1] = __uint_as_float(r.y);
@@ -155,18 +150,76 @@ __global__ __launch_bounds__(256) void k_colstats_z(const T* __restrict__ A, uin
       v[6] = bf16_bits_to_f32(r.w & 0xFFFFu);
       v[7] = bf16_bits_to_f32(r.w >> 16);
     }
-  };
+  }
+};
+
+// rows per wave of the column-resident passes (a multiple of 64 VEC)
+template <int V>
+__device__ __forceinline__ uint32_t resident_rows(uint32_t d) {
+  constexpr uint32_t step = 64u * V;
+  return (((d + 3) / 4) + step - 1) / step * step;
+}
+
+// acc += c * column, then (at the end) the partial out to OUT[blockIdx.x]
+template <typename T, int L>
+__device__ __forceinline__ void resident_axpy(float (&acc)[L * Elem<T>::VEC], float c,
+                                              const u32x4 (&col)[L]) {
+  constexpr int V = Elem<T>::VEC;
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    float v[V];
+    ColRegs<T, L>::unpack(col[l], v);
+#pragma unroll
+    for (int k = 0; k < V; ++k) acc[l * V + k] = fmaf(c, v[k], acc[l * V + k]);
+  }
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void resident_store(const float (&acc)[L * Elem<T>::VEC], float* out,
+                                               uint32_t rb, uint32_t q, int lane, uint64_t dpad) {
+  constexpr int V = Elem<T>::VEC;
+  // Each wave writes its own rows [rb, rb + q) only (L x 64 x VEC may exceed
+  // q when L was rounded up to a power of two); together they cover
+  // [0, dpad): 4 q is a multiple of 1024 >= d and dpad <= 4 q.
+#pragma unroll
+  for (int l = 0; l < L; ++l) {
+    const uint32_t row = rb + lane * V + l * ColRegs<T, L>::step;
+    if (row < rb + q && row < dpad) {
+      float v[V];
+#pragma unroll
+      for (int k = 0; k < V; ++k) v[k] = acc[l * V + k];
+      store_vecf<V>(out + row, v);
+    }
+  }
+}
+
+// Column statistics with z = A alpha in the same read of A (get_norms,
+// device_memory.h:34-59, and the z = A alpha product of
+// powerMethod_by_proposed.cuh:132-139): c_j = alpha_j, partials of z in
+// ZP[blockIdx.x] (k_sum_rows adds them in FP64).
+template <typename T, int L>
+__global__ __launch_bounds__(256) void k_colstats_z(const T* __restrict__ A, uint64_t ld, uint32_t d,
+                                                    uint64_t n, uint64_t dpad, float* __restrict__ alpha,
+                                                    double* __restrict__ nrm2, double* __restrict__ cnrm2,
+                                                    float* __restrict__ ZP) {
+  constexpr int V = Elem<T>::VEC;
+  __shared__ double part[2][2][4];  // [column parity][sum | sum of squares][wave]
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t q = resident_rows<V>(d);  // <= L * 64 * V (host)
+  const uint32_t rb = w * q;
+  const uint32_t re = rb + q < d ? rb + q : d;
+  ColRegs<T, L> cr;
   float zp[L * V];
 #pragma unroll
   for (int e = 0; e < L * V; ++e) zp[e] = 0.0f;
   uint64_t j = blockIdx.x;
-  if (j < n) load(j);
+  if (j < n) cr.load(A + j * ld, rb, re, lane);
   for (int par = 0; j < n; j += gridDim.x, par ^= 1) {
     double s = 0.0, ss = 0.0;
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       float v[V];
-      unpack(raw[l], v);
+      ColRegs<T, L>::unpack(cr.raw[l], v);
 #pragma unroll
       for (int k = 0; k < V; ++k) {
         const double b = v[k];
@@ -182,8 +235,8 @@ __global__ __launch_bounds__(256) void k_colstats_z(const T* __restrict__ A, uin
     }
     u32x4 cur[L];
 #pragma unroll
-    for (int l = 0; l < L; ++l) cur[l] = raw[l];
-    if (j + gridDim.x < n) load(j + gridDim.x);  // the next column streams in meanwhile
+    for (int l = 0; l < L; ++l) cur[l] = cr.raw[l];
+    if (j + gridDim.x < n) cr.load(A + (j + gridDim.x) * ld, rb, re, lane);  // streams in meanwhile
     __syncthreads();
     const double S = (part[par][0][0] + part[par][0][1]) + (part[par][0][2] + part[par][0][3]);
     const double SS = (part[par][1][0] + part[par][1][1]) + (part[par][1][2] + part[par][1][3]);
@@ -195,27 +248,65 @@ __global__ __launch_bounds__(256) void k_colstats_z(const T* __restrict__ A, uin
       c = c > 0.0 ? c : (c == c ? 0.0 : c);  // clamp rounding below 0, keep NaN
       cnrm2[j] = c;
     }
-    const float af = (float)mean;
+    resident_axpy<T, L>(zp, (float)mean, cur);
+  }
+  resident_store<T, L>(zp, ZP + (uint64_t)blockIdx.x * dpad, rb, q, lane, dpad);
+}
+
+// The exact product y = A (A^T x) in one read of A (the two cublasSgemv
+// calls of powerMethod_by_exact.cuh:46-62): c_j = a_j^T x (FP64 sum of
+// exact FP32 products, rounded to FP32 like the separate pass's t_j),
+// partials of y in P2[blockIdx.x] for k_finish.  x sits in LDS (zeros
+// beyond d).  Skipped when the gate says the step is not exact.
+template <typename T, int L>
+__global__ __launch_bounds__(256) void k_gram(const T* __restrict__ A, uint64_t ld, uint32_t d,
+                                              uint64_t n, uint64_t dpad, const float* __restrict__ x,
+                                              float* __restrict__ P2, const uint32_t* __restrict__ gate_mode,
+                                              uint32_t it, uint32_t gate_mask) {
+  if (gate_mode && !((gate_mask >> gate_mode[it]) & 1u)) return;
+  constexpr int V = Elem<T>::VEC;
+  constexpr uint32_t kRows = 4u * L * 64u * V;
+  __shared__ float xs[kRows];
+  __shared__ double part[2][4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t q = resident_rows<V>(d);
+  const uint32_t rb = w * q;
+  const uint32_t re = rb + q < d ? rb + q : d;
+  ColRegs<T, L> cr;
+  uint64_t j = blockIdx.x;
+  if (j < n) cr.load(A + j * ld, rb, re, lane);
+  for (uint32_t r = threadIdx.x; r < kRows; r += 256) xs[r] = r < d ? x[r] : 0.0f;
+  float yp[L * V];
+#pragma unroll
+  for (int e = 0; e < L * V; ++e) yp[e] = 0.0f;
+  __syncthreads();
+  for (int par = 0; j < n; j += gridDim.x, par ^= 1) {
+    double t = 0.0;
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       float v[V];
-      unpack(cur[l], v);
+      ColRegs<T, L>::unpack(cr.raw[l], v);
+      const float* xr = xs + rb + lane * V + l * ColRegs<T, L>::step;
 #pragma unroll
-      for (int k = 0; k < V; ++k) zp[l * V + k] = fmaf(af, v[k], zp[l * V + k]);
+      for (int k = 0; k < V; k += 4) {
+        const float4 xv = *reinterpret_cast<const float4*>(xr + k);
+        t += (double)v[k] * (double)xv.x;
+        t += (double)v[k + 1] * (double)xv.y;
+        t += (double)v[k + 2] * (double)xv.z;
+        t += (double)v[k + 3] * (double)xv.w;
+      }
     }
-  }
-  // rows [0, dpad) are covered: 4 q is a multiple of 1024 >= d, dpad <= 4 q
-  float* out = ZP + (uint64_t)blockIdx.x * dpad;
+    t = wave_sum(t);
+    if (lane == 0) part[par][w] = t;
+    u32x4 cur[L];
 #pragma unroll
-  for (int l = 0; l < L; ++l) {
-    const uint32_t row = rb + lane * V + l * step;
-    if (row < dpad) {
-      float v[V];
-#pragma unroll
-      for (int k = 0; k < V; ++k) v[k] = zp[l * V + k];
-      store_vecf<V>(out + row, v);
-    }
+    for (int l = 0; l < L; ++l) cur[l] = cr.raw[l];
+    if (j + gridDim.x < n) cr.load(A + (j + gridDim.x) * ld, rb, re, lane);  // streams in meanwhile
+    __syncthreads();
+    const float tj = (float)((part[par][0] + part[par][1]) + (part[par][2] + part[par][3]));
+    resident_axpy<T, L>(yp, tj, cur);
   }
+  resident_store<T, L>(yp, P2 + (uint64_t)blockIdx.x * dpad, rb, q, lane, dpad);
 }
 
 // ---------------------------------------------------------------------------
@@ -1130,36 +1221,90 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint
 // ===========================================================================
 static inline unsigned grid1(uint64_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
 
-// The fused statistics hold q = L x 64 x VEC rows per wave in registers,
-// L <= 16: d <= 16384 (FP32) / 32768 (BF16).
-static int colstats_L(int dtype, uint32_t d) {
+// The column-resident passes hold q = L x 64 x VEC rows per wave in
+// registers with L a power of two up to maxL: the statistics up to L = 16
+// (d <= 16384 FP32, 32768 BF16); the exact product keeps x in 64 KB of LDS
+// as well (d <= 16384 either way).
+static int resident_L(int dtype, uint32_t d, int maxL) {
   const uint32_t V = dtype == 1 ? 8u : 4u, step = 64u * V;
   const uint32_t q = ((d + 3) / 4 + step - 1) / step * step;
   const uint32_t L = q / step;
-  if (L <= 1) return 1;
-  if (L <= 2) return 2;
-  if (L <= 4) return 4;
-  if (L <= 8) return 8;
-  if (L <= 16) return 16;
-  return 0;
+  int r = 1;
+  while ((uint32_t)r < L) r *= 2;
+  return r <= maxL ? r : 0;
 }
+static int gram_maxL(int dtype) { return dtype == 1 ? 8 : 16; }
 
-// Persistent workgroups: as many as stay resident (two per CU at 16 loads
-// per lane, four below), at most one per column.
-static uint32_t colstats_reg_blocks(int L, uint64_t n) {
-  return (uint32_t)std::min<uint64_t>(n, L >= 16 ? 512 : 1024);
+// Persistent grid: as many workgroups as stay resident on the device (the
+// occupancy of the instance at 256 threads, times the CUs), at most one per
+// column.  Cached per kernel and device; the partial count of a pass is this
+// number, so it must not vary between calls.
+static uint32_t resident_grid(const void* fn, uint64_t n) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, uint32_t> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  uint32_t g = 0;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find({fn, dev});
+    if (it != cache.end()) {
+      g = it->second;
+    } else {
+      int per = 0, cus = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 256, 0) != hipSuccess || per < 1) per = 1;
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 1;
+      g = (uint32_t)(per * cus);
+      cache[{fn, dev}] = g;
+    }
+  }
+  return (uint32_t)std::min<uint64_t>(n, g);
 }
-
-uint32_t colstats_z_blocks(int dtype, uint32_t d, uint64_t n) {
-  return colstats_reg_blocks(colstats_L(dtype, d), n);
-}
-
-bool colstats_fuses_z(int dtype, uint32_t d) { return colstats_L(dtype, d) != 0; }
 
 template <typename T>
-static void cs_reg(int L, const void* A, uint64_t ld, uint32_t d, uint64_t n, uint64_t dpad, float* alpha,
-                   double* nrm2, double* cnrm2, float* ZP, hipStream_t s) {
-  const unsigned g = colstats_reg_blocks(L, n);
+static const void* colstats_z_fn(int L) {
+  switch (L) {
+    case 1: return (const void*)&k_colstats_z<T, 1>;
+    case 2: return (const void*)&k_colstats_z<T, 2>;
+    case 4: return (const void*)&k_colstats_z<T, 4>;
+    case 8: return (const void*)&k_colstats_z<T, 8>;
+    default: return (const void*)&k_colstats_z<T, 16>;
+  }
+}
+
+template <typename T>
+static const void* gram_fn(int L) {
+  switch (L) {
+    case 1: return (const void*)&k_gram<T, 1>;
+    case 2: return (const void*)&k_gram<T, 2>;
+    case 4: return (const void*)&k_gram<T, 4>;
+    case 8: return (const void*)&k_gram<T, 8>;
+    default:
+      if constexpr (Elem<T>::VEC == 4) return (const void*)&k_gram<T, 16>;
+      return nullptr;
+  }
+}
+
+bool colstats_fuses_z(int dtype, uint32_t d) { return resident_L(dtype, d, 16) != 0; }
+
+uint32_t colstats_z_blocks(int dtype, uint32_t d, uint64_t n) {
+  const int L = resident_L(dtype, d, 16);
+  if (!L) return 0;
+  return resident_grid(dtype == 1 ? colstats_z_fn<__hip_bfloat16>(L) : colstats_z_fn<float>(L), n);
+}
+
+bool gram_fits(int dtype, uint32_t d) { return resident_L(dtype, d, gram_maxL(dtype)) != 0; }
+
+uint32_t gram_blocks(int dtype, uint32_t d, uint64_t n) {
+  const int L = resident_L(dtype, d, gram_maxL(dtype));
+  if (!L) return 0;
+  return resident_grid(dtype == 1 ? gram_fn<__hip_bfloat16>(L) : gram_fn<float>(L), n);
+}
+
+template <typename T>
+static void cs_reg(int L, unsigned g, const void* A, uint64_t ld, uint32_t d, uint64_t n, uint64_t dpad,
+                   float* alpha, double* nrm2, double* cnrm2, float* ZP, hipStream_t s) {
   const T* a = (const T*)A;
   switch (L) {
     case 1: k_colstats_z<T, 1><<<g, 256, 0, s>>>(a, ld, d, n, dpad, alpha, nrm2, cnrm2, ZP); break;
@@ -1174,12 +1319,42 @@ hipError_t launch_colstats_z(int dtype, const void* A, uint64_t ld, uint32_t d, 
                              float* alpha, double* nrm2, double* cnrm2, float* ZP, DevState* st,
                              hipStream_t s) {
   (void)st;
-  const int L = colstats_L(dtype, d);
-  if (!L || !ZP) return hipErrorInvalidValue;  // z cannot be fused at this d
+  const int L = resident_L(dtype, d, 16);
+  if (!L || !ZP || n == 0) return hipErrorInvalidValue;  // z cannot be fused at this d
+  const unsigned g = colstats_z_blocks(dtype, d, n);
   if (dtype == 1)
-    cs_reg<__hip_bfloat16>(L, A, ld, d, n, dpad, alpha, nrm2, cnrm2, ZP, s);
+    cs_reg<__hip_bfloat16>(L, g, A, ld, d, n, dpad, alpha, nrm2, cnrm2, ZP, s);
   else
-    cs_reg<float>(L, A, ld, d, n, dpad, alpha, nrm2, cnrm2, ZP, s);
+    cs_reg<float>(L, g, A, ld, d, n, dpad, alpha, nrm2, cnrm2, ZP, s);
+  return hipGetLastError();
+}
+
+template <typename T>
+static void gram_launch(int L, unsigned g, const void* A, uint64_t ld, uint32_t d, uint64_t n, uint64_t dpad,
+                        const float* x, float* P2, const uint32_t* gm, uint32_t it, uint32_t mask,
+                        hipStream_t s) {
+  const T* a = (const T*)A;
+  switch (L) {
+    case 1: k_gram<T, 1><<<g, 256, 0, s>>>(a, ld, d, n, dpad, x, P2, gm, it, mask); break;
+    case 2: k_gram<T, 2><<<g, 256, 0, s>>>(a, ld, d, n, dpad, x, P2, gm, it, mask); break;
+    case 4: k_gram<T, 4><<<g, 256, 0, s>>>(a, ld, d, n, dpad, x, P2, gm, it, mask); break;
+    case 8: k_gram<T, 8><<<g, 256, 0, s>>>(a, ld, d, n, dpad, x, P2, gm, it, mask); break;
+    default:
+      if constexpr (Elem<T>::VEC == 4) k_gram<T, 16><<<g, 256, 0, s>>>(a, ld, d, n, dpad, x, P2, gm, it, mask);
+      break;
+  }
+}
+
+hipError_t launch_gram(int dtype, const void* A, uint64_t ld, uint32_t d, uint64_t n, uint64_t dpad,
+                       const float* x, float* P2, const uint32_t* gate_mode, uint32_t it, uint32_t gate_mask,
+                       hipStream_t s) {
+  const int L = resident_L(dtype, d, gram_maxL(dtype));
+  if (!L || !P2 || n == 0) return hipErrorInvalidValue;
+  const unsigned g = gram_blocks(dtype, d, n);
+  if (dtype == 1)
+    gram_launch<__hip_bfloat16>(L, g, A, ld, d, n, dpad, x, P2, gate_mode, it, gate_mask, s);
+  else
+    gram_launch<float>(L, g, A, ld, d, n, dpad, x, P2, gate_mode, it, gate_mask, s);
   return hipGetLastError();
 }
 

@@ -122,6 +122,14 @@ hipError_t launch_colstats_z(int dtype, const void* A, uint64_t ld, uint32_t d, 
                              hipStream_t s);
 uint32_t colstats_z_blocks(int dtype, uint32_t d, uint64_t n);
 bool colstats_fuses_z(int dtype, uint32_t d);
+// The exact product y = A (A^T x) in one read of A: partials
+// P2[gram_blocks(dtype, d, n)][dpad] for the finish kernel.  Needs
+// gram_fits(dtype, d); skipped unless the gate admits mode[it].
+hipError_t launch_gram(int dtype, const void* A, uint64_t ld, uint32_t d, uint64_t n, uint64_t dpad,
+                       const float* x, float* P2, const uint32_t* gate_mode, uint32_t it, uint32_t gate_mask,
+                       hipStream_t s);
+uint32_t gram_blocks(int dtype, uint32_t d, uint64_t n);
+bool gram_fits(int dtype, uint32_t d);
 hipError_t launch_cdf(const double* w, uint64_t n, uint64_t* cdf, DevState* st, int which,
                       const float* alpha, uint64_t* tile_max, uint64_t* tile_sum, uint64_t* tile_lp,
                       double* tile_a2, hipStream_t s);

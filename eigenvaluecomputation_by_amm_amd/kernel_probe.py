@@ -4,6 +4,7 @@
 
 Launches, in order: a plain device copy of 1 GiB (read 1 GiB, write 1 GiB:
 the scale for FETCH_SIZE / WRITE_SIZE), the column-statistics pass, the
+statistics pass with z = A alpha, the one-pass exact product, the two-pass
 exact product (both contractions), one sampled product of the headline
 configuration (previousAMM, c = 1024) and one batched 16-trial product in
 BF16 with every column present (both MFMA contractions).  Each runs
@@ -34,6 +35,8 @@ def main() -> None:
     A = heavy_tailed(args.d, args.n)
     with AmmContext(A) as ctx:
         ctx.time_column_stats(iters=args.reps)
+        ctx.time_pass("stats_z", iters=args.reps)
+        ctx.time_pass("exact", iters=args.reps)
         ctx.time_product("exact", 0, iters=args.reps)
         ctx.time_product("previousAMM", 1024, seed=1, iters=args.reps)
     torch.cuda.synchronize()

@@ -154,6 +154,18 @@ int amm_time_product(amm_ctx *ctx, int32_t kernel, uint32_t c, uint64_t seed, ui
  * (one read of A) over `iters` launches, HIP events on the context stream. */
 int amm_time_column_stats(amm_ctx *ctx, uint32_t iters, double *ms);
 
+/* Measurement helper: mean duration (ms) over `iters` launches of one pass
+ * as a run issues it (HIP events on the context stream).
+ *   AMM_PASS_STATS_Z: column statistics with z = A alpha in the same read
+ *     (get_norms, device_memory.h:34-59, plus powerMethod_by_proposed.cuh:
+ *     132-139), the kernel and the partial sum; d <= 16384 FP32 / 32768 BF16.
+ *   AMM_PASS_EXACT: the exact product y = A (A^T x) (the two cublasSgemv
+ *     calls of powerMethod_by_exact.cuh:46-62) without its finish: one read
+ *     of A for d <= 16384, else the two contraction kernels.
+ * AMM_E_INVALID when the pass does not apply to the context's shape. */
+enum { AMM_PASS_STATS_Z = 1, AMM_PASS_EXACT = 2 };
+int amm_time_pass(amm_ctx *ctx, int32_t pass, uint32_t iters, double *ms);
+
 /* Fill the m x m FP32 device matrix A (leading dimension ld) with a
  * symmetric random matrix -- entries gaussian(mean p1, std p2),
  * lognormal(p1, p2) or uniform[p1, p2) -- and add the Gershgorin offset

@@ -89,13 +89,20 @@ def test_sample_bitwise(c, centred, iteration, trial):
     assert not np.isin([0, 100, 24999], idx[: segc[0]]).any()
 
 
-@pytest.mark.parametrize("d,n,bf16", [(512, 3000, False), (1000, 777, False), (640, 900, True)])
-def test_exact_product(d, n, bf16):
-    banner(f"exact_product d={d} n={n} bf16={bf16}")
+# rows per wave of the one-pass form: 1, 2, 4, 8, 16 loads per lane, and
+# d = 16392 past it (the two-pass form); ragged d and padded ld included
+@pytest.mark.parametrize("d,n,ld,bf16", [(512, 3000, None, False), (1000, 777, 1004, False),
+                                         (1025, 300, 1028, False), (3001, 257, 3004, False),
+                                         (8000, 130, None, False), (16384, 70, None, False),
+                                         (16392, 50, None, False), (640, 900, None, True),
+                                         (5000, 200, 5008, True), (16384, 60, None, True),
+                                         (16392, 40, None, True)])
+def test_exact_product(d, n, ld, bf16):
+    banner(f"exact_product d={d} n={n} ld={ld} bf16={bf16}")
     AmmContext, *_ = _amm()
     A = lowrank_plus_noise(d, n, seed=4)
-    t, cols = dev_matrix(A, bf16=bf16)
-    Ah = (bf16_to_f32(cols).T if bf16 else A).astype(np.float64)
+    t, cols = dev_matrix(A, ld=ld, bf16=bf16)
+    Ah = (bf16_to_f32(cols)[:, :d].T if bf16 else A).astype(np.float64)
     x = np.random.default_rng(5).standard_normal(d).astype(np.float32)
     with AmmContext(t, d=d) as ctx:
         y = ctx.exact_product(torch.from_numpy(x).cuda()).cpu().numpy()
@@ -214,17 +221,19 @@ def test_graph_and_stream_launch_agree_bitwise():
 
 @pytest.mark.parametrize("kernel", ["proposedAMM", "proposedAMM_v2"])
 @pytest.mark.parametrize("bf16", [False, True])
-def test_fused_z_equals_separate_pass(kernel, bf16):
+@pytest.mark.parametrize("d,ld", [(1000, 1000), (3001, 3008)])
+def test_fused_z_equals_separate_pass(kernel, bf16, d, ld):
     """z = A alpha formed inside the column-statistics pass (a fresh run)
     against z from its own gather pass (a keep_stats run reuses the
     statistics, so only z's summation order differs): same samples, same
-    modes, trajectories within 1e-5."""
-    banner(f"fused z {kernel} bf16={bf16}")
+    modes, trajectories within 1e-5.  d = 3001 splits the rows unevenly
+    over the waves."""
+    banner(f"fused z {kernel} bf16={bf16} d={d}")
     AmmContext, RunParams, _ = _amm()
-    A = lowrank_plus_noise(1000, 3000, seed=13) + 0.2
-    t, _ = dev_matrix(A, bf16=bf16)
+    A = lowrank_plus_noise(d, 3000, seed=13) + 0.2
+    t, _ = dev_matrix(A, ld=ld, bf16=bf16)
     p = dict(kernel=kernel, c=400, step=20, vr_period=5, seed=9)
-    with AmmContext(t) as ctx:
+    with AmmContext(t, d=d) as ctx:
         a = ctx.power_method(RunParams(**p))
         ma = ctx.last_modes(20)
         b = ctx.power_method(RunParams(keep_stats=True, **p))
@@ -247,6 +256,23 @@ def test_proposed_beyond_register_rows():
         res = ctx.power_method(RunParams(seed=21, **run))
     ref = O.power_method(cols, d, seed=21, w_plain=nrm2, w_cent=cnrm2, **run)
     assert np.max(np.abs(res.eig - ref["eig"]) / np.abs(ref["eig"])) < 1e-4
+
+
+def test_time_pass():
+    """amm_time_pass times the statistics+z pass and the exact product as a
+    run issues them, and refuses the fused statistics past 16384 FP32 rows."""
+    banner("time_pass")
+    AmmContext, _, AmmError = _amm()
+    for d, fused in ((2000, True), (16392, False)):
+        A = lowrank_plus_noise(d, 64, seed=15)
+        t, _ = dev_matrix(A)
+        with AmmContext(t) as ctx:
+            assert ctx.time_pass("exact", iters=3) > 0
+            if fused:
+                assert ctx.time_pass("stats_z", iters=3) > 0
+            else:
+                with pytest.raises(AmmError):
+                    ctx.time_pass("stats_z", iters=3)
 
 
 def test_lambda_small_workload_vs_eigvalsh():
