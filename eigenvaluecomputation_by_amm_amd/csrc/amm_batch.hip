@@ -233,35 +233,6 @@ __global__ __launch_bounds__(256) void k_bnorm(const BProb* __restrict__ P, cons
 // counts the draw in cnt[j][k].  Items: trials x ceil(c / 2048) per problem.
 // The first block of a proposed trial also finishes s, q of its product.
 // ---------------------------------------------------------------------------
-// Coarse table: kCoarse bucket ends e_m = floor((m + 1) n / kCoarse) - 1 and
-// their CDF values in LDS; a target is first located among the buckets in
-// LDS, then within its bucket (about n / kCoarse entries) in global memory.
-// Same result as the plain search: the smallest j with cdf[j] > t.
-constexpr uint32_t kCoarse = 1024;
-
-__device__ __forceinline__ uint32_t bucket_end(uint32_t m, uint32_t n) {
-  return (uint32_t)(((uint64_t)(m + 1) * n) / kCoarse) - 1u;  // may wrap for empty leading buckets
-}
-
-__device__ __forceinline__ uint32_t bcdf_search(const uint64_t* __restrict__ cdf, const uint64_t* cs,
-                                                uint32_t n, uint64_t t, uint64_t last_pos) {
-  // smallest bucket m with cs[m] > t (cs[m] = cdf at its end; empty buckets
-  // repeat the previous end's value)
-  uint32_t lo = 0, hi = kCoarse;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (cs[mid] > t) hi = mid; else lo = mid + 1;
-  }
-  if (lo >= kCoarse) return (uint32_t)last_pos;
-  uint32_t jlo = lo ? bucket_end(lo - 1, n) + 1u : 0u;
-  uint32_t jhi = bucket_end(lo, n) + 1u;
-  while (jlo < jhi) {
-    const uint32_t mid = jlo + ((jhi - jlo) >> 1);
-    if (cdf[mid] > t) jhi = mid; else jlo = mid + 1;
-  }
-  return jlo < n ? jlo : (uint32_t)last_pos;
-}
-
 // Open-addressing count table in LDS: key = column + 1 (0 = empty).  A block
 // draws at most 512 columns, so 1024 entries stay at most half full.
 constexpr uint32_t kSlotsPerBlock = 512;
@@ -312,11 +283,7 @@ __global__ __launch_bounds__(256) void k_bsample(const BProb* __restrict__ P, co
   const uint64_t total = pr.st->total[which];
   const uint64_t last_pos = pr.st->last_pos[which];
   const uint64_t* cdf = which ? pr.cdf1 : pr.cdf0;
-  for (uint32_t m = threadIdx.x; m < kCoarse; m += 256) {
-    const uint32_t e = bucket_end(m, pr.n);
-    // an empty bucket before the first column ends at "-1": cdf value 0
-    cs[m] = ((uint64_t)(m + 1) * pr.n) / kCoarse == 0 ? 0ull : cdf[e];
-  }
+  coarse_fill(cs, cdf, pr.n, threadIdx.x, 256);
   for (uint32_t e = threadIdx.x; e < kHash; e += 256) {
     hkey[e] = 0u;
     hcnt[e] = 0u;

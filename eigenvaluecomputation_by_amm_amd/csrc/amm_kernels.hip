@@ -461,20 +461,12 @@ __global__ __launch_bounds__(256) void k_cdf_offsets(uint64_t* __restrict__ cdf,
 // device_memory.h:506-545, powerMethod_by_previous.cuh:56-70,
 // common.h:42-53).  Slot k of iteration it takes the 64-bit uniform u from
 // Philox counter (k>>1, it, trial, kStreamSample), words 2(k&1), 2(k&1)+1;
-// t = floor(u * total / 2^64); column = min{j : cdf_j > t}.  One workgroup
-// per (segment, iteration) sorts its slots in LDS (bitonic), merges
-// duplicates and writes the unique columns in ascending order with weight
-// mult * total / (c * q_j), q_j = cdf_j - cdf_{j-1}.
+// t = floor(u * total / 2^64); column = min{j : cdf_j > t} (located in a
+// coarse table of the CDF in LDS first, then within its bucket).  One
+// workgroup per (segment, iteration) sorts its slots in LDS (bitonic),
+// merges duplicates and writes the unique columns in ascending order with
+// weight mult * total / (c * q_j), q_j = cdf_j - cdf_{j-1}.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t cdf_search(const uint64_t* __restrict__ cdf, uint64_t n,
-                                               uint64_t t, uint64_t last_pos) {
-  uint64_t lo = 0, hi = n;
-  while (lo < hi) {
-    const uint64_t mid = lo + ((hi - lo) >> 1);
-    if (cdf[mid] > t) hi = mid; else lo = mid + 1;
-  }
-  return (uint32_t)(lo < n ? lo : last_pos);
-}
 
 __global__ __launch_bounds__(1024) void k_sample(const uint64_t* __restrict__ cdf, uint64_t n,
                                                  const DevState* st, int which, uint32_t it0,
@@ -484,6 +476,7 @@ __global__ __launch_bounds__(1024) void k_sample(const uint64_t* __restrict__ cd
                                                  uint32_t* __restrict__ seg_count) {
   __shared__ uint32_t keys[kSampleSeg];
   __shared__ uint32_t scan[2][1024];
+  __shared__ uint64_t cs[kCoarse];
   const uint32_t tid = threadIdx.x;
   const uint32_t g = blockIdx.x;
   const uint32_t it = it0 + blockIdx.y;
@@ -495,6 +488,8 @@ __global__ __launch_bounds__(1024) void k_sample(const uint64_t* __restrict__ cd
   const uint64_t seed = st->seed;
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   const uint32_t trial = st->trial;
+  coarse_fill(cs, cdf, (uint32_t)n, tid, 1024);
+  __syncthreads();
 
   // draw: one Philox call serves two consecutive slots
   for (uint32_t p = tid; 2 * p < kSampleSeg; p += 1024) {
@@ -503,9 +498,10 @@ __global__ __launch_bounds__(1024) void k_sample(const uint64_t* __restrict__ cd
       const uint32_t k = b + l0;
       uint32_t r[4];
       philox4x32_10(k >> 1, it, trial, kStreamSample, k0, k1, r);
-      keys[l0] = cdf_search(cdf, n, draw_target(r[0], r[1], total), last_pos);
-      keys[l0 + 1] = (l0 + 1 < cnt) ? cdf_search(cdf, n, draw_target(r[2], r[3], total), last_pos)
-                                    : 0xFFFFFFFFu;
+      keys[l0] = bcdf_search(cdf, cs, (uint32_t)n, draw_target(r[0], r[1], total), last_pos);
+      keys[l0 + 1] = (l0 + 1 < cnt)
+                         ? bcdf_search(cdf, cs, (uint32_t)n, draw_target(r[2], r[3], total), last_pos)
+                         : 0xFFFFFFFFu;
     } else {
       keys[l0] = 0xFFFFFFFFu;
       keys[l0 + 1] = 0xFFFFFFFFu;
@@ -878,6 +874,7 @@ __global__ __launch_bounds__(256) void k_finish(
     const uint64_t r0 = (uint64_t)blockIdx.x * kFinishRows + l16 * 4;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     if (r0 < dpad) {
+#pragma unroll 8
       for (uint32_t s = g; s < S; s += 16) {
         const float4 p = *reinterpret_cast<const float4*>(P2 + (uint64_t)s * dpad + r0);
         a0 += p.x;
@@ -1008,6 +1005,7 @@ __global__ __launch_bounds__(256) void k_tail(
   if (norm) {
     const double scale = scale_l;
     double ps = 0.0, pq = 0.0;
+#pragma unroll 8
     for (uint64_t r = (uint64_t)t * 4; r < dpad; r += 1024) {
       const float4 yv4 = *reinterpret_cast<const float4*>(y + r);
       float xv[4] = {(float)(scale * (double)yv4.x), (float)(scale * (double)yv4.y),
