@@ -38,7 +38,8 @@ HBM_PEAK_GBS = 8000.0
 # `python3 -m eigenvaluecomputation_by_amm_amd.kernel_probe`), each scaled by
 # what the counter reads for a 1 GiB device copy; used for roofline.traffic.
 PMC_TRAFFIC_16384x65536 = {
-    "k_colstats<float> (column statistics)": 4.2964e9,
+    "k_gram<float,16> (exact product A A^T x, one read of A)": 4.3290e9,
+    "k_colstats<float> (column statistics)": 4.3011e9,
     "k_gather_dot<float,4,dense> (exact pass A^T x)": 4.2956e9,
     "k_gather_axpy<float,dense> (exact pass A t)": 4.3011e9,
 }
@@ -57,7 +58,7 @@ def gram_L(d: int) -> int:
 
 
 # headline AMM configuration: the fastest settled time of the kernel x c x vr sweep (--sweep)
-HEADLINE = dict(kernel="previousAMM", c=1024, vr_period=5)
+HEADLINE = dict(kernel="previousAMM", c=512, vr_period=4)
 
 
 def log(*a):

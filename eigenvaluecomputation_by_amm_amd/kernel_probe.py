@@ -6,7 +6,7 @@ Launches, in order: a plain device copy of 1 GiB (read 1 GiB, write 1 GiB:
 the scale for FETCH_SIZE / WRITE_SIZE), the column-statistics pass, the
 statistics pass with z = A alpha, the one-pass exact product, the two-pass
 exact product (both contractions), one sampled product of the headline
-configuration (previousAMM, c = 1024) and one batched 16-trial product in
+configuration (previousAMM, c = 512) and one batched 16-trial product in
 BF16 with every column present (both MFMA contractions).  Each runs
 `--reps` times.
 """
@@ -38,7 +38,7 @@ def main() -> None:
         ctx.time_pass("stats_z", iters=args.reps)
         ctx.time_pass("exact", iters=args.reps)
         ctx.time_product("exact", 0, iters=args.reps)
-        ctx.time_product("previousAMM", 1024, seed=1, iters=args.reps)
+        ctx.time_product("previousAMM", 512, seed=1, iters=args.reps)
     torch.cuda.synchronize()
     Ab = A.to(torch.bfloat16)
     del A
