@@ -219,6 +219,37 @@ def test_graph_and_stream_launch_agree_bitwise():
     assert np.array_equal(a.eig, b.eig) and np.array_equal(a.eig, c.eig)
 
 
+def test_concurrent_contexts_on_own_streams():
+    """Two contexts on two streams driven from two host threads at once give
+    the same bits as the same runs one after another: no state is shared
+    between contexts beyond read-only launch shapes."""
+    import threading
+    banner("concurrent contexts")
+    AmmContext, RunParams, _ = _amm()
+    mats = [dev_matrix(heavy_columns(700, 5000, seed=30 + i))[0] for i in range(2)]
+    runs = [dict(kernel="proposedAMM", c=600, step=25, vr_period=4, seed=5),
+            dict(kernel="previousAMM", c=900, step=25, vr_period=5, seed=6)]
+
+    def solve(i, out, stream):
+        with AmmContext(mats[i], stream=stream) as ctx:
+            out[i] = [ctx.power_method(RunParams(trial=t, **runs[i])).eig for t in range(3)]
+
+    seq = [None, None]
+    for i in range(2):
+        solve(i, seq, 0)
+    streams = [torch.cuda.Stream() for _ in range(2)]
+    par = [None, None]
+    th = [threading.Thread(target=solve, args=(i, par, streams[i].cuda_stream)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for i in range(2):
+        assert par[i] is not None
+        for a, b in zip(seq[i], par[i]):
+            assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("kernel", ["proposedAMM", "proposedAMM_v2"])
 @pytest.mark.parametrize("bf16", [False, True])
 @pytest.mark.parametrize("d,ld", [(1000, 1000), (3001, 3008)])
