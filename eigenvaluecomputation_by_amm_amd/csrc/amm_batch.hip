@@ -5,6 +5,7 @@
 // amm_kernels.hip): the same columns, multiplicities and weights; only the
 // summation order of the contractions differs.
 #include "amm_batch.hpp"
+#include "amm_setup.hpp"
 
 namespace amm {
 
@@ -933,6 +934,77 @@ __global__ __launch_bounds__(256) void k_bsigma(const BProb* __restrict__ P, BRu
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Grouped setup (the single-matrix statistics and CDF passes of every
+// problem in one launch each; same device functions, same bits).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bcolstats(const BSetup* __restrict__ S, const uint32_t* off,
+                                                   uint32_t np, uint32_t skip_fusable) {
+  const uint32_t p = find_problem(off, np, blockIdx.x);
+  const BSetup& q = S[p];
+  if (skip_fusable && q.fusable) return;
+  const uint64_t j = blockIdx.x - off[p];
+  if (q.dtype == 1)
+    colstats_column<__hip_bfloat16>(reinterpret_cast<const __hip_bfloat16*>(q.A) + j * q.ld, q.d, j, q.alpha,
+                                    q.nrm2, q.cnrm2);
+  else
+    colstats_column<float>(reinterpret_cast<const float*>(q.A) + j * q.ld, q.d, j, q.alpha, q.nrm2, q.cnrm2);
+}
+
+__global__ __launch_bounds__(256) void k_bcdf_max(const BSetup* __restrict__ S, const uint32_t* off,
+                                                  uint32_t np, int which) {
+  const uint32_t p = find_problem(off, np, blockIdx.x);
+  const BSetup& q = S[p];
+  cdf_max_tile(which ? q.cnrm2 : q.nrm2, q.n, q.tile_max, blockIdx.x - off[p]);
+}
+
+__global__ __launch_bounds__(256) void k_bcdf_scan(const BSetup* __restrict__ S, const uint32_t* off,
+                                                   uint32_t np, int which) {
+  const uint32_t p = find_problem(off, np, blockIdx.x);
+  const BSetup& q = S[p];
+  cdf_scan_tile(which ? q.cnrm2 : q.nrm2, q.n, q.cdf[which], q.st, which ? q.alpha : nullptr, q.tile_max,
+                q.ntiles, q.tile_sum, q.tile_lp, q.tile_a2, blockIdx.x - off[p]);
+}
+
+__global__ __launch_bounds__(256) void k_bcdf_offsets(const BSetup* __restrict__ S, const uint32_t* off,
+                                                      uint32_t np, int which) {
+  const uint32_t p = find_problem(off, np, blockIdx.x);
+  const BSetup& q = S[p];
+  cdf_offsets_tile(q.cdf[which], q.n, q.tile_sum, q.tile_lp, q.tile_a2, q.st, which, which, blockIdx.x - off[p],
+                   q.ntiles);
+}
+
+// v = z - ||alpha||^2 (generate_v, core_proposedAMM.cuh:4-15) for the
+// problems whose z came from the statistics pass; 256 rows per block.
+__global__ __launch_bounds__(256) void k_bzv(const BSetup* __restrict__ S, const uint32_t* off, uint32_t np) {
+  const uint32_t p = find_problem(off, np, blockIdx.x);
+  const BSetup& q = S[p];
+  if (!q.fusable) return;
+  const uint64_t r = (uint64_t)(blockIdx.x - off[p]) * 256 + threadIdx.x;
+  if (r >= q.dpad) return;
+  q.v[r] = r < q.d ? (float)((double)q.z[r] - q.st->alpha2) : 0.0f;
+}
+
+hipError_t launch_bcolstats(const BSetup* S, BGroup g, uint32_t np, uint32_t skip_fusable, hipStream_t s) {
+  if (!g.total) return hipSuccess;
+  k_bcolstats<<<g.total, 256, 0, s>>>(S, g.off, np, skip_fusable);
+  return hipGetLastError();
+}
+
+hipError_t launch_bcdf(const BSetup* S, BGroup g, uint32_t np, int which, hipStream_t s) {
+  if (!g.total) return hipSuccess;
+  k_bcdf_max<<<g.total, 256, 0, s>>>(S, g.off, np, which);
+  k_bcdf_scan<<<g.total, 256, 0, s>>>(S, g.off, np, which);
+  k_bcdf_offsets<<<g.total, 256, 0, s>>>(S, g.off, np, which);
+  return hipGetLastError();
+}
+
+hipError_t launch_bzv(const BSetup* S, BGroup g, uint32_t np, hipStream_t s) {
+  if (!g.total) return hipSuccess;
+  k_bzv<<<g.total, 256, 0, s>>>(S, g.off, np);
+  return hipGetLastError();
+}
+
 hipError_t launch_bbegin(const BProb* P, BGroup g, BRun r, hipStream_t s) {
   if (g.total) k_bbegin<<<g.total, 256, 0, s>>>(P, g.off, r);
   return hipGetLastError();

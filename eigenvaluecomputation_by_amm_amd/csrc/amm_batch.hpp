@@ -136,6 +136,37 @@ struct BGroup {
   uint32_t total;
 };
 
+// Per-problem setup descriptor of the grouped statistics / CDF launches.
+struct BSetup {
+  const void* A;
+  uint64_t ld;
+  uint32_t d, n;
+  int32_t dtype;
+  uint32_t fusable;  // the statistics pass can form z = A alpha (register-held columns)
+  float* alpha;
+  double* nrm2;
+  double* cnrm2;
+  uint64_t* cdf[2];
+  DevState* st;
+  float* z;
+  float* v;
+  uint64_t dpad;
+  uint64_t* tile_max;  // this problem's CDF tile scratch (ntiles each)
+  uint64_t* tile_sum;
+  uint64_t* tile_lp;
+  double* tile_a2;
+  uint32_t ntiles;
+  uint32_t pad;
+};
+
+// Grouped setup: column statistics of every problem (one workgroup per
+// column; problems with fusable set are skipped when skip_fusable), the
+// CDF of weights `which` (0: nrm2, 1: cnrm2 with ||alpha||^2) of every
+// problem, and v = z - ||alpha||^2 of the fusable problems.
+hipError_t launch_bcolstats(const BSetup* S, BGroup g, uint32_t np, uint32_t skip_fusable, hipStream_t s);
+hipError_t launch_bcdf(const BSetup* S, BGroup g, uint32_t np, int which, hipStream_t s);
+hipError_t launch_bzv(const BSetup* S, BGroup g, uint32_t np, hipStream_t s);
+
 hipError_t launch_bbegin(const BProb* P, BGroup g, BRun r, hipStream_t s);
 hipError_t launch_bx0(const BProb* P, BGroup g, BRun r, hipStream_t s);
 hipError_t launch_btail(const BProb* P, BRun r, uint32_t op, uint32_t it, hipStream_t s);

@@ -101,6 +101,10 @@ struct amm_batch {
   double* tile_a2 = nullptr;
   float* zP2 = nullptr;
   double* zP2s = nullptr;
+  // grouped setup: descriptors (device) and their launch groups
+  amm::BSetup* dS = nullptr;
+  std::vector<uint32_t> tiles;  // CDF tiles per problem
+  amm::BGroup g_cols{}, g_tiles{}, g_zv{};
   // grouped-launch item offsets
   uint32_t* offs = nullptr;
   size_t offs_cap = 0;
@@ -216,6 +220,9 @@ int upload_plan(amm_batch* b, int kernel) {
     return acc < (1ull << 31);
   };
   bool ok = true;
+  ok &= add_group(b->g_cols, [](const BProb& p) { return p.n; });
+  ok &= add_group(b->g_tiles, [](const BProb& p) { return (uint32_t)amm::cdf_tiles(p.n); });
+  ok &= add_group(b->g_zv, [](const BProb& p) { return (uint32_t)cdiv(rup(p.d, 8), 256); });
   ok &= add_group(b->g_vec, [](const BProb& p) { return p.nvblk; });
   ok &= add_group(b->g_sample, [&](const BProb& p) {
     return kernel == AMM_KERNEL_EXACT ? 0u : b->ntr * (uint32_t)cdiv(p.c, 512);
@@ -327,34 +334,35 @@ amm::BRun run_args(amm_batch* b, const amm_params& p) {
   return r;
 }
 
-// Column statistics, CDFs and z = A alpha of every problem (the
-// single-matrix kernels, one problem after another on the stream).  z rides
-// in the statistics pass where the columns fit it (the partials of one
-// problem are summed before the next problem's pass reuses the buffer).
+// Column statistics, CDFs and z = A alpha of every problem.  Statistics
+// and CDFs are grouped launches over all problems.  For the centred
+// kernels, z rides in each problem's own statistics pass where the columns
+// fit it (its partials summed before the next problem's pass reuses the
+// buffer; v = z - ||alpha||^2 once the grouped centred CDF has set
+// ||alpha||^2), else z comes from a separate pass after the CDF.
 int enq_setup(amm_batch* b, int K) {
   const bool need_plain = K == AMM_KERNEL_PREVIOUS || K == AMM_KERNEL_PROPOSED_V2;
   const bool need_cent = K == AMM_KERNEL_PROPOSED || K == AMM_KERNEL_PROPOSED_V2;
-  for (uint32_t i = 0; i < b->np; ++i) {
-    const BProb& p = b->hp[i];
-    const SetupBufs& s = b->sb[i];
-    const bool fused = need_cent && amm::colstats_fuses_z(p.dtype, p.d);
-    if (fused)
+  BHIPCHK(amm::launch_bcolstats(b->dS, b->g_cols, b->np, need_cent ? 1u : 0u, b->stream));
+  if (need_cent) {
+    for (uint32_t i = 0; i < b->np; ++i) {
+      const BProb& p = b->hp[i];
+      const SetupBufs& s = b->sb[i];
+      if (!amm::colstats_fuses_z(p.dtype, p.d)) continue;
       BHIPCHK(amm::launch_colstats_z(p.dtype, p.A, p.ld, p.d, p.n, s.dpad8, s.alpha, s.nrm2, s.cnrm2,
                                      b->zP2, s.st, b->stream));
-    else
-      BHIPCHK(amm::launch_colstats(p.dtype, p.A, p.ld, p.d, p.n, s.alpha, s.nrm2, s.cnrm2, s.st,
-                                   b->stream));
-    if (need_plain)
-      BHIPCHK(amm::launch_cdf(s.nrm2, p.n, s.cdf0, s.st, 0, nullptr, b->tile_max, b->tile_sum,
-                              b->tile_lp, b->tile_a2, b->stream));
-    if (need_cent) {
-      BHIPCHK(amm::launch_cdf(s.cnrm2, p.n, s.cdf1, s.st, 1, s.alpha, b->tile_max, b->tile_sum,
-                              b->tile_lp, b->tile_a2, b->stream));
-      if (fused) {
-        BHIPCHK(amm::launch_sum_rows(b->zP2, amm::colstats_z_blocks(p.dtype, p.d, p.n), s.dpad8, p.d, s.z,
-                                     s.v, s.st, b->stream));
-        continue;
-      }
+      BHIPCHK(amm::launch_sum_rows(b->zP2, amm::colstats_z_blocks(p.dtype, p.d, p.n), s.dpad8, p.d, s.z,
+                                   nullptr, s.st, b->stream));
+    }
+  }
+  if (need_plain) BHIPCHK(amm::launch_bcdf(b->dS, b->g_tiles, b->np, 0, b->stream));
+  if (need_cent) {
+    BHIPCHK(amm::launch_bcdf(b->dS, b->g_tiles, b->np, 1, b->stream));
+    BHIPCHK(amm::launch_bzv(b->dS, b->g_zv, b->np, b->stream));
+    for (uint32_t i = 0; i < b->np; ++i) {
+      const BProb& p = b->hp[i];
+      const SetupBufs& s = b->sb[i];
+      if (amm::colstats_fuses_z(p.dtype, p.d)) continue;
       amm::GatherAxpyArgs a{};
       a.A = p.A;
       a.ld = p.ld;
@@ -660,15 +668,54 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     p.v = s.v;
     p.eigvec = nullptr;
   }
-  const uint64_t tiles = amm::cdf_tiles(max_n);
+  uint64_t tiles = 0;
+  b->tiles.resize(count);
+  for (uint32_t i = 0; i < count; ++i) {
+    b->tiles[i] = (uint32_t)amm::cdf_tiles(b->hp[i].n);
+    tiles += b->tiles[i];
+  }
   if (hipMalloc((void**)&b->tile_max, tiles * 8) != hipSuccess ||
       hipMalloc((void**)&b->tile_sum, tiles * 8) != hipSuccess ||
       hipMalloc((void**)&b->tile_lp, tiles * 8) != hipSuccess ||
       hipMalloc((void**)&b->tile_a2, tiles * 8) != hipSuccess ||
       hipMalloc((void**)&b->zP2, std::max<uint64_t>(max_zs, 1) * 4) != hipSuccess ||
       hipMalloc((void**)&b->zP2s, std::max<uint64_t>(cdiv(max_n, kZChunk), 1) * 8) != hipSuccess ||
-      hipMalloc((void**)&b->dP, count * sizeof(BProb)) != hipSuccess)
+      hipMalloc((void**)&b->dP, count * sizeof(BProb)) != hipSuccess ||
+      hipMalloc((void**)&b->dS, count * sizeof(amm::BSetup)) != hipSuccess)
     return bail(fail(AMM_E_NOMEM, "device allocation failed"));
+  {
+    std::vector<amm::BSetup> hs(count);
+    uint64_t t0 = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+      const BProb& p = b->hp[i];
+      const SetupBufs& s = b->sb[i];
+      amm::BSetup& q = hs[i];
+      q = amm::BSetup{};
+      q.A = p.A;
+      q.ld = p.ld;
+      q.d = p.d;
+      q.n = p.n;
+      q.dtype = p.dtype;
+      q.fusable = amm::colstats_fuses_z(p.dtype, p.d) ? 1u : 0u;
+      q.alpha = s.alpha;
+      q.nrm2 = s.nrm2;
+      q.cnrm2 = s.cnrm2;
+      q.cdf[0] = s.cdf0;
+      q.cdf[1] = s.cdf1;
+      q.st = s.st;
+      q.z = s.z;
+      q.v = s.v;
+      q.dpad = s.dpad8;
+      q.tile_max = b->tile_max + t0;
+      q.tile_sum = b->tile_sum + t0;
+      q.tile_lp = b->tile_lp + t0;
+      q.tile_a2 = b->tile_a2 + t0;
+      q.ntiles = b->tiles[i];
+      t0 += b->tiles[i];
+    }
+    if (hipMemcpy(b->dS, hs.data(), count * sizeof(amm::BSetup), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(AMM_E_HIP, "hipMemcpy failed"));
+  }
   if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
     return bail(fail(AMM_E_HIP, "hipEventCreate failed"));
   default_tuning(b);
@@ -682,7 +729,7 @@ int amm_batch_destroy(amm_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph(b);
   void* bufs[] = {b->spool, b->tpool,  b->rpool, b->tile_max, b->tile_sum, b->tile_lp,
-                  b->tile_a2, b->zP2, b->zP2s, b->offs,     b->dP};
+                  b->tile_a2, b->zP2, b->zP2s, b->offs,     b->dP,       b->dS};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
