@@ -329,65 +329,69 @@ __global__ __launch_bounds__(256) void k_bprep(const BProb* __restrict__ P, cons
   if (threadIdx.x < kBT) mode_s[threadIdx.x] = threadIdx.x < r.ntr ? pr.bt[threadIdx.x].mode : 0u;
   if (threadIdx.x < 2) total_s[threadIdx.x] = pr.st->total[threadIdx.x];
   __syncthreads();
-  const uint32_t tile = (blockIdx.x - off[p]) * 4 + (threadIdx.x >> 6);
-  if (tile >= pr.ntiles) return;
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t j = tile * kBTile + lane;
-  const bool inb = j < pr.n;
-  uint32_t cnt[kBT];
-  uint4* cp = reinterpret_cast<uint4*>(pr.cnt + (uint64_t)j * kBT);
-  if (inb) {
+  const uint32_t nwaves = (off[p + 1] - off[p]) * 4;
+  // wave-uniform loop: each wave takes tiles w, w + nwaves, ...
+  for (uint32_t tile = (blockIdx.x - off[p]) * 4 + (threadIdx.x >> 6); tile < pr.ntiles; tile += nwaves) {
+    const uint32_t j = tile * kBTile + lane;
+    const bool inb = j < pr.n;
+    uint32_t cnt[kBT];
+    uint4* cp = reinterpret_cast<uint4*>(pr.cnt + (uint64_t)j * kBT);
+    bool drawn = false;
+    if (inb) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const uint4 c4 = cp[v];
-      cnt[4 * v] = c4.x;
-      cnt[4 * v + 1] = c4.y;
-      cnt[4 * v + 2] = c4.z;
-      cnt[4 * v + 3] = c4.w;
-    }
-  } else {
-#pragma unroll
-    for (int v = 0; v < (int)kBT; ++v) cnt[v] = 0;
-  }
-  float w[kBT];
-  bool present = false;
-  uint64_t q0 = 0, q1 = 0;
-  bool have_q = false;
-#pragma unroll
-  for (int k = 0; k < (int)kBT; ++k) {
-    w[k] = 0.0f;
-    if ((uint32_t)k >= r.ntr || !inb) continue;
-    const uint32_t md = mode_s[k];
-    if (md == kModeExactMemo || md == kModeExactNoMemo) {
-      w[k] = 1.0f;
-      present = true;
-    } else if (cnt[k]) {
-      if (!have_q) {
-        q0 = pr.cdf0 ? pr.cdf0[j] - (j ? pr.cdf0[j - 1] : 0ull) : 0ull;
-        q1 = pr.cdf1 ? pr.cdf1[j] - (j ? pr.cdf1[j - 1] : 0ull) : 0ull;
-        have_q = true;
+      for (int v = 0; v < 4; ++v) {
+        const uint4 c4 = cp[v];
+        cnt[4 * v] = c4.x;
+        cnt[4 * v + 1] = c4.y;
+        cnt[4 * v + 2] = c4.z;
+        cnt[4 * v + 3] = c4.w;
+        drawn |= (c4.x | c4.y | c4.z | c4.w) != 0u;
       }
-      const int which = md == kModeAmmProposed ? 1 : 0;
-      const uint64_t qj = which ? q1 : q0;
-      const double num = (double)cnt[k] * (double)total_s[which];
-      const double den = (double)pr.c * (double)qj;
-      w[k] = den > 0.0 ? (float)(num / den) : 0.0f;
-      present = true;
+    } else {
+#pragma unroll
+      for (int v = 0; v < (int)kBT; ++v) cnt[v] = 0;
     }
-  }
-  const uint64_t mask = __ballot(present);
-  const uint32_t pos = __popcll(mask & ((1ull << lane) - 1ull));
-  if (present) {
-    const uint64_t slot = (uint64_t)tile * kBTile + pos;
-    pr.list[slot] = j;
-    float4* wp = reinterpret_cast<float4*>(pr.W + slot * kBT);
+    float w[kBT];
+    bool present = false;
+    uint64_t q0 = 0, q1 = 0;
+    bool have_q = false;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) wp[v] = make_float4(w[4 * v], w[4 * v + 1], w[4 * v + 2], w[4 * v + 3]);
-  }
-  if (lane == 0) pr.tcount[tile] = (uint32_t)__popcll(mask);
-  if (inb) {
+    for (int k = 0; k < (int)kBT; ++k) {
+      w[k] = 0.0f;
+      if ((uint32_t)k >= r.ntr || !inb) continue;
+      const uint32_t md = mode_s[k];
+      if (md == kModeExactMemo || md == kModeExactNoMemo) {
+        w[k] = 1.0f;
+        present = true;
+      } else if (cnt[k]) {
+        if (!have_q) {
+          q0 = pr.cdf0 ? pr.cdf0[j] - (j ? pr.cdf0[j - 1] : 0ull) : 0ull;
+          q1 = pr.cdf1 ? pr.cdf1[j] - (j ? pr.cdf1[j - 1] : 0ull) : 0ull;
+          have_q = true;
+        }
+        const int which = md == kModeAmmProposed ? 1 : 0;
+        const uint64_t qj = which ? q1 : q0;
+        const double num = (double)cnt[k] * (double)total_s[which];
+        const double den = (double)pr.c * (double)qj;
+        w[k] = den > 0.0 ? (float)(num / den) : 0.0f;
+        present = true;
+      }
+    }
+    const uint64_t mask = __ballot(present);
+    const uint32_t pos = __popcll(mask & ((1ull << lane) - 1ull));
+    if (present) {
+      const uint64_t slot = (uint64_t)tile * kBTile + pos;
+      pr.list[slot] = j;
+      float4* wp = reinterpret_cast<float4*>(pr.W + slot * kBT);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) cp[v] = make_uint4(0u, 0u, 0u, 0u);
+      for (int v = 0; v < 4; ++v) wp[v] = make_float4(w[4 * v], w[4 * v + 1], w[4 * v + 2], w[4 * v + 3]);
+    }
+    if (lane == 0) pr.tcount[tile] = (uint32_t)__popcll(mask);
+    if (drawn) {  // the counts start the next draw at zero; untouched ones already are
+#pragma unroll
+      for (int v = 0; v < 4; ++v) cp[v] = make_uint4(0u, 0u, 0u, 0u);
+    }
   }
 }
 
@@ -451,17 +455,19 @@ __global__ __launch_bounds__(256) void k_bcompact(const BProb* __restrict__ P, c
                                                   uint32_t np) {
   const uint32_t p = find_problem(off, np, blockIdx.x);
   const BProb& pr = P[p];
-  const uint32_t tile = (blockIdx.x - off[p]) * 4 + (threadIdx.x >> 6);
-  if (tile >= pr.ntiles) return;
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t cnt = pr.tcount[tile];
-  const uint32_t base = pr.tpre[tile];
-  // 64 slots x 16 weights: lane moves slot (i >> 4)'s weight (i & 15)
-  for (uint32_t i = lane; i < cnt * kBT; i += 64) {
-    const uint32_t slot = i >> 4;
-    pr.Wc[(uint64_t)(base + slot) * kBT + (i & 15)] = pr.W[((uint64_t)tile * kBTile + slot) * kBT + (i & 15)];
+  const uint32_t nwaves = (off[p + 1] - off[p]) * 4;
+  // the tiles of k_bprep's launch shape: wave w takes tiles w, w + nwaves, ...
+  for (uint32_t tile = (blockIdx.x - off[p]) * 4 + (threadIdx.x >> 6); tile < pr.ntiles; tile += nwaves) {
+    const uint32_t cnt = pr.tcount[tile];
+    const uint32_t base = pr.tpre[tile];
+    // 64 slots x 16 weights: lane moves slot (i >> 4)'s weight (i & 15)
+    for (uint32_t i = lane; i < cnt * kBT; i += 64) {
+      const uint32_t slot = i >> 4;
+      pr.Wc[(uint64_t)(base + slot) * kBT + (i & 15)] = pr.W[((uint64_t)tile * kBTile + slot) * kBT + (i & 15)];
+    }
+    if (lane < cnt) pr.colc[base + lane] = pr.list[(uint64_t)tile * kBTile + lane];
   }
-  if (lane < cnt) pr.colc[base + lane] = pr.list[(uint64_t)tile * kBTile + lane];
 }
 
 // ---------------------------------------------------------------------------

@@ -227,8 +227,11 @@ int upload_plan(amm_batch* b, int kernel) {
   ok &= add_group(b->g_sample, [&](const BProb& p) {
     return kernel == AMM_KERNEL_EXACT ? 0u : b->ntr * (uint32_t)cdiv(p.c, 512);
   });
-  ok &= add_group(b->g_prep, [](const BProb& p) { return (uint32_t)cdiv(p.ntiles, 4); });
-  ok &= add_group(b->g_weight, [](const BProb& p) { return std::min<uint32_t>(p.ntiles * 4u, 1024u); });
+  // a wave per 4 tiles / a thread per ~16 (slot, trial) entries: enough
+  // workgroups to fill the device on one large problem, few enough that
+  // many small problems do not pay per-workgroup start-up for one item each
+  ok &= add_group(b->g_prep, [](const BProb& p) { return (uint32_t)cdiv(p.ntiles, 16); });
+  ok &= add_group(b->g_weight, [](const BProb& p) { return std::min<uint32_t>((uint32_t)cdiv(p.ntiles, 4), 1024u); });
   for (int dt = 0; dt < 2; ++dt)
     ok &= add_group(b->g_dot[dt], [dt](const BProb& p) {
       // vtiles * kslices(|U|) < dot_blocks + vtiles for every union size
