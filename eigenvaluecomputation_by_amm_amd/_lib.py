@@ -53,6 +53,8 @@ _SIGS = {
     "amm_sigma_max": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "amm_rank1_update": (C.c_int, [_vp, C.c_int32, C.c_uint64, C.c_uint64, C.c_uint64, C.c_float,
                                    _vp, _vp, _vp]),
+    "amm_rank1_update_batched": (C.c_int, [_vp, C.c_int32, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
+                                           C.c_uint64, _vp, _vp, C.c_uint64, _vp, C.c_uint64, _vp]),
     "amm_batch_create": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _vp, C.POINTER(_vp)]),
     "amm_batch_destroy": (C.c_int, [_vp]),
     "amm_batch_set_exact_eigenvector": (C.c_int, [_vp, C.c_uint32, _vp, C.c_uint64]),

@@ -286,6 +286,32 @@ def rank1_update(out, g: float, u, v, d: int, n: int, ld: int) -> None:
                                C.c_void_p(stream) if stream else None))
 
 
+def rank1_update_batched(out, g, U, V, d: int, n: int, ld: int) -> None:
+    """out[k][:, j] = g[k] * U[k] * V[k, j] for a stack of column-major
+    d x n matrices held by the contiguous device tensor `out` of shape
+    (count, n, ld) (float32 or bfloat16), in one launch; g (count,), U
+    (count, d) and V (count, n) are float32 device tensors."""
+    import torch
+    lib = _lib.load()
+    if out.dtype == torch.float32:
+        dt = _lib.AMM_DTYPE_F32
+    elif out.dtype == torch.bfloat16:
+        dt = _lib.AMM_DTYPE_BF16
+    else:
+        raise ValueError("out must be float32 or bfloat16")
+    count = out.shape[0]
+    if out.dim() != 3 or tuple(out.shape[1:]) != (n, ld) or not out.is_contiguous():
+        raise ValueError("out must be a contiguous (count, n, ld) device tensor")
+    for t, shape in ((g, (count,)), (U, (count, d)), (V, (count, n))):
+        if t.dtype != torch.float32 or tuple(t.shape) != shape or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError("g, U, V must be contiguous float32 device tensors of shapes "
+                             "(count,), (count, d), (count, n)")
+    stream = torch.cuda.current_stream(out.device).cuda_stream
+    check(lib.amm_rank1_update_batched(C.c_void_p(out.data_ptr()), dt, d, n, ld, count, n * ld,
+                                       C.c_void_p(g.data_ptr()), C.c_void_p(U.data_ptr()), d,
+                                       C.c_void_p(V.data_ptr()), n, C.c_void_p(stream) if stream else None))
+
+
 def _run(kernel: str, ctx: AmmContext, **kw) -> RunResult:
     return ctx.power_method(RunParams(kernel=kernel, **kw))
 

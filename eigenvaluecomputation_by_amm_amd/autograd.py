@@ -21,7 +21,7 @@ from collections import OrderedDict
 
 import torch
 
-from .api import AmmBatch, AmmContext, RunParams, rank1_update
+from .api import AmmBatch, AmmContext, RunParams, rank1_update, rank1_update_batched
 
 _CACHE_MAX = 4
 _contexts: "OrderedDict[tuple, AmmContext]" = OrderedDict()
@@ -120,7 +120,8 @@ def _batch(mats, cols: int, params: RunParams) -> AmmBatch:
 class SigmaMaxStack(torch.autograd.Function):
     """sigma_k = sigma_max(W[k]) for a stack W (B, rows, cols), all matrices
     in one batched run (one grouped launch per kernel); d sigma_k / d W[k] =
-    v_k u_k^T (row-major view).  Matrix k uses the Philox key params.seed + k."""
+    v_k u_k^T (row-major view), all B adjoints in one launch.  Matrix k uses
+    the Philox key params.seed + k."""
 
     @staticmethod
     def forward(ctx, W: torch.Tensor, params: RunParams):
@@ -143,9 +144,8 @@ class SigmaMaxStack(torch.autograd.Function):
         U, V = ctx.saved_tensors
         B, rows, cols = ctx.shape
         dW = torch.empty(B, rows, cols, dtype=ctx.dtype, device=U.device)
-        gh = g.detach().float().cpu().tolist()
-        for k in range(B):
-            rank1_update(dW[k], gh[k], U[k], V[k], d=cols, n=rows, ld=cols)
+        gd = g.detach().to(device=U.device, dtype=torch.float32).contiguous()
+        rank1_update_batched(dW, gd, U.contiguous(), V.contiguous(), d=cols, n=rows, ld=cols)
         return dW, None
 
 

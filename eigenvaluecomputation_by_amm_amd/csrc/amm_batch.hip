@@ -33,11 +33,6 @@ __device__ __forceinline__ T sum16(T v) {
   return v;
 }
 
-__device__ __forceinline__ uint16_t bf16_rne(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
-
 __device__ __forceinline__ bool is_amm(uint32_t md) {
   return md == kModeAmmPrevious || md == kModeAmmProposed;
 }

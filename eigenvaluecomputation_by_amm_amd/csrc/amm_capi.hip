@@ -1048,6 +1048,21 @@ int amm_rank1_update(void* out, int32_t dtype, uint64_t d, uint64_t n, uint64_t 
   return AMM_OK;
 }
 
+int amm_rank1_update_batched(void* out, int32_t dtype, uint64_t d, uint64_t n, uint64_t ld, uint64_t count,
+                             uint64_t out_stride, const float* g_dev, const float* u_dev, uint64_t u_stride,
+                             const float* v_dev, uint64_t v_stride, void* stream) {
+  if (!out || !g_dev || !u_dev || !v_dev) return fail(AMM_E_INVALID, "NULL argument");
+  if (dtype != AMM_DTYPE_F32 && dtype != AMM_DTYPE_BF16) return fail(AMM_E_INVALID, "unknown dtype");
+  if (d < 1 || n < 1 || ld < d || d >= (1ull << 31) || n >= (1ull << 31))
+    return fail(AMM_E_INVALID, "need 1 <= d, n < 2^31 and ld >= d");
+  if (count < 1 || count >= 65536) return fail(AMM_E_INVALID, "count must be in [1, 65535]");
+  if (count > 1 && (out_stride < (n - 1) * ld + d || u_stride < d || v_stride < n))
+    return fail(AMM_E_INVALID, "strides must separate the matrices and vectors of the stack");
+  HIPCHK(amm::launch_rank1_batched(dtype, out, (uint32_t)d, n, ld, count, out_stride, 0.0f, g_dev, u_dev,
+                                   u_stride, v_dev, v_stride, (hipStream_t)stream));
+  return AMM_OK;
+}
+
 int amm_f32_to_bf16(const float* src, void* dst, uint64_t count, void* stream) {
   if (!src || !dst) return fail(AMM_E_INVALID, "NULL argument");
   HIPCHK(amm::launch_f32_to_bf16(src, (uint16_t*)dst, count, (hipStream_t)stream));

@@ -120,6 +120,12 @@ __device__ __forceinline__ T block256_sum(T v, T* lds4) {
   return r;
 }
 
+// FP32 -> BF16 bits, round to nearest even (finite inputs).
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
 __device__ __forceinline__ float bf16_bits_to_f32(uint32_t h16) {
   return __uint_as_float(h16 << 16);
 }

@@ -985,45 +985,62 @@ __global__ __launch_bounds__(256) void k_unit(const float* __restrict__ t, uint6
 }
 
 // Backward of sigma_max: out[:, j] = g * a * b_j for a column-major d x n
-// matrix (the rank-1 adjoint u v^T of d sigma / d A); 16-byte stores when
-// vec (out 16-byte aligned, ld a multiple of the vector width).  Grid-stride
-// over the 16-byte chunks of the matrix, column by column.
+// matrix (the rank-1 adjoint u v^T of d sigma / d A), FP32 products rounded
+// once (BF16 output: round to nearest even); 16-byte loads and stores when
+// vec (every matrix and vector 16-byte aligned, ld and the strides
+// multiples of the vector width).  Matrix k = blockIdx.y of a strided stack
+// uses g_k = gs[k] (or g), a + k astride, b + k bstride, out + k ostride.  A wave per column; its lanes stride over
+// the column's 16-byte chunks.
 template <typename T>
-__global__ __launch_bounds__(256) void k_rank1(T* __restrict__ out, uint32_t d, uint64_t ld,
-                                               uint64_t n, float g, const float* __restrict__ a,
-                                               const float* __restrict__ b, int vec) {
+__global__ __launch_bounds__(256) void k_rank1(T* __restrict__ out, uint32_t d, uint64_t ld, uint64_t n,
+                                               float g, const float* __restrict__ gs,
+                                               const float* __restrict__ a, const float* __restrict__ b,
+                                               int vec, uint64_t ostride, uint64_t astride,
+                                               uint64_t bstride) {
   constexpr uint32_t V = Elem<T>::VEC;
   const uint32_t cpc = (d + V - 1) / V;  // chunks per column
-  const uint64_t total = (uint64_t)cpc * n;
-  for (uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x; q < total; q += (uint64_t)gridDim.x * 256) {
-    const uint64_t j = q / cpc;
-    const uint32_t row = (uint32_t)(q - j * cpc) * V;
-    const float s = g * b[j];
-    T* col = out + j * ld + row;
-    float v[V];
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t k = blockIdx.y;
+  const float gk = gs ? gs[k] : g;
+  T* base = out + k * ostride;
+  const float* ak = a + k * astride;
+  const float* bk = b + k * bstride;
+  // wave per column, lanes over the column's 16-byte chunks
+  for (uint64_t j = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); j < n; j += (uint64_t)gridDim.x * 4) {
+    const float s = gk * bk[j];
+    T* colp = base + j * ld;
+    for (uint32_t c = lane; c < cpc; c += 64) {
+      const uint32_t row = c * V;
+      T* col = colp + row;
+      float v[V];
+      if (vec && row + V <= d) {
 #pragma unroll
-    for (uint32_t k = 0; k < V; ++k) v[k] = (row + k < d) ? s * a[row + k] : 0.0f;
-    if (vec && row + V <= d) {
-      if constexpr (V == 4) {
-        *reinterpret_cast<float4*>(col) = make_float4(v[0], v[1], v[2], v[3]);
-      } else {
-        uint32_t w[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const uint32_t lo = __float_as_uint(v[2 * k]), hi = __float_as_uint(v[2 * k + 1]);
-          const uint32_t rl = (lo + 0x7FFFu + ((lo >> 16) & 1u)) >> 16;
-          const uint32_t rh = (hi + 0x7FFFu + ((hi >> 16) & 1u)) >> 16;
-          w[k] = rl | (rh << 16);
+        for (uint32_t e = 0; e < V; e += 4) {  // 16-byte loads of a (cached: every column reuses it)
+          const float4 a4 = *reinterpret_cast<const float4*>(ak + row + e);
+          v[e] = s * a4.x;
+          v[e + 1] = s * a4.y;
+          v[e + 2] = s * a4.z;
+          v[e + 3] = s * a4.w;
         }
-        *reinterpret_cast<uint4*>(col) = make_uint4(w[0], w[1], w[2], w[3]);
+      } else {
+#pragma unroll
+        for (uint32_t e = 0; e < V; ++e) v[e] = (row + e < d) ? s * ak[row + e] : 0.0f;
       }
-    } else {
-      for (uint32_t k = 0; k < V && row + k < d; ++k) {
+      if (vec && row + V <= d) {
         if constexpr (V == 4) {
-          reinterpret_cast<float*>(col)[k] = v[k];
+          *reinterpret_cast<float4*>(col) = make_float4(v[0], v[1], v[2], v[3]);
         } else {
-          const uint32_t u = __float_as_uint(v[k]);
-          reinterpret_cast<uint16_t*>(col)[k] = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+          uint32_t w[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) w[q] = bf16_rne(v[2 * q]) | ((uint32_t)bf16_rne(v[2 * q + 1]) << 16);
+          *reinterpret_cast<uint4*>(col) = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+      } else {
+        for (uint32_t e = 0; e < V && row + e < d; ++e) {
+          if constexpr (V == 4)
+            reinterpret_cast<float*>(col)[e] = v[e];
+          else
+            reinterpret_cast<uint16_t*>(col)[e] = bf16_rne(v[e]);
         }
       }
     }
@@ -1325,15 +1342,25 @@ hipError_t launch_unit(const float* t, uint64_t n, const DevState* st, float* v,
 
 hipError_t launch_rank1(int dtype, void* out, uint32_t d, uint64_t n, uint64_t ld, float g,
                         const float* a, const float* b, hipStream_t s) {
-  if (n == 0 || d == 0) return hipSuccess;
+  return launch_rank1_batched(dtype, out, d, n, ld, 1, 0, g, nullptr, a, 0, b, 0, s);
+}
+
+hipError_t launch_rank1_batched(int dtype, void* out, uint32_t d, uint64_t n, uint64_t ld, uint64_t count,
+                                uint64_t ostride, float g, const float* gs, const float* a, uint64_t astride,
+                                const float* b, uint64_t bstride, hipStream_t s) {
+  if (n == 0 || d == 0 || count == 0) return hipSuccess;
   const uint64_t V = dtype == 1 ? 8 : 4;
-  const int vec = (ld % V == 0) && (((uintptr_t)out & 15u) == 0);
-  const uint64_t chunks = (d + V - 1) / V * n;
-  const unsigned blocks = (unsigned)std::min<uint64_t>((chunks + 255) / 256, 8192);
+  const uint64_t esz = dtype == 1 ? 2 : 4;
+  const int vec = (ld % V == 0) && (((uintptr_t)out & 15u) == 0) && (((uintptr_t)a & 15u) == 0) &&
+                  (count == 1 || ((ostride * esz) % 16 == 0 && astride % 4 == 0));
+  // about 2048 workgroups over the whole stack, a wave per column
+  const uint64_t per = std::max<uint64_t>(1, (2048 + count - 1) / count);
+  const dim3 grid((unsigned)std::min<uint64_t>((n + 3) / 4, per), (unsigned)count);
   if (dtype == 1)
-    k_rank1<__hip_bfloat16><<<blocks, 256, 0, s>>>((__hip_bfloat16*)out, d, ld, n, g, a, b, vec);
+    k_rank1<__hip_bfloat16><<<grid, 256, 0, s>>>((__hip_bfloat16*)out, d, ld, n, g, gs, a, b, vec, ostride, astride,
+                                                bstride);
   else
-    k_rank1<float><<<blocks, 256, 0, s>>>((float*)out, d, ld, n, g, a, b, vec);
+    k_rank1<float><<<grid, 256, 0, s>>>((float*)out, d, ld, n, g, gs, a, b, vec, ostride, astride, bstride);
   return hipGetLastError();
 }
 

@@ -156,6 +156,11 @@ hipError_t launch_reduce_p1(const float* P1, uint32_t R, uint64_t n, float* t, d
 hipError_t launch_unit(const float* t, uint64_t n, const DevState* st, float* v, hipStream_t s);
 hipError_t launch_rank1(int dtype, void* out, uint32_t d, uint64_t n, uint64_t ld, float g,
                         const float* a, const float* b, hipStream_t s);
+// count matrices of a strided stack in one launch (count < 65536): matrix k
+// at out + k ostride (elements), g_k = gs[k] when gs is given (device), else g.
+hipError_t launch_rank1_batched(int dtype, void* out, uint32_t d, uint64_t n, uint64_t ld, uint64_t count,
+                                uint64_t ostride, float g, const float* gs, const float* a, uint64_t astride,
+                                const float* b, uint64_t bstride, hipStream_t s);
 hipError_t launch_generate_symmetric(float* A, uint64_t m, uint64_t ld, int dist, float p1, float p2,
                                     uint64_t seed, uint32_t* delta_bits, hipStream_t s);
 hipError_t launch_f32_to_bf16(const float* src, uint16_t* dst, uint64_t count, hipStream_t s);

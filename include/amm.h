@@ -188,6 +188,15 @@ int amm_sigma_max(amm_ctx *ctx, const amm_params *p, double *sigma, float *u_dev
 int amm_rank1_update(void *out_dev, int32_t dtype, uint64_t d, uint64_t n, uint64_t ld, float g,
                      const float *u_dev, const float *v_dev, void *stream);
 
+/* The same for a strided stack of `count` matrices in one launch (no
+ * reference counterpart; the backward of a batched sigma_max): matrix k at
+ * out_dev + k out_stride elements gets g_dev[k] u_k v_k^T with u_k = u_dev +
+ * k u_stride, v_k = v_dev + k v_stride (g_dev: count FP32 values on the
+ * device).  1 <= count < 65536.  Asynchronous on stream. */
+int amm_rank1_update_batched(void *out_dev, int32_t dtype, uint64_t d, uint64_t n, uint64_t ld, uint64_t count,
+                             uint64_t out_stride, const float *g_dev, const float *u_dev, uint64_t u_stride,
+                             const float *v_dev, uint64_t v_stride, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Batched form (no reference counterpart: the reference solves one matrix
  * with one right-hand side per run, AMM.cu:7-72).  Many matrices of uneven

@@ -150,6 +150,30 @@ def test_rank1_update_bitwise(d, n, ld, dtype):
         assert torch.all(got[:, d:] == 7.0)
 
 
+@pytest.mark.parametrize("d,n,ld,dtype", [(256, 100, 256, torch.float32), (203, 77, 208, torch.float32),
+                                          (201, 33, 201, torch.bfloat16), (64, 40, 64, torch.bfloat16),
+                                          (13, 9, 16, torch.bfloat16)])
+def test_rank1_update_batched_matches_single(d, n, ld, dtype):
+    """One launch over a stack equals the per-matrix calls bit for bit
+    (per-matrix g from a device tensor; padding rows untouched)."""
+    banner(f"rank1 batched d={d} n={n} ld={ld} {dtype}")
+    from eigenvaluecomputation_by_amm_amd.api import rank1_update, rank1_update_batched
+    B = 5
+    gen = torch.Generator().manual_seed(d + n)
+    U = torch.randn(B, d, generator=gen).cuda()
+    V = torch.randn(B, n, generator=gen).cuda()
+    g = torch.randn(B, generator=gen).cuda()
+    out = torch.full((B, n, ld), 3.0, dtype=dtype, device="cuda")
+    rank1_update_batched(out, g, U, V, d=d, n=n, ld=ld)
+    ref = torch.full((B, n, ld), 3.0, dtype=dtype, device="cuda")
+    gh = g.cpu().tolist()
+    for k in range(B):
+        rank1_update(ref[k], gh[k], U[k], V[k], d=d, n=n, ld=ld)
+    torch.cuda.synchronize()
+    bits = torch.int16 if dtype == torch.bfloat16 else torch.int32
+    assert torch.equal(out.cpu().view(bits), ref.cpu().view(bits))
+
+
 @pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
 def test_sigma_stack(dtype):
     """A stack of planted matrices in one batched run: sigma and the
