@@ -117,11 +117,46 @@ def cpu_baseline(A_cols_host, d, lam, cfg, seed, iters_needed, budget_s=30.0):
     dt = time.perf_counter() - t0
     rel_err = abs(r["eig"][-1] - lam) / lam
     cores = O.num_threads()
-    return dict(value=dt * 1e3, unit="ms", cores=cores, kind="port",
-                sample=(f"oracle C power method, OpenMP over {cores} threads ({cfg['kernel']}, "
-                        f"c={cfg.get('c')}, vr={cfg.get('vr_period')}), seed {seed}, {step} iterations "
-                        f"on the full 16384x65536 matrix incl. column statistics; final rel err "
-                        f"{rel_err:.2e}"))
+    out = dict(value=dt * 1e3, unit="ms", cores=cores, kind="port",
+               sample=(f"oracle C power method, OpenMP over {cores} threads ({cfg['kernel']}, "
+                       f"c={cfg.get('c')}, vr={cfg.get('vr_period')}), seed {seed}, {step} iterations "
+                       f"on the full 16384x65536 matrix incl. column statistics; final rel err "
+                       f"{rel_err:.2e}"))
+    out["numpy"] = numpy_baseline(A_cols_host)
+    return out
+
+
+def numpy_baseline(A_cols_host):
+    """The reference's numpy path beside the port: one FP32 exact power
+    iteration y = A (A^T x) with numpy's BLAS on the full host matrix, and
+    the FP64 eigh of A A^T that calculate_largest_eigenvalue.py runs, on the
+    1024 x 2048 CPU-size configuration."""
+    try:
+        from threadpoolctl import threadpool_info
+        blas_threads = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
+    except Exception:
+        blas_threads = None
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(A_cols_host.shape[1]).astype(np.float32)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        t = A_cols_host @ x            # A^T x (row j of A_cols_host is column j of A)
+        y = A_cols_host.T @ t          # A t
+        x = (y / np.linalg.norm(y)).astype(np.float32)
+        times.append(time.perf_counter() - t0)
+    d1, n1 = 1024, 2048
+    U, _ = np.linalg.qr(rng.standard_normal((d1, d1)))
+    V, _ = np.linalg.qr(rng.standard_normal((n1, d1)))
+    M = ((U * (10.0 / np.arange(1, d1 + 1))) @ V.T + 0.01 * rng.standard_normal((d1, n1))).astype(np.float32)
+    t0 = time.perf_counter()
+    w = np.linalg.eigh((M.astype(np.float64) @ M.T.astype(np.float64)))[0]
+    t_eigh = time.perf_counter() - t0
+    return dict(exact_iteration_ms=round(statistics.median(times) * 1e3, 2),
+                eigh_1024x2048_ms=round(t_eigh * 1e3, 2), eigh_lambda_max=float(w[-1]),
+                blas_threads=blas_threads,
+                sample="numpy FP32 A (A^T x) on the full 16384x65536 matrix (median of 3 iterations); "
+                       "numpy FP64 eigh(A A^T) of the 1024x2048 configuration")
 
 
 def batched_trials(A, d, n, lam_f32, iters, cfg, reps, trials=16):
