@@ -96,7 +96,8 @@ def test_sample_bitwise(c, centred, iteration, trial):
                                          (8000, 130, None, False), (16384, 70, None, False),
                                          (16392, 50, None, False), (640, 900, None, True),
                                          (5000, 200, 5008, True), (16384, 60, None, True),
-                                         (16392, 40, None, True)])
+                                         (16392, 40, None, True), (1500, 1, None, False),
+                                         (2049, 3, 2056, True), (7, 2, 8, False)])
 def test_exact_product(d, n, ld, bf16):
     banner(f"exact_product d={d} n={n} ld={ld} bf16={bf16}")
     AmmContext, *_ = _amm()
@@ -271,6 +272,23 @@ def test_fused_z_equals_separate_pass(kernel, bf16, d, ld):
         mb = ctx.last_modes(20)
     assert np.array_equal(ma, mb)
     assert np.max(np.abs(a.eig - b.eig) / np.abs(b.eig)) < 1e-5
+
+
+@pytest.mark.parametrize("d,n,ld", [(600, 1, 600), (9, 2, 12), (4100, 5, 4100)])
+def test_fused_z_few_columns(d, n, ld):
+    """Statistics + z with fewer columns than resident workgroups (one
+    workgroup per column): proposed trajectories against the oracle."""
+    banner(f"fused z d={d} n={n}")
+    AmmContext, RunParams, _ = _amm()
+    A = lowrank_plus_noise(d, n, seed=16) + 0.3
+    t, cols = dev_matrix(A, ld=ld)
+    run = dict(kernel="proposedAMM", c=n, step=6, vr_period=3)
+    with AmmContext(t, d=d) as ctx:
+        assert ctx.time_pass("stats_z", iters=1) > 0   # statistics + z in one pass
+        _, nrm2, cnrm2, _ = ctx.column_stats()
+        res = ctx.power_method(RunParams(seed=3, **run))  # z from the statistics pass
+    ref = O.power_method(cols, d, seed=3, w_plain=nrm2, w_cent=cnrm2, **run)
+    assert np.max(np.abs(res.eig - ref["eig"]) / np.abs(ref["eig"])) < 1e-4
 
 
 def test_proposed_beyond_register_rows():
