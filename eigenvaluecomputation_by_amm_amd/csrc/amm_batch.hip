@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void k_bnorm(const BProb* __restrict__ P, cons
   for (uint32_t sub = 0; sub < kBVecRows / 64; ++sub) {
     const uint32_t row = blk * kBVecRows + sub * 64 + (threadIdx.x & 15) * 4;
     const uint64_t o = (uint64_t)k * pr.dstage + row;
-    const float4 y4 = *reinterpret_cast<const float4*>(pr.Y + o);
+    const float4 y4 = k < r.ntr ? *reinterpret_cast<const float4*>(pr.Y + o) : make_float4(0.f, 0.f, 0.f, 0.f);
     float4 m4 = make_float4(0.f, 0.f, 0.f, 0.f);
     if (memo) m4 = *reinterpret_cast<const float4*>(pr.XM + o);
     const float yv[4] = {y4.x, y4.y, y4.z, y4.w};
@@ -650,6 +650,10 @@ __global__ __launch_bounds__(256) void k_bweight(const BProb* __restrict__ P, co
   const uint64_t slab = (uint64_t)pr.ntiles * kBTile * kBT;
   for (uint32_t s = local * 16 + (threadIdx.x >> 4); s < up.U; s += nblk * 16) {
     const uint64_t e = (uint64_t)s * kBT + k;
+    if (k >= r.ntr) {  // unused trial slot: its weight is 0
+      pr.Tc[e] = 0.0f;
+      continue;
+    }
     float part[kMaxSlicesDev];
 #pragma unroll
     for (uint32_t q = 0; q < kMaxSlicesDev; ++q) part[q] = q < up.kslices ? pr.P1[q * slab + e] : 0.0f;
@@ -850,6 +854,10 @@ __global__ __launch_bounds__(256) void k_bfinish(const BProb* __restrict__ P, co
   for (uint32_t sub = 0; sub < kBVecRows / 64; ++sub) {
     const uint32_t row = blk * kBVecRows + sub * 64 + (threadIdx.x & 15) * 4;
     const uint64_t o = (uint64_t)k * pr.dstage + row;
+    if (k >= r.ntr) {  // unused trial slot: zeros, no reads
+      *reinterpret_cast<float4*>(pr.Y + o) = make_float4(0.f, 0.f, 0.f, 0.f);
+      continue;
+    }
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     for (uint32_t ch = 0; ch < active; ++ch) {
       const float4 q = *reinterpret_cast<const float4*>(pr.P2 + ((uint64_t)ch * kBT + k) * pr.p2pitch + row);
