@@ -227,11 +227,21 @@ int upload_plan(amm_batch* b, int kernel) {
   ok &= add_group(b->g_sample, [&](const BProb& p) {
     return kernel == AMM_KERNEL_EXACT ? 0u : b->ntr * (uint32_t)cdiv(p.c, 512);
   });
-  // a wave per 4 tiles / a thread per ~16 (slot, trial) entries: enough
-  // workgroups to fill the device on one large problem, few enough that
-  // many small problems do not pay per-workgroup start-up for one item each
-  ok &= add_group(b->g_prep, [](const BProb& p) { return (uint32_t)cdiv(p.ntiles, 16); });
-  ok &= add_group(b->g_weight, [](const BProb& p) { return std::min<uint32_t>((uint32_t)cdiv(p.ntiles, 4), 1024u); });
+  // Weights / compaction: a wave per tile and up to 1024 weight workgroups
+  // per problem fill the device on one large problem; across many problems
+  // both counts shrink by a common factor (each wave then loops over a few
+  // tiles) so that small problems do not pay workgroup start-up per item.
+  uint64_t tot_tiles = 0, tot_weight = 0;
+  for (const BProb& p : b->hp) {
+    tot_tiles += p.ntiles;
+    tot_weight += std::min<uint32_t>(p.ntiles * 4u, 1024u);
+  }
+  const uint64_t f_prep = std::max<uint64_t>(1, cdiv(tot_tiles, 16384));   // <= 4096 workgroups
+  const uint64_t f_weight = std::max<uint64_t>(1, cdiv(tot_weight, 8192));
+  ok &= add_group(b->g_prep, [f_prep](const BProb& p) { return (uint32_t)cdiv(p.ntiles, 4 * f_prep); });
+  ok &= add_group(b->g_weight, [f_weight](const BProb& p) {
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint32_t>(p.ntiles * 4u, 1024u) / f_weight);
+  });
   for (int dt = 0; dt < 2; ++dt)
     ok &= add_group(b->g_dot[dt], [dt](const BProb& p) {
       // vtiles * kslices(|U|) < dot_blocks + vtiles for every union size
@@ -346,7 +356,10 @@ amm::BRun run_args(amm_batch* b, const amm_params& p) {
 int enq_setup(amm_batch* b, int K) {
   const bool need_plain = K == AMM_KERNEL_PREVIOUS || K == AMM_KERNEL_PROPOSED_V2;
   const bool need_cent = K == AMM_KERNEL_PROPOSED || K == AMM_KERNEL_PROPOSED_V2;
-  BHIPCHK(amm::launch_bcolstats(b->dS, b->g_cols, b->np, need_cent ? 1u : 0u, b->stream));
+  bool any_plain = !need_cent;  // with z fused, only the problems it cannot fuse need this pass
+  for (uint32_t i = 0; i < b->np && !any_plain; ++i)
+    any_plain = !amm::colstats_fuses_z(b->hp[i].dtype, b->hp[i].d);
+  if (any_plain) BHIPCHK(amm::launch_bcolstats(b->dS, b->g_cols, b->np, need_cent ? 1u : 0u, b->stream));
   if (need_cent) {
     for (uint32_t i = 0; i < b->np; ++i) {
       const BProb& p = b->hp[i];
