@@ -291,6 +291,24 @@ def test_fused_z_few_columns(d, n, ld):
     assert np.max(np.abs(res.eig - ref["eig"]) / np.abs(ref["eig"])) < 1e-4
 
 
+@pytest.mark.parametrize("kernel", ["proposedAMM", "proposedAMM_v2"])
+def test_bf16_fused_z_beyond_exact_form(kernel):
+    """BF16 with 16384 < d <= 32768: z still rides in the statistics pass
+    (16 loads per lane) while the exact step takes the two-pass form;
+    trajectories against the oracle on the stored BF16 values."""
+    banner(f"bf16 d=20000 {kernel}")
+    AmmContext, RunParams, _ = _amm()
+    d, n = 20000, 120
+    A = lowrank_plus_noise(d, n, seed=17) + 0.2
+    t, bits = dev_matrix(A, bf16=True)
+    run = dict(kernel=kernel, c=60, step=8, vr_period=3)
+    with AmmContext(t, d=d) as ctx:
+        _, nrm2, cnrm2, _ = ctx.column_stats()
+        res = ctx.power_method(RunParams(seed=4, **run))
+    ref = O.power_method(bf16_to_f32(bits), d, seed=4, w_plain=nrm2, w_cent=cnrm2, **run)
+    assert np.max(np.abs(res.eig - ref["eig"]) / np.abs(ref["eig"])) < 1e-4
+
+
 def test_proposed_beyond_register_rows():
     """d above the register-held statistics form (16384 FP32 rows): the
     streamed statistics and a separate z pass, against the oracle."""
