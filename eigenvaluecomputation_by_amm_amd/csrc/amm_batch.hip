@@ -393,10 +393,10 @@ __global__ __launch_bounds__(256) void k_bprep(const BProb* __restrict__ P, cons
 // ---------------------------------------------------------------------------
 // 16 bytes of column `col` from row `row` (rows at or beyond d read 0).
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool NT>
 __device__ __forceinline__ u32x4 load_a16(const T* col, uint32_t row, uint32_t d) {
   constexpr uint32_t V = Elem<T>::VEC;
-  if (row + V <= d) return *reinterpret_cast<const u32x4*>(col + row);
+  if (row + V <= d) return load16<NT>(col + row);
   u32x4 z = {0u, 0u, 0u, 0u};
   if (row >= d) return z;
   if constexpr (V == 4) {
@@ -471,7 +471,7 @@ __global__ __launch_bounds__(256) void k_bcompact(const BProb* __restrict__ P, c
 // or 1024 contiguous bytes), prefetched into registers one round ahead and
 // stored to LDS at column pitch RB*sizeof(T) + 16 bytes.
 // ---------------------------------------------------------------------------
-template <typename T, uint32_t C, uint32_t RB>
+template <typename T, uint32_t C, uint32_t RB, bool NT>
 struct TileStage {
   static constexpr uint32_t kColBytes = RB * (uint32_t)sizeof(T);
   static constexpr uint32_t kLanesPerCol = kColBytes / 16;
@@ -490,7 +490,7 @@ struct TileStage {
 #pragma unroll
     for (uint32_t q = 0; q < kLoads; ++q) {
       const uint32_t j = cols[q * kColsPerPass + c0];
-      r[q] = j != 0xFFFFFFFFu ? load_a16<T>(A + (uint64_t)j * ld, row, d) : u32x4{0u, 0u, 0u, 0u};
+      r[q] = j != 0xFFFFFFFFu ? load_a16<T, NT>(A + (uint64_t)j * ld, row, d) : u32x4{0u, 0u, 0u, 0u};
     }
   }
   __device__ __forceinline__ void store(char* lds) const {
@@ -513,12 +513,12 @@ struct TileStage {
 // (FP32) of the step; B-operand lane (i, g): the same rows of dx_i;
 // accumulator lane (i, g) element e: slot 16G + 4g + e, trial i.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool NT>
 __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const uint32_t* off,
                                               uint32_t np) {
   constexpr bool BF = Elem<T>::VEC == 8;
   constexpr uint32_t RB = BF ? 256 : 128;
-  using Stage = TileStage<T, 64, RB>;
+  using Stage = TileStage<T, 64, RB, NT>;
   __shared__ __align__(16) char lds[Stage::kBytes];
   __shared__ uint32_t cols[64];
   const uint32_t p = find_problem(off, np, blockIdx.x);
@@ -682,14 +682,14 @@ __global__ __launch_bounds__(256) void k_bweight(const BProb* __restrict__ P, co
 // The proposed scalar sum_j alpha_j t_j of the chunk goes to P2s (row block
 // 0 only).
 // ---------------------------------------------------------------------------
-template <typename T, uint32_t PASSES>
+template <typename T, uint32_t PASSES, bool NT>
 __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, const uint32_t* off,
                                                uint32_t np, uint32_t proposed_any) {
   constexpr bool BF = Elem<T>::VEC == 8;
   constexpr uint32_t C = BF ? 32 : 16;
   constexpr uint32_t RB = 256 * PASSES;
   constexpr uint32_t WR = 64 * PASSES;  // rows per wave
-  using Stage = TileStage<T, C, RB>;
+  using Stage = TileStage<T, C, RB, NT>;
   __shared__ __align__(16) char lds[Stage::kBytes];
   __shared__ uint32_t cols[2][C];                // column indices, double-buffered by round
   __shared__ __align__(16) uint16_t th[kBT][C];  // BF16 t, high part [trial][column]
@@ -719,16 +719,16 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
   // Every load of round r + 1 (A tile, t, and the column indices of round
   // r + 2) is issued right after round r's first barrier, so it overlaps
   // round r's MFMAs; one barrier pair per round.
-  constexpr uint32_t NT = C * kBT / 256;  // t entries per thread
+  constexpr uint32_t NTE = C * kBT / 256;  // t entries per thread
   const float* Tc = pr.Tc;
   if (threadIdx.x < C) cols[0][threadIdx.x] = (sb + threadIdx.x < se) ? pr.colc[sb + threadIdx.x] : 0xFFFFFFFFu;
   uint32_t colnext = (threadIdx.x < C && sb + C + threadIdx.x < se) ? pr.colc[sb + C + threadIdx.x] : 0xFFFFFFFFu;
   __syncthreads();
   Stage st;
   st.load(A, pr.ld, cols[0], R0, d);
-  float tr[NT];
+  float tr[NTE];
 #pragma unroll
-  for (uint32_t q = 0; q < NT; ++q) {
+  for (uint32_t q = 0; q < NTE; ++q) {
     const uint32_t i = threadIdx.x + 256 * q, c = i >> 4;
     tr[q] = (sb + c < se) ? Tc[(uint64_t)(sb + c) * kBT + (i & 15)] : 0.0f;
   }
@@ -736,7 +736,7 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
   for (uint32_t s = sb; s < se; s += C, buf ^= 1u) {
     // this round's t into LDS (BF16: hi + lo), its A tile, next round's columns
 #pragma unroll
-    for (uint32_t q = 0; q < NT; ++q) {
+    for (uint32_t q = 0; q < NTE; ++q) {
       const uint32_t i = threadIdx.x + 256 * q, c = i >> 4, k = i & 15;
       const float tt = tr[q];
       if (proposed_any && rb == 0 && s + c < se) corr += (double)pr.alpha[cols[buf][c]] * (double)tt;
@@ -755,7 +755,7 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
     if (sn < se) {
       st.load(A, pr.ld, cols[buf ^ 1u], R0, d);
 #pragma unroll
-      for (uint32_t q = 0; q < NT; ++q) {
+      for (uint32_t q = 0; q < NTE; ++q) {
         const uint32_t i = threadIdx.x + 256 * q, c = i >> 4;
         tr[q] = (sn + c < se) ? Tc[(uint64_t)(sn + c) * kBT + (i & 15)] : 0.0f;
       }
@@ -1059,29 +1059,46 @@ hipError_t launch_bweight(const BProb* P, BGroup g, BRun r, uint32_t proposed_an
   return hipGetLastError();
 }
 
-hipError_t launch_bdot(int dtype, const BProb* P, BGroup g, uint32_t np, hipStream_t s) {
-  if (!g.total) return hipSuccess;
+template <bool NT>
+static void bdot_launch(int dtype, const BProb* P, BGroup g, uint32_t np, hipStream_t s) {
   if (dtype == 1)
-    k_bdot<__hip_bfloat16><<<g.total, 256, 0, s>>>(P, g.off, np);
+    k_bdot<__hip_bfloat16, NT><<<g.total, 256, 0, s>>>(P, g.off, np);
   else
-    k_bdot<float><<<g.total, 256, 0, s>>>(P, g.off, np);
+    k_bdot<float, NT><<<g.total, 256, 0, s>>>(P, g.off, np);
+}
+
+hipError_t launch_bdot(int dtype, bool nt, const BProb* P, BGroup g, uint32_t np, hipStream_t s) {
+  if (!g.total) return hipSuccess;
+  if (nt)
+    bdot_launch<true>(dtype, P, g, np, s);
+  else
+    bdot_launch<false>(dtype, P, g, np, s);
   return hipGetLastError();
 }
 
-hipError_t launch_baxpy(int dtype, uint32_t passes, const BProb* P, BGroup g, uint32_t np,
-                        uint32_t proposed_any, hipStream_t s) {
-  if (!g.total) return hipSuccess;
+template <bool NT>
+static void baxpy_launch(int dtype, uint32_t passes, const BProb* P, BGroup g, uint32_t np,
+                         uint32_t proposed_any, hipStream_t s) {
   if (dtype == 1) {
     if (passes == 1)
-      k_baxpy<__hip_bfloat16, 1><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any);
+      k_baxpy<__hip_bfloat16, 1, NT><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any);
     else
-      k_baxpy<__hip_bfloat16, 2><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any);
+      k_baxpy<__hip_bfloat16, 2, NT><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any);
   } else {
     if (passes == 1)
-      k_baxpy<float, 1><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any);
+      k_baxpy<float, 1, NT><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any);
     else
-      k_baxpy<float, 2><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any);
+      k_baxpy<float, 2, NT><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any);
   }
+}
+
+hipError_t launch_baxpy(int dtype, uint32_t passes, bool nt, const BProb* P, BGroup g, uint32_t np,
+                        uint32_t proposed_any, hipStream_t s) {
+  if (!g.total) return hipSuccess;
+  if (nt)
+    baxpy_launch<true>(dtype, passes, P, g, np, proposed_any, s);
+  else
+    baxpy_launch<false>(dtype, passes, P, g, np, proposed_any, s);
   return hipGetLastError();
 }
 

@@ -179,9 +179,10 @@ hipError_t launch_bcompact(const BProb* P, BGroup g, uint32_t np, hipStream_t s)
 // items: any count per problem (the blocks stride over the union slots)
 hipError_t launch_bweight(const BProb* P, BGroup g, BRun r, uint32_t proposed_any, hipStream_t s);
 // dtype-uniform groups: all problems of a group share the element type
-hipError_t launch_bdot(int dtype, const BProb* P, BGroup g, uint32_t np, hipStream_t s);
+// nt: A read with non-temporal loads (streams that no later pass reuses)
+hipError_t launch_bdot(int dtype, bool nt, const BProb* P, BGroup g, uint32_t np, hipStream_t s);
 // passes: 1 or 2 (a group shares it)
-hipError_t launch_baxpy(int dtype, uint32_t passes, const BProb* P, BGroup g, uint32_t np,
+hipError_t launch_baxpy(int dtype, uint32_t passes, bool nt, const BProb* P, BGroup g, uint32_t np,
                         uint32_t proposed_any, hipStream_t s);
 hipError_t launch_bfinish(const BProb* P, BGroup g, BRun r, hipStream_t s);
 hipError_t launch_bset_mode(const BProb* P, uint32_t np, uint32_t mode, hipStream_t s);
