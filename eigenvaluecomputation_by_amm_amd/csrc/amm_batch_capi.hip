@@ -970,8 +970,8 @@ int amm_batch_autotune(amm_batch* b, uint32_t iters, double* best_ms) {
     double ks_mult, ch_mult;
     uint32_t passes;
   };
-  const Cand cands[] = {{1, 1, 0},   {0.5, 0.5, 0}, {2, 2, 0},    {1, 1, 1},  {1, 1, 2},
-                        {2, 0.5, 0}, {0.5, 2, 0},   {0.25, 0.25, 0}, {4, 4, 0}};
+  const Cand cands[] = {{1, 1, 0},   {0.5, 0.5, 0}, {2, 2, 0},       {1, 1, 1}, {1, 1, 2}, {2, 0.5, 0},
+                        {0.5, 2, 0}, {0.25, 0.25, 0}, {4, 4, 0},    {4, 4, 1}, {4, 4, 2}, {8, 8, 0}};
   double tot = 0.0;
   for (const BProb& q : b->hp) tot += (double)q.ntiles * (double)q.d;
   double best = 1e30;
