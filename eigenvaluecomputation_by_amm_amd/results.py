@@ -11,7 +11,9 @@ of A A^T.
         (--exact VALUE | --matrix matrix.dat | --filepath F) [--tol 1e-3]
 
 prints, per repetition, the first step after which |Eigenvalue - exact| <=
-tol * exact holds for good and its time, then the median.
+tol * exact holds for good and its time, then the median.  With --plot
+PATH.png it also draws the relative error of every repetition against
+time (log scale), with the tolerance as a line.
 """
 from __future__ import annotations
 
@@ -47,6 +49,26 @@ def time_to_tolerance(runs: np.ndarray, exact: float, tol: float = 1e-3):
     return out
 
 
+def plot(runs: np.ndarray, exact: float, tol: float, path: str, label: str = "") -> None:
+    """Relative error |Eigenvalue - exact| / |exact| of every repetition
+    against cumulative time, log scale, tolerance as a dashed line."""
+    import matplotlib
+    matplotlib.use("Agg")
+    import matplotlib.pyplot as plt
+    fig, ax = plt.subplots(figsize=(6.4, 4.0))
+    for r, run in enumerate(runs):
+        rel = np.abs(run[:, 1] - exact) / abs(exact)
+        ax.semilogy(run[:, 0], np.maximum(rel, 1e-17), lw=0.8, alpha=0.6,
+                    color="C0", label=(label or "repetitions") if r == 0 else None)
+    ax.axhline(tol, color="k", ls="--", lw=0.8, label=f"tolerance {tol:g}")
+    ax.set_xlabel("time (ms)")
+    ax.set_ylabel("relative error of Eigenvalue")
+    ax.legend()
+    fig.tight_layout()
+    fig.savefig(path, dpi=120)
+    plt.close(fig)
+
+
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("binary")
@@ -55,6 +77,7 @@ def main(argv=None) -> None:
     ap.add_argument("--matrix", default=None, help="matrix.dat to solve for the reference")
     ap.add_argument("--filepath", default=None, help="-filepath matrix to solve for the reference")
     ap.add_argument("--tol", type=float, default=1e-3)
+    ap.add_argument("--plot", default=None, metavar="PATH.png")
     args = ap.parse_args(argv)
     exact = args.exact
     if exact is None:
@@ -69,6 +92,8 @@ def main(argv=None) -> None:
     ts = [t for _, t in res if t is not None]
     print(f"# exact {exact:.12f}; settled {len(ts)}/{len(res)}; "
           f"median time {statistics.median(ts) if ts else float('nan'):.6f} ms")
+    if args.plot:
+        plot(runs, exact, args.tol, args.plot, label=args.binary.rsplit("/", 1)[-1])
 
 
 if __name__ == "__main__":

@@ -70,6 +70,18 @@ def test_results_time_to_tolerance(tmp_path):
         results.read_binary(str(tmp_path / "r.binary"), 5)
 
 
+def test_results_plot(tmp_path, capsys):
+    _, results, _ = _tools()
+    step = 5
+    t = np.cumsum(np.full((3, step), 0.25), axis=1)
+    eig = 10.0 + np.array([[1.0, 0.1, 0.01, 1e-4, 1e-5]] * 3)
+    np.stack([t, eig, np.zeros_like(eig)], axis=-1).astype(np.float64).tofile(tmp_path / "r.binary")
+    out = tmp_path / "r.png"
+    results.main([str(tmp_path / "r.binary"), "--step", str(step), "--exact", "10.0", "--plot", str(out)])
+    assert "settled 3/3" in capsys.readouterr().out
+    assert out.read_bytes()[:8] == b"\x89PNG\r\n\x1a\n"
+
+
 def test_covariance_file(tmp_path):
     _, _, cov = _tools()
     rng = np.random.default_rng(2)
