@@ -232,3 +232,39 @@ def test_batch_edge_shapes(shapes, trials, c, bf16):
             e, _, m = _single(t, d, p, 2 + k)
             assert np.array_equal(modes[i, k], m)
             assert relmax(r.eig[i, k], e) < 1e-4, (i, k, r.eig[i, k], e)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_batch_random_mix(seed):
+    """Seeded random batches: 2-5 problems of random ragged shapes, FP32 and
+    BF16 mixed in one batch, random trial count, kernel, c and VR period;
+    every trial against the single-matrix run with the same seed and trial
+    word (same samples; step kinds identical, Rayleigh quotients within
+    1e-4)."""
+    rng = np.random.default_rng(7000 + seed)
+    AmmBatch, _, RunParams, _ = _api()
+    kernel = ["previousAMM", "proposedAMM", "proposedAMM_v2", "exact"][seed % 4]
+    trials = int(rng.integers(1, 17))
+    vr = int(rng.integers(2, 6))
+    probs, mats = [], []
+    for i in range(int(rng.integers(2, 6))):
+        d, n = int(rng.integers(1, 3000)), int(rng.integers(1, 3000))
+        bf16 = bool(rng.integers(2))
+        A = heavy_columns(d, n, seed=50 * seed + i, rank=min(4, d, n)) + 0.1
+        ld = (d + 7) // 8 * 8 + 8 * int(rng.integers(0, 2))
+        t = dev(A, bf16, ld)
+        mats.append((t, d))
+        probs.append(dict(A=t, d=d, c=int(rng.integers(1, n + 1)), seed=int(rng.integers(1, 2**40))))
+    banner(f"batch random seed={seed} {kernel} trials={trials} vr={vr} "
+           f"shapes={[(m[1], int(p['A'].shape[0]), p['A'].dtype) for m, p in zip(mats, probs)]}")
+    params = RunParams(kernel=kernel, step=8, vr_period=vr, trial=3)
+    with AmmBatch(probs, trials=trials) as b:
+        r = b.power_method(params)
+        modes = b.last_modes(params.step)
+    for i, (t, d) in enumerate(mats):
+        for k in range(trials):
+            p = RunParams(kernel=kernel, c=probs[i]["c"] if kernel != "exact" else 0, step=8, vr_period=vr,
+                          trial=3 + k, seed=probs[i]["seed"])
+            e, _, m = _single(t, d, p, 3 + k)
+            assert np.array_equal(modes[i, k], m), (i, k)
+            assert relmax(r.eig[i, k], e) < 1e-4, (i, k, r.eig[i, k], e)
