@@ -307,13 +307,13 @@ __global__ __launch_bounds__(256) void k_bsample(const BProb* __restrict__ P, co
 }
 
 // ---------------------------------------------------------------------------
-// Union of the trials' samples, one wave per tile of 64 columns: column j is
-// present when a sampled trial drew it or a trial takes an exact step.  The
-// wave compacts the present columns of its tile (ballot + prefix count) into
+// Union of the trials' samples, tile by tile of 64 columns: column j is
+// present when a sampled trial drew it or a trial takes an exact step.  A
+// wave compacts the present columns of a tile (ballot + prefix count) into
 // list[tile*64 ...], writes their weights W[slot][k] -- mult * total /
 // (c * q_j) for a sampled trial (update_weights, common.h:42-53, as
-// k_sample), 1 for an exact trial, 0 otherwise -- and clears the counts.
-// Items: ceil(ntiles / 4) per problem.
+// k_sample), 1 for an exact trial, 0 otherwise -- and clears the counts
+// that were drawn.  The problem's waves take tiles w, w + waves, ...
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bprep(const BProb* __restrict__ P, const uint32_t* off,
                                                BRun r) {
