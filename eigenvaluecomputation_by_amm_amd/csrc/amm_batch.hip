@@ -226,7 +226,7 @@ __global__ __launch_bounds__(256) void k_bnorm(const BProb* __restrict__ P, cons
 // slot i of trial k takes the 64-bit uniform of Philox counter (i >> 1, it,
 // trial0 + k, sample stream), words 2(i & 1), 2(i & 1) + 1, the integer
 // target floor(u * total / 2^64) and column min{j : cdf_j > target}, then
-// counts the draw in cnt[j][k].  Items: trials x ceil(c / 2048) per problem.
+// counts the draw in cnt[j][k].  Items: trials x ceil(c / 512) per problem.
 // The first block of a proposed trial also finishes s, q of its product.
 // ---------------------------------------------------------------------------
 // Open-addressing count table in LDS: key = column + 1 (0 = empty).  A block
@@ -444,7 +444,7 @@ __global__ __launch_bounds__(1024) void k_bscan(const BProb* __restrict__ P) {
 
 // ---------------------------------------------------------------------------
 // Compacted union: tile t's present columns and weights move to union slots
-// tpre[t] ... (one wave per tile, 4 tiles per workgroup).
+// tpre[t] ... (the tiles of k_bprep's launch: waves take tiles w, w + waves, ...).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bcompact(const BProb* __restrict__ P, const uint32_t* off,
                                                   uint32_t np) {
