@@ -101,7 +101,9 @@ def summarize(rows):
                 if any(r["t_first"] is not None for r in rows) else None,
                 median_iters=statistics.median([r["i_settled"] for r in rows if r["i_settled"] >= 0])
                 if ts else None,
-                median_run_ms=statistics.median([r["run_ms"] for r in rows]))
+                median_run_ms=statistics.median([r["run_ms"] for r in rows]),
+                final_rel_err_median=statistics.median([r["final_rel"] for r in rows]),
+                final_rel_err_max=max(r["final_rel"] for r in rows))
 
 
 def cpu_baseline(A_cols_host, d, lam, cfg, seed, iters_needed, budget_s=30.0):
@@ -586,7 +588,10 @@ def main():
                    "amm_p10_ms": am["p10_ms"], "amm_p90_ms": am["p90_ms"],
                    "amm_not_reached": am["not_reached"], "amm_median_iters": am["median_iters"],
                    "amm_first_hit_ms": am["median_first_ms"],
+                   "amm_final_rel_err_median": am["final_rel_err_median"],
+                   "amm_final_rel_err_max": am["final_rel_err_max"],
                    "exact_ms": ex["median_ms"], "exact_not_reached": ex["not_reached"],
+                   "exact_final_rel_err_max": ex["final_rel_err_max"],
                    "exact_median_iters": ex["median_iters"],
                    "speedup_vs_exact": (round(speed, 3) if speed else None)},
         "roofline": roof,
