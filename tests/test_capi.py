@@ -63,3 +63,32 @@ def test_null_context_calls_fail_cleanly():
     assert lib.amm_ctx_destroy(None) == 0
     assert lib.amm_power_method(None, None, None, None, None) == -1
     assert "NULL" in lib.amm_last_error().decode()
+
+
+def test_header_compiles_as_c_and_links(tmp_path):
+    """include/amm.h is plain C (C99): a C program that calls the library's
+    host-only entry points compiles with gcc, links against libamm.so and
+    runs without a GPU (argument validation happens before any device call)."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no gcc")
+    from eigenvaluecomputation_by_amm_amd import _lib
+    lib = Path(_lib.load()._name)
+    src = tmp_path / "probe.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <string.h>\n#include "amm.h"\n'
+        "int main(void) {\n"
+        "  amm_ctx *ctx = 0;\n"
+        "  if (amm_abi_version() != 1) return 1;\n"
+        "  if (amm_ctx_create(0, AMM_DTYPE_F32, 10, 10, 10, 0, &ctx) == 0) return 2;\n"
+        '  if (!strstr(amm_last_error(), "NULL")) return 3;\n'
+        '  printf("ok %s\\n", amm_last_error());\n'
+        "  return 0;\n}\n")
+    exe = tmp_path / "probe"
+    subprocess.run([gcc, "-std=c99", "-Wall", "-Werror", f"-I{ROOT / 'include'}", str(src), "-o", str(exe),
+                    f"-L{lib.parent}", "-lamm", f"-Wl,-rpath,{lib.parent}"], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+    assert r.stdout.startswith("ok")
