@@ -695,7 +695,7 @@ __global__ __launch_bounds__(256) void k_finish(
     const uint64_t r0 = (uint64_t)blockIdx.x * kFinishRows + l16 * 4;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     if (r0 < dpad) {
-#pragma unroll 8
+#pragma unroll 32  // 512 partials after the one-pass exact product: all loads in flight
       for (uint32_t s = g; s < S; s += 16) {
         const float4 p = *reinterpret_cast<const float4*>(P2 + (uint64_t)s * dpad + r0);
         a0 += p.x;
