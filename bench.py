@@ -548,8 +548,16 @@ def main():
     e1.record()
     torch.cuda.synchronize()
     copy_gbs = 10 * 2 * buf.numel() * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9
-    del buf, out
-    log(f"torch device copy: {copy_gbs:.0f} GB/s (read + write)")
+    # and a plain device reduction (read only), the library's streaming-read rate
+    s = buf.sum()
+    e0.record()
+    for _ in range(10):
+        s = buf.sum()
+    e1.record()
+    torch.cuda.synchronize()
+    sum_gbs = 10 * buf.numel() * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del buf, out, s
+    log(f"torch device copy: {copy_gbs:.0f} GB/s (read + write); torch sum: {sum_gbs:.0f} GB/s (read)")
     traffic = PMC_TRAFFIC_16384x65536.get(dom_name) if (d, n) == (16384, 65536) else None
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=dom_name,
@@ -557,7 +565,8 @@ def main():
                 share_of_run_us=round(dom_tot * 1e3, 1),
                 kernels_us={k: round(v[1] * 1e3, 2) for k, v in kern.items()},
                 not_in_run_us=two_pass,
-                sampled_columns=ucols, torch_copy_GBs=round(copy_gbs, 1))
+                sampled_columns=ucols, torch_copy_GBs=round(copy_gbs, 1),
+                torch_sum_GBs=round(sum_gbs, 1))
 
     cpu = None
     if not args.no_cpu and am["median_ms"] is not None:

@@ -450,16 +450,30 @@ class AmmBatch:
         check(self._lib.amm_batch_autotune(self._h, iters, C.c_void_p(_ptr(ms))))
         return float(ms[0])
 
-    def sigma_max(self, params: RunParams, vectors: bool = True):
+    def sigma_max(self, params: RunParams, vectors: bool = True, out_u=None, out_v=None):
         """sigma_max of every (problem, trial) ([problems][trials] float64)
         and, for trial 0 of each problem, the unit singular vectors u
-        (length d) and v (length n) as lists of device tensors."""
+        (length d) and v (length n) as lists of device tensors, or as the
+        rows of out_u (problems, d) and out_v (problems, n) when those
+        contiguous float32 device tensors are given (problems of one shape)."""
         import torch
         p = params.to_c()
         sig = np.zeros((self.count, self.trials), np.float64)
         U = V = None
         up = vp = None
-        if vectors:
+        if vectors and (out_u is not None or out_v is not None):
+            d, n = self.dims[0]
+            if any(dm != (d, n) for dm in self.dims):
+                raise ValueError("out_u / out_v need problems of one shape")
+            for t, m in ((out_u, d), (out_v, n)):
+                if (t is None or t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous()
+                        or tuple(t.shape) != (self.count, m)):
+                    raise ValueError(f"out_u / out_v must be contiguous float32 device tensors of "
+                                     f"shapes ({self.count}, {d}) and ({self.count}, {n})")
+            U, V = out_u, out_v
+            up = (C.c_void_p * self.count)(*[out_u.data_ptr() + 4 * d * i for i in range(self.count)])
+            vp = (C.c_void_p * self.count)(*[out_v.data_ptr() + 4 * n * i for i in range(self.count)])
+        elif vectors:
             dev = self._keep[0].device
             U = [torch.empty(d, dtype=torch.float32, device=dev) for d, _ in self.dims]
             V = [torch.empty(n, dtype=torch.float32, device=dev) for _, n in self.dims]

@@ -131,10 +131,17 @@ class SigmaMaxStack(torch.autograd.Function):
             raise ValueError("W must be float32 or bfloat16")
         B, rows, cols = W.shape
         Wd = W.detach()
-        mats, cols_ = zip(*[_as_columns(Wd[k]) for k in range(B)])
-        b = _batch(list(mats), cols_[0], params)
-        sig, U, V = b.sigma_max(params)
-        ctx.save_for_backward(torch.stack(U), torch.stack(V))
+        width = 4 if W.dtype == torch.float32 else 8
+        if (Wd.stride(2) == 1 and Wd.stride(1) >= cols and Wd.stride(1) % width == 0
+                and Wd.stride(0) % width == 0 and Wd.data_ptr() % 16 == 0):
+            mats = Wd.unbind(0)  # every matrix already qualifies as is
+        else:
+            mats = [_as_columns(Wd[k])[0] for k in range(B)]
+        b = _batch(list(mats), cols, params)
+        U = torch.empty(B, cols, dtype=torch.float32, device=W.device)
+        V = torch.empty(B, rows, dtype=torch.float32, device=W.device)
+        sig, _, _ = b.sigma_max(params, out_u=U, out_v=V)
+        ctx.save_for_backward(U, V)
         ctx.shape = (B, rows, cols)
         ctx.dtype = W.dtype
         return torch.tensor(sig[:, 0], dtype=torch.float32, device=W.device)

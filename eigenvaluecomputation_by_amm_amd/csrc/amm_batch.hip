@@ -919,7 +919,8 @@ __global__ __launch_bounds__(256) void k_bsigma(const BProb* __restrict__ P, BRu
   const BProb& pr = P[blockIdx.x];
   const uint32_t k = threadIdx.x >> 4, sub = threadIdx.x & 15;
   double acc = 0.0;
-  for (uint32_t j = sub; j < pr.n; j += 16) {
+#pragma unroll 8
+  for (uint32_t j = sub; j < pr.n; j += 16) {  // eight 1 KB rows of Tc in flight
     const double t = pr.Tc[(uint64_t)j * kBT + k];
     acc += t * t;
   }

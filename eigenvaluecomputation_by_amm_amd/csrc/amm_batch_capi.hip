@@ -93,6 +93,9 @@ struct amm_batch {
   char* tpool = nullptr;  // tuning-dependent partials (P1, P2, P2s)
   size_t tpool_bytes = 0;
   char* rpool = nullptr;  // per-run outputs (eig, err, modes, ts, schedule)
+  size_t rpool_bytes = 0;
+  double* dsig = nullptr;  // sigma_max outputs [np][ntr] and u, v pointer table [2 np]
+  float** dvec = nullptr;
   uint64_t rstep = 0;
   // shared setup scratch
   uint64_t* tile_max = nullptr;
@@ -301,6 +304,7 @@ int layout_run(amm_batch* b, uint64_t step) {
   b->t_start = reinterpret_cast<int64_t*>(b->rpool + ot0);
   b->modes_static = reinterpret_cast<uint32_t*>(b->rpool + oms);
   b->rstep = step;
+  b->rpool_bytes = pl.bytes;
   b->plan_valid = false;  // descriptors changed
   drop_graph(b);
   return AMM_OK;
@@ -461,10 +465,10 @@ int validate(const amm_batch* b, const amm_params* p) {
 }
 
 int check_status(amm_batch* b) {
+  std::vector<DevState> sts(b->np);
+  BHIPCHK(hipMemcpy(sts.data(), b->sb[0].st, b->np * sizeof(DevState), hipMemcpyDeviceToHost));
   for (uint32_t i = 0; i < b->np; ++i) {
-    uint32_t stt = 0;
-    BHIPCHK(hipMemcpy(&stt, (char*)b->sb[i].st + offsetof(DevState, status), sizeof(uint32_t),
-                      hipMemcpyDeviceToHost));
+    const uint32_t stt = sts[i].status;
     if (stt & amm::kStatusNonFinite)
       return fail(AMM_E_INVALID, "problem " + std::to_string(i) +
                                      ": the matrix has non-finite entries (NaN or Inf column norms)");
@@ -586,13 +590,15 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
   Pool pl;
   struct Off {
     size_t X, Y, XM, YM, DX, DXh, DXl, cnt, W, list, tcount, tpre, colc, Wc, Tc, BP, BQ, bt, alpha, nrm2,
-        cnrm2, cdf0, cdf1, z, v, st, eigvec;
+        cnrm2, cdf0, cdf1, z, v, eigvec;
   };
   std::vector<Off> o(count);
   b->hp.resize(count);
   b->sb.resize(count);
   b->tune.resize(count);
   uint64_t max_n = 0, max_zs = 0;
+  // the problems' device states side by side: one copy reads every status
+  const size_t st_all = pl.take<DevState>(count);
   for (uint32_t i = 0; i < count; ++i) {
     const amm_problem& q = probs[i];
     BProb& p = b->hp[i];
@@ -636,7 +642,6 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     f.cdf1 = pl.take<uint64_t>(q.n);
     f.z = pl.take<float>(dpad8);
     f.v = pl.take<float>(dpad8);
-    f.st = pl.take<DevState>(1);
     f.eigvec = pl.take<double>(dpad8);
     b->sb[i].dpad8 = dpad8;
     max_n = std::max<uint64_t>(max_n, q.n);
@@ -678,7 +683,7 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     s.cdf1 = reinterpret_cast<uint64_t*>(s0 + f.cdf1);
     s.z = reinterpret_cast<float*>(s0 + f.z);
     s.v = reinterpret_cast<float*>(s0 + f.v);
-    s.st = reinterpret_cast<DevState*>(s0 + f.st);
+    s.st = reinterpret_cast<DevState*>(s0 + st_all) + i;
     s.eigvec = reinterpret_cast<double*>(s0 + f.eigvec);
     p.st = s.st;
     p.cdf0 = s.cdf0;
@@ -750,8 +755,8 @@ int amm_batch_destroy(amm_batch* b) {
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph(b);
-  void* bufs[] = {b->spool, b->tpool,  b->rpool, b->tile_max, b->tile_sum, b->tile_lp,
-                  b->tile_a2, b->zP2, b->zP2s, b->offs,     b->dP,       b->dS};
+  void* bufs[] = {b->spool, b->tpool, b->rpool, b->tile_max, b->tile_sum, b->tile_lp, b->tile_a2,
+                  b->zP2,   b->zP2s,  b->offs,  b->dP,       b->dS,       b->dsig,    b->dvec};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -787,14 +792,17 @@ int amm_batch_power_method(amm_batch* b, const amm_params* p, double* time_ms, d
     BHIPCHK(hipMemcpy(t.data(), b->ts, S * sizeof(int64_t), hipMemcpyDeviceToHost));
     for (uint64_t i = 0; i < S; ++i) time_ms[i] = (double)t[i] / (double)b->wall_khz;
   }
-  std::vector<double> tmp(amm::kBT * S);
+  if (!eig && !err) return AMM_OK;
+  // one copy of the whole output pool, then each problem's first ntr trials
+  std::vector<char> h(b->rpool_bytes);
+  BHIPCHK(hipMemcpy(h.data(), b->rpool, h.size(), hipMemcpyDeviceToHost));
   for (int which = 0; which < 2; ++which) {
     double* dst = which ? err : eig;
     if (!dst) continue;
     for (uint32_t i = 0; i < b->np; ++i) {
-      BHIPCHK(hipMemcpy(tmp.data(), which ? b->hp[i].err : b->hp[i].eig, tmp.size() * sizeof(double),
-                        hipMemcpyDeviceToHost));
-      std::memcpy(dst + (uint64_t)i * b->ntr * S, tmp.data(), (size_t)b->ntr * S * sizeof(double));
+      const char* src = reinterpret_cast<const char*>(which ? b->hp[i].err : b->hp[i].eig);
+      std::memcpy(dst + (uint64_t)i * b->ntr * S, h.data() + (src - b->rpool),
+                  (size_t)b->ntr * S * sizeof(double));
     }
   }
   return AMM_OK;
@@ -805,10 +813,12 @@ int amm_batch_last_modes(amm_batch* b, uint32_t* modes, uint64_t len) {
   if (!b->ran) return fail(AMM_E_STATE, "no run yet");
   const uint64_t S = b->last_step;
   if (len < (uint64_t)b->np * b->ntr * S) return fail(AMM_E_INVALID, "modes buffer too small");
-  std::vector<uint32_t> tmp(amm::kBT * S);
+  std::vector<char> h(b->rpool_bytes);
+  BHIPCHK(hipMemcpy(h.data(), b->rpool, h.size(), hipMemcpyDeviceToHost));
   for (uint32_t i = 0; i < b->np; ++i) {
-    BHIPCHK(hipMemcpy(tmp.data(), b->hp[i].modes, tmp.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    std::memcpy(modes + (uint64_t)i * b->ntr * S, tmp.data(), (size_t)b->ntr * S * sizeof(uint32_t));
+    const char* src = reinterpret_cast<const char*>(b->hp[i].modes);
+    std::memcpy(modes + (uint64_t)i * b->ntr * S, h.data() + (src - b->rpool),
+                (size_t)b->ntr * S * sizeof(uint32_t));
   }
   return AMM_OK;
 }
@@ -911,27 +921,26 @@ int amm_batch_sigma_max(amm_batch* b, const amm_params* p, double* sigma_host, f
   if ((rc = run(b, p))) return rc;
   const amm::BRun r = run_args(b, *p);
   const size_t ns = (size_t)b->np * b->ntr;
-  double* dsig = nullptr;
+  // kept for the batch's lifetime (np and ntr are fixed at creation)
+  if (!b->dsig && hipMalloc((void**)&b->dsig, ns * sizeof(double)) != hipSuccess) {
+    b->dsig = nullptr;
+    return fail(AMM_E_NOMEM, "device allocation failed");
+  }
+  if (!b->dvec && hipMalloc((void**)&b->dvec, 2 * (size_t)b->np * sizeof(float*)) != hipSuccess) {
+    b->dvec = nullptr;
+    return fail(AMM_E_NOMEM, "device allocation failed");
+  }
+  double* dsig = b->dsig;
   float** dptr = nullptr;
-  BHIPCHK(hipMalloc((void**)&dsig, ns * sizeof(double)));
-  auto cleanup = [&]() {
-    (void)hipFree(dsig);
-    if (dptr) (void)hipFree(dptr);
-  };
+  std::vector<float*> hp;
   if (u_dev || v_dev) {
-    std::vector<float*> hp(2 * (size_t)b->np, nullptr);
+    hp.assign(2 * (size_t)b->np, nullptr);
     for (uint32_t i = 0; i < b->np; ++i) {
       if (u_dev) hp[i] = u_dev[i];
       if (v_dev) hp[b->np + i] = v_dev[i];
     }
-    if (hipMalloc((void**)&dptr, hp.size() * sizeof(float*)) != hipSuccess) {
-      cleanup();
-      return fail(AMM_E_NOMEM, "device allocation failed");
-    }
-    if (hipMemcpy(dptr, hp.data(), hp.size() * sizeof(float*), hipMemcpyHostToDevice) != hipSuccess) {
-      cleanup();
-      return fail(AMM_E_HIP, "hipMemcpy failed");
-    }
+    dptr = b->dvec;
+    BHIPCHK(hipMemcpyAsync(dptr, hp.data(), hp.size() * sizeof(float*), hipMemcpyHostToDevice, b->stream));
   }
   // t = A^T x for every trial: one exact half-step over the dense union
   hipError_t e = amm::launch_bset_mode(b->dP, b->np, amm::kModeExactNoMemo, b->stream);
@@ -944,9 +953,9 @@ int amm_batch_sigma_max(amm_batch* b, const amm_params* p, double* sigma_host, f
   if (e == hipSuccess)
     e = amm::launch_bsigma(b->dP, r, dsig, u_dev && dptr ? dptr : nullptr,
                            v_dev && dptr ? dptr + b->np : nullptr, b->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-  if (e == hipSuccess && sigma_host) e = hipMemcpy(sigma_host, dsig, ns * sizeof(double), hipMemcpyDeviceToHost);
-  cleanup();
+  if (e == hipSuccess && sigma_host)
+    e = hipMemcpyAsync(sigma_host, dsig, ns * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);  // also keeps hp alive until its copy ran
   if (e != hipSuccess) return fail(AMM_E_HIP, std::string("batched sigma_max failed: ") + hipGetErrorString(e));
   return AMM_OK;
 }

@@ -158,11 +158,15 @@ __global__ __launch_bounds__(256) void k_colstats_z(const T* __restrict__ A, uin
   uint64_t j = blockIdx.x;
   if (j < n) cr.load(A + j * ld, rb, re, lane);
   for (int par = 0; j < n; j += gridDim.x, par ^= 1) {
+    u32x4 cur[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) cur[l] = cr.raw[l];
+    if (j + gridDim.x < n) cr.load(A + (j + gridDim.x) * ld, rb, re, lane);  // streams in meanwhile
     double s = 0.0, ss = 0.0;
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       float v[V];
-      ColRegs<T, L>::unpack(cr.raw[l], v);
+      ColRegs<T, L>::unpack(cur[l], v);
 #pragma unroll
       for (int k = 0; k < V; ++k) {
         const double b = v[k];
@@ -176,10 +180,6 @@ __global__ __launch_bounds__(256) void k_colstats_z(const T* __restrict__ A, uin
       part[par][0][w] = s;
       part[par][1][w] = ss;
     }
-    u32x4 cur[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) cur[l] = cr.raw[l];
-    if (j + gridDim.x < n) cr.load(A + (j + gridDim.x) * ld, rb, re, lane);  // streams in meanwhile
     __syncthreads();
     const double S = (part[par][0][0] + part[par][0][1]) + (part[par][0][2] + part[par][0][3]);
     const double SS = (part[par][1][0] + part[par][1][1]) + (part[par][1][2] + part[par][1][3]);
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(256) void k_colstats_z(const T* __restrict__ A, uin
 // partials of y in P2[blockIdx.x] for k_finish.  x sits in LDS (zeros
 // beyond d).  Skipped when the gate says the step is not exact.
 template <typename T, int L>
-__global__ __launch_bounds__(256) void k_gram(const T* __restrict__ A, uint64_t ld, uint32_t d,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gram(const T* __restrict__ A, uint64_t ld, uint32_t d,
                                               uint64_t n, uint64_t dpad, const float* __restrict__ x,
                                               float* __restrict__ P2, const uint32_t* __restrict__ gate_mode,
                                               uint32_t it, uint32_t gate_mask) {
@@ -224,11 +224,15 @@ __global__ __launch_bounds__(256) void k_gram(const T* __restrict__ A, uint64_t 
   for (int e = 0; e < L * V; ++e) yp[e] = 0.0f;
   __syncthreads();
   for (int par = 0; j < n; j += gridDim.x, par ^= 1) {
+    u32x4 cur[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) cur[l] = cr.raw[l];
+    if (j + gridDim.x < n) cr.load(A + (j + gridDim.x) * ld, rb, re, lane);  // streams in meanwhile
     double t = 0.0;
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       float v[V];
-      ColRegs<T, L>::unpack(cr.raw[l], v);
+      ColRegs<T, L>::unpack(cur[l], v);
       const float* xr = xs + rb + lane * V + l * ColRegs<T, L>::step;
 #pragma unroll
       for (int k = 0; k < V; k += 4) {
@@ -241,10 +245,6 @@ __global__ __launch_bounds__(256) void k_gram(const T* __restrict__ A, uint64_t 
     }
     t = wave_sum(t);
     if (lane == 0) part[par][w] = t;
-    u32x4 cur[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) cur[l] = cr.raw[l];
-    if (j + gridDim.x < n) cr.load(A + (j + gridDim.x) * ld, rb, re, lane);  // streams in meanwhile
     __syncthreads();
     const float tj = (float)((part[par][0] + part[par][1]) + (part[par][2] + part[par][3]));
     resident_axpy<T, L>(yp, tj, cur);
