@@ -174,7 +174,9 @@ def batched_trials(A, d, n, lam_f32, iters, cfg, reps, trials=16):
     lam_b, _ = lambda_max_lanczos(Ab, d, k=64)
     for name, M, lam, s_A in (("bf16", Ab, lam_b, 2), ("f32", A, lam_f32, 4)):
         with AmmBatch([dict(A=M, c=cfg["c"], seed=1)], trials=trials) as b:
-            dense_ms = b.autotune(3)
+            # shapes tuned on short runs (two VR periods) of the same configuration
+            b.autotune_run(RunParams(step=2 * cfg["vr_period"], trial=20_000, **cfg), reps=2)
+            dense_ms = b.time_dense(3)
             b.power_method(RunParams(step=iters, trial=10_000, **cfg))  # graph build + warm-up
             alls, per, nr = [], [], 0
             for rep in range(reps):
@@ -221,7 +223,8 @@ def batch_config(args):
     probs = [dict(A=m16[i], c=max(1, n // 8), seed=77 + i) for i, (d, n) in enumerate(shapes)]
     with AmmBatch(probs, trials=16) as b:
         log("autotune")
-        dense_ms = b.autotune(2)
+        b.autotune_run(RunParams(step=2 * cfg["vr_period"], trial=20_000, **cfg), reps=1)
+        dense_ms = b.time_dense(2)
         b.power_method(RunParams(step=iters, trial=10_000, **cfg))  # graph build + warm-up
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -292,7 +295,7 @@ def bf16_config(args):
     lam_ex32 = float(rx[0].eig[-1])
     cfg = dict(kernel="proposedAMM", c=2048, vr_period=5)
     with AmmBatch([dict(A=Ab, c=cfg["c"], seed=3)], trials=16) as b:
-        b.autotune(3)
+        b.autotune_run(RunParams(step=2 * cfg["vr_period"], trial=20_000, **cfg), reps=2)
         b.power_method(RunParams(step=iters, trial=10_000, **cfg))
         alls, fin = [], []
         for rep in range(args.batch_reps):

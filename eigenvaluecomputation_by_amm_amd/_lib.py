@@ -65,6 +65,8 @@ _SIGS = {
     "amm_batch_set_tuning": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "amm_batch_get_tuning": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp]),
     "amm_batch_autotune": (C.c_int, [_vp, C.c_uint32, _vp]),
+    "amm_batch_autotune_run": (C.c_int, [_vp, _vp, C.c_uint32, _vp]),
+    "amm_batch_time_dense": (C.c_int, [_vp, C.c_uint32, _vp]),
     "amm_batch_sample": (C.c_int, [_vp, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, _vp, _vp, _vp]),
     "amm_batch_sigma_max": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "amm_f32_to_bf16": (C.c_int, [_vp, _vp, C.c_uint64, _vp]),

@@ -446,8 +446,24 @@ class AmmBatch:
         return int(a[0]), int(a[1]), int(a[2])
 
     def autotune(self, iters: int = 5) -> float:
+        """Shapes tuned on the dense pass; returns its time (ms)."""
         ms = np.zeros(1, np.float64)
         check(self._lib.amm_batch_autotune(self._h, iters, C.c_void_p(_ptr(ms))))
+        return float(ms[0])
+
+    def autotune_run(self, params: RunParams, reps: int = 2) -> float:
+        """Shapes tuned on complete runs of `params`; returns the best run's
+        device time (ms)."""
+        p = params.to_c()
+        ms = np.zeros(1, np.float64)
+        check(self._lib.amm_batch_autotune_run(self._h, C.byref(p), reps, C.c_void_p(_ptr(ms))))
+        return float(ms[0])
+
+    def time_dense(self, iters: int = 5) -> float:
+        """Mean time (ms) of one dense pass of both contractions with the
+        shapes in place."""
+        ms = np.zeros(1, np.float64)
+        check(self._lib.amm_batch_time_dense(self._h, iters, C.c_void_p(_ptr(ms))))
         return float(ms[0])
 
     def sigma_max(self, params: RunParams, vectors: bool = True, out_u=None, out_v=None):

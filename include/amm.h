@@ -260,6 +260,19 @@ int amm_batch_get_tuning(amm_batch *b, uint32_t problem, uint32_t *dot_blocks, u
  * fastest; *best_ms = one pass of both contractions with it. */
 int amm_batch_autotune(amm_batch *b, uint32_t iters, double *best_ms);
 
+/* Autotuner on whole runs: times complete runs of `params` (graph launch,
+ * device clock, best of `reps` after one warm-up each) for the shapes in
+ * place and along a coordinate descent over the default targets (chunks of
+ * the second contraction x 1/4 .. 16, then workgroups of the first x 1/4 ..
+ * 4, then passes 1 or 2) and keeps the fastest, so that sampled and exact
+ * steps weigh as often as the run takes them; *best_ms = that run's time.
+ * Leaves no run results behind that a caller should read. */
+int amm_batch_autotune_run(amm_batch *b, const amm_params *params, uint32_t reps, double *best_ms);
+
+/* Mean time (ms) of one pass of both contractions over every column (all
+ * trials exact) with the shapes in place; `iters` timed repetitions. */
+int amm_batch_time_dense(amm_batch *b, uint32_t iters, double *ms);
+
 /* Union sample of iteration `iter` for one problem with every trial in the
  * step kind of `kernel` (previousAMM: plain CDF, proposedAMM: centred),
  * trial words trial0 + k: the distinct columns (ascending, host, up to n)

@@ -167,6 +167,29 @@ def test_autotune_keeps_results():
     assert relmax(r1.eig, r0.eig) < 1e-4
 
 
+def test_autotune_run_keeps_results():
+    """Tuning on whole runs picks shapes that only change summation order;
+    the chosen shapes, set explicitly, reproduce the tuned run bit for bit."""
+    banner("batch autotune on runs")
+    AmmBatch, _, RunParams, _ = _api()
+    mats = [lowrank_plus_noise(640, 3000, seed=21, noise=0.02), lowrank_plus_noise(1000, 1500, seed=22, noise=0.02)]
+    params = RunParams(kernel="previousAMM", step=12, vr_period=4, seed=5)
+    probs = [dict(A=dev(M, True), c=M.shape[1] // 4, seed=30 + i) for i, M in enumerate(mats)]
+    with AmmBatch(probs, trials=8) as b:
+        r0 = b.power_method(params)
+        ms = b.autotune_run(params, reps=1)
+        dense = b.time_dense(2)
+        tuned = [b.get_tuning(i) for i in range(2)]
+        r1 = b.power_method(params)
+    assert ms > 0 and dense > 0
+    assert relmax(r1.eig, r0.eig) < 1e-4
+    with AmmBatch(probs, trials=8) as b2:
+        for i, t in enumerate(tuned):
+            b2.set_tuning(i, *t)
+        r2 = b2.power_method(params)
+    assert np.array_equal(r2.eig, r1.eig)
+
+
 def test_batch_invalid_arguments():
     banner("batch invalid arguments")
     AmmBatch, _, RunParams, AmmError = _api()
