@@ -350,7 +350,7 @@ def sigma_config(args):
     torch.cuda.synchronize()
     params = RunParams(kernel="previousAMM", c=1024, step=50, vr_period=5, seed=1)
     Wg = W.requires_grad_(True)
-    s = sigma_max_stack(Wg, params)  # warm-up: context + graph
+    s = sigma_max_stack(Wg, params, tune=True)  # warm-up: context, tuned shapes, graph
     s.sum().backward()
     Wg.grad = None
     torch.cuda.synchronize()

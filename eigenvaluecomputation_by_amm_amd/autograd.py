@@ -98,8 +98,10 @@ def sigma_max(W: torch.Tensor, params: RunParams | None = None) -> torch.Tensor:
 _batches: "OrderedDict[tuple, AmmBatch]" = OrderedDict()
 
 
-def _batch(mats, cols: int, params: RunParams) -> AmmBatch:
-    """Batched contexts cached per (pointers, shapes, dtype, c, seed, stream)."""
+def _batch(mats, cols: int, params: RunParams, tune: bool = False) -> AmmBatch:
+    """Batched contexts cached per (pointers, shapes, dtype, c, seed, stream);
+    with `tune`, a new one first tunes its launch shapes on runs of
+    `params` (once, when it is created)."""
     stream = torch.cuda.current_stream(mats[0].device).cuda_stream
     key = (tuple(m.data_ptr() for m in mats), tuple(mats[0].shape), mats[0].stride(0), cols,
            mats[0].dtype, int(params.c), int(params.seed), stream)
@@ -107,6 +109,8 @@ def _batch(mats, cols: int, params: RunParams) -> AmmBatch:
     if b is None:
         probs = [dict(A=m, d=cols, c=int(params.c), seed=int(params.seed) + i) for i, m in enumerate(mats)]
         b = AmmBatch(probs, trials=1)
+        if tune:
+            b.autotune_run(params, reps=1)
         _batches[key] = b
         while len(_batches) > _CACHE_MAX:
             _, old = _batches.popitem(last=False)
@@ -124,7 +128,7 @@ class SigmaMaxStack(torch.autograd.Function):
     the Philox key params.seed + k."""
 
     @staticmethod
-    def forward(ctx, W: torch.Tensor, params: RunParams):
+    def forward(ctx, W: torch.Tensor, params: RunParams, tune: bool = False):
         if not W.is_cuda or W.dim() != 3:
             raise ValueError("W must be a 3-D device tensor (stack, rows, cols)")
         if W.dtype not in (torch.float32, torch.bfloat16):
@@ -137,7 +141,7 @@ class SigmaMaxStack(torch.autograd.Function):
             mats = Wd.unbind(0)  # every matrix already qualifies as is
         else:
             mats = [_as_columns(Wd[k])[0] for k in range(B)]
-        b = _batch(list(mats), cols, params)
+        b = _batch(list(mats), cols, params, tune)
         U = torch.empty(B, cols, dtype=torch.float32, device=W.device)
         V = torch.empty(B, rows, dtype=torch.float32, device=W.device)
         sig, _, _ = b.sigma_max(params, out_u=U, out_v=V)
@@ -153,12 +157,14 @@ class SigmaMaxStack(torch.autograd.Function):
         dW = torch.empty(B, rows, cols, dtype=ctx.dtype, device=U.device)
         gd = g.detach().to(device=U.device, dtype=torch.float32).contiguous()
         rank1_update_batched(dW, gd, U.contiguous(), V.contiguous(), d=cols, n=rows, ld=cols)
-        return dW, None
+        return dW, None, None
 
 
-def sigma_max_stack(W: torch.Tensor, params: RunParams | None = None) -> torch.Tensor:
+def sigma_max_stack(W: torch.Tensor, params: RunParams | None = None, tune: bool = False) -> torch.Tensor:
     """Largest singular value of every matrix of W (B, rows, cols),
-    differentiable in W.  Default: 200 exact iterations."""
+    differentiable in W.  Default: 200 exact iterations.  tune: on the first
+    call for this stack, tune the batched launch shapes on runs of `params`
+    (about 30 runs, once; later calls reuse the cached batch)."""
     if params is None:
         params = RunParams(kernel="exact", step=200)
-    return SigmaMaxStack.apply(W, params)
+    return SigmaMaxStack.apply(W, params, tune)

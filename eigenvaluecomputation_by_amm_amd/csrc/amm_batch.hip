@@ -545,7 +545,9 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
   f32x4_t acc[4];
 #pragma unroll
   for (int G = 0; G < 4; ++G) acc[G] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-  // the next round's A tile and this wave's dx rows are prefetched together
+  // the next round's A tile is prefetched right after this round's store
+  // to LDS, this wave's dx rows of the next round right after the MFMAs
+  // have used this round's (no register copy: 4 workgroups per CU fit)
   Stage st;
   u32x4 xa[2], xb[2];  // BF16: dx hi, lo of K-steps 0, 1; FP32: dx of K-steps 0, 1 (xa)
   auto load_dx = [&](uint32_t row0) {
@@ -568,22 +570,14 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
   for (uint32_t row0 = r0; row0 < r1; row0 += RB) {
     st.store(lds);
     __syncthreads();
-    u32x4 ca[2], cb[2];
-#pragma unroll
-    for (uint32_t s = 0; s < 2; ++s) {
-      ca[s] = xa[s];
-      if constexpr (BF) cb[s] = xb[s];
-    }
-    if (row0 + RB < r1) {
-      st.load(A, pr.ld, cols, row0 + RB, pr.d);
-      load_dx(row0 + RB);
-    }
+    const bool more = row0 + RB < r1;
+    if (more) st.load(A, pr.ld, cols, row0 + RB, pr.d);
     if constexpr (BF) {
 #pragma unroll
       for (uint32_t s = 0; s < 2; ++s) {
         const uint32_t rr = wave * 64 + s * 32 + 8 * lg;  // row within the round
-        const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, ca[s]);
-        const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, cb[s]);
+        const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, xa[s]);
+        const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, xb[s]);
 #pragma unroll
         for (uint32_t G = 0; G < 4; ++G) {
           if (G < ngroups) {
@@ -598,7 +592,7 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
 #pragma unroll
       for (uint32_t s = 0; s < 2; ++s) {
         const uint32_t rr = wave * 32 + s * 16 + 4 * lg;
-        const u32x4 b = ca[s];
+        const u32x4 b = xa[s];
 #pragma unroll
         for (uint32_t G = 0; G < 4; ++G) {
           if (G < ngroups) {
@@ -611,6 +605,7 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
         }
       }
     }
+    if (more) load_dx(row0 + RB);
     __syncthreads();
   }
   // cross-wave sum in LDS (reusing the tile buffer), one coalesced write

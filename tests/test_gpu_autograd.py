@@ -207,3 +207,21 @@ def test_sigma_stack_equals_single_exact():
     for k in range(4):
         ss = sigma_max(Ws[k].contiguous(), p)
         assert abs(sb[k].item() - ss.item()) <= 1e-6 * ss.item()
+
+
+def test_sigma_stack_tuned():
+    """tune=True tunes the batched shapes on runs first; sigma and the
+    gradient keep the untuned accuracy against FP64 svdvals."""
+    banner("sigma_max stack tuned")
+    from eigenvaluecomputation_by_amm_amd import RunParams
+    from eigenvaluecomputation_by_amm_amd.autograd import sigma_max_stack
+    Ws = torch.stack([planted(384, 320, seed=80 + k, top=9.0 + k, second=5.0) for k in range(4)])
+    W = Ws.float().cuda().requires_grad_(True)
+    s = sigma_max_stack(W, RunParams(kernel="previousAMM", c=160, step=60, vr_period=5, seed=4), tune=True)
+    s.sum().backward()
+    Wq = W.detach().double().cpu()
+    for k in range(4):
+        ref = torch.linalg.svdvals(Wq[k])[0].item()
+        assert abs(s[k].item() - ref) <= 5e-6 * ref, (k, s[k].item(), ref)
+        Gref = _svd_grad(Wq[k])
+        assert (W.grad[k].double().cpu() - Gref).abs().max().item() <= 2e-5 * Gref.abs().max().item() + 1e-6, k
