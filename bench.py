@@ -154,11 +154,44 @@ def numpy_baseline(A_cols_host):
     t0 = time.perf_counter()
     w = np.linalg.eigh((M.astype(np.float64) @ M.T.astype(np.float64)))[0]
     t_eigh = time.perf_counter() - t0
+    c1 = numpy_power_c1(M, float(w[-1]), rng)
     return dict(exact_iteration_ms=round(statistics.median(times) * 1e3, 2),
                 eigh_1024x2048_ms=round(t_eigh * 1e3, 2), eigh_lambda_max=float(w[-1]),
-                blas_threads=blas_threads,
+                power_1024x2048=c1, blas_threads=blas_threads,
                 sample="numpy FP32 A (A^T x) on the full 16384x65536 matrix (median of 3 iterations); "
-                       "numpy FP64 eigh(A A^T) of the 1024x2048 configuration")
+                       "numpy FP64 eigh(A A^T) of the 1024x2048 configuration; numpy FP32 power "
+                       "methods (exact, previous AMM c=256 with p_j ~ ||a_j||^2) on it, 100 iterations each")
+
+
+def numpy_power_c1(M, lam, rng, iters=100, c=256):
+    """A numpy power method on the 1024 x 2048 configuration, exact and
+    previous AMM (c draws with p_j ~ ||a_j||^2, weights 1/(c p_j)), from one
+    x0 ~ N(0, I): wall time of the 100 iterations and the final relative error
+    of the Rayleigh quotient against the FP64 eigh value."""
+    d1, n1 = M.shape
+    x0 = rng.standard_normal(d1).astype(np.float32)
+    x0 /= np.linalg.norm(x0)
+    p = (M.astype(np.float64) ** 2).sum(axis=0)
+    p /= p.sum()
+    cdf = np.cumsum(p)
+    Mt = np.ascontiguousarray(M.T)  # row j = column j: contiguous gathers
+    out = {}
+    for name in ("exact", "previousAMM"):
+        x = x0.copy()
+        lam_i = 0.0
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            if name == "exact":
+                y = M @ (M.T @ x)
+            else:
+                j = np.minimum(np.searchsorted(cdf, rng.random(c) * cdf[-1], side="right"), n1 - 1)
+                S = Mt[j]
+                y = S.T @ ((1.0 / (c * p[j])).astype(np.float32) * (S @ x))
+            lam_i = float(np.dot(x.astype(np.float64), y.astype(np.float64)))
+            x = (y / np.linalg.norm(y)).astype(np.float32)
+        dt = time.perf_counter() - t0
+        out[name] = dict(ms=round(dt * 1e3, 2), final_rel_err=abs(lam_i - lam) / lam)
+    return out
 
 
 def batched_trials(A, d, n, lam_f32, iters, cfg, reps, trials=16):
