@@ -469,15 +469,15 @@ __global__ __launch_bounds__(256) void k_bcompact(const BProb* __restrict__ P, c
 // LDS tile of C columns x RB rows staged from HBM with long contiguous runs:
 // each column's RB rows are read by RB*sizeof(T)/16 consecutive lanes (512
 // or 1024 contiguous bytes), prefetched into registers one round ahead and
-// stored to LDS at column pitch RB*sizeof(T) + 16 bytes.
+// stored to LDS at column pitch RB*sizeof(T) + PAD bytes.
 // ---------------------------------------------------------------------------
-template <typename T, uint32_t C, uint32_t RB, bool NT>
+template <typename T, uint32_t C, uint32_t RB, bool NT, uint32_t PAD = 16>
 struct TileStage {
   static constexpr uint32_t kColBytes = RB * (uint32_t)sizeof(T);
   static constexpr uint32_t kLanesPerCol = kColBytes / 16;
   static constexpr uint32_t kColsPerPass = 256 / kLanesPerCol;
   static constexpr uint32_t kLoads = C / kColsPerPass;
-  static constexpr uint32_t kPitch = kColBytes + 16;
+  static constexpr uint32_t kPitch = kColBytes + PAD;
   static constexpr uint32_t kBytes = C * kPitch;
   static constexpr uint32_t kElems16 = 16 / (uint32_t)sizeof(T);
   u32x4 r[kLoads];
@@ -669,7 +669,8 @@ __global__ __launch_bounds__(256) void k_bweight(const BProb* __restrict__ P, co
 // round: C columns of the chunk staged in LDS (contiguous runs of 512 / 1024
 // bytes per column) and their t in LDS.
 //   BF16: C = 32 = one v_mfma_f32_16x16x32_bf16 K step; A-operand lane
-//     (i, g) = row i, columns 8g..8g+7 read transposed from LDS; t = hi + lo
+//     (i, g) = row i, columns g, g + 4, ..., g + 28 read transposed from LDS
+//     (K slot 8g + m = column g + 4m, t in the same order); t = hi + lo
 //     (two MFMAs); accumulator lane (i, g) element e = row 4g + e, trial i.
 //   FP32: C = 16, v_mfma_f32_16x16x4_f32 on 4 columns: A-operand lane (i, g)
 //     = column 4q + g, rows 4i + u (element u of a 16-byte LDS read);
@@ -684,7 +685,12 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
   constexpr uint32_t C = BF ? 32 : 16;
   constexpr uint32_t RB = 256 * PASSES;
   constexpr uint32_t WR = 64 * PASSES;  // rows per wave
-  using Stage = TileStage<T, C, RB, NT>;
+  // BF16: column pitch = 8 dwords mod 64 banks, and lane group g of the
+  // A operand reads columns g, g + 4, ..., g + 28 (K slot 8 g + m holds
+  // column g + 4 m; t is stored in the same order): the four groups then
+  // read disjoint banks, where columns 8g .. 8g + 7 at a 16-byte pad had
+  // groups 0 and 2 (1 and 3) on the same banks
+  using Stage = TileStage<T, C, RB, NT, BF ? 32u : 16u>;
   __shared__ __align__(16) char lds[Stage::kBytes];
   __shared__ uint32_t cols[2][C];                // column indices, double-buffered by round
   __shared__ __align__(16) uint16_t th[kBT][C];  // BF16 t, high part [trial][column]
@@ -736,9 +742,10 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
       const float tt = tr[q];
       if (proposed_any && rb == 0 && s + c < se) corr += (double)pr.alpha[cols[buf][c]] * (double)tt;
       if constexpr (BF) {
+        const uint32_t slot = 8 * (c & 3) + (c >> 2);  // K slot of column c
         const uint16_t h = bf16_rne(tt);
-        th[k][c] = h;
-        tlo[k][c] = bf16_rne(tt - bf16_bits_to_f32(h));
+        th[k][slot] = h;
+        tlo[k][slot] = bf16_rne(tt - bf16_bits_to_f32(h));
       } else {
         tf[c][k] = tt;
       }
@@ -766,9 +773,9 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
         uint32_t w4[4];
 #pragma unroll
         for (uint32_t e = 0; e < 4; ++e) {
-          const uint32_t c0 = 8 * lg + 2 * e;
+          const uint32_t c0 = lg + 8 * e;  // K slots 8 lg + 2e, + 1: columns lg + 8e, + 4
           const uint32_t lo16 = l16[(c0 * Stage::kPitch) / 2 + row];
-          const uint32_t hi16 = l16[((c0 + 1) * Stage::kPitch) / 2 + row];
+          const uint32_t hi16 = l16[((c0 + 4) * Stage::kPitch) / 2 + row];
           w4[e] = lo16 | (hi16 << 16);
         }
         const bf16x8_t a = __builtin_bit_cast(bf16x8_t, u32x4{w4[0], w4[1], w4[2], w4[3]});

@@ -1038,10 +1038,17 @@ int amm_batch_autotune(amm_batch* b, uint32_t iters, double* best_ms) {
   int rc;
   double best = 1e30;
   std::vector<Tuning> best_t = b->tune;
+  const std::vector<Tuning> orig = b->tune;
+  auto restore = [&](int r) {  // a failed candidate leaves the shapes as they were
+    b->tune = orig;
+    b->plan_valid = false;
+    drop_graph(b);
+    return r;
+  };
   for (const Cand& cd : kCands) {
     set_candidate(b, cd);
     double per = 0.0;
-    if ((rc = time_dense(b, iters, &per))) return rc;
+    if ((rc = time_dense(b, iters, &per))) return restore(rc);
     if (per < best) {
       best = per;
       best_t = b->tune;
@@ -1062,6 +1069,13 @@ int amm_batch_autotune_run(amm_batch* b, const amm_params* p, uint32_t reps, dou
   if ((rc = validate(b, p))) return rc;
   double best = 1e30;
   std::vector<Tuning> best_t = b->tune;
+  const std::vector<Tuning> orig = b->tune;
+  auto restore = [&](int r) {  // a failed run leaves the shapes as they were
+    b->tune = orig;
+    b->plan_valid = false;
+    drop_graph(b);
+    return r;
+  };
   // best of `reps` runs after the one that captures the graph
   auto eval = [&](double* t) -> int {
     *t = 1e30;
@@ -1079,7 +1093,7 @@ int amm_batch_autotune_run(amm_batch* b, const amm_params* p, uint32_t reps, dou
     }
   };
   double t = 0.0;
-  if ((rc = eval(&t))) return rc;  // the shapes in place (kept on a tie)
+  if ((rc = eval(&t))) return restore(rc);  // the shapes in place (kept on a tie)
   consider(t);
   // coordinate descent over the multipliers of the default targets: the
   // chunk count of the second contraction first (it moves run times most),
@@ -1098,16 +1112,16 @@ int amm_batch_autotune_run(amm_batch* b, const amm_params* p, uint32_t reps, dou
     }
     return AMM_OK;
   };
-  if ((rc = step(cur))) return rc;
+  if ((rc = step(cur))) return restore(rc);
   const Cand base_ch = cur;
   for (double ch : {0.25, 0.5, 2.0, 4.0, 8.0, 16.0})
-    if ((rc = step(Cand{base_ch.ks_mult, ch, base_ch.passes}))) return rc;
+    if ((rc = step(Cand{base_ch.ks_mult, ch, base_ch.passes}))) return restore(rc);
   const Cand base_ks = cur;
   for (double ks : {0.25, 0.5, 2.0, 4.0})
-    if ((rc = step(Cand{ks, base_ks.ch_mult, base_ks.passes}))) return rc;
+    if ((rc = step(Cand{ks, base_ks.ch_mult, base_ks.passes}))) return restore(rc);
   const Cand base_pa = cur;
   for (uint32_t pa : {1u, 2u})
-    if ((rc = step(Cand{base_pa.ks_mult, base_pa.ch_mult, pa}))) return rc;
+    if ((rc = step(Cand{base_pa.ks_mult, base_pa.ch_mult, pa}))) return restore(rc);
   b->tune = best_t;
   b->plan_valid = false;
   drop_graph(b);
