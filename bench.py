@@ -94,6 +94,7 @@ def run_config(ctx, RunParams, lam, seeds, step, **cfg):
 def summarize(rows):
     ts = [r["t_settled"] for r in rows if r["t_settled"] is not None]
     return dict(median_ms=(statistics.median(ts) if ts else None),
+                mean_ms=(float(np.mean(ts)) if ts else None),
                 p10_ms=(float(np.percentile(ts, 10)) if ts else None),
                 p90_ms=(float(np.percentile(ts, 90)) if ts else None),
                 not_reached=1.0 - len(ts) / len(rows),
@@ -322,7 +323,7 @@ def bf16_config(args):
     log(f"lambda* (FP64, BF16 matrix) {lam_b:.12e}; (FP64, FP32 matrix) {lam_32:.12e}")
     iters = args.iterations
     with AmmContext(A32) as ctx:
-        rx = [ctx.power_method(RunParams(kernel="exact", step=iters, seed=s)) for s in range(3)]
+        rx = [ctx.power_method(RunParams(kernel="exact", step=iters, seed=s)) for s in range(10)]
     ex_idx = [settled_index(r.eig, lam_32) for r in rx]
     ex_ms = statistics.median(float(r.time_ms[i]) for r, i in zip(rx, ex_idx) if i >= 0)
     lam_ex32 = float(rx[0].eig[-1])
@@ -440,13 +441,48 @@ def sigma_config(args):
     print(json.dumps(line), flush=True)
 
 
+def seeds_config(args, ctx, RunParams, lam, iters):
+    """The distribution over seeds on the headline matrix: previousAMM,
+    proposedAMM and proposedAMM_v2 at c = 1024, 4096, 16384 (variance
+    reduction every 4 steps), `--seeds` seeds each, 200 iterations: time to
+    settle within 1e-3 (median, p10, p90, fraction not reached) and the
+    final relative error against the FP64 Lanczos value (median, p90, max);
+    the exact power method beside them (10 seeds)."""
+    seeds = list(range(1000, 1000 + args.seeds))
+    cells = {}
+    t0 = time.perf_counter()
+    ex = run_config(ctx, RunParams, lam, seeds[:10], iters, kernel="exact")
+    cells["exact"] = dict(summarize(ex), seeds=len(ex))
+    for kern in ("previousAMM", "proposedAMM", "proposedAMM_v2"):
+        for c in (1024, 4096, 16384):
+            rows = run_config(ctx, RunParams, lam, seeds, iters, kernel=kern, c=c, vr_period=4)
+            sm = summarize(rows)
+            fr = np.array([r["final_rel"] for r in rows])
+            cells[f"{kern} c={c}"] = dict(
+                median_ms=sm["median_ms"], p10_ms=sm["p10_ms"], p90_ms=sm["p90_ms"],
+                not_reached=sm["not_reached"], median_iters=sm["median_iters"],
+                final_rel_err_median=float(np.median(fr)), final_rel_err_p90=float(np.percentile(fr, 90)),
+                final_rel_err_max=float(fr.max()), seeds=len(rows))
+            log(f"seeds {kern:15s} c={c:6d}: {cells[f'{kern} c={c}']}")
+    wall = time.perf_counter() - t0
+    line = {
+        "metric": "distribution over seeds of the time to lambda_max within 1e-3 rel err, 1 MI355X",
+        "value": cells["previousAMM c=1024"]["median_ms"], "unit": "ms", "n_gpus": 1, "steps": args.seeds,
+        "warmup": 0, "ms_per_step": round(wall * 1e3 / max(1, 9 * args.seeds), 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{args.d}x{args.n} FP32 heavy-tailed columns (lognormal 1.5) + planted rank 8",
+                   "iterations": iters, "vr_period": 4, "tolerance": TOL, "lambda_star": lam, "cells": cells},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed AMM runs (seeds)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--iterations", type=int, default=200)
-    ap.add_argument("--exact-seeds", type=int, default=3)
+    ap.add_argument("--exact-seeds", type=int, default=20)
     ap.add_argument("--sweep", action="store_true", help="also sweep kernel x c x vr (stderr table)")
     ap.add_argument("--sweep-seeds", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
@@ -454,9 +490,11 @@ def main():
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--kernel-iters", type=int, default=200)
     ap.add_argument("--batch-reps", type=int, default=3, help="batched runs (16 seeds each)")
-    ap.add_argument("--config", choices=["headline", "batch", "bf16", "sigma"], default="headline",
-                    help="batch: many uneven matrices x 16 seeds; bf16: image-like BF16 workload "
-                         "(separate JSON lines)")
+    ap.add_argument("--config", choices=["headline", "batch", "bf16", "sigma", "seeds"], default="headline",
+                    help="batch: many uneven matrices x 16 seeds; bf16: image-like BF16 workload; "
+                         "sigma: weight stack; seeds: distribution over --seeds seeds of every "
+                         "kernel x c on the headline matrix (separate JSON lines)")
+    ap.add_argument("--seeds", type=int, default=100, help="seeds per cell of --config seeds")
     ap.add_argument("--count", type=int, default=256, help="matrices of --config batch")
     ap.add_argument("--batch-iters", type=int, default=150)
     ap.add_argument("--exact-iters", type=int, default=40)
@@ -494,6 +532,9 @@ def main():
 
     ctx = AmmContext(A)
     iters = args.iterations
+    if args.config == "seeds":
+        seeds_config(args, ctx, RunParams, lam, iters)
+        return
 
     # exact FP32 power method (comparator)
     log("exact power method")
@@ -630,12 +671,14 @@ def main():
         "config": {"workload": f"{d}x{n} FP32 heavy-tailed columns (lognormal 1.5) + planted rank 8",
                    "kernel": cfg["kernel"], "c": cfg["c"], "vr_period": cfg["vr_period"],
                    "iterations": iters, "tolerance": TOL, "lambda_star": lam,
-                   "amm_p10_ms": am["p10_ms"], "amm_p90_ms": am["p90_ms"],
+                   "amm_p10_ms": am["p10_ms"], "amm_p90_ms": am["p90_ms"], "amm_mean_ms": am["mean_ms"],
                    "amm_not_reached": am["not_reached"], "amm_median_iters": am["median_iters"],
                    "amm_first_hit_ms": am["median_first_ms"],
                    "amm_final_rel_err_median": am["final_rel_err_median"],
                    "amm_final_rel_err_max": am["final_rel_err_max"],
                    "exact_ms": ex["median_ms"], "exact_not_reached": ex["not_reached"],
+                   "exact_seeds": args.exact_seeds, "exact_p10_ms": ex["p10_ms"], "exact_p90_ms": ex["p90_ms"],
+                   "exact_mean_ms": ex["mean_ms"],
                    "exact_final_rel_err_max": ex["final_rel_err_max"],
                    "exact_median_iters": ex["median_iters"],
                    "speedup_vs_exact": (round(speed, 3) if speed else None)},
