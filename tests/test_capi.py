@@ -63,6 +63,12 @@ def test_null_context_calls_fail_cleanly():
     assert lib.amm_ctx_destroy(None) == 0
     assert lib.amm_power_method(None, None, None, None, None) == -1
     assert "NULL" in lib.amm_last_error().decode()
+    # the batched tuners and timers check their handles before any HIP call
+    assert lib.amm_batch_autotune_run(None, None, 1, None) == -1
+    assert "NULL" in lib.amm_last_error().decode()
+    assert lib.amm_batch_time_dense(None, 1, None) == -1
+    assert lib.amm_batch_autotune(None, 1, None) == -1
+    assert lib.amm_batch_destroy(None) == 0
 
 
 def test_header_compiles_as_c_and_links(tmp_path):
