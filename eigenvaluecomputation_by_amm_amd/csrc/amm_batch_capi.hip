@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -22,6 +23,7 @@
 
 namespace amm_host {
 int fail(int code, const std::string& msg);
+std::recursive_mutex& device_mutex();  // amm_capi.hip: capture vs. allocation
 }
 
 using amm::BProb;
@@ -187,6 +189,7 @@ int check_tuning(const BProb& p, const Tuning& t) {
 // Descriptors from the tuning, (re)allocation of the partial buffers and
 // the grouped-launch offsets.
 int upload_plan(amm_batch* b, int kernel) {
+  std::lock_guard<std::recursive_mutex> lk(amm_host::device_mutex());
   const uint32_t np = b->np;
   BHIPCHK(hipStreamSynchronize(b->stream));
   Pool pl;
@@ -285,6 +288,7 @@ int upload_plan(amm_batch* b, int kernel) {
 // per-run outputs: eig, err, modes [16][step] per problem; ts; schedule
 int layout_run(amm_batch* b, uint64_t step) {
   if (b->rpool && b->rstep == step) return AMM_OK;
+  std::lock_guard<std::recursive_mutex> lk(amm_host::device_mutex());
   Pool pl;
   std::vector<size_t> oe(b->np), oerr(b->np), om(b->np);
   for (uint32_t i = 0; i < b->np; ++i) {
@@ -506,6 +510,7 @@ int run(amm_batch* b, const amm_params* p) {
       (!(p->flags & AMM_FLAG_KEEP_STATS) || !b->stats_valid || b->stats_kernel != p->kernel);
   if (use_graph) {
     if (!b->graph_valid || !same_key(b->key, *p) || b->graph_setup != setup_included) {
+    std::lock_guard<std::recursive_mutex> lk(amm_host::device_mutex());
       drop_graph(b);
       BHIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
       rc = enqueue_run(b, *p, setup_included);
@@ -573,6 +578,7 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     if (((uintptr_t)q.A) & 15u) return fail(AMM_E_INVALID, w + "A must be 16-byte aligned");
     if (q.c > q.n) return fail(AMM_E_INVALID, w + "c must be <= n");
   }
+  std::lock_guard<std::recursive_mutex> lk(amm_host::device_mutex());
   amm_batch* b = new amm_batch();
   auto bail = [&](int r) {
     amm_batch_destroy(b);
@@ -758,6 +764,7 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
 
 int amm_batch_destroy(amm_batch* b) {
   if (!b) return AMM_OK;
+  std::lock_guard<std::recursive_mutex> lk(amm_host::device_mutex());
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph(b);
@@ -928,6 +935,7 @@ int amm_batch_sigma_max(amm_batch* b, const amm_params* p, double* sigma_host, f
   const amm::BRun r = run_args(b, *p);
   const size_t ns = (size_t)b->np * b->ntr;
   // kept for the batch's lifetime (np and ntr are fixed at creation)
+  std::unique_lock<std::recursive_mutex> lk(amm_host::device_mutex());
   if (!b->dsig && hipMalloc((void**)&b->dsig, ns * sizeof(double)) != hipSuccess) {
     b->dsig = nullptr;
     return fail(AMM_E_NOMEM, "device allocation failed");
@@ -936,6 +944,7 @@ int amm_batch_sigma_max(amm_batch* b, const amm_params* p, double* sigma_host, f
     b->dvec = nullptr;
     return fail(AMM_E_NOMEM, "device allocation failed");
   }
+  lk.unlock();
   double* dsig = b->dsig;
   float** dptr = nullptr;
   std::vector<float*> hp;

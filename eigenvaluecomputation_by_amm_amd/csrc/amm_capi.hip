@@ -17,12 +17,17 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "amm.h"
 #include "amm_device.hpp"
 #include "amm_launch.hpp"
+
+namespace amm_host {
+std::recursive_mutex& device_mutex();
+}
 
 using amm::DevState;
 
@@ -45,6 +50,7 @@ int fail(int code, const std::string& msg) {
 template <typename T>
 int grow(T*& p, size_t& cap, size_t need) {
   if (need <= cap && p) return AMM_OK;
+  std::lock_guard<std::recursive_mutex> lk(amm_host::device_mutex());
   if (p) (void)hipFree(p);
   p = nullptr;
   cap = 0;
@@ -79,6 +85,15 @@ struct AxpyPlan {
 // error reporting shared with the batched engine (amm_batch_capi.hip)
 namespace amm_host {
 int fail(int code, const std::string& msg) { return ::fail(code, msg); }
+
+// Held while any context or batch of this process captures a graph and
+// while one allocates, frees or clears device memory with the synchronous
+// calls (hipMalloc / hipFree / hipMemset), so that those never fall inside
+// another thread's capture.
+std::recursive_mutex& device_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
 }  // namespace amm_host
 
 struct amm_ctx {
@@ -662,6 +677,7 @@ int amm_ctx_create(const void* A, int32_t dtype, uint64_t d, uint64_t n, uint64_
                    amm_ctx** out) {
   if (!out) return fail(AMM_E_INVALID, "out is NULL");
   *out = nullptr;
+  std::lock_guard<std::recursive_mutex> lk(amm_host::device_mutex());
   if (!A) return fail(AMM_E_INVALID, "A is NULL");
   if (dtype != AMM_DTYPE_F32 && dtype != AMM_DTYPE_BF16) return fail(AMM_E_INVALID, "unknown dtype");
   if (d < 1 || n < 1) return fail(AMM_E_INVALID, "d and n must be >= 1");
@@ -722,6 +738,7 @@ int amm_ctx_create(const void* A, int32_t dtype, uint64_t d, uint64_t n, uint64_
 
 int amm_ctx_destroy(amm_ctx* c) {
   if (!c) return AMM_OK;
+  std::lock_guard<std::recursive_mutex> lk(amm_host::device_mutex());
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
@@ -890,6 +907,10 @@ int amm_power_method(amm_ctx* c, const amm_params* p, double* time_ms, double* e
   const bool use_graph = !(p->flags & AMM_FLAG_NO_GRAPH);
   const uint32_t cc = p->kernel == AMM_KERNEL_EXACT ? 0 : p->c;
   const bool rebuild = !c->graph_valid || !same_key(c->graph_key, *p) || c->list_c != cc;
+  // buffers (re)sized and the graph captured under the device lock: no
+  // other thread of this process allocates or frees meanwhile
+  std::unique_lock<std::recursive_mutex> lk(amm_host::device_mutex(), std::defer_lock);
+  if (rebuild) lk.lock();
   if (rebuild) {
     drop_graph(c);
     if ((rc = ensure_plans(c, cc))) return rc;
@@ -930,6 +951,7 @@ int amm_power_method(amm_ctx* c, const amm_params* p, double* time_ms, double* e
     c->graph_valid = true;
     c->graph_key = *p;
   }
+  if (lk.owns_lock()) lk.unlock();
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   if (use_graph) {
     HIPCHK(hipGraphLaunch(c->exec, c->stream));

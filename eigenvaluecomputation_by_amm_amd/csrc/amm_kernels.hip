@@ -740,10 +740,13 @@ __global__ __launch_bounds__(256) void k_finish(
 }
 
 // ---------------------------------------------------------------------------
-// Single-block tail of every grid-wide reduction: sums the block partials in
-// a fixed order and updates the scalar state.  A separate launch (rather
-// than a last-block-done epilogue) keeps every block free of the
-// device-scope fences that cross-XCD visibility would need.
+// Tail of every grid-wide reduction: sums the block partials in a fixed
+// order and updates the scalar state.  A separate launch (rather than a
+// last-block-done epilogue) keeps every block free of the device-scope
+// fences that cross-XCD visibility would need.  With normalisation and no
+// s, q dots it runs one block per 1024 rows, each repeating the reduction
+// (a few KB of partials) so that x = y / ||y|| needs no second launch;
+// block 0 writes the scalars.
 //   kTailFinish: lambda_i = y^T x, err_i = y^T v* / ||y||, cos_i, 1/||y||,
 //                last exact step, the next step kind (adaptive variance
 //                reduction), the time stamp
@@ -764,6 +767,9 @@ __global__ __launch_bounds__(256) void k_tail(
   __shared__ double red[4];
   __shared__ double scale_l;
   const uint32_t t = threadIdx.x;
+  // Every block repeats the (small, fixed-order) reduction of BP and
+  // normalises its own rows; block 0 alone writes the scalar state.
+  const bool lead = blockIdx.x == 0;
   if (op == kTailFinish) {
     double a = 0.0, b = 0.0, c = 0.0;
     for (uint32_t i = t; i < nb; i += 256) {
@@ -775,12 +781,15 @@ __global__ __launch_bounds__(256) void k_tail(
     b = block256_sum(b, red);
     c = block256_sum(c, red);
     if (t == 0) {
-      const uint32_t md = mode[it];
       const double scale = b > 0.0 ? 1.0 / sqrt(b) : 0.0;
+      scale_l = scale;
+    }
+    if (t == 0 && lead) {
+      const uint32_t md = mode[it];
+      const double scale = scale_l;
       eig[it] = a;
       err[it] = c * scale;
       st->scale = scale;
-      scale_l = scale;
       const double cs = a * scale;
       st->cos_prev = cs;
       uint64_t le = st->last_exact;
@@ -799,7 +808,7 @@ __global__ __launch_bounds__(256) void k_tail(
     s = block256_sum(s, red);
     if (t == 0) {
       const double scale = s > 0.0 ? 1.0 / sqrt(s) : 0.0;
-      st->scale = scale;
+      if (lead) st->scale = scale;
       scale_l = scale;
     }
   } else if (op == kTailSQ) {
@@ -810,7 +819,7 @@ __global__ __launch_bounds__(256) void k_tail(
     }
     a = block256_sum(a, red);
     b = block256_sum(b, red);
-    if (t == 0) {
+    if (t == 0 && lead) {
       st->s = a;
       st->q = b;
     }
@@ -818,7 +827,7 @@ __global__ __launch_bounds__(256) void k_tail(
     double s = 0.0;
     for (uint32_t i = t; i < nb; i += 256) s += BP[i];
     s = block256_sum(s, red);
-    if (t == 0) st->sigma = sqrt(s);
+    if (t == 0 && lead) st->sigma = sqrt(s);
   } else if (t == 0) {
     scale_l = st->scale;
   }
@@ -827,7 +836,7 @@ __global__ __launch_bounds__(256) void k_tail(
     const double scale = scale_l;
     double ps = 0.0, pq = 0.0;
 #pragma unroll 8
-    for (uint64_t r = (uint64_t)t * 4; r < dpad; r += 1024) {
+    for (uint64_t r = ((uint64_t)blockIdx.x * 256 + t) * 4; r < dpad; r += (uint64_t)gridDim.x * 1024) {
       const float4 yv4 = *reinterpret_cast<const float4*>(y + r);
       float xv[4] = {(float)(scale * (double)yv4.x), (float)(scale * (double)yv4.y),
                      (float)(scale * (double)yv4.z), (float)(scale * (double)yv4.w)};
@@ -844,16 +853,16 @@ __global__ __launch_bounds__(256) void k_tail(
         }
       }
     }
-    if (z) {
+    if (z) {  // one block only (the host launches a single block when z is given)
       ps = block256_sum(ps, red);
       pq = block256_sum(pq, red);
-      if (t == 0) {
+      if (t == 0 && lead) {
         st->s = ps;
         st->q = pq;
       }
     }
   }
-  if (stamp && t == 0) ts[it] = wall_clock64();
+  if (stamp && t == 0 && lead) ts[it] = wall_clock64();
 }
 
 // ---------------------------------------------------------------------------
@@ -1304,7 +1313,12 @@ uint64_t normalize_blocks(uint64_t dpad) { return grid1(dpad, 1024); }
 uint64_t dots_blocks(uint64_t dpad) { return grid1(dpad, 256); }
 
 hipError_t launch_tail(const TailArgs& a, hipStream_t s) {
-  k_tail<<<1, 256, 0, s>>>(a.op, a.BP, a.nb, a.st, a.norm, a.y, a.x, a.xm, a.z, a.d, a.dpad, a.mode,
+  // normalisation without the s, q dots spreads over blocks of 1024 rows
+  // (each block repeats the reduction); the dots need the single block
+  unsigned g = 1;
+  if (a.norm && !a.z && (a.op == kTailFinish || a.op == kTailScale || a.op == kTailNorm))
+    g = (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, (a.dpad + 1023) / 1024), 64);
+  k_tail<<<g, 256, 0, s>>>(a.op, a.BP, a.nb, a.st, a.norm, a.y, a.x, a.xm, a.z, a.d, a.dpad, a.mode,
                            a.it, a.eig, a.err, a.ts, a.stamp, a.adapt_kernel, a.period, a.thr, a.step);
   return hipGetLastError();
 }
