@@ -213,28 +213,31 @@ __device__ __forceinline__ void store_vecf(float* p, const float (&v)[V]) {
     *reinterpret_cast<float4*>(p + k) = make_float4(v[k], v[k + 1], v[k + 2], v[k + 3]);
 }
 
-// Coarse table: kCoarse bucket ends e_m = floor((m + 1) n / kCoarse) - 1 and
-// their CDF values in LDS; a target is first located among the buckets in
-// LDS, then within its bucket (about n / kCoarse entries) in global memory.
-// Same result as the plain search: the smallest j with cdf[j] > t.
-constexpr uint32_t kCoarse = 1024;
+// Coarse table: K bucket ends e_m = floor((m + 1) n / K) - 1 and their CDF
+// values in LDS; a target is first located among the buckets in LDS, then
+// within its bucket (about n / K entries) in global memory.  Same result as
+// the plain search for any K: the smallest j with cdf[j] > t.
+constexpr uint32_t kCoarse = 1024;        // the batched sampler (256-thread blocks)
+constexpr uint32_t kCoarseSample = 8192;  // the single-vector sampler: 64 KB of LDS, buckets of n / 8192
 
+template <uint32_t K = kCoarse>
 __device__ __forceinline__ uint32_t bucket_end(uint32_t m, uint32_t n) {
-  return (uint32_t)(((uint64_t)(m + 1) * n) / kCoarse) - 1u;  // may wrap for empty leading buckets
+  return (uint32_t)(((uint64_t)(m + 1) * n) / K) - 1u;  // may wrap for empty leading buckets
 }
 
+template <uint32_t K = kCoarse>
 __device__ __forceinline__ uint32_t bcdf_search(const uint64_t* __restrict__ cdf, const uint64_t* cs,
                                                 uint32_t n, uint64_t t, uint64_t last_pos) {
   // smallest bucket m with cs[m] > t (cs[m] = cdf at its end; empty buckets
   // repeat the previous end's value)
-  uint32_t lo = 0, hi = kCoarse;
+  uint32_t lo = 0, hi = K;
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
     if (cs[mid] > t) hi = mid; else lo = mid + 1;
   }
-  if (lo >= kCoarse) return (uint32_t)last_pos;
-  uint32_t jlo = lo ? bucket_end(lo - 1, n) + 1u : 0u;
-  uint32_t jhi = bucket_end(lo, n) + 1u;
+  if (lo >= K) return (uint32_t)last_pos;
+  uint32_t jlo = lo ? bucket_end<K>(lo - 1, n) + 1u : 0u;
+  uint32_t jhi = bucket_end<K>(lo, n) + 1u;
   while (jlo < jhi) {
     const uint32_t mid = jlo + ((jhi - jlo) >> 1);
     if (cdf[mid] > t) jhi = mid; else jlo = mid + 1;
@@ -242,12 +245,13 @@ __device__ __forceinline__ uint32_t bcdf_search(const uint64_t* __restrict__ cdf
   return jlo < n ? jlo : (uint32_t)last_pos;
 }
 
-// Fill the coarse table cs[kCoarse] (LDS) by threads tid = 0, stride, ...;
-// an empty bucket before the first column ends at "-1": CDF value 0.
+// Fill the coarse table cs[K] (LDS) by threads tid = 0, stride, ...; an
+// empty bucket before the first column ends at "-1": CDF value 0.
+template <uint32_t K = kCoarse>
 __device__ __forceinline__ void coarse_fill(uint64_t* cs, const uint64_t* __restrict__ cdf, uint32_t n,
                                             uint32_t tid, uint32_t stride) {
-  for (uint32_t m = tid; m < kCoarse; m += stride)
-    cs[m] = ((uint64_t)(m + 1) * n) / kCoarse == 0 ? 0ull : cdf[bucket_end(m, n)];
+  for (uint32_t m = tid; m < K; m += stride)
+    cs[m] = ((uint64_t)(m + 1) * n) / K == 0 ? 0ull : cdf[bucket_end<K>(m, n)];
 }
 
 }  // namespace amm

@@ -297,7 +297,7 @@ __global__ __launch_bounds__(1024) void k_sample(const uint64_t* __restrict__ cd
                                                  uint32_t* __restrict__ seg_count) {
   __shared__ uint32_t keys[kSampleSeg];
   __shared__ uint32_t scan[2][1024];
-  __shared__ uint64_t cs[kCoarse];
+  __shared__ uint64_t cs[kCoarseSample];
   const uint32_t tid = threadIdx.x;
   const uint32_t g = blockIdx.x;
   const uint32_t it = it0 + blockIdx.y;
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(1024) void k_sample(const uint64_t* __restrict__ cd
   const uint64_t seed = st->seed;
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   const uint32_t trial = st->trial;
-  coarse_fill(cs, cdf, (uint32_t)n, tid, 1024);
+  coarse_fill<kCoarseSample>(cs, cdf, (uint32_t)n, tid, 1024);
   __syncthreads();
 
   // draw: one Philox call serves two consecutive slots
@@ -319,9 +319,9 @@ __global__ __launch_bounds__(1024) void k_sample(const uint64_t* __restrict__ cd
       const uint32_t k = b + l0;
       uint32_t r[4];
       philox4x32_10(k >> 1, it, trial, kStreamSample, k0, k1, r);
-      keys[l0] = bcdf_search(cdf, cs, (uint32_t)n, draw_target(r[0], r[1], total), last_pos);
+      keys[l0] = bcdf_search<kCoarseSample>(cdf, cs, (uint32_t)n, draw_target(r[0], r[1], total), last_pos);
       keys[l0 + 1] = (l0 + 1 < cnt)
-                         ? bcdf_search(cdf, cs, (uint32_t)n, draw_target(r[2], r[3], total), last_pos)
+                         ? bcdf_search<kCoarseSample>(cdf, cs, (uint32_t)n, draw_target(r[2], r[3], total), last_pos)
                          : 0xFFFFFFFFu;
     } else {
       keys[l0] = 0xFFFFFFFFu;
@@ -333,20 +333,44 @@ __global__ __launch_bounds__(1024) void k_sample(const uint64_t* __restrict__ cd
   // bitonic sort of the first N = next_pow2(cnt) keys (the rest are sentinels)
   uint32_t N = 1;
   while (N < cnt) N <<= 1;
-  for (uint32_t k = 2; k <= N; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = tid; i < N; i += 1024) {
-        const uint32_t ixj = i ^ j;
-        if (ixj > i) {
-          const uint32_t a = keys[i], bb = keys[ixj];
-          const bool up = (i & k) == 0;
-          if ((a > bb) == up) {
-            keys[i] = bb;
-            keys[ixj] = a;
+  if (N <= 1024) {
+    // one key per thread: partners closer than 64 are in the same wave
+    // (register shuffles), farther ones meet in LDS
+    uint32_t v = keys[tid];
+    for (uint32_t k = 2; k <= N; k <<= 1) {
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        uint32_t p;
+        if (j >= 64) {
+          __syncthreads();
+          keys[tid] = v;
+          __syncthreads();
+          p = keys[tid ^ j];
+        } else {
+          p = (uint32_t)__shfl_xor((int)v, (int)j);
+        }
+        const bool up = (tid & k) == 0, lower = (tid & j) == 0;
+        v = (lower == up) ? (v < p ? v : p) : (v < p ? p : v);
+      }
+    }
+    __syncthreads();
+    keys[tid] = v;
+    __syncthreads();
+  } else {
+    for (uint32_t k = 2; k <= N; k <<= 1) {
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        for (uint32_t i = tid; i < N; i += 1024) {
+          const uint32_t ixj = i ^ j;
+          if (ixj > i) {
+            const uint32_t a = keys[i], bb = keys[ixj];
+            const bool up = (i & k) == 0;
+            if ((a > bb) == up) {
+              keys[i] = bb;
+              keys[ixj] = a;
+            }
           }
         }
+        __syncthreads();
       }
-      __syncthreads();
     }
   }
 
