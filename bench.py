@@ -641,6 +641,11 @@ def main():
     sum_gbs = 10 * buf.numel() * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9
     del buf, out, s
     log(f"torch device copy: {copy_gbs:.0f} GB/s (read + write); torch sum: {sum_gbs:.0f} GB/s (read)")
+    # the library's plain streaming read of the same matrix: the read ceiling
+    read_ms = ctx.time_pass("read", iters=10)
+    read_gbs = d * n * 4 / (read_ms * 1e-3) / 1e9
+    log(f"streaming read of A: {read_ms * 1e3:.1f} us, {read_gbs:.0f} GB/s; dominant kernel at "
+        f"{achieved / read_gbs:.3f} of it")
     traffic = PMC_TRAFFIC_16384x65536.get(dom_name) if (d, n) == (16384, 65536) else None
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=dom_name,
@@ -649,7 +654,8 @@ def main():
                 kernels_us={k: round(v[1] * 1e3, 2) for k, v in kern.items()},
                 not_in_run_us=two_pass,
                 sampled_columns=ucols, torch_copy_GBs=round(copy_gbs, 1),
-                torch_sum_GBs=round(sum_gbs, 1))
+                torch_sum_GBs=round(sum_gbs, 1), read_ceiling_GBs=round(read_gbs, 1),
+                frac_of_read_ceiling=round(achieved / read_gbs, 4))
 
     cpu = None
     if not args.no_cpu and am["median_ms"] is not None:

@@ -233,8 +233,9 @@ class AmmContext:
     def time_pass(self, pass_: str, iters: int = 20) -> float:
         """Mean duration (ms) of one pass as a run issues it: "stats_z"
         (column statistics with z = A alpha, kernel and partial sum) or
-        "exact" (the exact product without its finish)."""
-        code = {"stats_z": 1, "exact": 2}[pass_]
+        "exact" (the exact product without its finish); or "read", a plain
+        streaming read of the matrix, the read ceiling for the others."""
+        code = {"stats_z": 1, "exact": 2, "read": 3}[pass_]
         t = np.zeros(1, np.float64)
         check(self._lib.amm_time_pass(self._h, code, iters, C.c_void_p(_ptr(t))))
         return float(t[0])

@@ -1112,10 +1112,26 @@ int amm_time_column_stats(amm_ctx* c, uint32_t iters, double* ms) {
 int amm_time_pass(amm_ctx* c, int32_t pass, uint32_t iters, double* ms) {
   if (!c || !ms) return fail(AMM_E_INVALID, "NULL argument");
   if (iters < 1 || iters > 100000) return fail(AMM_E_INVALID, "iters must be in [1, 100000]");
-  if (pass != AMM_PASS_STATS_Z && pass != AMM_PASS_EXACT) return fail(AMM_E_INVALID, "unknown pass");
+  if (pass != AMM_PASS_STATS_Z && pass != AMM_PASS_EXACT && pass != AMM_PASS_READ)
+    return fail(AMM_E_INVALID, "unknown pass");
   int rc;
   if ((rc = set_device(c))) return rc;
   drop_graph(c);
+  if (pass == AMM_PASS_READ) {
+    // the matrix's bytes up to the end of its last column's d rows
+    const uint64_t es = c->dtype == AMM_DTYPE_BF16 ? 2 : 4;
+    const uint64_t bytes = ((c->n - 1) * c->ld + c->d) * es / 16 * 16;
+    uint32_t* sink = reinterpret_cast<uint32_t*>(c->BP);
+    HIPCHK(amm::launch_read(c->A, bytes, sink, c->stream));  // warm-up
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (uint32_t i = 0; i < iters; ++i) HIPCHK(amm::launch_read(c->A, bytes, sink, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, c->ev0, c->ev1));
+    *ms = t / iters;
+    return AMM_OK;
+  }
   if ((rc = ensure_plans(c, c->list_c))) return rc;
   if (pass == AMM_PASS_STATS_Z) {
     if (!amm::colstats_fuses_z(c->dtype, (uint32_t)c->d))

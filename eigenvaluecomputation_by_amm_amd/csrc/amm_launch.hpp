@@ -145,6 +145,10 @@ hipError_t launch_begin(float* xm, float* ym, uint64_t dpad, DevState* st, uint3
 hipError_t launch_gather_dot(int dtype, const GatherDotArgs& a, hipStream_t s);
 hipError_t launch_gather_axpy(int dtype, const GatherAxpyArgs& a, hipStream_t s);
 hipError_t launch_finish(const FinishArgs& a, hipStream_t s);
+// plain streaming read of `bytes` (a multiple of 16 is read) from the
+// 16-byte aligned A; sink: one writable word
+hipError_t launch_read(const void* A, uint64_t bytes, uint32_t* sink, hipStream_t s);
+
 hipError_t launch_normalize(const float* y, float* x, uint32_t d, uint64_t dpad, const DevState* st,
                             const float* xm, const float* z, double* BP, hipStream_t s);
 hipError_t launch_sum_rows(const float* P2, uint32_t S, uint64_t dpad, uint32_t d, float* z,

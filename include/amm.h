@@ -162,8 +162,12 @@ int amm_time_column_stats(amm_ctx *ctx, uint32_t iters, double *ms);
  *   AMM_PASS_EXACT: the exact product y = A (A^T x) (the two cublasSgemv
  *     calls of powerMethod_by_exact.cuh:46-62) without its finish: one read
  *     of A for d <= 16384, else the two contraction kernels.
+ *   AMM_PASS_READ: no counterpart in the reference; a plain streaming read
+ *     of the matrix (every byte up to its last column's d-th row, 16-byte
+ *     non-temporal loads, 1 MiB per workgroup) and nothing else: the read
+ *     ceiling the passes above are measured against.
  * AMM_E_INVALID when the pass does not apply to the context's shape. */
-enum { AMM_PASS_STATS_Z = 1, AMM_PASS_EXACT = 2 };
+enum { AMM_PASS_STATS_Z = 1, AMM_PASS_EXACT = 2, AMM_PASS_READ = 3 };
 int amm_time_pass(amm_ctx *ctx, int32_t pass, uint32_t iters, double *ms);
 
 /* Fill the m x m FP32 device matrix A (leading dimension ld) with a

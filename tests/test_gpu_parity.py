@@ -326,8 +326,9 @@ def test_proposed_beyond_register_rows():
 
 
 def test_time_pass():
-    """amm_time_pass times the statistics+z pass and the exact product as a
-    run issues them, and refuses the fused statistics past 16384 FP32 rows."""
+    """amm_time_pass times the statistics+z pass, the exact product as a run
+    issues them and the plain read of A, and refuses the fused statistics
+    past 16384 FP32 rows."""
     banner("time_pass")
     AmmContext, _, AmmError = _amm()
     for d, fused in ((2000, True), (16392, False)):
@@ -335,6 +336,7 @@ def test_time_pass():
         t, _ = dev_matrix(A)
         with AmmContext(t) as ctx:
             assert ctx.time_pass("exact", iters=3) > 0
+            assert ctx.time_pass("read", iters=3) > 0   # the plain streaming read of A
             if fused:
                 assert ctx.time_pass("stats_z", iters=3) > 0
             else:
