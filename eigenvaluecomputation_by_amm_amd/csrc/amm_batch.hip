@@ -850,7 +850,15 @@ __global__ __launch_bounds__(256) void k_bfinish(const BProb* __restrict__ P, co
   if (prop) {
     sx = pr.bt[k].s;
     qx = pr.bt[k].q;
-    for (uint32_t ch = 0; ch < active; ++ch) corr += pr.P2s[(uint64_t)ch * kBT + k];
+    uint32_t ch = 0;
+    for (; ch + 8 <= active; ch += 8) {
+      double c8[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) c8[u] = pr.P2s[(uint64_t)(ch + u) * kBT + k];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) corr += c8[u];
+    }
+    for (; ch < active; ++ch) corr += pr.P2s[(uint64_t)ch * kBT + k];
   }
   double yx = 0.0, yy = 0.0, yw = 0.0;
   for (uint32_t sub = 0; sub < kBVecRows / 64; ++sub) {
@@ -861,7 +869,23 @@ __global__ __launch_bounds__(256) void k_bfinish(const BProb* __restrict__ P, co
       continue;
     }
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t ch = 0; ch < active; ++ch) {
+    // batches of 8 chunk partials, loads first (a loop with a bound per
+    // element gets one load in flight at a time); sums in chunk order
+    uint32_t ch = 0;
+    for (; ch + 8 <= active; ch += 8) {
+      float4 q[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        q[u] = *reinterpret_cast<const float4*>(pr.P2 + ((uint64_t)(ch + u) * kBT + k) * pr.p2pitch + row);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        a[0] += q[u].x;
+        a[1] += q[u].y;
+        a[2] += q[u].z;
+        a[3] += q[u].w;
+      }
+    }
+    for (; ch < active; ++ch) {
       const float4 q = *reinterpret_cast<const float4*>(pr.P2 + ((uint64_t)ch * kBT + k) * pr.p2pitch + row);
       a[0] += q.x;
       a[1] += q.y;

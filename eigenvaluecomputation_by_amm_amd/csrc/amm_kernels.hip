@@ -739,8 +739,23 @@ __global__ __launch_bounds__(256) void k_finish(
     const uint64_t r0 = (uint64_t)blockIdx.x * kFinishRows + l16 * 4;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     if (r0 < dpad) {
-#pragma unroll 32  // 512 partials after the one-pass exact product: all loads in flight
-      for (uint32_t s = g; s < S; s += 16) {
+      // batches of 16 partials with every load issued before the sums (a
+      // loop with a bound per element gets one load in flight at a time);
+      // the sums keep the order s = g, g + 16, ...
+      uint32_t s = g;
+      for (; s + 15 * 16 < S; s += 16 * 16) {
+        float4 p[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) p[u] = *reinterpret_cast<const float4*>(P2 + (uint64_t)(s + 16 * u) * dpad + r0);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          a0 += p[u].x;
+          a1 += p[u].y;
+          a2 += p[u].z;
+          a3 += p[u].w;
+        }
+      }
+      for (; s < S; s += 16) {
         const float4 p = *reinterpret_cast<const float4*>(P2 + (uint64_t)s * dpad + r0);
         a0 += p.x;
         a1 += p.y;
@@ -987,8 +1002,23 @@ __global__ __launch_bounds__(1024) void k_sum_rows(const float* __restrict__ P2,
   const uint64_t r0 = (uint64_t)blockIdx.x * 256 + lane * 4;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (r0 < dpad) {  // dpad is a multiple of 8, so r0 + 4 <= dpad
-#pragma unroll 4
-    for (uint32_t s = w; s < S; s += kSumRowsWaves) {
+    // batches of 16 partials per wave, loads first (the sums keep the order
+    // s = w, w + waves, ...)
+    uint32_t s = w;
+    for (; s + 15 * kSumRowsWaves < S; s += 16 * kSumRowsWaves) {
+      float4 q[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        q[u] = *reinterpret_cast<const float4*>(P2 + (uint64_t)(s + kSumRowsWaves * u) * dpad + r0);
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        acc[0] += (double)q[u].x;
+        acc[1] += (double)q[u].y;
+        acc[2] += (double)q[u].z;
+        acc[3] += (double)q[u].w;
+      }
+    }
+    for (; s < S; s += kSumRowsWaves) {
       const float4 q = *reinterpret_cast<const float4*>(P2 + (uint64_t)s * dpad + r0);
       acc[0] += (double)q.x;
       acc[1] += (double)q.y;
