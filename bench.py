@@ -253,7 +253,7 @@ def batch_config(args):
         m16.append(M.to(torch.bfloat16))
     torch.cuda.synchronize()
     iters = args.batch_iters
-    cfg = dict(kernel="previousAMM", vr_period=5)
+    cfg = dict(kernel="previousAMM", vr_period=args.batch_vr)
     probs = [dict(A=m16[i], c=max(1, n // 8), seed=77 + i) for i, (d, n) in enumerate(shapes)]
     with AmmBatch(probs, trials=16) as b:
         log("autotune")
@@ -497,6 +497,8 @@ def main():
     ap.add_argument("--seeds", type=int, default=100, help="seeds per cell of --config seeds")
     ap.add_argument("--count", type=int, default=256, help="matrices of --config batch")
     ap.add_argument("--batch-iters", type=int, default=150)
+    ap.add_argument("--batch-vr", type=int, default=3,
+                    help="variance-reduction period of --config batch (3: fastest of 2-5 measured)")
     ap.add_argument("--exact-iters", type=int, default=40)
     ap.add_argument("--stack", type=int, default=64, help="matrices of --config sigma")
     ap.add_argument("--sigma-reps", type=int, default=3)

@@ -851,12 +851,12 @@ __global__ __launch_bounds__(256) void k_bfinish(const BProb* __restrict__ P, co
     sx = pr.bt[k].s;
     qx = pr.bt[k].q;
     uint32_t ch = 0;
-    for (; ch + 8 <= active; ch += 8) {
-      double c8[8];
+    for (; ch + 4 <= active; ch += 4) {
+      double c4[4];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) c8[u] = pr.P2s[(uint64_t)(ch + u) * kBT + k];
+      for (int u = 0; u < 4; ++u) c4[u] = pr.P2s[(uint64_t)(ch + u) * kBT + k];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) corr += c8[u];
+      for (int u = 0; u < 4; ++u) corr += c4[u];
     }
     for (; ch < active; ++ch) corr += pr.P2s[(uint64_t)ch * kBT + k];
   }
@@ -869,16 +869,17 @@ __global__ __launch_bounds__(256) void k_bfinish(const BProb* __restrict__ P, co
       continue;
     }
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-    // batches of 8 chunk partials, loads first (a loop with a bound per
-    // element gets one load in flight at a time); sums in chunk order
+    // batches of 4 chunk partials, loads first (a loop with a bound per
+    // element gets one load in flight at a time; 4 keeps 8 waves per SIMD);
+    // sums in chunk order
     uint32_t ch = 0;
-    for (; ch + 8 <= active; ch += 8) {
-      float4 q[8];
+    for (; ch + 4 <= active; ch += 4) {
+      float4 q[4];
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
+      for (int u = 0; u < 4; ++u)
         q[u] = *reinterpret_cast<const float4*>(pr.P2 + ((uint64_t)(ch + u) * kBT + k) * pr.p2pitch + row);
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < 4; ++u) {
         a[0] += q[u].x;
         a[1] += q[u].y;
         a[2] += q[u].z;
