@@ -304,8 +304,9 @@ __global__ __launch_bounds__(256) void k_cdf_offsets(uint64_t* __restrict__ cdf,
 // Philox counter (k>>1, it, trial, kStreamSample), words 2(k&1), 2(k&1)+1;
 // t = floor(u * total / 2^64); column = min{j : cdf_j > t} (located in a
 // coarse table of the CDF in LDS first, then within its bucket).  One
-// workgroup per (segment, iteration) sorts its slots in LDS (bitonic),
-// merges duplicates and writes the unique columns in ascending order with
+// workgroup per (segment, iteration) sorts its slots (bitonic: register
+// shuffles between partners of one wave, LDS between waves), merges
+// duplicates and writes the unique columns in ascending order with
 // weight mult * total / (c * q_j), q_j = cdf_j - cdf_{j-1}.
 // ---------------------------------------------------------------------------
 
