@@ -1,7 +1,7 @@
 // amm_batch.hip -- kernels of the batched AMM power method (gfx950, wave64).
 //
 // Trial k of problem p is, sample for sample, the single-vector run with
-// Philox key seed_p and counter word 2 = trial0 + k (k_sample, k_x0 in
+// Philox key seed_p and counter word 2 = trial0 + k (k_sample, k_begin in
 // amm_kernels.hip): the same columns, multiplicities and weights; only the
 // summation order of the contractions differs.
 #include "amm_batch.hpp"
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void k_bbegin(const BProb* __restrict__ P, con
 
 // ---------------------------------------------------------------------------
 // x0 ~ N(0, I) per trial: Box-Muller on Philox counter (m, 0, trial, x0
-// stream) for rows 2m, 2m+1 (as k_x0).  Raw vector into Y; per-block sums of
+// stream) for rows 2m, 2m+1 (as k_begin).  Raw vector into Y; per-block sums of
 // squares into BP[., ., 1].
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bx0(const BProb* __restrict__ P, const uint32_t* off,

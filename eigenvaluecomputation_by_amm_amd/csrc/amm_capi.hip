@@ -475,8 +475,9 @@ int enq_norm(amm_ctx* c, uint32_t it, const float* zq, bool stamp) {
 }
 
 // x0 ~ N(0, I), normalised (curandGenerateNormal + normalize<512>)
+// x0's normalisation (the raw x0 and its block sums come from the run
+// prologue, k_begin; nothing in between touches y or BP)
 int enq_x0(amm_ctx* c, const float* zq) {
-  HIPCHK(amm::launch_x0(c->y, (uint32_t)c->d, c->dpad, c->st, c->BP, c->stream));
   const bool small = c->dpad <= kFuseNormRows;
   HIPCHK(amm::launch_tail(tail_args(c, amm::kTailScale, (uint32_t)amm::x0_blocks(c->dpad), 0, small,
                                     zq, false),
@@ -569,7 +570,8 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
   const bool adaptive = is_adaptive(p);
   const bool sanity = K != AMM_KERNEL_EXACT && p.sanity && p.c == c->n;
   const uint32_t step = (uint32_t)p.step;
-  HIPCHK(amm::launch_begin(c->xm, c->ym, c->dpad, c->st, c->mode, adaptive ? modes[0] : 0, c->stream));
+  HIPCHK(amm::launch_begin(c->xm, c->ym, c->dpad, c->st, c->mode, adaptive ? modes[0] : 0, c->y,
+                           (uint32_t)c->d, c->BP, c->stream));
   const bool need_plain = K == AMM_KERNEL_PREVIOUS || K == AMM_KERNEL_PROPOSED_V2;
   const bool need_cent = K == AMM_KERNEL_PROPOSED || K == AMM_KERNEL_PROPOSED_V2;
   // the exact power method needs no column statistics; the centred kernels

@@ -440,18 +440,37 @@ __global__ __launch_bounds__(1024) void k_sample(const uint64_t* __restrict__ cd
 }
 
 // ---------------------------------------------------------------------------
-// x0 ~ N(0, I) (curandGenerateNormal, powerMethod_by_exact.cuh:32): Box-Muller
-// on Philox counter (m, 0, trial, kStreamX0) for rows 2m, 2m+1.  Writes the
-// raw vector into y and per-block sums of squares into BP; k_tail then forms
-// 1/||y|| and x (normalize<512>, common.h:156-183).
+// Run prologue: zero x_memo / y_memo (zero_clear, device_memory.h:345-355),
+// reset the scalar state, stamp the start time and draw x0 in the same
+// launch.  Thread m owns rows 2m, 2m + 1.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_x0(float* __restrict__ y, uint32_t d, uint64_t dpad,
-                                            const DevState* st, double* __restrict__ BP) {
+__global__ __launch_bounds__(256) void k_begin(float* __restrict__ xm, float* __restrict__ ym,
+                                               uint64_t dpad, DevState* st,
+                                               uint32_t* __restrict__ mode, uint32_t mode0,
+                                               float* __restrict__ y, uint32_t d,
+                                               double* __restrict__ BP) {
   __shared__ double red[4];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->t_start = wall_clock64();
+    st->cos_prev = 0.0;
+    st->last_exact = 0;
+    st->s = 0.0;
+    st->q = 0.0;
+    st->status = 0;
+    if (mode0) mode[0] = mode0;
+  }
+  // x0 ~ N(0, I) (curandGenerateNormal, powerMethod_by_exact.cuh:32):
+  // Box-Muller on Philox counter (m, 0, trial, kStreamX0) for rows 2m,
+  // 2m+1; the raw vector goes to y and per-block sums of squares to BP,
+  // from which k_tail forms 1/||y|| and x (normalize<512>, common.h:156-183).
   const uint64_t m = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   const uint64_t seed = st->seed;
   double acc = 0.0;
-  if (2 * m < dpad) {
+  if (2 * m < dpad) {  // dpad is even: rows 2m and 2m + 1 both exist
+    xm[2 * m] = 0.0f;
+    xm[2 * m + 1] = 0.0f;
+    ym[2 * m] = 0.0f;
+    ym[2 * m + 1] = 0.0f;
     float g0 = 0.0f, g1 = 0.0f;
     if (2 * m < d) {
       uint32_t r[4];
@@ -470,29 +489,6 @@ __global__ __launch_bounds__(256) void k_x0(float* __restrict__ y, uint32_t d, u
   }
   acc = block256_sum(acc, red);
   if (threadIdx.x == 0) BP[blockIdx.x] = acc;
-}
-
-// ---------------------------------------------------------------------------
-// Run prologue: zero x_memo / y_memo (zero_clear, device_memory.h:345-355),
-// reset the scalar state and stamp the start time.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_begin(float* __restrict__ xm, float* __restrict__ ym,
-                                               uint64_t dpad, DevState* st,
-                                               uint32_t* __restrict__ mode, uint32_t mode0) {
-  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->t_start = wall_clock64();
-    st->cos_prev = 0.0;
-    st->last_exact = 0;
-    st->s = 0.0;
-    st->q = 0.0;
-    st->status = 0;
-    if (mode0) mode[0] = mode0;
-  }
-  if (i < dpad) {
-    xm[i] = 0.0f;
-    ym[i] = 0.0f;
-  }
 }
 
 __device__ __forceinline__ bool gate_open(const uint32_t* mode, uint32_t it, uint32_t mask) {
@@ -1312,15 +1308,10 @@ hipError_t launch_sample(const uint64_t* cdf, uint64_t n, const DevState* st,
   return hipGetLastError();
 }
 
-hipError_t launch_x0(float* y, uint32_t d, uint64_t dpad, const DevState* st, double* BP,
-                     hipStream_t s) {
-  k_x0<<<(unsigned)x0_blocks(dpad), 256, 0, s>>>(y, d, dpad, st, BP);
-  return hipGetLastError();
-}
 
 hipError_t launch_begin(float* xm, float* ym, uint64_t dpad, DevState* st, uint32_t* mode,
-                        uint32_t mode0, hipStream_t s) {
-  k_begin<<<grid1(dpad, 256), 256, 0, s>>>(xm, ym, dpad, st, mode, mode0);
+                        uint32_t mode0, float* y, uint32_t d, double* BP, hipStream_t s) {
+  k_begin<<<(unsigned)x0_blocks(dpad), 256, 0, s>>>(xm, ym, dpad, st, mode, mode0, y, d, BP);
   return hipGetLastError();
 }
 

@@ -138,10 +138,10 @@ hipError_t launch_sample(const uint64_t* cdf, uint64_t n, const DevState* st, in
                          uint32_t* seg_count, hipStream_t s);
 hipError_t launch_dots(const float* x, const float* xm, const float* z, uint64_t dpad, double* BP,
                        hipStream_t s);
-hipError_t launch_x0(float* y, uint32_t d, uint64_t dpad, const DevState* st, double* BP,
-                     hipStream_t s);
+// run prologue with x0: raw x0 into y, its per-block sums of squares into
+// BP (x0_blocks(dpad) of them, for k_tail's kTailScale)
 hipError_t launch_begin(float* xm, float* ym, uint64_t dpad, DevState* st, uint32_t* mode,
-                        uint32_t mode0, hipStream_t s);
+                        uint32_t mode0, float* y, uint32_t d, double* BP, hipStream_t s);
 hipError_t launch_gather_dot(int dtype, const GatherDotArgs& a, hipStream_t s);
 hipError_t launch_gather_axpy(int dtype, const GatherAxpyArgs& a, hipStream_t s);
 hipError_t launch_finish(const FinishArgs& a, hipStream_t s);
