@@ -39,36 +39,13 @@ __device__ __forceinline__ bool is_amm(uint32_t md) {
 
 // ---------------------------------------------------------------------------
 // Run prologue: x_memo = y_memo = 0 (zero_clear, device_memory.h:345-355),
-// per-trial state, start stamp.  Items: nvblk row blocks per problem.
+// per-trial state, start stamp, and x0 ~ N(0, I) per trial in the same
+// launch: Box-Muller on Philox counter (m, 0, trial, x0 stream) for rows
+// 2m, 2m+1 (as k_begin); raw vector into Y, per-block sums of squares into
+// BP[., ., 1].  Items: nvblk row blocks per problem.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bbegin(const BProb* __restrict__ P, const uint32_t* off,
                                                 BRun r) {
-  const uint32_t p = find_problem(off, r.np, blockIdx.x);
-  const BProb& pr = P[p];
-  const uint32_t blk = blockIdx.x - off[p];
-  const uint32_t k = threadIdx.x >> 4;
-  for (uint32_t sub = 0; sub < kBVecRows / 64; ++sub) {
-    const uint64_t row = (uint64_t)blk * kBVecRows + sub * 64 + (threadIdx.x & 15) * 4;
-    const uint64_t o = (uint64_t)k * pr.dstage + row;
-    *reinterpret_cast<float4*>(pr.XM + o) = make_float4(0.f, 0.f, 0.f, 0.f);
-    *reinterpret_cast<float4*>(pr.YM + o) = make_float4(0.f, 0.f, 0.f, 0.f);
-  }
-  if (blk == 0 && threadIdx.x < kBT) {
-    BTrial t{};
-    t.mode = r.modes_static[0];
-    pr.bt[threadIdx.x] = t;
-    if (threadIdx.x == 0) pr.st->status = 0;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) *r.t_start = wall_clock64();
-}
-
-// ---------------------------------------------------------------------------
-// x0 ~ N(0, I) per trial: Box-Muller on Philox counter (m, 0, trial, x0
-// stream) for rows 2m, 2m+1 (as k_begin).  Raw vector into Y; per-block sums of
-// squares into BP[., ., 1].
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bx0(const BProb* __restrict__ P, const uint32_t* off,
-                                             BRun r) {
   const uint32_t p = find_problem(off, r.np, blockIdx.x);
   const BProb& pr = P[p];
   const uint32_t blk = blockIdx.x - off[p];
@@ -77,6 +54,9 @@ __global__ __launch_bounds__(256) void k_bx0(const BProb* __restrict__ P, const 
   double yy = 0.0;
   for (uint32_t sub = 0; sub < kBVecRows / 64; ++sub) {
     const uint32_t row = blk * kBVecRows + sub * 64 + (threadIdx.x & 15) * 4;
+    const uint64_t o = (uint64_t)k * pr.dstage + row;
+    *reinterpret_cast<float4*>(pr.XM + o) = make_float4(0.f, 0.f, 0.f, 0.f);
+    *reinterpret_cast<float4*>(pr.YM + o) = make_float4(0.f, 0.f, 0.f, 0.f);
     float g[4] = {0.f, 0.f, 0.f, 0.f};
     if (k < r.ntr) {
 #pragma unroll
@@ -94,7 +74,7 @@ __global__ __launch_bounds__(256) void k_bx0(const BProb* __restrict__ P, const 
         }
       }
     }
-    *reinterpret_cast<float4*>(pr.Y + (uint64_t)k * pr.dstage + row) = make_float4(g[0], g[1], g[2], g[3]);
+    *reinterpret_cast<float4*>(pr.Y + o) = make_float4(g[0], g[1], g[2], g[3]);
 #pragma unroll
     for (int e = 0; e < 4; ++e) yy += (double)g[e] * (double)g[e];
   }
@@ -105,6 +85,13 @@ __global__ __launch_bounds__(256) void k_bx0(const BProb* __restrict__ P, const 
     b[1] = yy;
     b[2] = 0.0;
   }
+  if (blk == 0 && threadIdx.x < kBT) {
+    BTrial t{};
+    t.mode = r.modes_static[0];
+    pr.bt[threadIdx.x] = t;
+    if (threadIdx.x == 0) pr.st->status = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *r.t_start = wall_clock64();
 }
 
 // ---------------------------------------------------------------------------
@@ -1044,11 +1031,6 @@ hipError_t launch_bzv(const BSetup* S, BGroup g, uint32_t np, hipStream_t s) {
 
 hipError_t launch_bbegin(const BProb* P, BGroup g, BRun r, hipStream_t s) {
   if (g.total) k_bbegin<<<g.total, 256, 0, s>>>(P, g.off, r);
-  return hipGetLastError();
-}
-
-hipError_t launch_bx0(const BProb* P, BGroup g, BRun r, hipStream_t s) {
-  if (g.total) k_bx0<<<g.total, 256, 0, s>>>(P, g.off, r);
   return hipGetLastError();
 }
 

@@ -168,7 +168,6 @@ hipError_t launch_bcdf(const BSetup* S, BGroup g, uint32_t np, int which, hipStr
 hipError_t launch_bzv(const BSetup* S, BGroup g, uint32_t np, hipStream_t s);
 
 hipError_t launch_bbegin(const BProb* P, BGroup g, BRun r, hipStream_t s);
-hipError_t launch_bx0(const BProb* P, BGroup g, BRun r, hipStream_t s);
 hipError_t launch_btail(const BProb* P, BRun r, uint32_t op, uint32_t it, hipStream_t s);
 hipError_t launch_bnorm(const BProb* P, BGroup g, BRun r, hipStream_t s);
 hipError_t launch_bsample(const BProb* P, BGroup g, BRun r, uint32_t it, hipStream_t s);

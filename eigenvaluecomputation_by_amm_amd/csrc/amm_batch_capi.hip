@@ -443,7 +443,7 @@ int enqueue_run(amm_batch* b, const amm_params& p, bool setup) {
   BHIPCHK(amm::launch_bbegin(b->dP, b->g_vec, r, b->stream));
   if (setup)
     if ((rc = enq_setup(b, K))) return rc;
-  BHIPCHK(amm::launch_bx0(b->dP, b->g_vec, r, b->stream));
+  // x0 and its block sums came with the prologue (the setup leaves Y and BP alone)
   BHIPCHK(amm::launch_btail(b->dP, r, amm::kBTailScale, 0, b->stream));
   BHIPCHK(amm::launch_bnorm(b->dP, b->g_vec, r, b->stream));
   for (uint32_t it = 0; it < (uint32_t)p.step; ++it)
