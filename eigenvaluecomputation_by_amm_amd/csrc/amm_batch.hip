@@ -823,44 +823,45 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
 // exact steps with memo snapshot [x, y] -> [x_memo, y_memo].
 // Items: nvblk per problem; thread (k, 4 rows).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bfinish(const BProb* __restrict__ P, const uint32_t* off,
-                                                 BRun r) {
+// Four wave groups of 256 threads split the chunk partials into four
+// contiguous ranges (more loads in flight where a problem has few row
+// blocks); group 0 adds the ranges in group order through LDS and finishes.
+constexpr uint32_t kFinGroups = 4;
+
+__global__ __launch_bounds__(256 * kFinGroups) void k_bfinish(const BProb* __restrict__ P, const uint32_t* off,
+                                                              BRun r) {
+  __shared__ double red[kFinGroups - 1][256][5];
   const uint32_t p = find_problem(off, r.np, blockIdx.x);
   const BProb& pr = P[p];
   const uint32_t blk = blockIdx.x - off[p];
-  const uint32_t k = threadIdx.x >> 4;
+  const uint32_t grp = threadIdx.x >> 8, lt = threadIdx.x & 255;
+  const uint32_t k = lt >> 4;
   const uint32_t md = k < r.ntr ? pr.bt[k].mode : kModeExactNoMemo;
   const bool memo_base = is_amm(md);
   const bool prop = md == kModeAmmProposed;
   const uint32_t active = union_plan(pr, pr.tpre[pr.ntiles]).active;
-  double sx = 0.0, qx = 0.0, corr = 0.0;
-  if (prop) {
-    sx = pr.bt[k].s;
-    qx = pr.bt[k].q;
-    uint32_t ch = 0;
-    for (; ch + 4 <= active; ch += 4) {
-      double c4[4];
+  const uint32_t c0 = (uint32_t)(((uint64_t)active * grp) / kFinGroups);
+  const uint32_t c1 = (uint32_t)(((uint64_t)active * (grp + 1)) / kFinGroups);
+  const uint32_t row = blk * kBVecRows + (lt & 15) * 4;
+  static_assert(kBVecRows == 64, "one 64-row slab per block");
+  double corr = 0.0;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  if (k < r.ntr) {
+    if (prop) {
+      uint32_t ch = c0;
+      for (; ch + 4 <= c1; ch += 4) {
+        double c4[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) c4[u] = pr.P2s[(uint64_t)(ch + u) * kBT + k];
+        for (int u = 0; u < 4; ++u) c4[u] = pr.P2s[(uint64_t)(ch + u) * kBT + k];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) corr += c4[u];
+        for (int u = 0; u < 4; ++u) corr += c4[u];
+      }
+      for (; ch < c1; ++ch) corr += pr.P2s[(uint64_t)ch * kBT + k];
     }
-    for (; ch < active; ++ch) corr += pr.P2s[(uint64_t)ch * kBT + k];
-  }
-  double yx = 0.0, yy = 0.0, yw = 0.0;
-  for (uint32_t sub = 0; sub < kBVecRows / 64; ++sub) {
-    const uint32_t row = blk * kBVecRows + sub * 64 + (threadIdx.x & 15) * 4;
-    const uint64_t o = (uint64_t)k * pr.dstage + row;
-    if (k >= r.ntr) {  // unused trial slot: zeros, no reads
-      *reinterpret_cast<float4*>(pr.Y + o) = make_float4(0.f, 0.f, 0.f, 0.f);
-      continue;
-    }
-    double a[4] = {0.0, 0.0, 0.0, 0.0};
     // batches of 4 chunk partials, loads first (a loop with a bound per
-    // element gets one load in flight at a time; 4 keeps 8 waves per SIMD);
-    // sums in chunk order
-    uint32_t ch = 0;
-    for (; ch + 4 <= active; ch += 4) {
+    // element gets one load in flight at a time); sums in chunk order
+    uint32_t ch = c0;
+    for (; ch + 4 <= c1; ch += 4) {
       float4 q[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u)
@@ -873,13 +874,42 @@ __global__ __launch_bounds__(256) void k_bfinish(const BProb* __restrict__ P, co
         a[3] += q[u].w;
       }
     }
-    for (; ch < active; ++ch) {
+    for (; ch < c1; ++ch) {
       const float4 q = *reinterpret_cast<const float4*>(pr.P2 + ((uint64_t)ch * kBT + k) * pr.p2pitch + row);
       a[0] += q.x;
       a[1] += q.y;
       a[2] += q.z;
       a[3] += q.w;
     }
+  }
+  if (grp > 0) {
+    double* s = red[grp - 1][lt];
+    s[0] = a[0];
+    s[1] = a[1];
+    s[2] = a[2];
+    s[3] = a[3];
+    s[4] = corr;
+  }
+  __syncthreads();
+  if (grp > 0) return;
+  for (uint32_t g = 0; g + 1 < kFinGroups; ++g) {
+    const double* s = red[g][lt];
+    a[0] += s[0];
+    a[1] += s[1];
+    a[2] += s[2];
+    a[3] += s[3];
+    corr += s[4];
+  }
+  double sx = 0.0, qx = 0.0;
+  if (prop) {
+    sx = pr.bt[k].s;
+    qx = pr.bt[k].q;
+  }
+  double yx = 0.0, yy = 0.0, yw = 0.0;
+  const uint64_t o = (uint64_t)k * pr.dstage + row;
+  if (k >= r.ntr) {  // unused trial slot: zeros, no reads
+    *reinterpret_cast<float4*>(pr.Y + o) = make_float4(0.f, 0.f, 0.f, 0.f);
+  } else {
     float4 ym4 = make_float4(0.f, 0.f, 0.f, 0.f);
     if (memo_base) ym4 = *reinterpret_cast<const float4*>(pr.YM + o);
     const float ymv[4] = {ym4.x, ym4.y, ym4.z, ym4.w};
@@ -1124,7 +1154,7 @@ hipError_t launch_bsigma(const BProb* P, BRun r, double* sigma, float* const* u,
 }
 
 hipError_t launch_bfinish(const BProb* P, BGroup g, BRun r, hipStream_t s) {
-  if (g.total) k_bfinish<<<g.total, 256, 0, s>>>(P, g.off, r);
+  if (g.total) k_bfinish<<<g.total, 256 * kFinGroups, 0, s>>>(P, g.off, r);
   return hipGetLastError();
 }
 
