@@ -34,9 +34,9 @@ TOL = 1e-3
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch on the 16384 x 65536 FP32 workload from two rocprofv3
-# PMC passes (FETCH_SIZE, then WRITE_SIZE, over
-# `python3 -m eigenvaluecomputation_by_amm_amd.kernel_probe`), each scaled by
-# what the counter reads for a 1 GiB device copy; used for roofline.traffic.
+# PMC passes (FETCH_SIZE, then WRITE_SIZE, over `python3 bench.py --config
+# probe`), each scaled by what the counter reads for a 1 GiB device copy;
+# used for roofline.traffic.
 PMC_TRAFFIC_16384x65536 = {
     "k_gram<float,16> (exact product A A^T x, one read of A)": 4.3291e9,
     "k_colstats<float> (column statistics)": 4.3012e9,
@@ -57,8 +57,10 @@ def gram_L(d: int) -> int:
     return L
 
 
-# headline AMM configuration: the fastest settled time of the kernel x c x vr sweep (--sweep)
-HEADLINE = dict(kernel="previousAMM", c=512, vr_period=4)
+# headline AMM configuration: previousAMM at c = 1024, the smallest sample
+# count of the stated c set {1024, 4096, 16384}, with an exact step every 4
+# iterations (the fastest period of the --sweep table)
+HEADLINE = dict(kernel="previousAMM", c=1024, vr_period=4)
 
 
 def log(*a):
@@ -476,6 +478,43 @@ def seeds_config(args, ctx, RunParams, lam, iters):
     print(json.dumps(line), flush=True)
 
 
+def probe_config(args):
+    """Each hot kernel a few times on the 16384 x 65536 heavy-tailed
+    workload, for hardware-counter profiling (`rocprofv3 --pmc ... --
+    python3 bench.py --config probe`).  Launches, in order: a plain device
+    copy of 1 GiB (read 1 GiB, write 1 GiB: the scale for FETCH_SIZE /
+    WRITE_SIZE), the column-statistics pass, the statistics pass with
+    z = A alpha, the one-pass exact product, the two-pass exact product (both
+    contractions), one sampled product of the headline configuration and
+    one batched 16-trial product in BF16 with every column present (both
+    MFMA contractions); each `--probe-reps` times."""
+    import torch
+    from eigenvaluecomputation_by_amm_amd import AmmBatch, AmmContext, RunParams
+    from eigenvaluecomputation_by_amm_amd.workloads import heavy_tailed
+    reps = args.probe_reps
+    buf = torch.empty(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+    out = torch.empty_like(buf)
+    for _ in range(reps):
+        out.copy_(buf)
+    torch.cuda.synchronize()
+    del buf, out
+    A = heavy_tailed(args.d, args.n)
+    with AmmContext(A) as ctx:
+        ctx.time_column_stats(iters=reps)
+        ctx.time_pass("stats_z", iters=reps)
+        ctx.time_pass("exact", iters=reps)
+        ctx.time_product("exact", 0, iters=reps)
+        ctx.time_product(HEADLINE["kernel"], HEADLINE["c"], seed=1, iters=reps)
+    torch.cuda.synchronize()
+    Ab = A.to(torch.bfloat16)
+    del A
+    with AmmBatch([dict(A=Ab, c=HEADLINE["c"], seed=1)], trials=16) as b:
+        b.power_method(RunParams(kernel="exact", step=reps))
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "done", "reps": reps, "workload": f"{args.d}x{args.n} heavy-tailed FP32"}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -490,7 +529,9 @@ def main():
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--kernel-iters", type=int, default=200)
     ap.add_argument("--batch-reps", type=int, default=3, help="batched runs (16 seeds each)")
-    ap.add_argument("--config", choices=["headline", "batch", "bf16", "sigma", "seeds"], default="headline",
+    ap.add_argument("--probe-reps", type=int, default=3, help="launches per kernel of --config probe")
+    ap.add_argument("--config", choices=["headline", "batch", "bf16", "sigma", "seeds", "probe"],
+                    default="headline",
                     help="batch: many uneven matrices x 16 seeds; bf16: image-like BF16 workload; "
                          "sigma: weight stack; seeds: distribution over --seeds seeds of every "
                          "kernel x c on the headline matrix (separate JSON lines)")
@@ -517,6 +558,9 @@ def main():
         return
     if args.config == "sigma":
         sigma_config(args)
+        return
+    if args.config == "probe":
+        probe_config(args)
         return
 
     import torch

@@ -19,6 +19,7 @@ AMM_KERNEL_PROPOSED = 2
 AMM_KERNEL_PROPOSED_V2 = 3
 AMM_FLAG_NO_GRAPH = 1
 AMM_FLAG_KEEP_STATS = 2
+AMM_FLAG_NO_VR = 4
 
 KERNEL_NAMES = {
     "exact": AMM_KERNEL_EXACT,

@@ -41,6 +41,9 @@ class RunParams:
     sanity: bool = False
     use_graph: bool = True
     keep_stats: bool = False
+    # the reference's variance_reduction = false (host_memory.h:108-120): no
+    # variance-reduction exact steps; proposedAMM_v2 keeps its transition
+    no_vr: bool = False
 
     def to_c(self) -> _lib.amm_params:
         if self.kernel not in _lib.KERNEL_NAMES:
@@ -50,6 +53,8 @@ class RunParams:
             flags |= _lib.AMM_FLAG_NO_GRAPH
         if self.keep_stats:
             flags |= _lib.AMM_FLAG_KEEP_STATS
+        if self.no_vr:
+            flags |= _lib.AMM_FLAG_NO_VR
         return _lib.amm_params(
             kernel=_lib.KERNEL_NAMES[self.kernel], c=int(self.c), step=int(self.step),
             vr_period=int(self.vr_period), vr_adaptive_threshold=float(self.vr_adaptive_threshold),

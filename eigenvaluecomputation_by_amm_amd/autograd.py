@@ -17,32 +17,56 @@ row-major (rows, cols) array is exactly dA = g * u_A v_A^T column-major.
 """
 from __future__ import annotations
 
+import weakref
 from collections import OrderedDict
 
 import torch
 
 from .api import AmmBatch, AmmContext, RunParams, rank1_update, rank1_update_batched
 
+# Contexts and batches are cached per caller tensor (pointer, shape, layout,
+# dtype, launch parameters, stream): they hold only the matrix's address and
+# every run re-reads the matrix.  The cache keeps a weak reference to the
+# caller's tensor and no strong one: when that tensor is collected its
+# entry is closed, so a cached entry never keeps a weight stack alive.  At
+# most _CACHE_MAX entries of each kind are kept (least recently used first
+# out).  A matrix whose layout needs a padded copy is not cached: the copy
+# lives for one forward call, with a context (or batch) of its own.
 _CACHE_MAX = 4
-_contexts: "OrderedDict[tuple, AmmContext]" = OrderedDict()
+_contexts: "OrderedDict[tuple, tuple]" = OrderedDict()
 
 
-def _context(Wc: torch.Tensor, cols: int) -> AmmContext:
-    """Contexts are cached per (pointer, shape, dtype, stream): a context
-    holds only the matrix's address, and every run re-reads the matrix."""
+def _evict(cache, key, ref) -> None:
+    ent = cache.get(key)
+    if ent is not None and ent[0] is ref:
+        del cache[key]
+        ent[1].close()
+
+
+def _cached(cache, key, owner: torch.Tensor, make):
+    """The cached object for `owner` under `key`, made by `make()` on a miss."""
+    ent = cache.get(key)
+    if ent is not None and ent[0]() is not None:
+        cache.move_to_end(key)
+        return ent[1]
+    if ent is not None:
+        del cache[key]
+        ent[1].close()
+    obj = make()
+    obj._keep = None  # the caller's tensor keeps the matrix alive, not the cache
+    ref = weakref.ref(owner)
+    cache[key] = (ref, obj)
+    weakref.finalize(owner, _evict, cache, key, ref)
+    while len(cache) > _CACHE_MAX:
+        _, (_, old) = cache.popitem(last=False)
+        old.close()
+    return obj
+
+
+def _context(owner: torch.Tensor, Wc: torch.Tensor, cols: int) -> AmmContext:
     stream = torch.cuda.current_stream(Wc.device).cuda_stream
     key = (Wc.data_ptr(), tuple(Wc.shape), Wc.stride(0), cols, Wc.dtype, Wc.device.index, stream)
-    ctx = _contexts.get(key)
-    if ctx is None:
-        ctx = AmmContext(Wc, d=cols)
-        _contexts[key] = ctx
-        while len(_contexts) > _CACHE_MAX:
-            _, old = _contexts.popitem(last=False)
-            old.close()
-    else:
-        _contexts.move_to_end(key)
-        ctx._keep = Wc
-    return ctx
+    return _cached(_contexts, key, owner, lambda: AmmContext(Wc, d=cols))
 
 
 def _as_columns(W: torch.Tensor) -> tuple[torch.Tensor, int]:
@@ -70,9 +94,13 @@ class SigmaMax(torch.autograd.Function):
             raise ValueError("W must be a 2-D device tensor")
         if W.dtype not in (torch.float32, torch.bfloat16):
             raise ValueError("W must be float32 or bfloat16")
-        Wc, cols = _as_columns(W.detach())
-        amm = _context(Wc, cols)
-        sigma, u, v = amm.sigma_max(params, device=W.device)
+        Wd = W.detach()
+        Wc, cols = _as_columns(Wd)
+        if Wc.data_ptr() == Wd.data_ptr():
+            sigma, u, v = _context(W, Wc, cols).sigma_max(params, device=W.device)
+        else:  # padded copy: a context for this call only
+            with AmmContext(Wc, d=cols) as amm:
+                sigma, u, v = amm.sigma_max(params, device=W.device)
         ctx.save_for_backward(u, v)
         ctx.shape = tuple(W.shape)
         ctx.dtype = W.dtype
@@ -95,30 +123,25 @@ def sigma_max(W: torch.Tensor, params: RunParams | None = None) -> torch.Tensor:
     return SigmaMax.apply(W, params)
 
 
-_batches: "OrderedDict[tuple, AmmBatch]" = OrderedDict()
+_batches: "OrderedDict[tuple, tuple]" = OrderedDict()
 
 
-def _batch(mats, cols: int, params: RunParams, tune: bool = False) -> AmmBatch:
-    """Batched contexts cached per (pointers, shapes, dtype, c, seed, stream);
-    with `tune`, a new one first tunes its launch shapes on runs of
-    `params` (once, when it is created)."""
+def _make_batch(mats, cols: int, params: RunParams, tune: bool) -> AmmBatch:
+    probs = [dict(A=m, d=cols, c=int(params.c), seed=int(params.seed) + i) for i, m in enumerate(mats)]
+    b = AmmBatch(probs, trials=1)
+    if tune:
+        b.autotune_run(params, reps=1)
+    return b
+
+
+def _batch(owner: torch.Tensor, mats, cols: int, params: RunParams, tune: bool = False) -> AmmBatch:
+    """Batched contexts cached per (caller stack, pointers, shapes, dtype,
+    c, seed, stream); with `tune`, a new one first tunes its launch shapes
+    on runs of `params` (once, when it is created)."""
     stream = torch.cuda.current_stream(mats[0].device).cuda_stream
     key = (tuple(m.data_ptr() for m in mats), tuple(mats[0].shape), mats[0].stride(0), cols,
            mats[0].dtype, int(params.c), int(params.seed), stream)
-    b = _batches.get(key)
-    if b is None:
-        probs = [dict(A=m, d=cols, c=int(params.c), seed=int(params.seed) + i) for i, m in enumerate(mats)]
-        b = AmmBatch(probs, trials=1)
-        if tune:
-            b.autotune_run(params, reps=1)
-        _batches[key] = b
-        while len(_batches) > _CACHE_MAX:
-            _, old = _batches.popitem(last=False)
-            old.close()
-    else:
-        _batches.move_to_end(key)
-        b._keep = list(mats)
-    return b
+    return _cached(_batches, key, owner, lambda: _make_batch(mats, cols, params, tune))
 
 
 class SigmaMaxStack(torch.autograd.Function):
@@ -136,15 +159,16 @@ class SigmaMaxStack(torch.autograd.Function):
         B, rows, cols = W.shape
         Wd = W.detach()
         width = 4 if W.dtype == torch.float32 else 8
-        if (Wd.stride(2) == 1 and Wd.stride(1) >= cols and Wd.stride(1) % width == 0
-                and Wd.stride(0) % width == 0 and Wd.data_ptr() % 16 == 0):
-            mats = Wd.unbind(0)  # every matrix already qualifies as is
-        else:
-            mats = [_as_columns(Wd[k])[0] for k in range(B)]
-        b = _batch(list(mats), cols, params, tune)
         U = torch.empty(B, cols, dtype=torch.float32, device=W.device)
         V = torch.empty(B, rows, dtype=torch.float32, device=W.device)
-        sig, _, _ = b.sigma_max(params, out_u=U, out_v=V)
+        if (Wd.stride(2) == 1 and Wd.stride(1) >= cols and Wd.stride(1) % width == 0
+                and Wd.stride(0) % width == 0 and Wd.data_ptr() % 16 == 0):
+            # every matrix already qualifies as is
+            sig, _, _ = _batch(W, list(Wd.unbind(0)), cols, params, tune).sigma_max(params, out_u=U, out_v=V)
+        else:  # padded copies: a batch for this call only
+            mats = [_as_columns(Wd[k])[0] for k in range(B)]
+            with _make_batch(mats, cols, params, tune) as b:
+                sig, _, _ = b.sigma_max(params, out_u=U, out_v=V)
         ctx.save_for_backward(U, V)
         ctx.shape = (B, rows, cols)
         ctx.dtype = W.dtype

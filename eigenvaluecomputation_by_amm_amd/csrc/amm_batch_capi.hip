@@ -324,9 +324,10 @@ std::vector<uint32_t> static_modes(const amm_params& p) {
   std::vector<uint32_t> m(p.step);
   uint64_t last_exact = 0;
   const uint64_t P = p.vr_period;
+  const bool vr = !(p.flags & AMM_FLAG_NO_VR);
   for (uint64_t i = 0; i < p.step; ++i) {
     uint32_t md;
-    const bool due = P > 0 && (i + 1) % P == 0;
+    const bool due = vr && P > 0 && (i + 1) % P == 0;
     if (p.kernel == AMM_KERNEL_EXACT) {
       md = amm::kModeExactNoMemo;
     } else if (p.kernel == AMM_KERNEL_PREVIOUS) {
@@ -345,7 +346,7 @@ std::vector<uint32_t> static_modes(const amm_params& p) {
 }
 
 bool is_adaptive(const amm_params& p) {
-  return p.vr_adaptive_threshold > 0.0 &&
+  return p.vr_adaptive_threshold > 0.0 && !(p.flags & AMM_FLAG_NO_VR) &&
          (p.kernel == AMM_KERNEL_PREVIOUS || p.kernel == AMM_KERNEL_PROPOSED);
 }
 

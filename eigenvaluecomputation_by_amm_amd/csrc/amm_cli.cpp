@@ -340,7 +340,9 @@ static void power_method(host_memory& p, device_memory& dm, int kernel, bool is_
   P.seed = 1234ull + p.seed;
   P.trial = dm.rep++;
   P.sanity = p.sanity_check ? 1 : 0;
-  P.flags = AMM_FLAG_KEEP_STATS;
+  // the last of -vr / -vr_adaptive decides whether any variance-reduction
+  // step happens (host_memory.h:108-120, powerMethod_by_previous.cuh:135)
+  P.flags = AMM_FLAG_KEEP_STATS | (p.variance_reduction ? 0u : AMM_FLAG_NO_VR);
   std::vector<double> t(step), e(step), r(step);
   if (amm_power_method(dm.ctx, &P, t.data(), e.data(), r.data())) die("amm_power_method");
   if (!is_output) return;

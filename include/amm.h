@@ -48,6 +48,13 @@ extern "C" {
 /* flags of amm_params.flags */
 #define AMM_FLAG_NO_GRAPH 1u     /* launch kernel by kernel instead of one hipGraph */
 #define AMM_FLAG_KEEP_STATS 2u   /* reuse column statistics of an earlier run */
+/* variance reduction off: the reference's variance_reduction = false, which
+ * the last of -vr / -vr_adaptive decides (host_memory.h:108-120) and which
+ * gates every variance-reduction exact step (powerMethod_by_previous.cuh:
+ * 135-138, powerMethod_by_proposed.cuh:159-161, powerMethod_by_proposed_v2.cuh:
+ * 305); proposedAMM_v2 still takes its transition step at vr_period
+ * (powerMethod_by_proposed_v2.cuh:234-277). */
+#define AMM_FLAG_NO_VR 4u
 
 typedef struct amm_ctx amm_ctx;
 

@@ -404,6 +404,8 @@ typedef struct {
   uint64_t seed;
   int32_t sanity;
   uint32_t trial;
+  int32_t no_vr; /* the reference's variance_reduction = false (host_memory.h:108-120) */
+  uint32_t pad;
 } orc_params;
 
 /* Returns 1 when iteration i is an exact (variance-reduction) step.
@@ -473,8 +475,12 @@ int orc_power_method(const float *A, uint64_t d, uint64_t n, uint64_t ld, const 
   double prev_cos = 0.0;
   uint64_t last_exact = 0;
   for (uint64_t i = 0; i < P->step; ++i) {
-    int ex = orc_is_exact_step(kern, i, P->vr_period, kern == 3 ? 0.0 : P->vr_adaptive_threshold,
-                               prev_cos, last_exact);
+    int ex;
+    if (P->no_vr && kern != 0) /* only v2's transition step stays (powerMethod_by_proposed_v2.cuh:255-277) */
+      ex = kern == 3 && i + 1 == P->vr_period;
+    else
+      ex = orc_is_exact_step(kern, i, P->vr_period, kern == 3 ? 0.0 : P->vr_adaptive_threshold,
+                             prev_cos, last_exact);
     int centred = (kern == 2) || (kern == 3 && i + 1 > P->vr_period);
     if (ex) {
       orc_exact_product(A, d, n, ld, x, y);

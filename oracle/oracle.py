@@ -64,6 +64,8 @@ class orc_params(C.Structure):
         ("seed", C.c_uint64),
         ("sanity", C.c_int32),
         ("trial", C.c_uint32),
+        ("no_vr", C.c_int32),
+        ("pad", C.c_uint32),
     ]
 
 
@@ -204,13 +206,13 @@ def is_exact_step(kernel: str, i: int, vr_period: int, thr: float, prev_cos: flo
 
 
 def power_method(A_cols, d, kernel="exact", c=0, step=10, vr_period=0, vr_adaptive_threshold=0.0,
-                 seed=0, trial=0, sanity=False, w_plain=None, w_cent=None, eigvec=None):
+                 seed=0, trial=0, sanity=False, w_plain=None, w_cent=None, eigvec=None, no_vr=False):
     """Returns dict(eig, err, modes, x)."""
     a = _as_f32_cols(A_cols)
     n, ld = a.shape
     P = orc_params(kernel=KERNELS[kernel], c=c, step=step, vr_period=vr_period,
                    vr_adaptive_threshold=vr_adaptive_threshold, seed=seed & 0xFFFFFFFFFFFFFFFF,
-                   sanity=1 if sanity else 0, trial=trial)
+                   sanity=1 if sanity else 0, trial=trial, no_vr=1 if no_vr else 0, pad=0)
     eig = np.zeros(step, np.float64)
     err = np.zeros(step, np.float64)
     modes = np.zeros(step, np.uint8)

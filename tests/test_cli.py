@@ -147,3 +147,73 @@ def test_generator_symmetric_with_gershgorin_offset():
             assert raw.min() >= -2.0 and raw.max() < 3.0
         if dist == 1:
             assert raw.min() > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dist,p1,p2", [(0, 0.0, 1.0), (0, 1.5, 0.5), (1, 0.0, 0.5), (1, -1.0, 0.25),
+                                        (2, -2.0, 3.0), (2, 0.5, 0.75)])
+def test_generator_entry_distribution(dist, p1, p2):
+    """The entries of the generated matrix (upper triangle with the diagonal,
+    before the offset) follow the law that -matrixType names with its two
+    parameters, as curandGenerateNormal(mean p1, std p2),
+    curandGenerateLogNormal(p1, p2) and uniform p1 + (p2 - p1) u do
+    (device_memory.h:21-32,402-424,487-497): one-sample KS at m = 2048
+    (2.1 M entries, alpha = 0.01) plus the first two moments, so a std read
+    as a variance or a swapped pair of parameters fails."""
+    print(f"\n[gpu-case] generator law dist={dist} p1={p1} p2={p2}", flush=True)
+    import torch
+    from scipy import stats
+    from eigenvaluecomputation_by_amm_amd import _lib
+    lib = _lib.load()
+    m = 2048
+    A = torch.empty((m, m), device="cuda")
+    delta = C.c_float()
+    assert lib.amm_generate_symmetric(C.c_void_p(A.data_ptr()), m, m, dist, p1, p2, 7 + dist, None,
+                                      C.byref(delta)) == 0
+    a = A.cpu().numpy()
+    np.fill_diagonal(a, np.diag(a) - np.float32(delta.value))
+    e = a[np.triu_indices(m)].astype(np.float64)
+    if dist == 0:
+        law = stats.norm(loc=p1, scale=p2)
+    elif dist == 1:
+        law = stats.lognorm(s=p2, scale=np.exp(p1))
+    else:
+        law = stats.uniform(loc=p1, scale=p2 - p1)
+    ks = stats.kstest(e, law.cdf)
+    assert ks.pvalue > 0.01, ks
+    se = law.std() / np.sqrt(e.size)
+    assert abs(e.mean() - law.mean()) < 5 * se
+    assert abs(e.std() / law.std() - 1.0) < 0.01
+    # the diagonal carries the Gershgorin offset on top of the same law
+    assert delta.value > 0
+
+
+@pytest.mark.gpu
+def test_cli_last_vr_flag_decides(tmp_path):
+    """-vr and -vr_adaptive each set the reference's variance_reduction, and
+    the last one parsed wins (host_memory.h:108-120); every variance-reduction
+    step is gated on it (powerMethod_by_previous.cuh:135-138).  So
+    '-vr 5 -vr_adaptive 0' runs no exact step at all (the same trajectory
+    as '-vr 0'), '-vr_adaptive 0 -vr 5' runs one every 5 steps, and
+    '-vr_adaptive 0.01 -vr 0' none either."""
+    print("\n[gpu-case] cli vr flag order", flush=True)
+    rng = np.random.default_rng(2)
+    n = 256
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = (Q * (10.0 / np.arange(1, n + 1))) @ Q.T
+    write_matrix_file(tmp_path / "m.bin", A.astype(np.float32))
+    (tmp_path / "result").mkdir()
+    common = ["-filepath", "m.bin", "-kernel", "previousAMM", "-c", "32", "-step", "20", "-r", "1",
+              "-display", "-seed", "9"]
+
+    def eig(*flags):
+        r = run_amm([*common, *flags], tmp_path)
+        assert r.returncode == 0, r.stderr
+        return parse_csv(r.stdout)[:, 1]
+
+    none = eig("-vr", "0")
+    assert np.array_equal(eig("-vr", "5", "-vr_adaptive", "0"), none)
+    assert np.array_equal(eig("-vr_adaptive", "0.01", "-vr", "0"), none)
+    periodic = eig("-vr_adaptive", "0", "-vr", "5")
+    assert np.array_equal(periodic[:4], none[:4])      # iterations 0-3 are sampled in both
+    assert not np.array_equal(periodic[4:], none[4:])  # iteration 4 is exact with -vr 5

@@ -352,3 +352,91 @@ def test_reference_style_sampler_unbiased():
     x0 = O.x0(1, 0, 48)
     eig = O.ref_power_method(to_cols(A), 48, "previousAMM", 48, 200, 7, x0)
     assert abs(np.mean(eig[50:]) - lam) / lam < 0.05
+
+
+def _lib_ref():
+    import ctypes as C
+    return C, O.lib()
+
+
+def _xorshift_py(y: int) -> int:
+    """Marsaglia's xorshift32 with shifts (13, 17, 5) in Python integers."""
+    y ^= (y << 13) & 0xFFFFFFFF
+    y ^= y >> 17
+    y ^= (y << 5) & 0xFFFFFFFF
+    return y
+
+
+def test_reference_uniform23_bit_mapping():
+    """orc_ref_uniform23 (get_rand, device_memory.h:506-517): the float
+    (y & 0x7FFFFF | 0x3F800000) - 1 is exactly k / 2^23 with k the low 23
+    bits of the xorshift output, and the state advances to that output.
+    Checked against integer arithmetic, including outputs whose low bits are
+    all zero (u = 0) and all one (u = 1 - 2^-23)."""
+    C, L = _lib_ref()
+    rng = np.random.default_rng(3)
+    states = [int(s) for s in rng.integers(1, 2**32, 2000)]
+    # states whose next output has low 23 bits all zero (u = 0) or all one
+    # (u = 1 - 2^-23): invert the three xorshift steps (each is a bijection)
+    def inv_shl(y, k):
+        x = y
+        for _ in range(32 // k + 1):
+            x = y ^ ((x << k) & 0xFFFFFFFF)
+        return x
+
+    def inv_shr(y, k):
+        x = y
+        for _ in range(32 // k + 1):
+            x = y ^ (x >> k)
+        return x
+
+    found = {}
+    for y in (0x12800000, 0xFF800000, 0x007FFFFF, 0xA57FFFFF):
+        s0 = inv_shl(inv_shr(inv_shl(y, 5), 17), 13)
+        assert _xorshift_py(s0) == y
+        found[y] = s0
+    states += list(found.values())
+    for s0 in states:
+        st = np.array([s0], np.uint32)
+        u = float(L.orc_ref_uniform23(C.c_void_p(st.ctypes.data)))
+        y = _xorshift_py(s0)
+        assert int(st[0]) == y
+        assert u == (y & 0x7FFFFF) / 2.0**23
+        assert 0.0 <= u < 1.0
+        if s0 in found.values():
+            assert u in (0.0, 1.0 - 2.0**-23)
+
+
+def test_reference_pick_brute_force():
+    """orc_ref_pick (pick_index, device_memory.h:519-545) against a linear
+    scan: min{j : cdf_j > float32(cdf_{n-1} * u)}, n - 1 when no entry
+    exceeds the target.  CDFs with leading and inner zero weights (ties),
+    targets exactly on a CDF value, u = 0 and u just below 1."""
+    C, L = _lib_ref()
+    rng = np.random.default_rng(4)
+    cases = [np.array([0, 0, 1, 1, 1, 3, 3, 6], np.float32),
+             np.array([5], np.float32),
+             np.array([0, 2], np.float32),
+             np.cumsum(rng.choice([0.0, 0.25, 1.0, 3.5], size=37)).astype(np.float32)]
+    for _ in range(20):
+        w = rng.exponential(size=int(rng.integers(1, 300))).astype(np.float32)
+        w[rng.random(w.size) < 0.3] = 0.0
+        if w[-1] == 0.0:
+            w[-1] = 1.0
+        cases.append(np.cumsum(w, dtype=np.float32))
+    for cdf in cases:
+        n = cdf.size
+        total = cdf[-1]
+        us = [0.0, 1.0 - 2.0**-23] + [float(v) for v in rng.random(50)]
+        # u that puts the target exactly on each distinct CDF value (when representable)
+        for v in np.unique(cdf):
+            u = np.float32(v / total) if total > 0 else np.float32(0)
+            if np.float32(total * u) == v and u < 1:
+                us.append(float(u))
+        for u in us:
+            u32 = np.float32(u)
+            t = np.float32(total * u32)
+            above = np.nonzero(cdf > t)[0]
+            want = int(above[0]) if above.size else n - 1
+            got = int(L.orc_ref_pick(C.c_void_p(cdf.ctypes.data), n, C.c_float(u32)))
+            assert got == want, (cdf, float(u32), float(t), got, want)
