@@ -42,6 +42,8 @@ _SIGS = {
     "amm_square_of_A": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_uint32, C.c_uint64, C.c_uint32,
                                   C.c_uint32, _vp, _vp, _vp, _vp]),
     "amm_exact_product": (C.c_int, [_vp, _vp, _vp]),
+    "amm_exact_product_noalloc": (C.c_int, [_vp, C.c_int32, C.c_uint64, C.c_uint64, C.c_uint64, _vp, _vp,
+                                            _vp]),
     "amm_power_method": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "amm_last_modes": (C.c_int, [_vp, _vp, C.c_uint64]),
     "amm_get_vector": (C.c_int, [_vp, C.c_int32, _vp, C.c_uint64]),

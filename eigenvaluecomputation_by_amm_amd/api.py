@@ -318,6 +318,35 @@ def rank1_update_batched(out, g, U, V, d: int, n: int, ld: int) -> None:
                                        C.c_void_p(V.data_ptr()), n, C.c_void_p(stream) if stream else None))
 
 
+def exact_product_noalloc(A, x, y=None, d: int | None = None):
+    """y = A (A^T x) for the matrix held by the device tensor A ((n, ld)
+    rows = columns, float32 or bfloat16) with no device allocation in the
+    library (amm_exact_product_noalloc: its intermediates live in a fixed
+    16 MiB device array per GPU).  Asynchronous on torch's current stream;
+    y (float32, length d) is allocated here by torch when not given."""
+    import torch
+    lib = _lib.load()
+    if A.dim() != 2 or not A.is_cuda or A.stride(1) != 1:
+        raise ValueError("A must be a 2-D device tensor (n, ld) with contiguous rows")
+    if A.dtype == torch.float32:
+        dt = _lib.AMM_DTYPE_F32
+    elif A.dtype == torch.bfloat16:
+        dt = _lib.AMM_DTYPE_BF16
+    else:
+        raise ValueError("A must be float32 or bfloat16")
+    n, ld = A.shape[0], A.stride(0)
+    d = A.shape[1] if d is None else int(d)
+    if y is None:
+        y = torch.empty(d, dtype=torch.float32, device=A.device)
+    for t in (x, y):
+        if t.dtype != torch.float32 or t.numel() != d or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError("x and y must be contiguous float32 device vectors of length d")
+    stream = torch.cuda.current_stream(A.device).cuda_stream
+    check(lib.amm_exact_product_noalloc(C.c_void_p(A.data_ptr()), dt, d, n, ld, C.c_void_p(x.data_ptr()),
+                                        C.c_void_p(y.data_ptr()), C.c_void_p(stream) if stream else None))
+    return y
+
+
 def _run(kernel: str, ctx: AmmContext, **kw) -> RunResult:
     return ctx.power_method(RunParams(kernel=kernel, **kw))
 

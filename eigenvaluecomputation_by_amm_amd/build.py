@@ -38,7 +38,8 @@ HIP_FLAGS = [
     f"-I{CSRC}",
 ]
 
-LIB_SOURCES = ["amm_kernels.hip", "amm_generate.hip", "amm_capi.hip", "amm_batch.hip", "amm_batch_capi.hip"]
+LIB_SOURCES = ["amm_kernels.hip", "amm_generate.hip", "amm_capi.hip", "amm_batch.hip", "amm_batch_capi.hip",
+               "amm_noalloc.hip"]
 CLI_SOURCES = ["amm_cli.cpp"]
 
 

@@ -127,6 +127,18 @@ int amm_square_of_A(amm_ctx *ctx, int32_t kernel, int32_t sanity, uint32_t c, ui
  * cublasSgemv calls (powerMethod_by_exact.cuh:37-53). */
 int amm_exact_product(amm_ctx *ctx, const float *x_dev, float *y_dev);
 
+/* The same exact product y = A (A^T x) with no device allocation and no
+ * context.  No counterpart in the reference: for callers that cannot
+ * allocate (code under hipGraph capture, serving loops, harnesses that
+ * forbid hipMalloc).  A as for amm_ctx_create; x_dev, y_dev: device, length
+ * d.  Every intermediate lives in one 16 MiB device array per GPU, held
+ * until the process ends; sizes whose intermediates do not fit are rejected
+ * with AMM_E_INVALID.  Asynchronous on `stream` (a hipStream_t, NULL: the
+ * null stream) on the current device.  Calls that overlap on one GPU share
+ * that array and race: order them on one stream. */
+int amm_exact_product_noalloc(const void *A, int32_t dtype, uint64_t d, uint64_t n, uint64_t ld,
+                              const float *x_dev, float *y_dev, void *stream);
+
 /* A full power-method run; host outputs of length p->step: cumulative device
  * time in ms from the run's first kernel, raw Rayleigh quotient
  * lambda_i = y^T x, and y^T v* / ||y||.  Replaces powerMethod_by_exact /

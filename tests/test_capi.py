@@ -99,3 +99,22 @@ def test_header_compiles_as_c_and_links(tmp_path):
     assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
     assert r.stdout.startswith("ok")
 
+
+def test_noalloc_exact_product_rejects_before_any_hip_call():
+    """amm_exact_product_noalloc validates its arguments and sizes its plan
+    against the fixed scratch before it touches the device."""
+    from eigenvaluecomputation_by_amm_amd import _lib
+    lib = _lib.load()
+    p = C.c_void_p(16)
+    assert lib.amm_exact_product_noalloc(None, 0, 8, 8, 8, p, p, None) == -1
+    assert "A is NULL" in lib.amm_last_error().decode()
+    assert lib.amm_exact_product_noalloc(p, 0, 8, 8, 8, None, p, None) == -1
+    assert lib.amm_exact_product_noalloc(p, 0, 10, 8, 10, p, p, None) == -1
+    assert "multiple of 16 bytes" in lib.amm_last_error().decode()
+    # 2^21 rows: the padded x and y alone fill the 16 MiB scratch
+    assert lib.amm_exact_product_noalloc(p, 0, 1 << 21, 4, 1 << 21, p, p, None) == -1
+    assert "scratch" in lib.amm_last_error().decode()
+    # 2^20 columns of 2^17 rows: even chunks of 2048 columns (the largest) leave
+    # 512 partials of A t at 512 KiB each, far beyond the scratch
+    assert lib.amm_exact_product_noalloc(p, 0, 1 << 17, 1 << 20, 1 << 17, p, p, None) == -1
+    assert "scratch" in lib.amm_last_error().decode()
