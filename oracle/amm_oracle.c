@@ -431,18 +431,22 @@ int orc_power_method(const float *A, uint64_t d, uint64_t n, uint64_t ld, const 
 {
   const int kern = P->kernel;
   if (kern == 3 && P->vr_period < 1) return -1; /* v2 needs 1 <= vr_period */
-  float *alpha = (float *)malloc(sizeof(float) * n);
-  double *nrm2 = (double *)malloc(sizeof(double) * n);
-  double *cnrm2 = (double *)malloc(sizeof(double) * n);
-  orc_colstats_f32(A, d, n, ld, alpha, nrm2, cnrm2);
-  const double *wp = w_plain_in ? w_plain_in : nrm2;
-  const double *wc = w_cent_in ? w_cent_in : cnrm2;
-  uint64_t *cdf_p = (uint64_t *)malloc(sizeof(uint64_t) * n);
-  uint64_t *cdf_c = (uint64_t *)malloc(sizeof(uint64_t) * n);
-  uint64_t tot_p, tot_c;
-  uint64_t lp_p, lp_c;
-  orc_cdf(wp, n, cdf_p, &tot_p, &lp_p);
-  orc_cdf(wc, n, cdf_c, &tot_c, &lp_c);
+  float *alpha = (float *)calloc(n, sizeof(float));
+  double *nrm2 = (double *)calloc(n, sizeof(double));
+  double *cnrm2 = (double *)calloc(n, sizeof(double));
+  uint64_t *cdf_p = (uint64_t *)calloc(n, sizeof(uint64_t));
+  uint64_t *cdf_c = (uint64_t *)calloc(n, sizeof(uint64_t));
+  uint64_t tot_p = 0, tot_c = 0;
+  uint64_t lp_p = 0, lp_c = 0;
+  /* the exact method samples nothing: no column statistics, as
+   * powerMethod_by_exact.cuh:4-100 reads no norms */
+  if (kern != 0) {
+    orc_colstats_f32(A, d, n, ld, alpha, nrm2, cnrm2);
+    const double *wp = w_plain_in ? w_plain_in : nrm2;
+    const double *wc = w_cent_in ? w_cent_in : cnrm2;
+    orc_cdf(wp, n, cdf_p, &tot_p, &lp_p);
+    orc_cdf(wc, n, cdf_c, &tot_c, &lp_c);
+  }
 
   float *x = (float *)malloc(sizeof(float) * d);
   float *y = (float *)malloc(sizeof(float) * d);
