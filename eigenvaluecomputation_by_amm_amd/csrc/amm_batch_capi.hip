@@ -38,6 +38,17 @@ using amm_host::fail;
       return fail(AMM_E_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
   } while (0)
 
+// Host copies run on the object's own stream and then wait for it.  A plain
+// hipMemcpy runs on the legacy null stream, which may not take a dependency
+// on a blocking stream that another thread of this process is capturing into
+// a graph; the failed copy would then abort that capture too.
+static hipError_t copy_sync(hipStream_t s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(s);
+}
+
+
 namespace {
 
 constexpr uint32_t kTargetItems = 2048;      // workgroups the first contraction aims for
@@ -276,9 +287,9 @@ int upload_plan(amm_batch* b, int kernel) {
     BHIPCHK(hipMalloc((void**)&b->offs, all.size() * sizeof(uint32_t)));
     b->offs_cap = all.size();
   }
-  BHIPCHK(hipMemcpy(b->offs, all.data(), all.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  BHIPCHK(copy_sync(b->stream, b->offs, all.data(), all.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   for (auto& f : fix) f.first->off = b->offs + f.second;
-  BHIPCHK(hipMemcpy(b->dP, b->hp.data(), np * sizeof(BProb), hipMemcpyHostToDevice));
+  BHIPCHK(copy_sync(b->stream, b->dP, b->hp.data(), np * sizeof(BProb), hipMemcpyHostToDevice));
   b->plan_valid = true;
   b->plan_kernel = kernel;
   drop_graph(b);
@@ -477,7 +488,7 @@ int validate(const amm_batch* b, const amm_params* p) {
 
 int check_status(amm_batch* b) {
   std::vector<DevState> sts(b->np);
-  BHIPCHK(hipMemcpy(sts.data(), b->sb[0].st, b->np * sizeof(DevState), hipMemcpyDeviceToHost));
+  BHIPCHK(copy_sync(b->stream, sts.data(), b->sb[0].st, b->np * sizeof(DevState), hipMemcpyDeviceToHost));
   for (uint32_t i = 0; i < b->np; ++i) {
     const uint32_t stt = sts[i].status;
     if (stt & amm::kStatusNonFinite)
@@ -753,7 +764,7 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
       q.ntiles = b->tiles[i];
       t0 += b->tiles[i];
     }
-    if (hipMemcpy(b->dS, hs.data(), count * sizeof(amm::BSetup), hipMemcpyHostToDevice) != hipSuccess)
+    if (copy_sync(b->stream, b->dS, hs.data(), count * sizeof(amm::BSetup), hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(AMM_E_HIP, "hipMemcpy failed"));
   }
   if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
@@ -788,7 +799,7 @@ int amm_batch_set_exact_eigenvector(amm_batch* b, uint32_t problem, const double
     p.eigvec = nullptr;
   } else {
     if (len != p.d) return fail(AMM_E_INVALID, "eigenvector length must equal d");
-    BHIPCHK(hipMemcpy(b->sb[problem].eigvec, v, len * sizeof(double), hipMemcpyHostToDevice));
+    BHIPCHK(copy_sync(b->stream, b->sb[problem].eigvec, v, len * sizeof(double), hipMemcpyHostToDevice));
     p.eigvec = b->sb[problem].eigvec;
   }
   b->plan_valid = false;
@@ -803,13 +814,13 @@ int amm_batch_power_method(amm_batch* b, const amm_params* p, double* time_ms, d
   const uint64_t S = p->step;
   if (time_ms) {
     std::vector<int64_t> t(S);
-    BHIPCHK(hipMemcpy(t.data(), b->ts, S * sizeof(int64_t), hipMemcpyDeviceToHost));
+    BHIPCHK(copy_sync(b->stream, t.data(), b->ts, S * sizeof(int64_t), hipMemcpyDeviceToHost));
     for (uint64_t i = 0; i < S; ++i) time_ms[i] = (double)t[i] / (double)b->wall_khz;
   }
   if (!eig && !err) return AMM_OK;
   // one copy of the whole output pool, then each problem's first ntr trials
   std::vector<char> h(b->rpool_bytes);
-  BHIPCHK(hipMemcpy(h.data(), b->rpool, h.size(), hipMemcpyDeviceToHost));
+  BHIPCHK(copy_sync(b->stream, h.data(), b->rpool, h.size(), hipMemcpyDeviceToHost));
   for (int which = 0; which < 2; ++which) {
     double* dst = which ? err : eig;
     if (!dst) continue;
@@ -828,7 +839,7 @@ int amm_batch_last_modes(amm_batch* b, uint32_t* modes, uint64_t len) {
   const uint64_t S = b->last_step;
   if (len < (uint64_t)b->np * b->ntr * S) return fail(AMM_E_INVALID, "modes buffer too small");
   std::vector<char> h(b->rpool_bytes);
-  BHIPCHK(hipMemcpy(h.data(), b->rpool, h.size(), hipMemcpyDeviceToHost));
+  BHIPCHK(copy_sync(b->stream, h.data(), b->rpool, h.size(), hipMemcpyDeviceToHost));
   for (uint32_t i = 0; i < b->np; ++i) {
     const char* src = reinterpret_cast<const char*>(b->hp[i].modes);
     std::memcpy(modes + (uint64_t)i * b->ntr * S, h.data() + (src - b->rpool),
@@ -845,7 +856,7 @@ int amm_batch_get_vector(amm_batch* b, uint32_t problem, uint32_t trial, int32_t
   if (len < p.d) return fail(AMM_E_INVALID, "buffer shorter than d");
   const float* src = (which == 1 ? p.Y : p.X) + (uint64_t)trial * p.dstage;
   BHIPCHK(hipStreamSynchronize(b->stream));
-  BHIPCHK(hipMemcpy(host, src, p.d * sizeof(float), hipMemcpyDeviceToHost));
+  BHIPCHK(copy_sync(b->stream, host, src, p.d * sizeof(float), hipMemcpyDeviceToHost));
   return AMM_OK;
 }
 
@@ -909,9 +920,9 @@ int amm_batch_sample(amm_batch* b, int32_t kernel, uint32_t iter, uint32_t trial
   const BProb& q = b->hp[problem];
   std::vector<uint32_t> tc(q.ntiles), list((size_t)q.ntiles * amm::kBTile);
   std::vector<float> W((size_t)q.ntiles * amm::kBTile * amm::kBT);
-  BHIPCHK(hipMemcpy(tc.data(), q.tcount, tc.size() * 4, hipMemcpyDeviceToHost));
-  BHIPCHK(hipMemcpy(list.data(), q.list, list.size() * 4, hipMemcpyDeviceToHost));
-  BHIPCHK(hipMemcpy(W.data(), q.W, W.size() * 4, hipMemcpyDeviceToHost));
+  BHIPCHK(copy_sync(b->stream, tc.data(), q.tcount, tc.size() * 4, hipMemcpyDeviceToHost));
+  BHIPCHK(copy_sync(b->stream, list.data(), q.list, list.size() * 4, hipMemcpyDeviceToHost));
+  BHIPCHK(copy_sync(b->stream, W.data(), q.W, W.size() * 4, hipMemcpyDeviceToHost));
   uint64_t m = 0;
   for (uint32_t t = 0; t < q.ntiles; ++t) {
     for (uint32_t s = 0; s < tc[t]; ++s) {

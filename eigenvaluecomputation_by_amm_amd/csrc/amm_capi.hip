@@ -47,6 +47,17 @@ int fail(int code, const std::string& msg) {
       return fail(AMM_E_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
   } while (0)
 
+// Host copies run on the object's own stream and then wait for it.  A plain
+// hipMemcpy runs on the legacy null stream, which may not take a dependency
+// on a blocking stream that another thread of this process is capturing into
+// a graph; the failed copy would then abort that capture too.
+hipError_t copy_sync(hipStream_t s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(s);
+}
+
+
 template <typename T>
 int grow(T*& p, size_t& cap, size_t need) {
   if (need <= cap && p) return AMM_OK;
@@ -192,6 +203,15 @@ DotPlan plan_dot(const amm_ctx* c, uint64_t entries, int cpw) {
   return DotPlan{(uint32_t)R, (uint32_t)rps};
 }
 
+// Sampled dot pass: rows split so that each wave covers one trip of
+// kListDotSteps wave-steps, every load of it in flight at once (a sampled
+// pass is a few tens of MB: latency, not bandwidth, sets its time).
+DotPlan plan_dot_list(const amm_ctx* c) {
+  const uint64_t trip = 64u * c->V * amm::kListDotSteps;
+  const uint64_t R = std::max<uint64_t>(1, (c->d + trip - 1) / trip);
+  return DotPlan{(uint32_t)R, (uint32_t)trip};
+}
+
 AxpyPlan plan_axpy(const amm_ctx* c, uint64_t entries, bool pow2) {
   const uint32_t rb = rowblocks(c);
   const uint64_t S_t = std::max<uint64_t>(1, (kTargetBlocks + rb - 1) / rb);
@@ -212,7 +232,7 @@ int ensure_plans(amm_ctx* c, uint32_t list_c) {
   c->axpy_dense = plan_axpy(c, c->n, false);
   c->list_c = list_c;
   if (list_c) {
-    c->dot_list = plan_dot(c, list_c, 2);
+    c->dot_list = plan_dot_list(c);
     c->axpy_list = plan_axpy(c, list_c, true);
   } else {
     c->dot_list = DotPlan{1, 0};
@@ -353,7 +373,13 @@ int enq_exact(amm_ctx* c, uint32_t it, const uint32_t* gate_mode, uint32_t gate_
   return AMM_OK;
 }
 
-// sampled product with iteration `it`'s list at list slot `slot`
+// A sampled product's second contraction carries the finish (y and the
+// block partials of the Rayleigh quotient) where the row blocks fill the
+// device; sanity mode (every column, unit weights) keeps the chunked form.
+bool fused_finish(const amm_ctx* c, bool sanity) { return !sanity && amm::axpy_finish_applies(c->dpad); }
+
+// sampled product with iteration `it`'s list at list slot `slot`; with
+// fused_finish the product also writes y = c->y and the finish partials
 int enq_amm(amm_ctx* c, uint32_t it, uint32_t slot, bool proposed, bool sanity,
             const uint32_t* gate_mode, uint32_t gate_mask) {
   const uint32_t cc = c->list_c;
@@ -377,6 +403,33 @@ int enq_amm(amm_ctx* c, uint32_t it, uint32_t slot, bool proposed, bool sanity,
   g.gate_mask = gate_mask;
   g.nt = (c->list_nt & 1u) != 0;
   HIPCHK(amm::launch_gather_dot(c->dtype, g, c->stream));
+  if (fused_finish(c, sanity)) {
+    amm::AxpyFinishArgs f{};
+    f.A = c->A;
+    f.ld = c->ld;
+    f.d = (uint32_t)c->d;
+    f.dpad = c->dpad;
+    f.idx = g.idx;
+    f.wgt = c->wgt + (size_t)slot * cc;
+    f.seg_count = g.seg_count;
+    f.cap = cc;
+    f.P1 = c->P1;
+    f.R = dp.R;
+    f.alpha = proposed ? c->alpha : nullptr;
+    f.st = c->st;
+    f.x = c->x;
+    f.y = c->y;
+    f.ym = c->ym;
+    f.v = c->v;
+    f.eigvec = c->has_eigvec ? c->eigvec : nullptr;
+    f.BP = c->BP;
+    f.mode = gate_mode;
+    f.it = it;
+    f.gate_mask = gate_mask;
+    f.nt = (c->list_nt & 2u) != 0;
+    HIPCHK(amm::launch_axpy_finish(c->dtype, f, c->stream));
+    return AMM_OK;
+  }
   amm::GatherAxpyArgs a{};
   a.nt = (c->list_nt & 2u) != 0;
   a.A = c->A;
@@ -425,11 +478,17 @@ amm::TailArgs tail_args(amm_ctx* c, uint32_t op, uint32_t nb, uint32_t it, bool 
   return t;
 }
 
+// How the finish kernel runs after a product: always (the product left
+// partials), never (a fused sampled product wrote y and the block partials
+// itself), or only on exact steps (adaptive runs whose sampled steps fuse).
+enum FinishMode { kFinishAlways = 0, kFinishNone = 1, kFinishExactOnly = 2 };
+
 // y = the product's partial sums (+ base), then the tail: lambda_i, err_i,
 // scale, step decision; with norm_inline also x = y / ||y|| (+ s, q)
 int enq_finish(amm_ctx* c, uint32_t it, bool sanity, float* y_out, const float* ym_base,
                uint32_t adapt_kernel, uint64_t period, double thr, uint32_t step,
-               bool norm_inline = false, const float* zq = nullptr, bool stamp = false) {
+               bool norm_inline = false, const float* zq = nullptr, bool stamp = false,
+               FinishMode fm = kFinishAlways) {
   amm::FinishArgs f{};
   f.P2 = c->P2;
   f.S_exact = c->gram_S ? c->gram_S : c->axpy_dense.S;
@@ -447,7 +506,8 @@ int enq_finish(amm_ctx* c, uint32_t it, bool sanity, float* y_out, const float* 
   f.st = c->st;
   f.mode = c->mode;
   f.it = it;
-  HIPCHK(amm::launch_finish(f, c->stream));
+  f.gate_mask = fm == kFinishExactOnly ? (1u << amm::kModeExactMemo) | (1u << amm::kModeExactNoMemo) : 0u;
+  if (fm != kFinishNone) HIPCHK(amm::launch_finish(f, c->stream));
   amm::TailArgs t = tail_args(c, amm::kTailFinish, (uint32_t)amm::finish_blocks(c->dpad), it,
                               norm_inline, zq, stamp);
   t.y = y_out;
@@ -460,8 +520,9 @@ int enq_finish(amm_ctx* c, uint32_t it, bool sanity, float* y_out, const float* 
 }
 
 // x = y / ||y|| with the 1/||y|| already in the state, plus s, q when zq is
-// given; `stamp` closes iteration it.  Small d: one tail block; large d: a
-// grid pass and a tail for s, q.
+// given; `stamp` closes iteration it.  Up to 65536 rows: the tail kernel's
+// blocks normalise (and form the parts of s, q); beyond: a grid pass and a
+// tail for s, q.
 int enq_norm(amm_ctx* c, uint32_t it, const float* zq, bool stamp) {
   if (c->dpad <= kFuseNormRows) {
     HIPCHK(amm::launch_tail(tail_args(c, amm::kTailNorm, 0, it, true, zq, stamp), c->stream));
@@ -607,11 +668,14 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
   if ((rc = enq_x0(c, K == AMM_KERNEL_PROPOSED ? c->z : nullptr))) return rc;
   const uint32_t gate_exact = 1u << amm::kModeExactMemo;
   const uint32_t gate_amm = (1u << amm::kModeAmmPrevious) | (1u << amm::kModeAmmProposed);
+  const bool fuse = fused_finish(c, sanity);
   for (uint32_t i = 0; i < step; ++i) {
     bool z_after = false;
+    FinishMode fm = kFinishAlways;
     if (adaptive) {
       if ((rc = enq_exact(c, i, c->mode, gate_exact))) return rc;
       if ((rc = enq_amm(c, i, i, K == AMM_KERNEL_PROPOSED, sanity, c->mode, gate_amm))) return rc;
+      if (fuse) fm = kFinishExactOnly;
     } else {
       const uint32_t md = modes[i];
       if (md == amm::kModeExactMemo || md == amm::kModeExactNoMemo) {
@@ -619,6 +683,7 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
         z_after = K == AMM_KERNEL_PROPOSED_V2 && i + 1 == P && !fused_z;
       } else {
         if ((rc = enq_amm(c, i, i, md == amm::kModeAmmProposed, sanity, nullptr, 0))) return rc;
+        if (fuse) fm = kFinishNone;
       }
     }
     const bool want_sq = K == AMM_KERNEL_PROPOSED || (K == AMM_KERNEL_PROPOSED_V2 && i + 1 >= P);
@@ -627,7 +692,7 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
     const bool inline_norm = c->dpad <= kFuseNormRows && !z_after;
     if ((rc = enq_finish(c, i, sanity, c->y, c->ym, adaptive ? (uint32_t)K : 0u, P,
                          p.vr_adaptive_threshold, step, inline_norm, want_sq ? c->z : nullptr,
-                         inline_norm)))
+                         inline_norm, fm)))
       return rc;
     if (z_after)
       if ((rc = enq_z(c))) return rc;  // [z, .] = A [alpha, .] at the v2 transition
@@ -640,11 +705,11 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
 // Reads and clears the sticky device status after a synchronisation.
 int check_status(amm_ctx* c) {
   uint32_t stt = 0;
-  HIPCHK(hipMemcpy(&stt, (char*)c->st + offsetof(DevState, status), sizeof(uint32_t),
+  HIPCHK(copy_sync(c->stream, &stt, (char*)c->st + offsetof(DevState, status), sizeof(uint32_t),
                    hipMemcpyDeviceToHost));
   if (stt) {
     const uint32_t zero = 0;
-    HIPCHK(hipMemcpy((char*)c->st + offsetof(DevState, status), &zero, sizeof(uint32_t),
+    HIPCHK(copy_sync(c->stream, (char*)c->st + offsetof(DevState, status), &zero, sizeof(uint32_t),
                      hipMemcpyHostToDevice));
     if (stt & amm::kStatusNonFinite)
       return fail(AMM_E_INVALID, "the matrix has non-finite entries (NaN or Inf column norms)");
@@ -770,7 +835,7 @@ int amm_set_exact_eigenvector(amm_ctx* c, const double* v, uint64_t len) {
     return AMM_OK;
   }
   if (len != c->d) return fail(AMM_E_INVALID, "eigenvector length must equal d");
-  HIPCHK(hipMemcpy(c->eigvec, v, sizeof(double) * len, hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c->stream, c->eigvec, v, sizeof(double) * len, hipMemcpyHostToDevice));
   c->has_eigvec = true;
   drop_graph(c);
   return AMM_OK;
@@ -784,11 +849,11 @@ int amm_column_stats(amm_ctx* c, float* alpha_h, double* nrm2_h, double* cnrm2_h
   if ((rc = enq_cdf(c, 1, false))) return rc;
   c->stats_valid = true;
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (alpha_h) HIPCHK(hipMemcpy(alpha_h, c->alpha, sizeof(float) * c->n, hipMemcpyDeviceToHost));
-  if (nrm2_h) HIPCHK(hipMemcpy(nrm2_h, c->nrm2, sizeof(double) * c->n, hipMemcpyDeviceToHost));
-  if (cnrm2_h) HIPCHK(hipMemcpy(cnrm2_h, c->cnrm2, sizeof(double) * c->n, hipMemcpyDeviceToHost));
+  if (alpha_h) HIPCHK(copy_sync(c->stream, alpha_h, c->alpha, sizeof(float) * c->n, hipMemcpyDeviceToHost));
+  if (nrm2_h) HIPCHK(copy_sync(c->stream, nrm2_h, c->nrm2, sizeof(double) * c->n, hipMemcpyDeviceToHost));
+  if (cnrm2_h) HIPCHK(copy_sync(c->stream, cnrm2_h, c->cnrm2, sizeof(double) * c->n, hipMemcpyDeviceToHost));
   if (alpha2_h)
-    HIPCHK(hipMemcpy(alpha2_h, (char*)c->st + offsetof(DevState, alpha2), sizeof(double),
+    HIPCHK(copy_sync(c->stream, alpha2_h, (char*)c->st + offsetof(DevState, alpha2), sizeof(double),
                      hipMemcpyDeviceToHost));
   return AMM_OK;
 }
@@ -805,12 +870,12 @@ int amm_build_cdf(amm_ctx* c, int32_t centred, uint64_t* cdf_h, uint64_t* total_
   if ((rc = enq_cdf(c, which, false))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   if ((rc = check_status(c))) return rc;
-  if (cdf_h) HIPCHK(hipMemcpy(cdf_h, c->cdf[which], sizeof(uint64_t) * c->n, hipMemcpyDeviceToHost));
+  if (cdf_h) HIPCHK(copy_sync(c->stream, cdf_h, c->cdf[which], sizeof(uint64_t) * c->n, hipMemcpyDeviceToHost));
   if (total_h)
-    HIPCHK(hipMemcpy(total_h, (char*)c->st + offsetof(DevState, total) + sizeof(uint64_t) * which,
+    HIPCHK(copy_sync(c->stream, total_h, (char*)c->st + offsetof(DevState, total) + sizeof(uint64_t) * which,
                      sizeof(uint64_t), hipMemcpyDeviceToHost));
   if (last_h)
-    HIPCHK(hipMemcpy(last_h, (char*)c->st + offsetof(DevState, last_pos) + sizeof(uint64_t) * which,
+    HIPCHK(copy_sync(c->stream, last_h, (char*)c->st + offsetof(DevState, last_pos) + sizeof(uint64_t) * which,
                      sizeof(uint64_t), hipMemcpyDeviceToHost));
   return AMM_OK;
 }
@@ -834,9 +899,9 @@ int amm_sample(amm_ctx* c, int32_t centred, uint64_t seed, uint32_t iter, uint32
   HIPCHK(hipStreamSynchronize(c->stream));
   if ((rc = check_status(c))) return rc;
   const size_t nseg = (cc + amm::kSampleSeg - 1) / amm::kSampleSeg;
-  if (idx_h) HIPCHK(hipMemcpy(idx_h, c->idx, sizeof(uint32_t) * cc, hipMemcpyDeviceToHost));
-  if (wgt_h) HIPCHK(hipMemcpy(wgt_h, c->wgt, sizeof(float) * cc, hipMemcpyDeviceToHost));
-  if (segc_h) HIPCHK(hipMemcpy(segc_h, c->segc, sizeof(uint32_t) * nseg, hipMemcpyDeviceToHost));
+  if (idx_h) HIPCHK(copy_sync(c->stream, idx_h, c->idx, sizeof(uint32_t) * cc, hipMemcpyDeviceToHost));
+  if (wgt_h) HIPCHK(copy_sync(c->stream, wgt_h, c->wgt, sizeof(float) * cc, hipMemcpyDeviceToHost));
+  if (segc_h) HIPCHK(copy_sync(c->stream, segc_h, c->segc, sizeof(uint32_t) * nseg, hipMemcpyDeviceToHost));
   return AMM_OK;
 }
 
@@ -880,7 +945,9 @@ int amm_square_of_A(amm_ctx* c, int32_t kernel, int32_t sanity, uint32_t cc, uin
   c->h_mode = prop ? amm::kModeAmmProposed : amm::kModeAmmPrevious;
   HIPCHK(hipMemcpyAsync(c->mode, &c->h_mode, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
   if ((rc = enq_amm(c, 0, 0, prop, san, nullptr, 0))) return rc;
-  if ((rc = enq_finish(c, 0, san, c->y, c->ym, 0, 0, 0.0, 1))) return rc;
+  if ((rc = enq_finish(c, 0, san, c->y, c->ym, 0, 0, 0.0, 1, false, nullptr, false,
+                       fused_finish(c, san) ? kFinishNone : kFinishAlways)))
+    return rc;
   HIPCHK(hipMemcpyAsync(y_dev, c->y, db, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return check_status(c);
@@ -972,13 +1039,13 @@ int amm_power_method(amm_ctx* c, const amm_params* p, double* time_ms, double* e
   if (p->kernel != AMM_KERNEL_EXACT) c->stats_valid = true;
   if ((rc = check_status(c))) return rc;
   const size_t S = p->step;
-  if (eig) HIPCHK(hipMemcpy(eig, c->eig, sizeof(double) * S, hipMemcpyDeviceToHost));
-  if (err) HIPCHK(hipMemcpy(err, c->err, sizeof(double) * S, hipMemcpyDeviceToHost));
+  if (eig) HIPCHK(copy_sync(c->stream, eig, c->eig, sizeof(double) * S, hipMemcpyDeviceToHost));
+  if (err) HIPCHK(copy_sync(c->stream, err, c->err, sizeof(double) * S, hipMemcpyDeviceToHost));
   if (time_ms) {
     std::vector<int64_t> ts(S);
     int64_t t0 = 0;
-    HIPCHK(hipMemcpy(ts.data(), c->ts, sizeof(int64_t) * S, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(&t0, (char*)c->st + offsetof(DevState, t_start), sizeof(int64_t),
+    HIPCHK(copy_sync(c->stream, ts.data(), c->ts, sizeof(int64_t) * S, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(c->stream, &t0, (char*)c->st + offsetof(DevState, t_start), sizeof(int64_t),
                      hipMemcpyDeviceToHost));
     for (size_t i = 0; i < S; ++i) time_ms[i] = (double)(ts[i] - t0) / (double)c->wall_khz;
   }
@@ -990,7 +1057,7 @@ int amm_last_modes(amm_ctx* c, uint32_t* modes_h, uint64_t len) {
   if (len > c->mode_cap) return fail(AMM_E_INVALID, "len exceeds the last run's step");
   int rc;
   if ((rc = set_device(c))) return rc;
-  HIPCHK(hipMemcpy(modes_h, c->mode, sizeof(uint32_t) * len, hipMemcpyDeviceToHost));
+  HIPCHK(copy_sync(c->stream, modes_h, c->mode, sizeof(uint32_t) * len, hipMemcpyDeviceToHost));
   return AMM_OK;
 }
 
@@ -1000,7 +1067,7 @@ int amm_get_vector(amm_ctx* c, int32_t which, float* host, uint64_t len) {
   int rc;
   if ((rc = set_device(c))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipMemcpy(host, which ? c->y : c->x, sizeof(float) * len, hipMemcpyDeviceToHost));
+  HIPCHK(copy_sync(c->stream, host, which ? c->y : c->x, sizeof(float) * len, hipMemcpyDeviceToHost));
   return AMM_OK;
 }
 
@@ -1060,7 +1127,7 @@ int amm_sigma_max(amm_ctx* c, const amm_params* p, double* sigma, float* u_dev, 
     HIPCHK(hipMemcpyAsync(u_dev, c->x, sizeof(float) * c->d, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (sigma)
-    HIPCHK(hipMemcpy(sigma, (char*)c->st + offsetof(DevState, sigma), sizeof(double),
+    HIPCHK(copy_sync(c->stream, sigma, (char*)c->st + offsetof(DevState, sigma), sizeof(double),
                      hipMemcpyDeviceToHost));
   return AMM_OK;
 }
@@ -1221,14 +1288,41 @@ int amm_time_product(amm_ctx* c, int32_t kernel, uint32_t cc, uint64_t seed, uin
   a.P2s = c->P2s;
   g.nt = (c->list_nt & 1u) != 0;
   a.nt = (c->list_nt & 2u) != 0;
+  // a sampled product's second contraction as a run launches it: with the
+  // finish fused in where that form applies
+  const bool fused = !exact && fused_finish(c, false);
+  amm::AxpyFinishArgs f{};
+  if (fused) {
+    f.A = c->A;
+    f.ld = c->ld;
+    f.d = (uint32_t)c->d;
+    f.dpad = c->dpad;
+    f.idx = c->idx;
+    f.wgt = c->wgt;
+    f.seg_count = c->segc;
+    f.cap = cc;
+    f.P1 = c->P1;
+    f.R = g.R;
+    f.alpha = a.alpha;
+    f.st = c->st;
+    f.x = c->x;
+    f.y = c->y;
+    f.ym = c->ym;
+    f.v = c->v;
+    f.BP = c->BP;
+    f.nt = a.nt;
+  }
+  auto second = [&]() {
+    return fused ? amm::launch_axpy_finish(dtype, f, c->stream) : amm::launch_gather_axpy(dtype, a, c->stream);
+  };
   HIPCHK(amm::launch_gather_dot(dtype, g, c->stream));
-  HIPCHK(amm::launch_gather_axpy(dtype, a, c->stream));
+  HIPCHK(second());
   hipEvent_t e2 = nullptr;
   HIPCHK(hipEventCreate(&e2));
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   for (uint32_t i = 0; i < iters; ++i) HIPCHK(amm::launch_gather_dot(dtype, g, c->stream));
   HIPCHK(hipEventRecord(c->ev1, c->stream));
-  for (uint32_t i = 0; i < iters; ++i) HIPCHK(amm::launch_gather_axpy(dtype, a, c->stream));
+  for (uint32_t i = 0; i < iters; ++i) HIPCHK(second());
   HIPCHK(hipEventRecord(e2, c->stream));
   const hipError_t se = hipEventSynchronize(e2);
   float t1 = 0.0f, t2 = 0.0f;
@@ -1245,7 +1339,7 @@ int amm_time_product(amm_ctx* c, int32_t kernel, uint32_t cc, uint64_t seed, uin
     } else {
       const size_t nseg = (cc + amm::kSampleSeg - 1) / amm::kSampleSeg;
       std::vector<uint32_t> sc(nseg);
-      HIPCHK(hipMemcpy(sc.data(), c->segc, sizeof(uint32_t) * nseg, hipMemcpyDeviceToHost));
+      HIPCHK(copy_sync(c->stream, sc.data(), c->segc, sizeof(uint32_t) * nseg, hipMemcpyDeviceToHost));
       uint64_t u = 0;
       for (uint32_t v : sc) u += v;
       *columns = u;

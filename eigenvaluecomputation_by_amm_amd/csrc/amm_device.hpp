@@ -43,12 +43,14 @@ __device__ __forceinline__ bool exact_step_due(uint32_t kernel, uint64_t i, uint
 // Status bits of DevState::status.
 constexpr uint32_t kStatusNonFinite = 1u;  // a column weight was NaN or Inf
 
+// blocks of a multi-block tail (k_tail), each writing one part of s and q
+constexpr uint32_t kTailMaxBlocks = 64;
+
 struct alignas(16) DevState {
   uint64_t total[2];    // integer CDF totals: [0] plain ||a_j||^2, [1] centred
   uint64_t last_pos[2]; // last positive-weight column per CDF
   double alpha2;        // ||alpha||^2
   double scale;         // 1 / ||y|| of the latest product
-  double s, q;          // proposed: 1^T dx and z^T dx for the next product
   double cos_prev;      // y^T x / ||y|| of the latest iteration
   double sigma;         // ||A^T u|| of the last sigma_max evaluation
   uint64_t last_exact;  // last exact iteration (proposed gate)
@@ -57,7 +59,31 @@ struct alignas(16) DevState {
   uint32_t status;      // kStatus* bits, sticky until the host clears them
   uint32_t pad0;
   int64_t t_start;      // wall clock at the run's first kernel
+  // proposed: s = 1^T dx and q = z^T dx for the next product, as sq_n
+  // per-block parts (summed by sq_total); no parts means s = q = 0
+  uint32_t sq_n;
+  uint32_t pad1;
+  double sq_part[kTailMaxBlocks][2];
 };
+
+// s and q from their per-block parts, summed in a fixed order (lane i takes
+// part i; the butterfly gives every lane the same bits).  Call with the
+// whole wave active.
+__device__ __forceinline__ double2 sq_total(const DevState* st) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t n = st->sq_n;
+  double a = 0.0, b = 0.0;
+  if (lane < n) {
+    a = st->sq_part[lane][0];
+    b = st->sq_part[lane][1];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    b += __shfl_xor(b, off, 64);
+  }
+  return make_double2(a, b);
+}
 
 __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) {
   return __umulhi(a, b);

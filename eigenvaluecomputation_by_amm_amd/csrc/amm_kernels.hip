@@ -454,8 +454,7 @@ __global__ __launch_bounds__(256) void k_begin(float* __restrict__ xm, float* __
     st->t_start = wall_clock64();
     st->cos_prev = 0.0;
     st->last_exact = 0;
-    st->s = 0.0;
-    st->q = 0.0;
+    st->sq_n = 0;
     st->status = 0;
     if (mode0) mode[0] = mode0;
   }
@@ -503,7 +502,7 @@ __device__ __forceinline__ bool gate_open(const uint32_t* mode, uint32_t it, uin
 // LIST: col(e) = idx[e], valid when (e % seg) < seg_count[e / seg];
 // dense: col(e) = e (exact step, sanity mode).
 // ---------------------------------------------------------------------------
-template <typename T, int CPW, bool LIST, bool NT>
+template <typename T, int CPW, int U, bool LIST, bool NT>
 __global__ __launch_bounds__(256) void k_gather_dot(
     const T* __restrict__ A, uint64_t ld, uint32_t d, const uint32_t* __restrict__ idx,
     const uint32_t* __restrict__ seg_count, uint32_t cap, const float* __restrict__ x,
@@ -538,26 +537,26 @@ __global__ __launch_bounds__(256) void k_gather_dot(
   for (int c = 0; c < CPW; ++c) acc[c] = 0.0f;
   uint32_t row = rb + lane * V;
   constexpr uint32_t STEP = 64u * V;
-  // two wave-steps per trip: all loads of both issued before the FMAs
-  for (; row + STEP + V <= re; row += 2 * STEP) {
-    float x0[V], x1[V], m0[V], m1[V];
-    load_vecf<V>(x + row, x0);
-    load_vecf<V>(x + row + STEP, x1);
-    if (xm) {
-      load_vecf<V>(xm + row, m0);
-      load_vecf<V>(xm + row + STEP, m1);
+  // U wave-steps per trip: all loads of the trip issued before the FMAs
+  for (; row + (U - 1) * STEP + V <= re; row += U * STEP) {
+    float xs[U][V];
 #pragma unroll
-      for (int k = 0; k < V; ++k) {
-        x0[k] -= m0[k];
-        x1[k] -= m1[k];
+    for (int u = 0; u < U; ++u) load_vecf<V>(x + row + u * STEP, xs[u]);
+    if (xm) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        float m[V];
+        load_vecf<V>(xm + row + u * STEP, m);
+#pragma unroll
+        for (int k = 0; k < V; ++k) xs[u][k] -= m[k];
       }
     }
-    float a0[CPW][V], a1[CPW][V];
+    float a[CPW][U][V];
 #pragma unroll
     for (int c = 0; c < CPW; ++c) {
       if (valid[c]) {
-        Elem<T>::template load_full<NT>(col[c] + row, a0[c]);
-        Elem<T>::template load_full<NT>(col[c] + row + STEP, a1[c]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) Elem<T>::template load_full<NT>(col[c] + row + u * STEP, a[c][u]);
       }
     }
 #pragma unroll
@@ -565,9 +564,10 @@ __global__ __launch_bounds__(256) void k_gather_dot(
       if (valid[c]) {
         float s0 = acc[c];
 #pragma unroll
-        for (int k = 0; k < V; ++k) s0 = fmaf(a0[c][k], x0[k], s0);
+        for (int u = 0; u < U; ++u) {
 #pragma unroll
-        for (int k = 0; k < V; ++k) s0 = fmaf(a1[c][k], x1[k], s0);
+          for (int k = 0; k < V; ++k) s0 = fmaf(a[c][u][k], xs[u][k], s0);
+        }
         acc[c] = s0;
       }
     }
@@ -632,7 +632,7 @@ __global__ __launch_bounds__(256) void k_gather_axpy(
     e1 = e1 < vend ? e1 : vend;
   }
   const uint32_t nv = e1 > e0 ? e1 - e0 : 0;
-  const double sx = alpha ? st->s : 0.0;
+  const double sx = alpha ? sq_total(st).x : 0.0;
   double corr = 0.0;
   for (uint32_t k = threadIdx.x; k < nv; k += 256) {
     const uint32_t e = e0 + k;
@@ -713,11 +713,12 @@ __global__ __launch_bounds__(256) void k_finish(
     const float* __restrict__ x, float* __restrict__ y, float* __restrict__ xm,
     float* __restrict__ ym, const float* __restrict__ v, const double* __restrict__ P2s,
     const double* __restrict__ eigvec, double* __restrict__ BP, const DevState* st,
-    const uint32_t* __restrict__ mode, uint32_t it) {
+    const uint32_t* __restrict__ mode, uint32_t it, uint32_t gate_mask) {
   __shared__ double part[16][kFinishRows];
   __shared__ double red[4];
   __shared__ double corr_l;
   const uint32_t md = mode[it];
+  if (gate_mask && !((gate_mask >> md) & 1u)) return;
   const bool prop = md == kModeAmmProposed;
   const bool memo_base = md == kModeAmmPrevious || md == kModeAmmProposed;
   const uint32_t S = memo_base ? S_amm : S_exact;
@@ -728,7 +729,8 @@ __global__ __launch_bounds__(256) void k_finish(
     if (threadIdx.x == 0) corr_l = c;
     __syncthreads();
   }
-  const double sx = prop ? st->s : 0.0, qx = prop ? st->q : 0.0;
+  const double2 sq = prop ? sq_total(st) : make_double2(0.0, 0.0);
+  const double sx = sq.x, qx = sq.y;
   const double corr = prop ? corr_l : 0.0;
   // 16 groups of 16 lanes; group g sums chunks g, g+16, ... for 64 rows
   {
@@ -752,12 +754,21 @@ __global__ __launch_bounds__(256) void k_finish(
           a3 += p[u].w;
         }
       }
-      for (; s < S; s += 16) {
-        const float4 p = *reinterpret_cast<const float4*>(P2 + (uint64_t)s * dpad + r0);
-        a0 += p.x;
-        a1 += p.y;
-        a2 += p.z;
-        a3 += p.w;
+      // the remaining (at most 15) partials: every load issued before the
+      // sums; absent ones read as +0, which leaves each sum unchanged
+      {
+        float4 p[15];
+#pragma unroll
+        for (int u = 0; u < 15; ++u)
+          p[u] = (s + 16 * u < S) ? *reinterpret_cast<const float4*>(P2 + (uint64_t)(s + 16 * u) * dpad + r0)
+                                  : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int u = 0; u < 15; ++u) {
+          a0 += p[u].x;
+          a1 += p[u].y;
+          a2 += p[u].z;
+          a3 += p[u].w;
+        }
       }
     }
     part[g][l16 * 4 + 0] = a0;
@@ -796,20 +807,188 @@ __global__ __launch_bounds__(256) void k_finish(
 }
 
 // ---------------------------------------------------------------------------
+// Second contraction of a sampled product with the finish fused in
+// (previousAMM_kernel2 / proposedAMM_kernel2, core_previousAMM.cuh:177-263,
+// core_proposedAMM.cuh:215-313, then the y half of dotProduct_and_generate_x,
+// common.h:89-154).  Workgroup b owns rows [64 b, 64 b + 64) and every entry
+// of the list, so y is complete for its rows without partials in memory:
+//   t_e = wgt_e (sum_r P1[r][e] - alpha_col s)      (chunks of kMaxChunk in LDS)
+//   group g (64 / V lanes, 16-byte loads of 64 rows) accumulates entries
+//   g, g + G, ... (G = 1024 V / 64 groups) in FP32;
+//   the groups' sums are added in FP64 in a fixed order (16 parts of G/16
+//   groups, then the parts), y = y_memo (+ s v + q - sum_e alpha_e t_e) + sum;
+//   block partials of y^T x, y^T y, y^T v* as k_finish writes them.
+// The list's column indices and t stay in LDS; A is read once per entry and
+// row, as in the chunked form, and nothing of size d x chunks is written.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kAfRows = 64;  // rows per workgroup (= kFinishRows: same block partials)
+
+template <typename T, bool NT>
+__global__ __launch_bounds__(1024) void k_axpy_finish(
+    const T* __restrict__ A, uint64_t ld, uint32_t d, uint64_t dpad,
+    const uint32_t* __restrict__ idx, const float* __restrict__ wgt,
+    const uint32_t* __restrict__ seg_count, uint32_t cap, const float* __restrict__ P1, uint32_t R,
+    const float* __restrict__ alpha, const DevState* __restrict__ st, const float* __restrict__ x,
+    float* __restrict__ y, const float* __restrict__ ym, const float* __restrict__ v,
+    const double* __restrict__ eigvec, double* __restrict__ BP, const uint32_t* __restrict__ mode,
+    uint32_t it, uint32_t gate_mask) {
+  if (!gate_open(mode, it, gate_mask)) return;
+  constexpr int V = Elem<T>::VEC;
+  constexpr uint32_t LPG = kAfRows / V;  // lanes per group: 16 (FP32), 8 (BF16)
+  constexpr uint32_t G = 1024 / LPG;     // groups: 64 (FP32), 128 (BF16)
+  constexpr uint32_t GP = G / 16;        // groups per part of the fixed-order sum
+  __shared__ float tl[kMaxChunk];
+  __shared__ uint32_t cl[kMaxChunk];
+  __shared__ float gsum[G][kAfRows];
+  __shared__ double psum[16][kAfRows];
+  __shared__ double red[16];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t g = tid / LPG, l = tid % LPG;
+  const uint64_t row64 = (uint64_t)blockIdx.x * kAfRows + l * V;
+  const bool live = row64 < dpad;
+  const uint32_t row = live ? (uint32_t)row64 : 0u;
+  const double2 sq = alpha ? sq_total(st) : make_double2(0.0, 0.0);
+  const double sx = sq.x;
+  double corr = 0.0;
+  float acc[V];
+#pragma unroll
+  for (int q = 0; q < V; ++q) acc[q] = 0.0f;
+  const T* Ar = A + row;
+  const bool full = live && row + V <= d;
+  for (uint32_t base = 0; base < cap; base += kMaxChunk) {
+    // a chunk lies inside one sample segment (kMaxChunk divides kSampleSeg)
+    const uint32_t sg = base / kSampleSeg;
+    const uint32_t vend = sg * kSampleSeg + seg_count[sg];
+    const uint32_t e1 = min(min(base + kMaxChunk, cap), vend);
+    const uint32_t nv = e1 > base ? e1 - base : 0u;
+    __syncthreads();  // the previous chunk's t and columns are consumed
+    for (uint32_t k = tid; k < nv; k += 1024) cl[k] = idx[base + k];
+    __syncthreads();
+    // the group's first 8 entries are loaded before t is formed, so the
+    // reads of A overlap the reads of the dot partials (absent entries: 0)
+    float a0[8][V];
+    if (full) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t k = g + u * G;
+        if (k < nv) {
+          Elem<T>::template load_full<NT>(Ar + (uint64_t)cl[k] * ld, a0[u]);
+        } else {
+#pragma unroll
+          for (int q = 0; q < V; ++q) a0[u][q] = 0.0f;
+        }
+      }
+    }
+    for (uint32_t k = tid; k < nv; k += 1024) {
+      const uint32_t e = base + k;
+      const uint32_t j = cl[k];
+      double rr = 0.0;
+      for (uint32_t q = 0; q < R; ++q) rr += (double)P1[(uint64_t)q * cap + e];
+      const double aj = alpha ? (double)alpha[j] : 0.0;
+      const float t = (float)((double)wgt[e] * (rr - aj * sx));
+      corr += aj * (double)t;
+      tl[k] = t;
+    }
+    __syncthreads();
+    if (full) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t k = g + u * G;
+        const float t = k < nv ? tl[k] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < V; ++q) acc[q] = fmaf(a0[u][q], t, acc[q]);
+      }
+      uint32_t k = g + 8 * G;
+      for (; k + 7 * G < nv; k += 8 * G) {
+        float a[8][V];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) Elem<T>::template load_full<NT>(Ar + (uint64_t)cl[k + u * G] * ld, a[u]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const float t = tl[k + u * G];
+#pragma unroll
+          for (int q = 0; q < V; ++q) acc[q] = fmaf(a[u][q], t, acc[q]);
+        }
+      }
+      for (; k < nv; k += G) {
+        float a[V];
+        Elem<T>::template load_full<NT>(Ar + (uint64_t)cl[k] * ld, a);
+        const float t = tl[k];
+#pragma unroll
+        for (int q = 0; q < V; ++q) acc[q] = fmaf(a[q], t, acc[q]);
+      }
+    } else if (live) {
+      for (uint32_t k = g; k < nv; k += G) {
+        float a[V];
+        load_col<T>(A + (uint64_t)cl[k] * ld, row, d, a);
+        const float t = tl[k];
+#pragma unroll
+        for (int q = 0; q < V; ++q) acc[q] = fmaf(a[q], t, acc[q]);
+      }
+    }
+  }
+  // proposed: sum_e alpha_e t_e over the whole list (every workgroup forms it)
+  if (alpha) {
+    corr = wave_sum(corr);
+    if ((tid & 63) == 0) red[tid >> 6] = corr;
+  }
+#pragma unroll
+  for (int q = 0; q < V; ++q) gsum[g][l * V + q] = acc[q];
+  __syncthreads();
+  {  // part p sums groups [p GP, (p + 1) GP) for row r, in order
+    const uint32_t r = tid % kAfRows, p = tid / kAfRows;
+    double a = 0.0;
+#pragma unroll
+    for (uint32_t k = 0; k < GP; ++k) a += (double)gsum[p * GP + k][r];
+    psum[p][r] = a;
+  }
+  __syncthreads();
+  if (tid >= 64) return;  // wave 0 finishes its 64 rows
+  double corr_all = 0.0;
+  if (alpha) {
+#pragma unroll
+    for (int w = 0; w < 16; ++w) corr_all += red[w];
+  }
+  const uint64_t r = (uint64_t)blockIdx.x * kAfRows + tid;
+  double yx = 0.0, yy = 0.0, yv = 0.0;
+  if (r < dpad) {
+    double a = (double)ym[r];
+    if (alpha) a += sx * (double)v[r] + sq.y - corr_all;
+#pragma unroll
+    for (int p = 0; p < 16; ++p) a += psum[p][tid];
+    const float yf = r < d ? (float)a : 0.0f;
+    const double yd = yf;
+    yx = yd * (double)x[r];
+    yy = yd * yd;
+    if (eigvec && r < d) yv = yd * eigvec[r];
+    y[r] = yf;
+  }
+  yx = wave_sum(yx);
+  yy = wave_sum(yy);
+  yv = wave_sum(yv);
+  if (tid == 0) {
+    BP[3 * blockIdx.x + 0] = yx;
+    BP[3 * blockIdx.x + 1] = yy;
+    BP[3 * blockIdx.x + 2] = yv;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Tail of every grid-wide reduction: sums the block partials in a fixed
 // order and updates the scalar state.  A separate launch (rather than a
 // last-block-done epilogue) keeps every block free of the device-scope
-// fences that cross-XCD visibility would need.  With normalisation and no
-// s, q dots it runs one block per 1024 rows, each repeating the reduction
-// (a few KB of partials) so that x = y / ||y|| needs no second launch;
-// block 0 writes the scalars.
+// fences that cross-XCD visibility would need.  With normalisation it runs
+// one block per 1024 rows (at most kTailMaxBlocks), each repeating the
+// reduction (a few KB of partials) so that x = y / ||y|| needs no second
+// launch; block 0 writes the scalars, and with z every block writes its
+// part of s and q (DevState::sq_part, summed by the next product).
 //   kTailFinish: lambda_i = y^T x, err_i = y^T v* / ||y||, cos_i, 1/||y||,
 //                last exact step, the next step kind (adaptive variance
 //                reduction), the time stamp
 //   kTailScale:  1/||y|| of the raw x0
 //   kTailSQ:     s, q of a stand-alone product or a grid normalisation
-// With norm != 0 the block then writes x = y / ||y|| for all d rows and,
-// with z given, s = 1^T (x - x_memo), q = z^T (x - x_memo).
+// With norm != 0 the blocks then write x = y / ||y|| over their rows and,
+// with z given, their parts of s = 1^T (x - x_memo), q = z^T (x - x_memo).
 // ---------------------------------------------------------------------------
 //   kTailNorm:   nothing to reduce; normalise with the stored 1/||y||
 // (TailOp is declared in amm_launch.hpp.)
@@ -876,8 +1055,9 @@ __global__ __launch_bounds__(256) void k_tail(
     a = block256_sum(a, red);
     b = block256_sum(b, red);
     if (t == 0 && lead) {
-      st->s = a;
-      st->q = b;
+      st->sq_part[0][0] = a;
+      st->sq_part[0][1] = b;
+      st->sq_n = 1;
     }
   } else if (op == kTailNorm2) {
     double s = 0.0;
@@ -909,12 +1089,13 @@ __global__ __launch_bounds__(256) void k_tail(
         }
       }
     }
-    if (z) {  // one block only (the host launches a single block when z is given)
+    if (z) {  // this block's part of s and q
       ps = block256_sum(ps, red);
       pq = block256_sum(pq, red);
-      if (t == 0 && lead) {
-        st->s = ps;
-        st->q = pq;
+      if (t == 0) {
+        st->sq_part[blockIdx.x][0] = ps;
+        st->sq_part[blockIdx.x][1] = pq;
+        if (lead) st->sq_n = gridDim.x;
       }
     }
   }
@@ -1319,21 +1500,23 @@ hipError_t launch_begin(float* xm, float* ym, uint64_t dpad, DevState* st, uint3
 // reads its columns twice in an iteration (dot, then axpy) and heavy
 // columns recur across iterations, so the list passes load non-temporally
 // only where a.nt says so.
-template <typename T, int CPW, bool LIST, bool NT>
+template <typename T, int CPW, int U, bool LIST, bool NT>
 static void gd(const GatherDotArgs& a, hipStream_t s) {
   const dim3 g((a.cap + 4 * CPW - 1) / (4 * CPW), a.R);
-  k_gather_dot<T, CPW, LIST, NT><<<g, 256, 0, s>>>((const T*)a.A, a.ld, a.d, a.idx, a.seg_count,
-                                                   a.cap, a.x, a.xm, a.rows_per_split, a.P1, a.mode,
-                                                   a.it, a.gate_mask);
+  k_gather_dot<T, CPW, U, LIST, NT><<<g, 256, 0, s>>>((const T*)a.A, a.ld, a.d, a.idx, a.seg_count,
+                                                      a.cap, a.x, a.xm, a.rows_per_split, a.P1, a.mode,
+                                                      a.it, a.gate_mask);
 }
 
 template <typename T>
 static void gd_any(const GatherDotArgs& a, hipStream_t s) {
-  // dense: 4 columns per wave (each x load serves 4 columns); measured
-  // faster than a workgroup per column or per 4 columns split by rows
-  if (!a.idx) gd<T, 4, false, true>(a, s);
-  else if (a.nt) gd<T, 2, true, true>(a, s);
-  else gd<T, 2, true, false>(a, s);
+  // dense: 4 columns per wave (each x load serves 4 columns), two steps per
+  // trip; measured faster than a workgroup per column or per 4 columns
+  // split by rows.  List: 2 columns per wave and kListDotSteps steps per
+  // trip, with the plan's splits sized so that a wave makes one trip.
+  if (!a.idx) gd<T, 4, 2, false, true>(a, s);
+  else if (a.nt) gd<T, 2, kListDotSteps, true, true>(a, s);
+  else gd<T, 2, kListDotSteps, true, false>(a, s);
 }
 
 hipError_t launch_gather_dot(int dtype, const GatherDotArgs& a, hipStream_t s) {
@@ -1369,7 +1552,28 @@ hipError_t launch_gather_axpy(int dtype, const GatherAxpyArgs& a, hipStream_t s)
 hipError_t launch_finish(const FinishArgs& a, hipStream_t s) {
   k_finish<<<grid1(a.dpad, kFinishRows), 256, 0, s>>>(a.P2, a.S_exact, a.S_amm, a.dpad, a.d, a.x,
                                                        a.y, a.xm, a.ym, a.v, a.P2s, a.eigvec, a.BP,
-                                                       a.st, a.mode, a.it);
+                                                       a.st, a.mode, a.it, a.gate_mask);
+  return hipGetLastError();
+}
+
+bool axpy_finish_applies(uint64_t dpad) { return grid1(dpad, kAfRows) >= kAfMinBlocks; }
+
+template <typename T, bool NT>
+static void af(const AxpyFinishArgs& a, hipStream_t s) {
+  k_axpy_finish<T, NT><<<grid1(a.dpad, kAfRows), 1024, 0, s>>>(
+      (const T*)a.A, a.ld, a.d, a.dpad, a.idx, a.wgt, a.seg_count, a.cap, a.P1, a.R, a.alpha, a.st, a.x,
+      a.y, a.ym, a.v, a.eigvec, a.BP, a.mode, a.it, a.gate_mask);
+}
+
+hipError_t launch_axpy_finish(int dtype, const AxpyFinishArgs& a, hipStream_t s) {
+  if (!a.idx || !a.wgt || !a.seg_count || !a.ym) return hipErrorInvalidValue;
+  if (dtype == 1) {
+    if (a.nt) af<__hip_bfloat16, true>(a, s);
+    else af<__hip_bfloat16, false>(a, s);
+  } else {
+    if (a.nt) af<float, true>(a, s);
+    else af<float, false>(a, s);
+  }
   return hipGetLastError();
 }
 
@@ -1379,11 +1583,11 @@ uint64_t normalize_blocks(uint64_t dpad) { return grid1(dpad, 1024); }
 uint64_t dots_blocks(uint64_t dpad) { return grid1(dpad, 256); }
 
 hipError_t launch_tail(const TailArgs& a, hipStream_t s) {
-  // normalisation without the s, q dots spreads over blocks of 1024 rows
-  // (each block repeats the reduction); the dots need the single block
+  // normalisation spreads over blocks of 1024 rows (each block repeats the
+  // reduction and writes its own part of the s, q dots)
   unsigned g = 1;
-  if (a.norm && !a.z && (a.op == kTailFinish || a.op == kTailScale || a.op == kTailNorm))
-    g = (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, (a.dpad + 1023) / 1024), 64);
+  if (a.norm && (a.op == kTailFinish || a.op == kTailScale || a.op == kTailNorm))
+    g = (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, (a.dpad + 1023) / 1024), kTailMaxBlocks);
   k_tail<<<g, 256, 0, s>>>(a.op, a.BP, a.nb, a.st, a.norm, a.y, a.x, a.xm, a.z, a.d, a.dpad, a.mode,
                            a.it, a.eig, a.err, a.ts, a.stamp, a.adapt_kernel, a.period, a.thr, a.step);
   return hipGetLastError();

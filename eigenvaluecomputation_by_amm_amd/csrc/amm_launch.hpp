@@ -8,6 +8,10 @@ namespace amm {
 
 struct DevState;
 
+// wave-steps (64 lanes x 16 bytes) per trip of the sampled dot pass; the
+// plan splits rows so that one wave covers one trip
+constexpr int kListDotSteps = 4;
+
 struct GatherDotArgs {
   const void* A;
   uint64_t ld;
@@ -67,7 +71,41 @@ struct FinishArgs {
   const DevState* st;
   const uint32_t* mode;
   uint32_t it;
+  uint32_t gate_mask;  // nonzero: run only when bit mode[it] is set
 };
+
+// Second contraction of a sampled product with the finish fused in
+// (k_axpy_finish): y over every row and the block partials of y^T x, y^T y,
+// y^T v* in the layout of k_finish (finish_blocks(dpad) blocks).  List mode
+// only; worthwhile where the row blocks fill the device
+// (axpy_finish_applies).
+struct AxpyFinishArgs {
+  const void* A;
+  uint64_t ld;
+  uint32_t d;
+  uint64_t dpad;
+  const uint32_t* idx;
+  const float* wgt;
+  const uint32_t* seg_count;
+  uint32_t cap;
+  const float* P1;
+  uint32_t R;
+  const float* alpha;  // proposed mean shift when non-null
+  const DevState* st;
+  const float* x;
+  float* y;
+  const float* ym;     // y_memo (the base of every sampled product)
+  const float* v;      // z - ||alpha||^2 (proposed)
+  const double* eigvec;
+  double* BP;
+  const uint32_t* mode;
+  uint32_t it;
+  uint32_t gate_mask;
+  bool nt;
+};
+constexpr uint64_t kAfMinBlocks = 128;  // row blocks of 64 rows below which the chunked form is used
+bool axpy_finish_applies(uint64_t dpad);
+hipError_t launch_axpy_finish(int dtype, const AxpyFinishArgs& a, hipStream_t s);
 
 // single-block tail of a grid reduction (k_tail)
 enum TailOp : uint32_t {

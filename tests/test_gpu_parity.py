@@ -476,6 +476,63 @@ def test_clearing_eigenvector_drops_captured_graph():
     assert np.array_equal(r2.eig, r3.eig)
 
 
+@pytest.mark.parametrize("kernel", ["previousAMM", "proposedAMM"])
+@pytest.mark.parametrize("c,bf16", [(700, False), (5000, False), (3000, True)])
+def test_square_of_A_fused_finish(kernel, c, bf16):
+    """Rows enough for the second contraction to carry the finish (d >= 8192,
+    64-row blocks): one sampled product against the FP64 product over the
+    same sample; c = 5000 spans three LDS chunks of the list."""
+    banner(f"square_of_A fused finish {kernel} c={c} bf16={bf16}")
+    AmmContext, *_ = _amm()
+    d, n = 8200, 6000
+    A = heavy_columns(d, n, seed=26) + 0.3
+    t, cols = dev_matrix(A, bf16=bf16)
+    colsf = bf16_to_f32(cols) if bf16 else cols
+    rng = np.random.default_rng(27)
+    x, xm, ym = (rng.standard_normal(d).astype(np.float32) for _ in range(3))
+    seed, it = 31, 2
+    prop = kernel == "proposedAMM"
+    with AmmContext(t) as ctx:
+        alpha, nrm2, cnrm2, a2 = ctx.column_stats()
+        y = ctx.square_of_A(kernel, torch.from_numpy(x).cuda(), c, it, seed,
+                            x_memo=torch.from_numpy(xm).cuda(), y_memo=torch.from_numpy(ym).cuda())
+        y = y.cpu().numpy()
+    idx, wgt, segc = O.sample(cnrm2 if prop else nrm2, c, seed, it)
+    A64 = colsf.T.astype(np.float64)
+    z = (A64 @ alpha.astype(np.float64)).astype(np.float32)
+    v = (z.astype(np.float64) - a2).astype(np.float32)
+    ref = O.amm_product(colsf, d, prop, c, idx, wgt, segc, False, x, xm, ym, alpha, v, z)
+    assert rel(y, ref) < 1e-5
+
+
+@pytest.mark.parametrize("run", [dict(kernel="previousAMM", c=600, step=20, vr_adaptive_threshold=0.05),
+                                 dict(kernel="proposedAMM", c=600, step=20, vr_adaptive_threshold=0.05),
+                                 dict(kernel="proposedAMM", c=3000, step=12, vr_period=4)],
+                         ids=["adaptive-previous", "adaptive-proposed", "proposed-c3000"])
+def test_fused_finish_runs_against_oracle(run):
+    """Whole runs where sampled steps carry the finish in their second
+    contraction: with adaptive variance reduction the finish kernel runs
+    only on exact steps and the fused kernel only on sampled ones (both
+    gated on the step kind the device decided).  Same samples as the
+    oracle: identical step kinds, trajectories within 1e-4."""
+    banner(f"fused finish run {run}")
+    AmmContext, RunParams, _ = _amm()
+    d, n = 8200, 4000
+    A = lowrank_plus_noise(d, n, seed=28) + 0.05
+    t, cols = dev_matrix(A)
+    with AmmContext(t) as ctx:
+        _, nrm2, cnrm2, _ = ctx.column_stats()
+        res = ctx.power_method(RunParams(seed=29, **run))
+        modes = ctx.last_modes(run["step"])
+        eager = ctx.power_method(RunParams(seed=29, use_graph=False, **run))
+    ref = O.power_method(cols, d, seed=29, w_plain=nrm2, w_cent=cnrm2, **run)
+    assert np.array_equal(modes, ref["modes"].astype(np.uint32))
+    if "vr_adaptive_threshold" in run:
+        assert (modes == 1).any() and (modes != 1).any()
+    assert np.max(np.abs(res.eig - ref["eig"]) / np.abs(ref["eig"])) < 1e-4
+    assert np.array_equal(res.eig, eager.eig)
+
+
 def test_lanczos_ground_truth_against_eigvalsh():
     """The benchmark's FP64 ground truth (workloads.lambda_max_lanczos, GPU)
     against FP64 eigvalsh of A A^T (the computation of
