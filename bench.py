@@ -414,30 +414,35 @@ def c4_section(args):
     matrix by matrix."""
     import torch
     from eigenvaluecomputation_by_amm_amd import AmmBatch, AmmContext, RunParams
-    from eigenvaluecomputation_by_amm_amd.workloads import (lambda_max_lanczos, lowrank_plus_noise_fast,
-                                                            uneven_problem_shapes)
+    from eigenvaluecomputation_by_amm_amd.workloads import (lambda_max_lanczos, lowrank_plus_noise_fast, padded_ld,
+                                                            uneven_problem_shapes, with_ld)
     shapes = uneven_problem_shapes(args.count)
     P = len(shapes)
     elems = sum(d * n for d, n in shapes)
     log(f"c4: generating {P} matrices, {elems / 1e9:.2f} G elements")
     m16, lam16, lam32 = [], [], []
     t_exact = []
+    # columns start on 512-byte boundaries (leading dimension padded): the
+    # batched contractions read 512-byte runs of each column
     for i, (d, n) in enumerate(shapes):
         M = lowrank_plus_noise_fast(d, n, seed=1000 + i)
         lam32.append(lambda_max_lanczos(M, d, k=40)[0])
-        with AmmContext(M) as ctx:  # comparator: FP32 exact power method on this matrix
+        M32 = with_ld(M, padded_ld(d, torch.float32))
+        with AmmContext(M32, d=d) as ctx:  # comparator: FP32 exact power method on this matrix
             rx = ctx.power_method(RunParams(kernel="exact", step=args.exact_iters, seed=5))
             j = settled_index(rx.eig, lam32[-1])
             t_exact.append(float(rx.time_ms[j]) if j >= 0 else float("nan"))
+        del M32
         Mb = M.to(torch.bfloat16)
         del M
         lam16.append(lambda_max_lanczos(Mb, d, k=40)[0])
-        m16.append(Mb)
+        m16.append(with_ld(Mb, padded_ld(d, torch.bfloat16)))
+        del Mb
     lam16 = np.array(lam16)
     tot_exact = float(np.nansum(t_exact))
     iters = args.batch_iters
     cfg = dict(kernel="previousAMM", vr_period=args.batch_vr)
-    probs = [dict(A=m16[i], c=max(1, n // 8), seed=77 + i) for i, (d, n) in enumerate(shapes)]
+    probs = [dict(A=m16[i], d=d, c=max(1, n // 8), seed=77 + i) for i, (d, n) in enumerate(shapes)]
 
     def settle_all(r, ntr):
         idx = np.array([[settled_index(r.eig[i, k], lam16[i]) for k in range(ntr)] for i in range(P)])
@@ -459,7 +464,7 @@ def c4_section(args):
     gbs = 2 * elems * 2 / (dense_ms * 1e-3) / 1e9
     out = {
         "workload": f"{P} matrices, d~U[512,8192], n~U[1024,32768], s_i=10/i rank-64 + 0.01 N(0,1), BF16 "
-                    "storage; 16 seeds each",
+                    "storage (leading dimension padded to 512 bytes); 16 seeds each",
         "elements": elems, "kernel": cfg["kernel"], "c": "n/8", "vr_period": cfg["vr_period"],
         "iterations": iters,
         "accounting": "per_solve = per (matrix, seed), 16 x matrices solves; per_matrix = run time / matrices",

@@ -14,15 +14,15 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr uint32_t kMaxSlicesDev = 16;  // contraction-1 row slices at most
 
-// largest p with off[p] <= b (off is non-decreasing, off[0] = 0, b < off[np])
+// Problem of workgroup b of a grouped launch: the group's offsets
+// off[0 .. np] (off[p] = first workgroup of problem p) are followed by a
+// map off[np + 1 + b] = problem of workgroup b, so a workgroup finds its
+// problem with one load (a binary search over the offsets made each
+// workgroup wait on about log2(np) dependent loads before its first read
+// of A; with 256 problems that was a sizeable share of a short workgroup).
 __device__ __forceinline__ uint32_t find_problem(const uint32_t* __restrict__ off, uint32_t np,
                                                  uint32_t b) {
-  uint32_t lo = 0, hi = np;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (off[mid] <= b) lo = mid; else hi = mid;
-  }
-  return lo;
+  return off[np + 1 + b];
 }
 
 // sum over the 16 lanes of a 16-lane group (groups aligned to 16 lanes)
@@ -213,17 +213,23 @@ __global__ __launch_bounds__(256) void k_bnorm(const BProb* __restrict__ P, cons
 // slot i of trial k takes the 64-bit uniform of Philox counter (i >> 1, it,
 // trial0 + k, sample stream), words 2(i & 1), 2(i & 1) + 1, the integer
 // target floor(u * total / 2^64) and column min{j : cdf_j > target}, then
-// counts the draw in cnt[j][k].  Items: trials x ceil(c / 512) per problem.
-// The first block of a proposed trial also finishes s, q of its product.
+// counts the draw in cnt[j][k].  Units: (trial, part of kBSampleSlots
+// slots), ntr ceil(c / kBSampleSlots) per problem; a workgroup takes `upb`
+// consecutive units of one problem (its start, a chain of dependent loads,
+// and the coarse table, shared by every unit on the same CDF, are paid once
+// per workgroup rather than per unit).  The part-0 unit of a proposed
+// trial also finishes s, q of its product.
 // ---------------------------------------------------------------------------
-// Open-addressing count table in LDS: key = column + 1 (0 = empty).  A block
-// draws at most 512 columns, so 1024 entries stay at most half full.
-constexpr uint32_t kSlotsPerBlock = 512;
-constexpr uint32_t kHash = 1024;
+// Open-addressing count table in LDS: key = column + 1 (0 = empty).  A unit
+// draws at most kBSampleSlots = 2048 columns (two per thread), so 4096
+// entries stay at most half full; the flush after each unit clears it.
+constexpr uint32_t kSlotsPerBlock = kBSampleSlots;
+constexpr uint32_t kHash = 2 * kBSampleSlots;
+constexpr uint32_t kSampleThreads = kBSampleSlots / 2;
 
 __device__ __forceinline__ void hash_count(uint32_t* hkey, uint32_t* hcnt, uint32_t j) {
   const uint32_t key = j + 1u;
-  uint32_t h = (j * 2654435761u) >> 22;  // top 10 bits: [0, 1024)
+  uint32_t h = (j * 2654435761u) >> 20;  // top 12 bits: [0, 4096)
   while (true) {
     const uint32_t old = atomicCAS(&hkey[h], 0u, key);
     if (old == 0u || old == key) {
@@ -234,9 +240,9 @@ __device__ __forceinline__ void hash_count(uint32_t* hkey, uint32_t* hcnt, uint3
   }
 }
 
-__global__ __launch_bounds__(256) void k_bsample(const BProb* __restrict__ P, const uint32_t* off,
-                                                 BRun r, uint32_t it) {
-  __shared__ double red[4];
+__global__ __launch_bounds__(kSampleThreads) void k_bsample(const BProb* __restrict__ P, const uint32_t* off,
+                                                            BRun r, uint32_t it, uint32_t upb) {
+  __shared__ double red[2][kSampleThreads / 64];
   __shared__ uint64_t cs[kCoarse];
   __shared__ uint32_t hkey[kHash];
   __shared__ uint32_t hcnt[kHash];
@@ -244,52 +250,83 @@ __global__ __launch_bounds__(256) void k_bsample(const BProb* __restrict__ P, co
   const BProb& pr = P[p];
   const uint32_t local = blockIdx.x - off[p];
   const uint32_t per = (pr.c + kSlotsPerBlock - 1) / kSlotsPerBlock;
-  const uint32_t k = local / per, part = local % per;
-  BTrial& t = pr.bt[k];
-  const uint32_t md = t.mode;
-  if (!is_amm(md)) return;
-  const int which = md == kModeAmmProposed ? 1 : 0;
-  if (which && part == 0) {
-    double s = 0.0, q = 0.0;
-    for (uint32_t blk = threadIdx.x; blk < pr.nvblk; blk += 256) {
-      const double* b = pr.BQ + ((uint64_t)blk * kBT + k) * 2;
-      s += b[0];
-      q += b[1];
-    }
-    s = block256_sum(s, red);
-    q = block256_sum(q, red);
-    if (threadIdx.x == 0) {
-      t.s = s;
-      t.q = q;
-    }
-  }
-  const uint64_t total = pr.st->total[which];
-  const uint64_t last_pos = pr.st->last_pos[which];
-  const uint64_t* cdf = which ? pr.cdf1 : pr.cdf0;
-  coarse_fill(cs, cdf, pr.n, threadIdx.x, 256);
-  for (uint32_t e = threadIdx.x; e < kHash; e += 256) {
+  const uint32_t units = r.ntr * per;
+  const uint32_t u0 = local * upb;
+  const uint32_t u1 = min(u0 + upb, units);
+  const uint32_t k0 = (uint32_t)pr.seed, k1 = (uint32_t)(pr.seed >> 32);
+  for (uint32_t e = threadIdx.x; e < kHash; e += kSampleThreads) {
     hkey[e] = 0u;
     hcnt[e] = 0u;
   }
-  __syncthreads();
-  const uint32_t k0 = (uint32_t)pr.seed, k1 = (uint32_t)(pr.seed >> 32);
-  {
-    const uint32_t pair = part * (kSlotsPerBlock / 2) + threadIdx.x;
-    const uint32_t slot = 2 * pair;
-    if (slot < pr.c) {
-      uint32_t q[4];
-      philox4x32_10(pair, it, r.trial0 + k, kStreamSample, k0, k1, q);
-      hash_count(hkey, hcnt, bcdf_search(cdf, cs, pr.n, draw_target(q[0], q[1], total), last_pos));
-      if (slot + 1 < pr.c)
-        hash_count(hkey, hcnt, bcdf_search(cdf, cs, pr.n, draw_target(q[2], q[3], total), last_pos));
+  int built = -1;  // the CDF whose coarse table cs holds
+  for (uint32_t u = u0; u < u1; ++u) {  // uniform over the workgroup
+    const uint32_t k = u / per, part = u % per;
+    BTrial& t = pr.bt[k];
+    const uint32_t md = t.mode;
+    if (!is_amm(md)) continue;
+    const int which = md == kModeAmmProposed ? 1 : 0;
+    if (which && part == 0) {
+      double sv = 0.0, qv = 0.0;
+      for (uint32_t blk = threadIdx.x; blk < pr.nvblk; blk += kSampleThreads) {
+        const double* bq = pr.BQ + ((uint64_t)blk * kBT + k) * 2;
+        sv += bq[0];
+        qv += bq[1];
+      }
+      sv = wave_sum(sv);
+      qv = wave_sum(qv);
+      if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = sv;
+        red[1][threadIdx.x >> 6] = qv;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double ts = 0.0, tq = 0.0;
+        for (uint32_t w = 0; w < kSampleThreads / 64; ++w) {
+          ts += red[0][w];
+          tq += red[1][w];
+        }
+        t.s = ts;
+        t.q = tq;
+      }
     }
-  }
-  __syncthreads();
-  // one global update per distinct column of this block (heavy columns
-  // would otherwise serialise on one address)
-  for (uint32_t e = threadIdx.x; e < kHash; e += 256) {
-    const uint32_t key = hkey[e];
-    if (key) atomicAdd(pr.cnt + (uint64_t)(key - 1u) * kBT + k, hcnt[e]);
+    const uint64_t* cdf = which ? pr.cdf1 : pr.cdf0;
+    if (which != built) {
+      __syncthreads();  // no thread still reads the previous table
+      coarse_fill(cs, cdf, pr.n, threadIdx.x, kSampleThreads);
+      built = which;
+    }
+    const uint64_t total = pr.st->total[which];
+    const uint64_t last_pos = pr.st->last_pos[which];
+    __syncthreads();  // the coarse table and the cleared count table
+    {
+      const uint32_t pair = part * (kSlotsPerBlock / 2) + threadIdx.x;
+      const uint32_t slot = 2 * pair;
+      if (slot < pr.c) {
+        uint32_t q[4];
+        philox4x32_10(pair, it, r.trial0 + k, kStreamSample, k0, k1, q);
+        hash_count(hkey, hcnt, bcdf_search(cdf, cs, pr.n, draw_target(q[0], q[1], total), last_pos));
+        if (slot + 1 < pr.c)
+          hash_count(hkey, hcnt, bcdf_search(cdf, cs, pr.n, draw_target(q[2], q[3], total), last_pos));
+      }
+    }
+    __syncthreads();
+    // one global update per distinct column of this unit (heavy columns
+    // would otherwise serialise on one address): a plain store where the
+    // trial's draws are one unit (the counts start at zero and no other
+    // workgroup adds to them), else an atomic add; the table is cleared
+    // for the next unit
+    for (uint32_t e = threadIdx.x; e < kHash; e += kSampleThreads) {
+      const uint32_t key = hkey[e];
+      if (key) {
+        uint32_t* cp = pr.cnt + (uint64_t)(key - 1u) * kBT + k;
+        if (per == 1u)
+          *cp = hcnt[e];
+        else
+          atomicAdd(cp, hcnt[e]);
+        hkey[e] = 0u;
+        hcnt[e] = 0u;
+      }
+    }
   }
 }
 
@@ -297,83 +334,45 @@ __global__ __launch_bounds__(256) void k_bsample(const BProb* __restrict__ P, co
 // Union of the trials' samples, tile by tile of 64 columns: column j is
 // present when a sampled trial drew it or a trial takes an exact step.  A
 // wave compacts the present columns of a tile (ballot + prefix count) into
-// list[tile*64 ...], writes their weights W[slot][k] -- mult * total /
-// (c * q_j) for a sampled trial (update_weights, common.h:42-53, as
-// k_sample), 1 for an exact trial, 0 otherwise -- and clears the counts
-// that were drawn.  The problem's waves take tiles w, w + waves, ...
+// list[tile*64 ...].  The per-trial weights are formed later from the same
+// draw counts (k_bweight), where the products need them; the counts are
+// cleared there too.  The problem's waves take tiles w, w + waves, ...
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bprep(const BProb* __restrict__ P, const uint32_t* off,
                                                BRun r) {
-  __shared__ uint32_t mode_s[kBT];
-  __shared__ uint64_t total_s[2];
+  __shared__ uint32_t exact_any;
   const uint32_t p = find_problem(off, r.np, blockIdx.x);
   const BProb& pr = P[p];
-  if (threadIdx.x < kBT) mode_s[threadIdx.x] = threadIdx.x < r.ntr ? pr.bt[threadIdx.x].mode : 0u;
-  if (threadIdx.x < 2) total_s[threadIdx.x] = pr.st->total[threadIdx.x];
+  if (threadIdx.x == 0) exact_any = 0u;
   __syncthreads();
+  if (threadIdx.x < r.ntr) {
+    const uint32_t md = pr.bt[threadIdx.x].mode;
+    if (md == kModeExactMemo || md == kModeExactNoMemo) exact_any = 1u;
+  }
+  __syncthreads();
+  const bool all_present = exact_any != 0u;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t nwaves = (off[p + 1] - off[p]) * 4;
   // wave-uniform loop: each wave takes tiles w, w + nwaves, ...
   for (uint32_t tile = (blockIdx.x - off[p]) * 4 + (threadIdx.x >> 6); tile < pr.ntiles; tile += nwaves) {
     const uint32_t j = tile * kBTile + lane;
-    const bool inb = j < pr.n;
-    uint32_t cnt[kBT];
-    uint4* cp = reinterpret_cast<uint4*>(pr.cnt + (uint64_t)j * kBT);
-    bool drawn = false;
-    if (inb) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const uint4 c4 = cp[v];
-        cnt[4 * v] = c4.x;
-        cnt[4 * v + 1] = c4.y;
-        cnt[4 * v + 2] = c4.z;
-        cnt[4 * v + 3] = c4.w;
-        drawn |= (c4.x | c4.y | c4.z | c4.w) != 0u;
-      }
-    } else {
-#pragma unroll
-      for (int v = 0; v < (int)kBT; ++v) cnt[v] = 0;
-    }
-    float w[kBT];
     bool present = false;
-    uint64_t q0 = 0, q1 = 0;
-    bool have_q = false;
+    if (j < pr.n) {
+      if (all_present) {
+        present = true;
+      } else {
+        const uint4* cp = reinterpret_cast<const uint4*>(pr.cnt + (uint64_t)j * kBT);
 #pragma unroll
-    for (int k = 0; k < (int)kBT; ++k) {
-      w[k] = 0.0f;
-      if ((uint32_t)k >= r.ntr || !inb) continue;
-      const uint32_t md = mode_s[k];
-      if (md == kModeExactMemo || md == kModeExactNoMemo) {
-        w[k] = 1.0f;
-        present = true;
-      } else if (cnt[k]) {
-        if (!have_q) {
-          q0 = pr.cdf0 ? pr.cdf0[j] - (j ? pr.cdf0[j - 1] : 0ull) : 0ull;
-          q1 = pr.cdf1 ? pr.cdf1[j] - (j ? pr.cdf1[j - 1] : 0ull) : 0ull;
-          have_q = true;
+        for (int v = 0; v < 4; ++v) {
+          const uint4 c4 = cp[v];
+          present |= (c4.x | c4.y | c4.z | c4.w) != 0u;
         }
-        const int which = md == kModeAmmProposed ? 1 : 0;
-        const uint64_t qj = which ? q1 : q0;
-        const double num = (double)cnt[k] * (double)total_s[which];
-        const double den = (double)pr.c * (double)qj;
-        w[k] = den > 0.0 ? (float)(num / den) : 0.0f;
-        present = true;
       }
     }
     const uint64_t mask = __ballot(present);
     const uint32_t pos = __popcll(mask & ((1ull << lane) - 1ull));
-    if (present) {
-      const uint64_t slot = (uint64_t)tile * kBTile + pos;
-      pr.list[slot] = j;
-      float4* wp = reinterpret_cast<float4*>(pr.W + slot * kBT);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) wp[v] = make_float4(w[4 * v], w[4 * v + 1], w[4 * v + 2], w[4 * v + 3]);
-    }
+    if (present) pr.list[(uint64_t)tile * kBTile + pos] = j;
     if (lane == 0) pr.tcount[tile] = (uint32_t)__popcll(mask);
-    if (drawn) {  // the counts start the next draw at zero; untouched ones already are
-#pragma unroll
-      for (int v = 0; v < 4; ++v) cp[v] = make_uint4(0u, 0u, 0u, 0u);
-    }
   }
 }
 
@@ -430,8 +429,8 @@ __global__ __launch_bounds__(1024) void k_bscan(const BProb* __restrict__ P) {
 }
 
 // ---------------------------------------------------------------------------
-// Compacted union: tile t's present columns and weights move to union slots
-// tpre[t] ... (the tiles of k_bprep's launch: waves take tiles w, w + waves, ...).
+// Compacted union: tile t's present columns move to union slots tpre[t] ...
+// (the tiles of k_bprep's launch: waves take tiles w, w + waves, ...).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bcompact(const BProb* __restrict__ P, const uint32_t* off,
                                                   uint32_t np) {
@@ -443,11 +442,6 @@ __global__ __launch_bounds__(256) void k_bcompact(const BProb* __restrict__ P, c
   for (uint32_t tile = (blockIdx.x - off[p]) * 4 + (threadIdx.x >> 6); tile < pr.ntiles; tile += nwaves) {
     const uint32_t cnt = pr.tcount[tile];
     const uint32_t base = pr.tpre[tile];
-    // 64 slots x 16 weights: lane moves slot (i >> 4)'s weight (i & 15)
-    for (uint32_t i = lane; i < cnt * kBT; i += 64) {
-      const uint32_t slot = i >> 4;
-      pr.Wc[(uint64_t)(base + slot) * kBT + (i & 15)] = pr.W[((uint64_t)tile * kBTile + slot) * kBT + (i & 15)];
-    }
     if (lane < cnt) pr.colc[base + lane] = pr.list[(uint64_t)tile * kBTile + lane];
   }
 }
@@ -615,35 +609,117 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
 // ---------------------------------------------------------------------------
 // Weights of the current product, once per union slot (previousAMM_kernel1 /
 // proposedAMM_kernel1 epilogue, core_previousAMM.cuh:4-175,
-// core_proposedAMM.cuh:17-213): Tc[s][k] = W[s][k] (sum_ks P1[ks][s][k] -
-// alpha_j s_k), s_k = 1^T dx_k for a proposed trial, else 0.  Thread
-// (slot of 16, trial); the problem's blocks stride over the union.
+// core_proposedAMM.cuh:17-213): Tc[s][k] = w_{s,k} (sum_ks P1[ks][s][k] -
+// alpha_j s_k), s_k = 1^T dx_k for a proposed trial, else 0, with w formed
+// from the draw counts as update_weights (common.h:42-53) and k_sample form
+// it: 1 on an exact step, mult total / (c q_j) for a draw count mult > 0 on
+// a sampled step (q_j = cdf_j - cdf_{j-1} of the step's CDF), else 0; the
+// counts are cleared.  (Forming t in
+// k_bdot's epilogue where it has one row slice measured slower: its reads
+// and writes compete with the stream of A.)  With out_w,
+// Tc[s][k] = w_{s,k} for every problem instead (the union sample and its
+// weights, for amm_batch_sample).  A thread per union slot, all 16 trials
+// (16-byte accesses of the counts, partials and t; 64 slots in flight per
+// wave); the problem's blocks stride over the union.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bweight(const BProb* __restrict__ P, const uint32_t* off,
-                                                 BRun r, uint32_t proposed_any) {
+                                                 BRun r, uint32_t proposed_any, uint32_t out_w) {
+  __shared__ double s_sk[kBT];
   const uint32_t p = find_problem(off, r.np, blockIdx.x);
   const BProb& pr = P[p];
   const uint32_t local = blockIdx.x - off[p];
   const uint32_t nblk = off[p + 1] - off[p];
-  const uint32_t k = threadIdx.x & 15;
   const UPlan up = union_plan(pr, pr.tpre[pr.ntiles]);
-  double sk = 0.0;
-  if (proposed_any && pr.bt[k].mode == kModeAmmProposed) sk = pr.bt[k].s;
   const uint64_t slab = (uint64_t)pr.ntiles * kBTile * kBT;
-  for (uint32_t s = local * 16 + (threadIdx.x >> 4); s < up.U; s += nblk * 16) {
-    const uint64_t e = (uint64_t)s * kBT + k;
-    if (k >= r.ntr) {  // unused trial slot: its weight is 0
-      pr.Tc[e] = 0.0f;
-      continue;
+  const uint32_t* __restrict__ colc = pr.colc;
+  uint32_t* __restrict__ cnt = pr.cnt;
+  const float* __restrict__ P1 = pr.P1;
+  float* __restrict__ Tc = pr.Tc;
+  const uint32_t ntr = r.ntr, ks = up.kslices;
+  // per-trial step kinds as bit masks (uniform over the workgroup); s_k
+  // (0 unless a proposed product) in LDS
+  uint32_t exact_m = 0u, prop_m = 0u;
+#pragma unroll
+  for (uint32_t k = 0; k < kBT; ++k) {
+    const uint32_t md = k < ntr ? pr.bt[k].mode : kModeExactMemo;
+    exact_m |= (md == kModeExactMemo || md == kModeExactNoMemo ? 1u : 0u) << k;
+    prop_m |= (md == kModeAmmProposed ? 1u : 0u) << k;
+  }
+  const uint32_t live_m = ntr >= 32 ? 0xFFFFFFFFu : (1u << ntr) - 1u;
+  const uint32_t need0 = live_m & ~exact_m & ~prop_m, need1 = live_m & ~exact_m & prop_m;
+  const bool use_sk = !out_w && proposed_any && (live_m & prop_m) != 0u;
+  if (threadIdx.x < kBT)
+    s_sk[threadIdx.x] = (use_sk && ((live_m & prop_m) >> threadIdx.x) & 1u) ? pr.bt[threadIdx.x].s : 0.0;
+  __syncthreads();
+  const double tot0 = need0 && pr.st ? pr.st->total[0] : 0.0;
+  const double tot1 = need1 && pr.st ? pr.st->total[1] : 0.0;
+  const double cd = (double)pr.c;
+  const uint32_t nq = (ntr + 3) / 4;  // 4-trial groups holding live trials
+  // the column of the next slot is loaded one iteration ahead (one
+  // dependent latency per slot instead of two)
+  const uint32_t stride = nblk * 256;
+  uint32_t sl = local * 256 + threadIdx.x;
+  uint32_t jn = sl < up.U ? colc[sl] : 0u;
+  for (; sl < up.U; sl += stride) {
+    const uint32_t j = jn;
+    if (sl + stride < up.U) jn = colc[sl + stride];
+    const uint64_t base = (uint64_t)sl * kBT;
+    f32x4_t pv[kBT / 4];  // the first slice's dots
+    if (!out_w) {
+#pragma unroll
+      for (uint32_t g = 0; g < kBT / 4; ++g) pv[g] = *reinterpret_cast<const f32x4_t*>(P1 + base + 4 * g);
     }
-    float part[kMaxSlicesDev];
+    uint32_t* cj = cnt + (uint64_t)j * kBT;
+    u32x4 m[kBT / 4];
 #pragma unroll
-    for (uint32_t q = 0; q < kMaxSlicesDev; ++q) part[q] = q < up.kslices ? pr.P1[q * slab + e] : 0.0f;
-    double a1 = 0.0;
+    for (uint32_t g = 0; g < kBT / 4; ++g) m[g] = g < nq ? *reinterpret_cast<const u32x4*>(cj + 4 * g) : u32x4{0u, 0u, 0u, 0u};
+    // q_j of each CDF a sampled trial uses
+    uint64_t q0 = 0ull, q1 = 0ull;
+    if (need0 && pr.cdf0) q0 = pr.cdf0[j] - (j ? pr.cdf0[j - 1] : 0ull);
+    if (need1 && pr.cdf1) q1 = pr.cdf1[j] - (j ? pr.cdf1[j - 1] : 0ull);
+    const double aj = use_sk ? (double)pr.alpha[j] : 0.0;
 #pragma unroll
-    for (uint32_t q = 0; q < kMaxSlicesDev; ++q) a1 += (double)part[q];
-    const double aj = sk != 0.0 ? (double)pr.alpha[pr.colc[s]] : 0.0;
-    pr.Tc[e] = (float)((double)pr.Wc[e] * (a1 - aj * sk));
+    for (uint32_t g = 0; g < kBT / 4; ++g) {
+      // clear the counts used (the next draw starts at 0)
+      if (g < nq && (m[g][0] | m[g][1] | m[g][2] | m[g][3]) != 0u)
+        *reinterpret_cast<u32x4*>(cj + 4 * g) = u32x4{0u, 0u, 0u, 0u};
+      double acc[4];
+      if (!out_w) {
+        // p = sum over the row slices of the dots, in slice order
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e) acc[e] = 0.0 + (double)pv[g][e];
+        for (uint32_t q = 1; q < ks; ++q) {
+          const f32x4_t v = *reinterpret_cast<const f32x4_t*>(P1 + q * slab + base + 4 * g);
+#pragma unroll
+          for (uint32_t e = 0; e < 4; ++e) acc[e] += (double)v[e];
+        }
+      }
+      f32x4_t t;
+#pragma unroll
+      for (uint32_t e = 0; e < 4; ++e) {
+        const uint32_t k = 4 * g + e;
+        const uint32_t mult = m[g][e];
+        // weight: 1 on an exact step, mult total / (c q_j) on a sampled one
+        float w = 0.0f;
+        if ((exact_m >> k) & 1u) {
+          w = 1.0f;
+        } else if (mult) {
+          const bool pp = (prop_m >> k) & 1u;
+          const double num = (double)mult * (pp ? tot1 : tot0);
+          const double den = cd * (double)(pp ? q1 : q0);
+          w = den > 0.0 ? (float)(num / den) : 0.0f;
+        }
+        if (!((live_m >> k) & 1u)) w = 0.0f;
+        if (!out_w) {
+          // t = w (p - alpha_j s_k)
+          const double skk = s_sk[k];
+          const double ajs = skk != 0.0 ? aj : 0.0;
+          w = ((live_m >> k) & 1u) ? (float)((double)w * (acc[e] - ajs * skk)) : 0.0f;
+        }
+        t[e] = w;
+      }
+      *reinterpret_cast<f32x4_t*>(Tc + base + 4 * g) = t;
+    }
   }
 }
 
@@ -821,126 +897,135 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
 // sum_ch P2[ch][k] with base 0 (exact), y_memo (previous) or y_memo + s v + q
 // - sum_ch P2s[ch][k] (proposed); block partials of y^T x, y^T y, y^T v*;
 // exact steps with memo snapshot [x, y] -> [x_memo, y_memo].
-// Items: nvblk per problem; thread (k, 4 rows).
+// Workgroups stride over the problem's 64-row slabs (a workgroup's start is
+// a chain of dependent loads that one slab would not amortise); block
+// partials stay per slab.  Thread (k, 4 rows).  With G > 1, G wave groups
+// of 256 threads split the chunk partials into G contiguous ranges (more
+// loads in flight where a problem has many chunks); group 0 adds the ranges
+// in group order through LDS and finishes.
 // ---------------------------------------------------------------------------
-// Four wave groups of 256 threads split the chunk partials into four
-// contiguous ranges (more loads in flight where a problem has few row
-// blocks); group 0 adds the ranges in group order through LDS and finishes.
-constexpr uint32_t kFinGroups = 4;
-
-__global__ __launch_bounds__(256 * kFinGroups) void k_bfinish(const BProb* __restrict__ P, const uint32_t* off,
-                                                              BRun r) {
-  __shared__ double red[kFinGroups - 1][256][5];
+template <uint32_t G>
+__global__ __launch_bounds__(256 * G) void k_bfinish(const BProb* __restrict__ P, const uint32_t* off, BRun r) {
+  __shared__ double red[G > 1 ? G - 1 : 1][256][5];
   const uint32_t p = find_problem(off, r.np, blockIdx.x);
   const BProb& pr = P[p];
-  const uint32_t blk = blockIdx.x - off[p];
+  const uint32_t local = blockIdx.x - off[p];
+  const uint32_t nblk = off[p + 1] - off[p];
   const uint32_t grp = threadIdx.x >> 8, lt = threadIdx.x & 255;
   const uint32_t k = lt >> 4;
-  const uint32_t md = k < r.ntr ? pr.bt[k].mode : kModeExactNoMemo;
+  const bool live = k < r.ntr;
+  const uint32_t md = live ? pr.bt[k].mode : kModeExactNoMemo;
   const bool memo_base = is_amm(md);
   const bool prop = md == kModeAmmProposed;
   const uint32_t active = union_plan(pr, pr.tpre[pr.ntiles]).active;
-  const uint32_t c0 = (uint32_t)(((uint64_t)active * grp) / kFinGroups);
-  const uint32_t c1 = (uint32_t)(((uint64_t)active * (grp + 1)) / kFinGroups);
-  const uint32_t row = blk * kBVecRows + (lt & 15) * 4;
-  static_assert(kBVecRows == 64, "one 64-row slab per block");
+  const uint32_t c0 = (uint32_t)(((uint64_t)active * grp) / G);
+  const uint32_t c1 = (uint32_t)(((uint64_t)active * (grp + 1)) / G);
+  const float* __restrict__ P2 = pr.P2;
+  const uint64_t pitch = pr.p2pitch;
+  const uint32_t d = pr.d, nvblk = pr.nvblk;
+  static_assert(kBVecRows == 64, "one 64-row slab per item");
+  // the proposed correction sum_ch P2s[ch][k] is the same for every slab
   double corr = 0.0;
-  double a[4] = {0.0, 0.0, 0.0, 0.0};
-  if (k < r.ntr) {
-    if (prop) {
-      uint32_t ch = c0;
-      for (; ch + 4 <= c1; ch += 4) {
-        double c4[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) c4[u] = pr.P2s[(uint64_t)(ch + u) * kBT + k];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) corr += c4[u];
-      }
-      for (; ch < c1; ++ch) corr += pr.P2s[(uint64_t)ch * kBT + k];
-    }
-    // batches of 4 chunk partials, loads first (a loop with a bound per
-    // element gets one load in flight at a time); sums in chunk order
-    uint32_t ch = c0;
-    for (; ch + 4 <= c1; ch += 4) {
-      float4 q[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        q[u] = *reinterpret_cast<const float4*>(pr.P2 + ((uint64_t)(ch + u) * kBT + k) * pr.p2pitch + row);
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        a[0] += q[u].x;
-        a[1] += q[u].y;
-        a[2] += q[u].z;
-        a[3] += q[u].w;
-      }
-    }
-    for (; ch < c1; ++ch) {
-      const float4 q = *reinterpret_cast<const float4*>(pr.P2 + ((uint64_t)ch * kBT + k) * pr.p2pitch + row);
-      a[0] += q.x;
-      a[1] += q.y;
-      a[2] += q.z;
-      a[3] += q.w;
-    }
-  }
-  if (grp > 0) {
-    double* s = red[grp - 1][lt];
-    s[0] = a[0];
-    s[1] = a[1];
-    s[2] = a[2];
-    s[3] = a[3];
-    s[4] = corr;
-  }
-  __syncthreads();
-  if (grp > 0) return;
-  for (uint32_t g = 0; g + 1 < kFinGroups; ++g) {
-    const double* s = red[g][lt];
-    a[0] += s[0];
-    a[1] += s[1];
-    a[2] += s[2];
-    a[3] += s[3];
-    corr += s[4];
+  if (live && prop) {
+    for (uint32_t ch = c0; ch < c1; ++ch) corr += pr.P2s[(uint64_t)ch * kBT + k];
   }
   double sx = 0.0, qx = 0.0;
   if (prop) {
     sx = pr.bt[k].s;
     qx = pr.bt[k].q;
   }
-  double yx = 0.0, yy = 0.0, yw = 0.0;
-  const uint64_t o = (uint64_t)k * pr.dstage + row;
-  if (k >= r.ntr) {  // unused trial slot: zeros, no reads
-    *reinterpret_cast<float4*>(pr.Y + o) = make_float4(0.f, 0.f, 0.f, 0.f);
-  } else {
-    float4 ym4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (memo_base) ym4 = *reinterpret_cast<const float4*>(pr.YM + o);
-    const float ymv[4] = {ym4.x, ym4.y, ym4.z, ym4.w};
-    const float4 x4 = *reinterpret_cast<const float4*>(pr.X + o);
-    const float xv[4] = {x4.x, x4.y, x4.z, x4.w};
-    float yv[4];
+  for (uint32_t blk = local; blk < nvblk; blk += nblk) {
+    const uint32_t row = blk * kBVecRows + (lt & 15) * 4;
+    const uint64_t o = (uint64_t)k * pr.dstage + row;
+    // loads that do not depend on the partials first
+    float4 ym4 = make_float4(0.f, 0.f, 0.f, 0.f), x4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (grp == 0 && live) {
+      if (memo_base) ym4 = *reinterpret_cast<const float4*>(pr.YM + o);
+      x4 = *reinterpret_cast<const float4*>(pr.X + o);
+    }
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    if (live) {
+      // batches of 4 chunk partials, loads first (a loop with a bound per
+      // element gets one load in flight at a time); sums in chunk order
+      uint32_t ch = c0;
+      for (; ch + 4 <= c1; ch += 4) {
+        float4 q[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      double acc = memo_base ? (double)ymv[e] : 0.0;
-      if (prop && row + e < pr.d) acc += sx * (double)pr.v[row + e] + qx - corr;
-      acc += a[e];
-      yv[e] = (row + e < pr.d && k < r.ntr) ? (float)acc : 0.0f;
-      const double yd = yv[e];
-      yx += yd * (double)xv[e];
-      yy += yd * yd;
-      if (pr.eigvec && row + e < pr.d) yw += yd * pr.eigvec[row + e];
+        for (int u = 0; u < 4; ++u)
+          q[u] = *reinterpret_cast<const float4*>(P2 + ((uint64_t)(ch + u) * kBT + k) * pitch + row);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          a[0] += q[u].x;
+          a[1] += q[u].y;
+          a[2] += q[u].z;
+          a[3] += q[u].w;
+        }
+      }
+      for (; ch < c1; ++ch) {
+        const float4 q = *reinterpret_cast<const float4*>(P2 + ((uint64_t)ch * kBT + k) * pitch + row);
+        a[0] += q.x;
+        a[1] += q.y;
+        a[2] += q.z;
+        a[3] += q.w;
+      }
     }
-    *reinterpret_cast<float4*>(pr.Y + o) = make_float4(yv[0], yv[1], yv[2], yv[3]);
-    if (md == kModeExactMemo && k < r.ntr) {
-      *reinterpret_cast<float4*>(pr.XM + o) = x4;
-      *reinterpret_cast<float4*>(pr.YM + o) = make_float4(yv[0], yv[1], yv[2], yv[3]);
+    double cr = corr;
+    if constexpr (G > 1) {
+      if (grp > 0) {
+        double* sp = red[grp - 1][lt];
+        sp[0] = a[0];
+        sp[1] = a[1];
+        sp[2] = a[2];
+        sp[3] = a[3];
+        sp[4] = corr;
+      }
+      __syncthreads();
+      if (grp == 0) {
+        for (uint32_t g = 0; g + 1 < G; ++g) {
+          const double* sp = red[g][lt];
+          a[0] += sp[0];
+          a[1] += sp[1];
+          a[2] += sp[2];
+          a[3] += sp[3];
+          cr += sp[4];
+        }
+      }
+      __syncthreads();  // red is rewritten by the next slab
+      if (grp > 0) continue;
     }
-  }
-  yx = sum16(yx);
-  yy = sum16(yy);
-  yw = sum16(yw);
-  if ((threadIdx.x & 15) == 0) {
-    double* b = pr.BP + ((uint64_t)blk * kBT + k) * 3;
-    b[0] = yx;
-    b[1] = yy;
-    b[2] = yw;
+    double yx = 0.0, yy = 0.0, yw = 0.0;
+    if (!live) {  // unused trial slot: zeros, no reads
+      *reinterpret_cast<float4*>(pr.Y + o) = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      const float ymv[4] = {ym4.x, ym4.y, ym4.z, ym4.w};
+      const float xv[4] = {x4.x, x4.y, x4.z, x4.w};
+      float yv[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        double acc = memo_base ? (double)ymv[e] : 0.0;
+        if (prop && row + e < d) acc += sx * (double)pr.v[row + e] + qx - cr;
+        acc += a[e];
+        yv[e] = row + e < d ? (float)acc : 0.0f;
+        const double yd = yv[e];
+        yx += yd * (double)xv[e];
+        yy += yd * yd;
+        if (pr.eigvec && row + e < d) yw += yd * pr.eigvec[row + e];
+      }
+      *reinterpret_cast<float4*>(pr.Y + o) = make_float4(yv[0], yv[1], yv[2], yv[3]);
+      if (md == kModeExactMemo) {
+        *reinterpret_cast<float4*>(pr.XM + o) = x4;
+        *reinterpret_cast<float4*>(pr.YM + o) = make_float4(yv[0], yv[1], yv[2], yv[3]);
+      }
+    }
+    yx = sum16(yx);
+    yy = sum16(yy);
+    yw = sum16(yw);
+    if ((lt & 15) == 0) {
+      double* b = pr.BP + ((uint64_t)blk * kBT + k) * 3;
+      b[0] = yx;
+      b[1] = yy;
+      b[2] = yw;
+    }
   }
 }
 
@@ -1074,8 +1159,8 @@ hipError_t launch_bnorm(const BProb* P, BGroup g, BRun r, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_bsample(const BProb* P, BGroup g, BRun r, uint32_t it, hipStream_t s) {
-  if (g.total) k_bsample<<<g.total, 256, 0, s>>>(P, g.off, r, it);
+hipError_t launch_bsample(const BProb* P, BGroup g, BRun r, uint32_t it, uint32_t upb, hipStream_t s) {
+  if (g.total) k_bsample<<<g.total, kSampleThreads, 0, s>>>(P, g.off, r, it, upb);
   return hipGetLastError();
 }
 
@@ -1094,8 +1179,8 @@ hipError_t launch_bcompact(const BProb* P, BGroup g, uint32_t np, hipStream_t s)
   return hipGetLastError();
 }
 
-hipError_t launch_bweight(const BProb* P, BGroup g, BRun r, uint32_t proposed_any, hipStream_t s) {
-  if (g.total) k_bweight<<<g.total, 256, 0, s>>>(P, g.off, r, proposed_any);
+hipError_t launch_bweight(const BProb* P, BGroup g, BRun r, uint32_t proposed_any, uint32_t out_w, hipStream_t s) {
+  if (g.total) k_bweight<<<g.total, 256, 0, s>>>(P, g.off, r, proposed_any, out_w);
   return hipGetLastError();
 }
 
@@ -1153,8 +1238,12 @@ hipError_t launch_bsigma(const BProb* P, BRun r, double* sigma, float* const* u,
   return hipGetLastError();
 }
 
-hipError_t launch_bfinish(const BProb* P, BGroup g, BRun r, hipStream_t s) {
-  if (g.total) k_bfinish<<<g.total, 256 * kFinGroups, 0, s>>>(P, g.off, r);
+hipError_t launch_bfinish(const BProb* P, BGroup g, BRun r, uint32_t groups, hipStream_t s) {
+  if (!g.total) return hipSuccess;
+  if (groups > 1)
+    k_bfinish<4><<<g.total, 1024, 0, s>>>(P, g.off, r);
+  else
+    k_bfinish<1><<<g.total, 256, 0, s>>>(P, g.off, r);
   return hipGetLastError();
 }
 

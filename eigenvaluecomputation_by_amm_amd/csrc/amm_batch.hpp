@@ -22,6 +22,7 @@ constexpr uint32_t kBT = 16;      // trials per right-hand-side block
 constexpr uint32_t kBTile = 64;   // columns per compaction tile
 constexpr uint32_t kBVecRows = 64;  // rows per block of the vector kernels
 constexpr uint32_t kBStageAlign = 256;  // staged vectors are padded to this many rows
+constexpr uint32_t kBSampleSlots = 2048;  // draws (slots) of one trial per sampler workgroup
 
 // Per (problem, trial) scalar state.
 struct alignas(16) BTrial {
@@ -58,12 +59,10 @@ struct alignas(16) BProb {
   uint16_t* DXh;        // [16][dstage] BF16 high part (BF16 A)
   uint16_t* DXl;        // [16][dstage] BF16 low part
   uint32_t* cnt;        // [ntiles*64][16] draw counts (zero between iterations)
-  float* W;             // [ntiles*64][16] weights of the compacted columns
   uint32_t* list;       // [ntiles*64] compacted column indices, per tile
   uint32_t* tcount;     // [ntiles] columns present per tile
   uint32_t* tpre;       // [ntiles + 1] prefix of tcount; tpre[ntiles] = |U|
   uint32_t* colc;       // [n] union columns in compacted order
-  float* Wc;            // [n][16] their weights
   float* Tc;            // [n][16] t = W (A_U^T dx - alpha s) of the current product
   float* P1;            // [kslices_max][ntiles*64][16], compacted slots
   float* P2;            // [chunks][16][p2pitch]
@@ -170,20 +169,23 @@ hipError_t launch_bzv(const BSetup* S, BGroup g, uint32_t np, hipStream_t s);
 hipError_t launch_bbegin(const BProb* P, BGroup g, BRun r, hipStream_t s);
 hipError_t launch_btail(const BProb* P, BRun r, uint32_t op, uint32_t it, hipStream_t s);
 hipError_t launch_bnorm(const BProb* P, BGroup g, BRun r, hipStream_t s);
-hipError_t launch_bsample(const BProb* P, BGroup g, BRun r, uint32_t it, hipStream_t s);
+// items: ceil(ntr ceil(c / kBSampleSlots) / upb) per problem (upb units each)
+hipError_t launch_bsample(const BProb* P, BGroup g, BRun r, uint32_t it, uint32_t upb, hipStream_t s);
 hipError_t launch_bprep(const BProb* P, BGroup g, BRun r, hipStream_t s);
 hipError_t launch_bscan(const BProb* P, uint32_t np, hipStream_t s);
 // items: ceil(ntiles / 4) per problem (the k_bprep group)
 hipError_t launch_bcompact(const BProb* P, BGroup g, uint32_t np, hipStream_t s);
-// items: any count per problem (the blocks stride over the union slots)
-hipError_t launch_bweight(const BProb* P, BGroup g, BRun r, uint32_t proposed_any, hipStream_t s);
+// items: any count per problem (the blocks stride over the union slots);
+// t of every union slot, or (out_w) the bare weights
+hipError_t launch_bweight(const BProb* P, BGroup g, BRun r, uint32_t proposed_any, uint32_t out_w, hipStream_t s);
 // dtype-uniform groups: all problems of a group share the element type
 // nt: A read with non-temporal loads (streams that no later pass reuses)
 hipError_t launch_bdot(int dtype, bool nt, const BProb* P, BGroup g, uint32_t np, hipStream_t s);
 // passes: 1 or 2 (a group shares it)
 hipError_t launch_baxpy(int dtype, uint32_t passes, bool nt, const BProb* P, BGroup g, uint32_t np,
                         uint32_t proposed_any, hipStream_t s);
-hipError_t launch_bfinish(const BProb* P, BGroup g, BRun r, hipStream_t s);
+// groups: 1 (256 threads per workgroup) or 4 (1024: four ranges of chunks)
+hipError_t launch_bfinish(const BProb* P, BGroup g, BRun r, uint32_t groups, hipStream_t s);
 hipError_t launch_bset_mode(const BProb* P, uint32_t np, uint32_t mode, hipStream_t s);
 // sigma: device [np][ntr]; u, v: device arrays of np device pointers (or null)
 hipError_t launch_bsigma(const BProb* P, BRun r, double* sigma, float* const* u, float* const* v,

@@ -53,6 +53,8 @@ namespace {
 
 constexpr uint32_t kTargetItems = 2048;      // workgroups the first contraction aims for
 constexpr uint32_t kTargetAxpyItems = 512;   // and the second: fewer chunks, fewer partials
+constexpr uint32_t kWeightBlocks = 2048;     // weight workgroups at most (about twice resident)
+constexpr uint32_t kFinishSlabs = 4096;      // finish workgroups of 256 threads at most
 constexpr uint32_t kMaxSlices = 16;      // contraction-1 row slices per problem
 constexpr uint32_t kZChunk = 2048;       // columns per chunk of z = A alpha
 
@@ -129,7 +131,9 @@ struct amm_batch {
   // grouped-launch item offsets
   uint32_t* offs = nullptr;
   size_t offs_cap = 0;
-  amm::BGroup g_vec{}, g_sample{}, g_prep{}, g_weight{}, g_dot[2]{}, g_axpy[2][2]{};
+  amm::BGroup g_vec{}, g_fin{}, g_sample{}, g_prep{}, g_weight{}, g_dot[2]{}, g_axpy[2][2]{};
+  uint32_t fin_groups = 1;  // wave groups of the finish (4 where a problem has many chunks)
+  uint32_t sample_upb = 1;  // sampler units per workgroup
   bool plan_valid = false;
   int plan_kernel = -1;
   // run state
@@ -234,6 +238,8 @@ int upload_plan(amm_batch* b, int kernel) {
   }
   std::vector<uint32_t> all;
   std::vector<std::pair<amm::BGroup*, size_t>> fix;
+  // a group: offsets off[0 .. np] of each problem's first workgroup, then
+  // the problem of every workgroup (find_problem reads it in one load)
   auto add_group = [&](amm::BGroup& g, auto count_fn) {
     const size_t base = all.size();
     uint64_t acc = 0;
@@ -242,33 +248,51 @@ int upload_plan(amm_batch* b, int kernel) {
       acc += count_fn(b->hp[i]);
       all.push_back((uint32_t)acc);
     }
+    if (acc >= (1ull << 31)) return false;
+    for (uint32_t i = 0; i < np; ++i) all.insert(all.end(), all[base + i + 1] - all[base + i], i);
     g.total = (uint32_t)acc;
     fix.emplace_back(&g, base);
-    return acc < (1ull << 31);
+    return true;
   };
   bool ok = true;
   ok &= add_group(b->g_cols, [](const BProb& p) { return p.n; });
   ok &= add_group(b->g_tiles, [](const BProb& p) { return (uint32_t)amm::cdf_tiles(p.n); });
   ok &= add_group(b->g_zv, [](const BProb& p) { return (uint32_t)cdiv(rup(p.d, 8), 256); });
   ok &= add_group(b->g_vec, [](const BProb& p) { return p.nvblk; });
+  // Sampler: a workgroup per unit (trial, kBSampleSlots draws).  Several
+  // units per workgroup, sharing its start and coarse table, measured
+  // slower on 256 problems x 16 trials (350 against 266 us): each unit's
+  // barrier then waits for the previous unit's count updates.
+  b->sample_upb = 1;
   ok &= add_group(b->g_sample, [&](const BProb& p) {
-    return kernel == AMM_KERNEL_EXACT ? 0u : b->ntr * (uint32_t)cdiv(p.c, 512);
+    return kernel == AMM_KERNEL_EXACT
+               ? 0u
+               : (uint32_t)cdiv((uint64_t)b->ntr * cdiv(p.c, amm::kBSampleSlots), b->sample_upb);
   });
-  // Weights / compaction: a wave per tile and up to 1024 weight workgroups
-  // per problem fill the device on one large problem; across many problems
-  // both counts shrink by a common factor (each wave then loops over a few
-  // tiles) so that small problems do not pay workgroup start-up per item.
-  uint64_t tot_tiles = 0, tot_weight = 0;
-  for (const BProb& p : b->hp) {
-    tot_tiles += p.ntiles;
-    tot_weight += std::min<uint32_t>(p.ntiles * 4u, 1024u);
-  }
-  const uint64_t f_prep = std::max<uint64_t>(1, cdiv(tot_tiles, 16384));   // <= 4096 workgroups
-  const uint64_t f_weight = std::max<uint64_t>(1, cdiv(tot_weight, 8192));
+  // Compaction: a wave per tile; weights: a thread per column (the union
+  // is at most n slots).  Across many problems both counts shrink by a
+  // common factor (each wave / thread then loops over several items): a
+  // workgroup's start is a chain of dependent loads (problem, descriptor,
+  // union size) that few items per workgroup would not amortise.
+  uint64_t tot_tiles = 0;
+  for (const BProb& p : b->hp) tot_tiles += p.ntiles;
+  const uint64_t f_prep = std::max<uint64_t>(1, cdiv(tot_tiles, 16384));  // <= 4096 workgroups
+  const uint64_t f_weight = std::max<uint64_t>(1, cdiv(tot_tiles, 4 * kWeightBlocks));
   ok &= add_group(b->g_prep, [f_prep](const BProb& p) { return (uint32_t)cdiv(p.ntiles, 4 * f_prep); });
-  ok &= add_group(b->g_weight, [f_weight](const BProb& p) {
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint32_t>(p.ntiles * 4u, 1024u) / f_weight);
-  });
+  // Finish: a workgroup per 64-row slab on small batches; across many
+  // problems workgroups take several slabs (about twice resident in all).
+  // Four wave groups per workgroup split the chunk partials where some
+  // problem has more than 8 chunks.
+  uint64_t tot_vblk = 0;
+  uint32_t max_chunks = 0;
+  for (const BProb& p : b->hp) {
+    tot_vblk += p.nvblk;
+    max_chunks = std::max(max_chunks, p.chunks);
+  }
+  b->fin_groups = max_chunks > 8 ? 4u : 1u;
+  const uint64_t f_fin = std::max<uint64_t>(1, cdiv(tot_vblk, kFinishSlabs / b->fin_groups));
+  ok &= add_group(b->g_fin, [f_fin](const BProb& p) { return (uint32_t)cdiv(p.nvblk, f_fin); });
+  ok &= add_group(b->g_weight, [f_weight](const BProb& p) { return (uint32_t)cdiv(p.ntiles, 4 * f_weight); });
   for (int dt = 0; dt < 2; ++dt)
     ok &= add_group(b->g_dot[dt], [dt](const BProb& p) {
       // vtiles * kslices(|U|) < dot_blocks + vtiles for every union size
@@ -432,17 +456,17 @@ int enq_setup(amm_batch* b, int K) {
 int enq_iteration(amm_batch* b, const amm::BRun& r, int K, uint32_t it) {
   const BProb* P = b->dP;
   hipStream_t s = b->stream;
-  if (K != AMM_KERNEL_EXACT) BHIPCHK(amm::launch_bsample(P, b->g_sample, r, it, s));
+  if (K != AMM_KERNEL_EXACT) BHIPCHK(amm::launch_bsample(P, b->g_sample, r, it, b->sample_upb, s));
   BHIPCHK(amm::launch_bprep(P, b->g_prep, r, s));
   BHIPCHK(amm::launch_bscan(P, b->np, s));
   BHIPCHK(amm::launch_bcompact(P, b->g_prep, b->np, s));
-  for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bdot(dt, b->nt_loads, P, b->g_dot[dt], b->np, s));
   const uint32_t prop = (K == AMM_KERNEL_PROPOSED || K == AMM_KERNEL_PROPOSED_V2) ? 1u : 0u;
-  BHIPCHK(amm::launch_bweight(P, b->g_weight, r, prop, s));
+  for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bdot(dt, b->nt_loads, P, b->g_dot[dt], b->np, s));
+  BHIPCHK(amm::launch_bweight(P, b->g_weight, r, prop, 0u, s));
   for (int dt = 0; dt < 2; ++dt)
     for (int pa = 0; pa < 2; ++pa)
       BHIPCHK(amm::launch_baxpy(dt, (uint32_t)pa + 1, b->nt_loads, P, b->g_axpy[dt][pa], b->np, prop, s));
-  BHIPCHK(amm::launch_bfinish(P, b->g_vec, r, s));
+  BHIPCHK(amm::launch_bfinish(P, b->g_fin, r, b->fin_groups, s));
   BHIPCHK(amm::launch_btail(P, r, amm::kBTailFinish, it, s));
   BHIPCHK(amm::launch_bnorm(P, b->g_vec, r, s));
   return AMM_OK;
@@ -613,7 +637,7 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
   // static per-problem layout
   Pool pl;
   struct Off {
-    size_t X, Y, XM, YM, DX, DXh, DXl, cnt, W, list, tcount, tpre, colc, Wc, Tc, BP, BQ, bt, alpha, nrm2,
+    size_t X, Y, XM, YM, DX, DXh, DXl, cnt, list, tcount, tpre, colc, Tc, BP, BQ, bt, alpha, nrm2,
         cnrm2, cdf0, cdf1, z, v, eigvec;
   };
   std::vector<Off> o(count);
@@ -649,12 +673,10 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     f.DXh = pl.take<uint16_t>(q.dtype == AMM_DTYPE_BF16 ? vec : 1);
     f.DXl = pl.take<uint16_t>(q.dtype == AMM_DTYPE_BF16 ? vec : 1);
     f.cnt = pl.take<uint32_t>(cols * amm::kBT);
-    f.W = pl.take<float>(cols * amm::kBT);
     f.list = pl.take<uint32_t>(cols);
     f.tcount = pl.take<uint32_t>(p.ntiles);
     f.tpre = pl.take<uint32_t>(p.ntiles + 1);
     f.colc = pl.take<uint32_t>(q.n);
-    f.Wc = pl.take<float>(q.n * amm::kBT);
     f.Tc = pl.take<float>(q.n * amm::kBT);
     f.BP = pl.take<double>((size_t)p.nvblk * amm::kBT * 3);
     f.BQ = pl.take<double>((size_t)p.nvblk * amm::kBT * 2);
@@ -690,12 +712,10 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     p.DXh = reinterpret_cast<uint16_t*>(s0 + f.DXh);
     p.DXl = reinterpret_cast<uint16_t*>(s0 + f.DXl);
     p.cnt = reinterpret_cast<uint32_t*>(s0 + f.cnt);
-    p.W = reinterpret_cast<float*>(s0 + f.W);
     p.list = reinterpret_cast<uint32_t*>(s0 + f.list);
     p.tcount = reinterpret_cast<uint32_t*>(s0 + f.tcount);
     p.tpre = reinterpret_cast<uint32_t*>(s0 + f.tpre);
     p.colc = reinterpret_cast<uint32_t*>(s0 + f.colc);
-    p.Wc = reinterpret_cast<float*>(s0 + f.Wc);
     p.Tc = reinterpret_cast<float*>(s0 + f.Tc);
     p.BP = reinterpret_cast<double*>(s0 + f.BP);
     p.BQ = reinterpret_cast<double*>(s0 + f.BQ);
@@ -914,26 +934,27 @@ int amm_batch_sample(amm_batch* b, int32_t kernel, uint32_t iter, uint32_t trial
   BHIPCHK(amm::launch_bbegin(b->dP, b->g_vec, r, b->stream));
   if ((rc = enq_setup(b, kernel))) return rc;
   b->stats_valid = false;
-  BHIPCHK(amm::launch_bsample(b->dP, b->g_sample, r, iter, b->stream));
+  BHIPCHK(amm::launch_bsample(b->dP, b->g_sample, r, iter, b->sample_upb, b->stream));
   BHIPCHK(amm::launch_bprep(b->dP, b->g_prep, r, b->stream));
-  BHIPCHK(hipStreamSynchronize(b->stream));
+  BHIPCHK(amm::launch_bscan(b->dP, b->np, b->stream));
+  BHIPCHK(amm::launch_bcompact(b->dP, b->g_prep, b->np, b->stream));
+  // the per-trial weights of every union slot, as the products form them
+  BHIPCHK(amm::launch_bweight(b->dP, b->g_weight, r, 0u, 1u, b->stream));
   const BProb& q = b->hp[problem];
-  std::vector<uint32_t> tc(q.ntiles), list((size_t)q.ntiles * amm::kBTile);
-  std::vector<float> W((size_t)q.ntiles * amm::kBTile * amm::kBT);
-  BHIPCHK(copy_sync(b->stream, tc.data(), q.tcount, tc.size() * 4, hipMemcpyDeviceToHost));
-  BHIPCHK(copy_sync(b->stream, list.data(), q.list, list.size() * 4, hipMemcpyDeviceToHost));
-  BHIPCHK(copy_sync(b->stream, W.data(), q.W, W.size() * 4, hipMemcpyDeviceToHost));
-  uint64_t m = 0;
-  for (uint32_t t = 0; t < q.ntiles; ++t) {
-    for (uint32_t s = 0; s < tc[t]; ++s) {
-      const size_t slot = (size_t)t * amm::kBTile + s;
-      if (cols_host) cols_host[m] = list[slot];
-      if (w_host)
-        for (uint32_t k = 0; k < b->ntr; ++k) w_host[m * b->ntr + k] = W[slot * amm::kBT + k];
-      ++m;
-    }
+  uint32_t U = 0;
+  BHIPCHK(copy_sync(b->stream, &U, q.tpre + q.ntiles, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  std::vector<uint32_t> cols(U);
+  std::vector<float> W((size_t)U * amm::kBT);
+  if (U) {
+    BHIPCHK(copy_sync(b->stream, cols.data(), q.colc, cols.size() * 4, hipMemcpyDeviceToHost));
+    BHIPCHK(copy_sync(b->stream, W.data(), q.Tc, W.size() * 4, hipMemcpyDeviceToHost));
   }
-  *count = m;
+  for (uint32_t m = 0; m < U; ++m) {
+    if (cols_host) cols_host[m] = cols[m];
+    if (w_host)
+      for (uint32_t k = 0; k < b->ntr; ++k) w_host[(size_t)m * b->ntr + k] = W[(size_t)m * amm::kBT + k];
+  }
+  *count = U;
   return AMM_OK;
 }
 
@@ -976,7 +997,7 @@ int amm_batch_sigma_max(amm_batch* b, const amm_params* p, double* sigma_host, f
   if (e == hipSuccess) e = amm::launch_bscan(b->dP, b->np, b->stream);
   if (e == hipSuccess) e = amm::launch_bcompact(b->dP, b->g_prep, b->np, b->stream);
   for (int dt = 0; dt < 2 && e == hipSuccess; ++dt) e = amm::launch_bdot(dt, b->nt_loads, b->dP, b->g_dot[dt], b->np, b->stream);
-  if (e == hipSuccess) e = amm::launch_bweight(b->dP, b->g_weight, r, 0u, b->stream);
+  if (e == hipSuccess) e = amm::launch_bweight(b->dP, b->g_weight, r, 0u, 0u, b->stream);
   if (e == hipSuccess)
     e = amm::launch_bsigma(b->dP, r, dsig, u_dev && dptr ? dptr : nullptr,
                            v_dev && dptr ? dptr + b->np : nullptr, b->stream);
@@ -1030,7 +1051,7 @@ int time_dense(amm_batch* b, uint32_t iters, double* ms_out) {
     BHIPCHK(hipEventRecord(b->ev0, b->stream));
     for (uint32_t k = 0; k < (w ? iters : 1); ++k) {
       for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bdot(dt, b->nt_loads, b->dP, b->g_dot[dt], b->np, b->stream));
-      BHIPCHK(amm::launch_bweight(b->dP, b->g_weight, r, 0u, b->stream));
+      BHIPCHK(amm::launch_bweight(b->dP, b->g_weight, r, 0u, 0u, b->stream));
       for (int dt = 0; dt < 2; ++dt)
         for (int pa = 0; pa < 2; ++pa)
           BHIPCHK(amm::launch_baxpy(dt, (uint32_t)pa + 1, b->nt_loads, b->dP, b->g_axpy[dt][pa], b->np, 0u, b->stream));
