@@ -449,7 +449,7 @@ def c4_section(args):
         return (float(r.time_ms[idx.max()]) if (idx >= 0).all() else None), idx
 
     with AmmBatch(probs, trials=16) as b:
-        b.autotune_run(RunParams(step=2 * cfg["vr_period"], trial=20_000, **cfg), reps=1)
+        b.autotune_run(RunParams(step=2 * cfg["vr_period"], trial=20_000, **cfg), reps=2)
         dense_ms = b.time_dense(2)
         b.power_method(RunParams(step=iters, trial=10_000, **cfg))  # graph build + warm-up
         r = b.power_method(RunParams(step=iters, trial=0, **cfg))
@@ -457,7 +457,7 @@ def c4_section(args):
         rx = b.power_method(RunParams(kernel="exact", step=args.exact_iters, trial=0))
         tx16, _ = settle_all(rx, 16)
     with AmmBatch(probs, trials=1) as b1:
-        b1.autotune_run(RunParams(step=2 * cfg["vr_period"], trial=20_000, **cfg), reps=1)
+        b1.autotune_run(RunParams(step=2 * cfg["vr_period"], trial=20_000, **cfg), reps=2)
         b1.power_method(RunParams(step=iters, trial=10_000, **cfg))
         r1 = b1.power_method(RunParams(step=iters, trial=0, **cfg))
         t1, _ = settle_all(r1, 1)
@@ -652,8 +652,8 @@ def kernel_roofline(ctx, A, d, n, n_exact, n_amm, kernel_iters):
         two_pass = {}
     kern.update({
         "k_colstats<float> (column statistics)": (ms_stats, ms_stats, d * n * s_A),
-        "k_gather_dot<float,2,list> (sampled pass A_S^T dx)": (n_amm * ms_dot, ms_dot, ucols * d * s_A),
-        "k_gather_axpy<float,list> (sampled pass A_S t)": (n_amm * ms_axpy, ms_axpy, ucols * d * s_A),
+        "k_gather_dot<float,2,4,list> (sampled pass A_S^T dx)": (n_amm * ms_dot, ms_dot, ucols * d * s_A),
+        "k_axpy_finish<float> (sampled pass A_S t with the finish)": (n_amm * ms_axpy, ms_axpy, ucols * d * s_A),
     })
     for k, (tot, t1, b) in kern.items():
         log(f"kernel {k}: {t1*1e3:.1f} us/launch, {b/(t1*1e-3)/1e9:.0f} GB/s, {tot*1e3:.0f} us in the run")
