@@ -524,8 +524,11 @@ __global__ __launch_bounds__(256) void k_gather_dot(
     bool v = e < cap;
     uint32_t j = e;
     if (LIST && v) {
+      // idx[e] is in bounds for every e < cap: loaded beside the segment
+      // count rather than after it, and used only where the entry is valid
+      const uint32_t jr = idx[e];
       v = (e % kSampleSeg) < seg_count[e / kSampleSeg];
-      if (v) j = idx[e];
+      j = jr;
     }
     valid[c] = v;
     col[c] = A + (uint64_t)(v ? j : 0) * ld;
@@ -598,6 +601,23 @@ __global__ __launch_bounds__(256) void k_gather_dot(
   }
 }
 
+// sum_r P1[r][e] over the R row splits in split order (FP64), the loads of
+// each batch of 8 issued together (a loop with a bound per element keeps one
+// load in flight at a time)
+__device__ __forceinline__ double split_sum(const float* __restrict__ P1, uint32_t R, uint32_t cap, uint32_t e) {
+  double rr = 0.0;
+  uint32_t q = 0;
+  for (; q + 8 <= R; q += 8) {
+    float p[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) p[u] = P1[(uint64_t)(q + u) * cap + e];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) rr += (double)p[u];
+  }
+  for (; q < R; ++q) rr += (double)P1[(uint64_t)q * cap + e];
+  return rr;
+}
+
 // ---------------------------------------------------------------------------
 // Contraction 2 (previousAMM_kernel2 / proposedAMM_kernel2,
 // core_previousAMM.cuh:177-263, core_proposedAMM.cuh:215-313): workgroup
@@ -641,10 +661,9 @@ __global__ __launch_bounds__(256) void k_gather_axpy(
     if (tdirect) {
       t = tdirect[j];
     } else {
-      double rr = 0.0;
-      for (uint32_t q = 0; q < R; ++q) rr += (double)P1[(uint64_t)q * cap + e];
       const double aj = alpha ? (double)alpha[j] : 0.0;
       const double wj = (LIST && wgt) ? (double)wgt[e] : 1.0;
+      const double rr = split_sum(P1, R, cap, e);
       const double tt = wj * (rr - aj * sx);
       t = (float)tt;
       corr += aj * (double)t;
@@ -858,11 +877,14 @@ __global__ __launch_bounds__(1024) void k_axpy_finish(
   for (uint32_t base = 0; base < cap; base += kMaxChunk) {
     // a chunk lies inside one sample segment (kMaxChunk divides kSampleSeg)
     const uint32_t sg = base / kSampleSeg;
-    const uint32_t vend = sg * kSampleSeg + seg_count[sg];
-    const uint32_t e1 = min(min(base + kMaxChunk, cap), vend);
-    const uint32_t nv = e1 > base ? e1 - base : 0u;
+    const uint32_t ce = min(base + kMaxChunk, cap);
     __syncthreads();  // the previous chunk's t and columns are consumed
-    for (uint32_t k = tid; k < nv; k += 1024) cl[k] = idx[base + k];
+    // every entry of the chunk up to cap (in bounds) is loaded without
+    // waiting for the segment's count; entries at or beyond it go unused
+    for (uint32_t k = tid; k < ce - base; k += 1024) cl[k] = idx[base + k];
+    const uint32_t vend = sg * kSampleSeg + seg_count[sg];
+    const uint32_t e1 = min(ce, vend);
+    const uint32_t nv = e1 > base ? e1 - base : 0u;
     __syncthreads();
     // the group's first 8 entries are loaded before t is formed, so the
     // reads of A overlap the reads of the dot partials (absent entries: 0)
@@ -882,10 +904,11 @@ __global__ __launch_bounds__(1024) void k_axpy_finish(
     for (uint32_t k = tid; k < nv; k += 1024) {
       const uint32_t e = base + k;
       const uint32_t j = cl[k];
-      double rr = 0.0;
-      for (uint32_t q = 0; q < R; ++q) rr += (double)P1[(uint64_t)q * cap + e];
+      // the per-entry loads first: they do not wait for the partial sums
       const double aj = alpha ? (double)alpha[j] : 0.0;
-      const float t = (float)((double)wgt[e] * (rr - aj * sx));
+      const double wj = (double)wgt[e];
+      const double rr = split_sum(P1, R, cap, e);
+      const float t = (float)(wj * (rr - aj * sx));
       corr += aj * (double)t;
       tl[k] = t;
     }
@@ -1000,11 +1023,25 @@ __global__ __launch_bounds__(256) void k_tail(
     double* __restrict__ eig, double* __restrict__ err, int64_t* __restrict__ ts, int stamp,
     uint32_t adapt_kernel, uint64_t period, double thr, uint32_t step) {
   __shared__ double red[4];
+  __shared__ double red3[3][4];
   __shared__ double scale_l;
   const uint32_t t = threadIdx.x;
   // Every block repeats the (small, fixed-order) reduction of BP and
   // normalises its own rows; block 0 alone writes the scalar state.
   const bool lead = blockIdx.x == 0;
+  // The first rows this thread normalises (its only ones where the grid
+  // covers dpad, as launch_tail arranges up to kTailMaxBlocks x 1024 rows):
+  // their loads do not depend on the reduction, so they go out first.
+  const uint64_t r0 = ((uint64_t)blockIdx.x * 256 + t) * 4;
+  const bool pre = norm && r0 < dpad;
+  float4 y0 = make_float4(0.f, 0.f, 0.f, 0.f), m0 = y0, z0 = y0;
+  if (pre) {
+    y0 = *reinterpret_cast<const float4*>(y + r0);
+    if (z) {
+      m0 = *reinterpret_cast<const float4*>(xm + r0);
+      z0 = *reinterpret_cast<const float4*>(z + r0);
+    }
+  }
   if (op == kTailFinish) {
     double a = 0.0, b = 0.0, c = 0.0;
     for (uint32_t i = t; i < nb; i += 256) {
@@ -1012,9 +1049,21 @@ __global__ __launch_bounds__(256) void k_tail(
       b += BP[3 * i + 1];
       c += BP[3 * i + 2];
     }
-    a = block256_sum(a, red);
-    b = block256_sum(b, red);
-    c = block256_sum(c, red);
+    // the three block sums in one exchange (each in block256_sum's order)
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    if ((t & 63) == 0) {
+      red3[0][t >> 6] = a;
+      red3[1][t >> 6] = b;
+      red3[2][t >> 6] = c;
+    }
+    __syncthreads();
+    if (t == 0) {
+      a = (red3[0][0] + red3[0][1]) + (red3[0][2] + red3[0][3]);
+      b = (red3[1][0] + red3[1][1]) + (red3[1][2] + red3[1][3]);
+      c = (red3[2][0] + red3[2][1]) + (red3[2][2] + red3[2][3]);
+    }
     if (t == 0) {
       const double scale = b > 0.0 ? 1.0 / sqrt(b) : 0.0;
       scale_l = scale;
@@ -1071,9 +1120,15 @@ __global__ __launch_bounds__(256) void k_tail(
   if (norm) {
     const double scale = scale_l;
     double ps = 0.0, pq = 0.0;
-#pragma unroll 8
-    for (uint64_t r = ((uint64_t)blockIdx.x * 256 + t) * 4; r < dpad; r += (uint64_t)gridDim.x * 1024) {
-      const float4 yv4 = *reinterpret_cast<const float4*>(y + r);
+    for (uint64_t r = r0; r < dpad; r += (uint64_t)gridDim.x * 1024) {
+      float4 yv4 = y0, m4 = m0, z4 = z0;
+      if (r != r0) {
+        yv4 = *reinterpret_cast<const float4*>(y + r);
+        if (z) {
+          m4 = *reinterpret_cast<const float4*>(xm + r);
+          z4 = *reinterpret_cast<const float4*>(z + r);
+        }
+      }
       float xv[4] = {(float)(scale * (double)yv4.x), (float)(scale * (double)yv4.y),
                      (float)(scale * (double)yv4.z), (float)(scale * (double)yv4.w)};
 #pragma unroll
@@ -1081,20 +1136,27 @@ __global__ __launch_bounds__(256) void k_tail(
         if (r + k >= d) xv[k] = 0.0f;
       *reinterpret_cast<float4*>(x + r) = make_float4(xv[0], xv[1], xv[2], xv[3]);
       if (z) {
+        const float mv[4] = {m4.x, m4.y, m4.z, m4.w};
+        const float zv[4] = {z4.x, z4.y, z4.z, z4.w};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const double dx = (double)xv[k] - (double)xm[r + k];
+          const double dx = (double)xv[k] - (double)mv[k];
           ps += dx;
-          pq += (double)z[r + k] * dx;
+          pq += (double)zv[k] * dx;
         }
       }
     }
-    if (z) {  // this block's part of s and q
-      ps = block256_sum(ps, red);
-      pq = block256_sum(pq, red);
+    if (z) {  // this block's part of s and q (both in one exchange, each in block256_sum's order)
+      ps = wave_sum(ps);
+      pq = wave_sum(pq);
+      if ((t & 63) == 0) {
+        red3[0][t >> 6] = ps;
+        red3[1][t >> 6] = pq;
+      }
+      __syncthreads();
       if (t == 0) {
-        st->sq_part[blockIdx.x][0] = ps;
-        st->sq_part[blockIdx.x][1] = pq;
+        st->sq_part[blockIdx.x][0] = (red3[0][0] + red3[0][1]) + (red3[0][2] + red3[0][3]);
+        st->sq_part[blockIdx.x][1] = (red3[1][0] + red3[1][1]) + (red3[1][2] + red3[1][3]);
         if (lead) st->sq_n = gridDim.x;
       }
     }
