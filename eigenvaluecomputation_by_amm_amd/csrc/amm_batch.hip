@@ -213,19 +213,24 @@ __global__ __launch_bounds__(256) void k_bnorm(const BProb* __restrict__ P, cons
 // slot i of trial k takes the 64-bit uniform of Philox counter (i >> 1, it,
 // trial0 + k, sample stream), words 2(i & 1), 2(i & 1) + 1, the integer
 // target floor(u * total / 2^64) and column min{j : cdf_j > target}, then
-// counts the draw in cnt[j][k].  Units: (trial, part of kBSampleSlots
-// slots), ntr ceil(c / kBSampleSlots) per problem; a workgroup takes `upb`
-// consecutive units of one problem (its start, a chain of dependent loads,
-// and the coarse table, shared by every unit on the same CDF, are paid once
-// per workgroup rather than per unit).  The part-0 unit of a proposed
+// counts the draw in trial k's row of 16-bit counts, cnt[k][j].  One
+// workgroup per (problem, trial) draws all c slots, 2048 per pass, and
+// rewrites the trial's whole row (so rows need no clearing between
+// iterations; a row is read only while its trial samples):
+//   n <= kDenseCols: the row is counted in LDS (16-bit halves of 32-bit
+//     words, LDS atomics) and written out whole;
+//   otherwise: an LDS open-addressing table (4096 entries, at most half
+//     full for 2048 draws) counts each pass, the row is zero-filled first
+//     and each pass adds its distinct columns to it.
+// (Scattered 4-byte count updates into a column-major [n][16] array cost
+// half of the sampler's time on 256 small problems: every update was a
+// partial write of its own 64-byte row.)  The workgroup of a proposed
 // trial also finishes s, q of its product.
 // ---------------------------------------------------------------------------
-// Open-addressing count table in LDS: key = column + 1 (0 = empty).  A unit
-// draws at most kBSampleSlots = 2048 columns (two per thread), so 4096
-// entries stay at most half full; the flush after each unit clears it.
-constexpr uint32_t kSlotsPerBlock = kBSampleSlots;
+constexpr uint32_t kSlotsPerPass = kBSampleSlots;
 constexpr uint32_t kHash = 2 * kBSampleSlots;
 constexpr uint32_t kSampleThreads = kBSampleSlots / 2;
+static_assert(kDenseCols / 2 >= 2 * kHash, "the hash table shares the dense row's LDS");
 
 __device__ __forceinline__ void hash_count(uint32_t* hkey, uint32_t* hcnt, uint32_t j) {
   const uint32_t key = j + 1u;
@@ -241,102 +246,111 @@ __device__ __forceinline__ void hash_count(uint32_t* hkey, uint32_t* hcnt, uint3
 }
 
 __global__ __launch_bounds__(kSampleThreads) void k_bsample(const BProb* __restrict__ P, const uint32_t* off,
-                                                            BRun r, uint32_t it, uint32_t upb) {
+                                                            BRun r, uint32_t it) {
   __shared__ double red[2][kSampleThreads / 64];
   __shared__ uint64_t cs[kCoarse];
-  __shared__ uint32_t hkey[kHash];
-  __shared__ uint32_t hcnt[kHash];
+  __shared__ __align__(16) uint32_t lc[kDenseCols / 2];  // dense row, or hash keys | counts
   const uint32_t p = find_problem(off, r.np, blockIdx.x);
   const BProb& pr = P[p];
-  const uint32_t local = blockIdx.x - off[p];
-  const uint32_t per = (pr.c + kSlotsPerBlock - 1) / kSlotsPerBlock;
-  const uint32_t units = r.ntr * per;
-  const uint32_t u0 = local * upb;
-  const uint32_t u1 = min(u0 + upb, units);
-  const uint32_t k0 = (uint32_t)pr.seed, k1 = (uint32_t)(pr.seed >> 32);
-  for (uint32_t e = threadIdx.x; e < kHash; e += kSampleThreads) {
-    hkey[e] = 0u;
-    hcnt[e] = 0u;
-  }
-  int built = -1;  // the CDF whose coarse table cs holds
-  for (uint32_t u = u0; u < u1; ++u) {  // uniform over the workgroup
-    const uint32_t k = u / per, part = u % per;
-    BTrial& t = pr.bt[k];
-    const uint32_t md = t.mode;
-    if (!is_amm(md)) continue;
-    const int which = md == kModeAmmProposed ? 1 : 0;
-    if (which && part == 0) {
-      double sv = 0.0, qv = 0.0;
-      for (uint32_t blk = threadIdx.x; blk < pr.nvblk; blk += kSampleThreads) {
-        const double* bq = pr.BQ + ((uint64_t)blk * kBT + k) * 2;
-        sv += bq[0];
-        qv += bq[1];
-      }
-      sv = wave_sum(sv);
-      qv = wave_sum(qv);
-      if ((threadIdx.x & 63) == 0) {
-        red[0][threadIdx.x >> 6] = sv;
-        red[1][threadIdx.x >> 6] = qv;
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double ts = 0.0, tq = 0.0;
-        for (uint32_t w = 0; w < kSampleThreads / 64; ++w) {
-          ts += red[0][w];
-          tq += red[1][w];
-        }
-        t.s = ts;
-        t.q = tq;
-      }
+  const uint32_t k = blockIdx.x - off[p];
+  if (k >= r.ntr) return;
+  BTrial& t = pr.bt[k];
+  const uint32_t md = t.mode;
+  if (!is_amm(md)) return;
+  const uint32_t tid = threadIdx.x;
+  const int which = md == kModeAmmProposed ? 1 : 0;
+  if (which) {
+    double sv = 0.0, qv = 0.0;
+    for (uint32_t blk = tid; blk < pr.nvblk; blk += kSampleThreads) {
+      const double* bq = pr.BQ + ((uint64_t)blk * kBT + k) * 2;
+      sv += bq[0];
+      qv += bq[1];
     }
-    const uint64_t* cdf = which ? pr.cdf1 : pr.cdf0;
-    if (which != built) {
-      __syncthreads();  // no thread still reads the previous table
-      coarse_fill(cs, cdf, pr.n, threadIdx.x, kSampleThreads);
-      built = which;
-    }
-    const uint64_t total = pr.st->total[which];
-    const uint64_t last_pos = pr.st->last_pos[which];
-    __syncthreads();  // the coarse table and the cleared count table
-    {
-      const uint32_t pair = part * (kSlotsPerBlock / 2) + threadIdx.x;
-      const uint32_t slot = 2 * pair;
-      if (slot < pr.c) {
-        uint32_t q[4];
-        philox4x32_10(pair, it, r.trial0 + k, kStreamSample, k0, k1, q);
-        hash_count(hkey, hcnt, bcdf_search(cdf, cs, pr.n, draw_target(q[0], q[1], total), last_pos));
-        if (slot + 1 < pr.c)
-          hash_count(hkey, hcnt, bcdf_search(cdf, cs, pr.n, draw_target(q[2], q[3], total), last_pos));
-      }
+    sv = wave_sum(sv);
+    qv = wave_sum(qv);
+    if ((tid & 63) == 0) {
+      red[0][tid >> 6] = sv;
+      red[1][tid >> 6] = qv;
     }
     __syncthreads();
-    // one global update per distinct column of this unit (heavy columns
-    // would otherwise serialise on one address): a plain store where the
-    // trial's draws are one unit (the counts start at zero and no other
-    // workgroup adds to them), else an atomic add; the table is cleared
-    // for the next unit
-    for (uint32_t e = threadIdx.x; e < kHash; e += kSampleThreads) {
-      const uint32_t key = hkey[e];
-      if (key) {
-        uint32_t* cp = pr.cnt + (uint64_t)(key - 1u) * kBT + k;
-        if (per == 1u)
-          *cp = hcnt[e];
-        else
-          atomicAdd(cp, hcnt[e]);
-        hkey[e] = 0u;
-        hcnt[e] = 0u;
+    if (tid == 0) {
+      double ts = 0.0, tq = 0.0;
+      for (uint32_t w = 0; w < kSampleThreads / 64; ++w) {
+        ts += red[0][w];
+        tq += red[1][w];
+      }
+      t.s = ts;
+      t.q = tq;
+    }
+  }
+  const uint32_t n = pr.n, c = pr.c;
+  const uint32_t npad = pr.ntiles * kBTile;
+  const bool dense = npad <= kDenseCols;
+  uint16_t* __restrict__ row = pr.cnt + (uint64_t)k * npad;
+  const uint64_t* cdf = which ? pr.cdf1 : pr.cdf0;
+  coarse_fill(cs, cdf, n, tid, kSampleThreads);
+  uint32_t* hkey = lc;
+  uint32_t* hcnt = lc + kHash;
+  if (dense) {
+    for (uint32_t e = tid; e < npad / 2; e += kSampleThreads) lc[e] = 0u;
+  } else {
+    for (uint32_t e = tid; e < 2 * kHash; e += kSampleThreads) lc[e] = 0u;
+    // the row is zero-filled before any pass adds to it (npad: multiple of 64)
+    uint4* r4 = reinterpret_cast<uint4*>(row);
+    for (uint32_t e = tid; e < npad / 8; e += kSampleThreads) r4[e] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  const uint64_t total = pr.st->total[which];
+  const uint64_t last_pos = pr.st->last_pos[which];
+  const uint32_t k0 = (uint32_t)pr.seed, k1 = (uint32_t)(pr.seed >> 32);
+  __syncthreads();  // the coarse table, the cleared counts, the zeroed row
+  for (uint32_t base = 0; base < c; base += kSlotsPerPass) {  // uniform over the workgroup
+    const uint32_t pair = base / 2 + tid;
+    const uint32_t slot = 2 * pair;
+    if (slot < c) {
+      uint32_t q[4];
+      philox4x32_10(pair, it, r.trial0 + k, kStreamSample, k0, k1, q);
+      const uint32_t j0 = bcdf_search(cdf, cs, n, draw_target(q[0], q[1], total), last_pos);
+      const bool two = slot + 1 < c;
+      const uint32_t j1 = two ? bcdf_search(cdf, cs, n, draw_target(q[2], q[3], total), last_pos) : 0u;
+      if (dense) {
+        atomicAdd(&lc[j0 >> 1], 1u << ((j0 & 1u) << 4));
+        if (two) atomicAdd(&lc[j1 >> 1], 1u << ((j1 & 1u) << 4));
+      } else {
+        hash_count(hkey, hcnt, j0);
+        if (two) hash_count(hkey, hcnt, j1);
       }
     }
+    if (!dense) {
+      __syncthreads();
+      // this pass's distinct columns added to the row (one writer per
+      // column: this workgroup), the table cleared for the next pass
+      for (uint32_t e = tid; e < kHash; e += kSampleThreads) {
+        const uint32_t key = hkey[e];
+        if (key) {
+          uint16_t* cp = row + (key - 1u);
+          *cp = (uint16_t)(*cp + hcnt[e]);
+          hkey[e] = 0u;
+          hcnt[e] = 0u;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (dense) {
+    __syncthreads();
+    const uint4* l4 = reinterpret_cast<const uint4*>(lc);
+    uint4* r4 = reinterpret_cast<uint4*>(row);
+    for (uint32_t e = tid; e < npad / 8; e += kSampleThreads) r4[e] = l4[e];
   }
 }
 
 // ---------------------------------------------------------------------------
 // Union of the trials' samples, tile by tile of 64 columns: column j is
-// present when a sampled trial drew it or a trial takes an exact step.  A
-// wave compacts the present columns of a tile (ballot + prefix count) into
-// list[tile*64 ...].  The per-trial weights are formed later from the same
-// draw counts (k_bweight), where the products need them; the counts are
-// cleared there too.  The problem's waves take tiles w, w + waves, ...
+// present when a sampled trial drew it (its row's count is nonzero) or a
+// trial takes an exact step.  The present columns of a tile are compacted
+// in column order into list[tile*64 ...] and counted in tcount[tile].  The
+// per-trial weights are formed later from the same draw counts
+// (k_bweight), where the products need them.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bprep(const BProb* __restrict__ P, const uint32_t* off,
                                                BRun r) {
@@ -353,26 +367,59 @@ __global__ __launch_bounds__(256) void k_bprep(const BProb* __restrict__ P, cons
   const bool all_present = exact_any != 0u;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t nwaves = (off[p + 1] - off[p]) * 4;
-  // wave-uniform loop: each wave takes tiles w, w + nwaves, ...
-  for (uint32_t tile = (blockIdx.x - off[p]) * 4 + (threadIdx.x >> 6); tile < pr.ntiles; tile += nwaves) {
-    const uint32_t j = tile * kBTile + lane;
-    bool present = false;
-    if (j < pr.n) {
+  const uint32_t npad = pr.ntiles * kBTile;
+  const uint32_t ngroups = (pr.ntiles + 7) / 8;
+  // A wave takes 8 tiles (512 columns): lane l holds columns 8 l .. 8 l + 7
+  // of the group (one 16-byte load per trial row); the 8 lanes of a tile
+  // add up their presence counts (prefix over the lane octet) and write the
+  // tile's compacted list.  Wave-uniform loop over groups w, w + nwaves, ...
+  for (uint32_t grp = (blockIdx.x - off[p]) * 4 + (threadIdx.x >> 6); grp < ngroups; grp += nwaves) {
+    const uint32_t tile = grp * 8 + (lane >> 3);
+    const uint32_t j0 = grp * 512 + lane * 8;  // < npad whenever tile < ntiles
+    uint32_t bits = 0u;
+    if (tile < pr.ntiles) {
       if (all_present) {
-        present = true;
+        bits = 0xFFu;
       } else {
-        const uint4* cp = reinterpret_cast<const uint4*>(pr.cnt + (uint64_t)j * kBT);
+        // every live trial samples (no trial is exact): its row is current
+        uint4 v[kBT];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const uint4 c4 = cp[v];
-          present |= (c4.x | c4.y | c4.z | c4.w) != 0u;
+        for (uint32_t k = 0; k < kBT; ++k)
+          v[k] = k < r.ntr ? *reinterpret_cast<const uint4*>(pr.cnt + (uint64_t)k * npad + j0) : make_uint4(0u, 0u, 0u, 0u);
+        uint4 o = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (uint32_t k = 0; k < kBT; ++k) {
+          o.x |= v[k].x;
+          o.y |= v[k].y;
+          o.z |= v[k].z;
+          o.w |= v[k].w;
+        }
+        const uint32_t w4[4] = {o.x, o.y, o.z, o.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          bits |= ((w4[q] & 0xFFFFu) != 0u ? 1u : 0u) << (2 * q);
+          bits |= ((w4[q] >> 16) != 0u ? 1u : 0u) << (2 * q + 1);
         }
       }
+      // columns at or beyond n are never present
+      if (j0 + 8 > pr.n) bits &= j0 >= pr.n ? 0u : (1u << (pr.n - j0)) - 1u;
     }
-    const uint64_t mask = __ballot(present);
-    const uint32_t pos = __popcll(mask & ((1ull << lane) - 1ull));
-    if (present) pr.list[(uint64_t)tile * kBTile + pos] = j;
-    if (lane == 0) pr.tcount[tile] = (uint32_t)__popcll(mask);
+    const uint32_t cntl = (uint32_t)__popc(bits);
+    // inclusive prefix of the counts over the lane octet of this tile
+    uint32_t incl = cntl;
+#pragma unroll
+    for (uint32_t o = 1; o < 8; o <<= 1) {
+      const uint32_t v = __shfl_up(incl, o, 8);
+      if ((lane & 7u) >= o) incl += v;
+    }
+    if (tile < pr.ntiles) {
+      uint32_t pos = incl - cntl;
+      uint32_t* out = pr.list + (uint64_t)tile * kBTile;
+#pragma unroll
+      for (uint32_t b2 = 0; b2 < 8; ++b2)
+        if ((bits >> b2) & 1u) out[pos++] = j0 + b2;
+      if ((lane & 7u) == 7u) pr.tcount[tile] = incl;
+    }
   }
 }
 
@@ -613,14 +660,14 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
 // alpha_j s_k), s_k = 1^T dx_k for a proposed trial, else 0, with w formed
 // from the draw counts as update_weights (common.h:42-53) and k_sample form
 // it: 1 on an exact step, mult total / (c q_j) for a draw count mult > 0 on
-// a sampled step (q_j = cdf_j - cdf_{j-1} of the step's CDF), else 0; the
-// counts are cleared.  (Forming t in
+// a sampled step (q_j = cdf_j - cdf_{j-1} of the step's CDF), else 0.  (Forming t in
 // k_bdot's epilogue where it has one row slice measured slower: its reads
 // and writes compete with the stream of A.)  With out_w,
 // Tc[s][k] = w_{s,k} for every problem instead (the union sample and its
-// weights, for amm_batch_sample).  A thread per union slot, all 16 trials
-// (16-byte accesses of the counts, partials and t; 64 slots in flight per
-// wave); the problem's blocks stride over the union.
+// weights, for amm_batch_sample).  A lane quad per union slot, four trials
+// per lane (16-byte accesses of the partials and t; the counts from the
+// trials' rows, adjacent columns of a row in adjacent quads); the problem's
+// blocks stride over the union, 64 slots per workgroup and pass.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bweight(const BProb* __restrict__ P, const uint32_t* off,
                                                  BRun r, uint32_t proposed_any, uint32_t out_w) {
@@ -632,7 +679,8 @@ __global__ __launch_bounds__(256) void k_bweight(const BProb* __restrict__ P, co
   const UPlan up = union_plan(pr, pr.tpre[pr.ntiles]);
   const uint64_t slab = (uint64_t)pr.ntiles * kBTile * kBT;
   const uint32_t* __restrict__ colc = pr.colc;
-  uint32_t* __restrict__ cnt = pr.cnt;
+  const uint16_t* __restrict__ cnt = pr.cnt;
+  const uint32_t npad = pr.ntiles * kBTile;
   const float* __restrict__ P1 = pr.P1;
   float* __restrict__ Tc = pr.Tc;
   const uint32_t ntr = r.ntr, ks = up.kslices;
@@ -655,71 +703,70 @@ __global__ __launch_bounds__(256) void k_bweight(const BProb* __restrict__ P, co
   const double tot1 = need1 && pr.st ? pr.st->total[1] : 0.0;
   const double cd = (double)pr.c;
   const uint32_t nq = (ntr + 3) / 4;  // 4-trial groups holding live trials
-  // the column of the next slot is loaded one iteration ahead (one
+  // lane quad per slot: lane g of the quad takes trials 4 g .. 4 g + 3, so
+  // the quad's 16-byte accesses of the counts, dots and t are one 64-byte
+  // run; the column of the next slot is loaded one iteration ahead (one
   // dependent latency per slot instead of two)
-  const uint32_t stride = nblk * 256;
-  uint32_t sl = local * 256 + threadIdx.x;
+  const uint32_t g = threadIdx.x & 3u;
+  const bool gl = g < nq;
+  const uint32_t stride = nblk * 64u;
+  uint32_t sl = local * 64u + (threadIdx.x >> 2);
   uint32_t jn = sl < up.U ? colc[sl] : 0u;
   for (; sl < up.U; sl += stride) {
     const uint32_t j = jn;
     if (sl + stride < up.U) jn = colc[sl + stride];
-    const uint64_t base = (uint64_t)sl * kBT;
-    f32x4_t pv[kBT / 4];  // the first slice's dots
-    if (!out_w) {
+    const uint64_t base = (uint64_t)sl * kBT + 4u * g;
+    f32x4_t pv = {0.f, 0.f, 0.f, 0.f};  // the first slice's dots
+    if (!out_w) pv = *reinterpret_cast<const f32x4_t*>(P1 + base);
+    // the counts of trials 4 g .. 4 g + 3 (rows of exact or unused trials
+    // are not read: their weights do not use them)
+    uint32_t m[4];
 #pragma unroll
-      for (uint32_t g = 0; g < kBT / 4; ++g) pv[g] = *reinterpret_cast<const f32x4_t*>(P1 + base + 4 * g);
+    for (uint32_t e = 0; e < 4; ++e) {
+      const uint32_t k = 4u * g + e;
+      m[e] = (gl && ((live_m & ~exact_m) >> k) & 1u) ? (uint32_t)cnt[(uint64_t)k * npad + j] : 0u;
     }
-    uint32_t* cj = cnt + (uint64_t)j * kBT;
-    u32x4 m[kBT / 4];
-#pragma unroll
-    for (uint32_t g = 0; g < kBT / 4; ++g) m[g] = g < nq ? *reinterpret_cast<const u32x4*>(cj + 4 * g) : u32x4{0u, 0u, 0u, 0u};
     // q_j of each CDF a sampled trial uses
     uint64_t q0 = 0ull, q1 = 0ull;
     if (need0 && pr.cdf0) q0 = pr.cdf0[j] - (j ? pr.cdf0[j - 1] : 0ull);
     if (need1 && pr.cdf1) q1 = pr.cdf1[j] - (j ? pr.cdf1[j - 1] : 0ull);
     const double aj = use_sk ? (double)pr.alpha[j] : 0.0;
+    double acc[4];
+    if (!out_w) {
+      // p = sum over the row slices of the dots, in slice order
 #pragma unroll
-    for (uint32_t g = 0; g < kBT / 4; ++g) {
-      // clear the counts used (the next draw starts at 0)
-      if (g < nq && (m[g][0] | m[g][1] | m[g][2] | m[g][3]) != 0u)
-        *reinterpret_cast<u32x4*>(cj + 4 * g) = u32x4{0u, 0u, 0u, 0u};
-      double acc[4];
-      if (!out_w) {
-        // p = sum over the row slices of the dots, in slice order
+      for (uint32_t e = 0; e < 4; ++e) acc[e] = 0.0 + (double)pv[e];
+      for (uint32_t q = 1; q < ks; ++q) {
+        const f32x4_t v = *reinterpret_cast<const f32x4_t*>(P1 + q * slab + base);
 #pragma unroll
-        for (uint32_t e = 0; e < 4; ++e) acc[e] = 0.0 + (double)pv[g][e];
-        for (uint32_t q = 1; q < ks; ++q) {
-          const f32x4_t v = *reinterpret_cast<const f32x4_t*>(P1 + q * slab + base + 4 * g);
-#pragma unroll
-          for (uint32_t e = 0; e < 4; ++e) acc[e] += (double)v[e];
-        }
+        for (uint32_t e = 0; e < 4; ++e) acc[e] += (double)v[e];
       }
-      f32x4_t t;
-#pragma unroll
-      for (uint32_t e = 0; e < 4; ++e) {
-        const uint32_t k = 4 * g + e;
-        const uint32_t mult = m[g][e];
-        // weight: 1 on an exact step, mult total / (c q_j) on a sampled one
-        float w = 0.0f;
-        if ((exact_m >> k) & 1u) {
-          w = 1.0f;
-        } else if (mult) {
-          const bool pp = (prop_m >> k) & 1u;
-          const double num = (double)mult * (pp ? tot1 : tot0);
-          const double den = cd * (double)(pp ? q1 : q0);
-          w = den > 0.0 ? (float)(num / den) : 0.0f;
-        }
-        if (!((live_m >> k) & 1u)) w = 0.0f;
-        if (!out_w) {
-          // t = w (p - alpha_j s_k)
-          const double skk = s_sk[k];
-          const double ajs = skk != 0.0 ? aj : 0.0;
-          w = ((live_m >> k) & 1u) ? (float)((double)w * (acc[e] - ajs * skk)) : 0.0f;
-        }
-        t[e] = w;
-      }
-      *reinterpret_cast<f32x4_t*>(Tc + base + 4 * g) = t;
     }
+    f32x4_t t;
+#pragma unroll
+    for (uint32_t e = 0; e < 4; ++e) {
+      const uint32_t k = 4u * g + e;
+      const uint32_t mult = m[e];
+      // weight: 1 on an exact step, mult total / (c q_j) on a sampled one
+      float w = 0.0f;
+      if ((exact_m >> k) & 1u) {
+        w = 1.0f;
+      } else if (mult) {
+        const bool pp = (prop_m >> k) & 1u;
+        const double num = (double)mult * (pp ? tot1 : tot0);
+        const double den = cd * (double)(pp ? q1 : q0);
+        w = den > 0.0 ? (float)(num / den) : 0.0f;
+      }
+      if (!((live_m >> k) & 1u)) w = 0.0f;
+      if (!out_w) {
+        // t = w (p - alpha_j s_k)
+        const double skk = s_sk[k];
+        const double ajs = skk != 0.0 ? aj : 0.0;
+        w = ((live_m >> k) & 1u) ? (float)((double)w * (acc[e] - ajs * skk)) : 0.0f;
+      }
+      t[e] = w;
+    }
+    *reinterpret_cast<f32x4_t*>(Tc + base) = t;
   }
 }
 
@@ -1159,8 +1206,8 @@ hipError_t launch_bnorm(const BProb* P, BGroup g, BRun r, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_bsample(const BProb* P, BGroup g, BRun r, uint32_t it, uint32_t upb, hipStream_t s) {
-  if (g.total) k_bsample<<<g.total, kSampleThreads, 0, s>>>(P, g.off, r, it, upb);
+hipError_t launch_bsample(const BProb* P, BGroup g, BRun r, uint32_t it, hipStream_t s) {
+  if (g.total) k_bsample<<<g.total, kSampleThreads, 0, s>>>(P, g.off, r, it);
   return hipGetLastError();
 }
 

@@ -22,7 +22,9 @@ constexpr uint32_t kBT = 16;      // trials per right-hand-side block
 constexpr uint32_t kBTile = 64;   // columns per compaction tile
 constexpr uint32_t kBVecRows = 64;  // rows per block of the vector kernels
 constexpr uint32_t kBStageAlign = 256;  // staged vectors are padded to this many rows
-constexpr uint32_t kBSampleSlots = 2048;  // draws (slots) of one trial per sampler workgroup
+constexpr uint32_t kBSampleSlots = 2048;  // draws (slots) per pass of a sampler workgroup
+constexpr uint32_t kDenseCols = 32768;    // columns whose 16-bit counts the sampler holds in LDS (64 KB)
+constexpr uint32_t kMaxBatchC = 65535;    // draws per trial and iteration (the counts are 16-bit)
 
 // Per (problem, trial) scalar state.
 struct alignas(16) BTrial {
@@ -58,7 +60,7 @@ struct alignas(16) BProb {
   float* DX;            // [16][dstage] FP32 staging of x - x_memo (FP32 A)
   uint16_t* DXh;        // [16][dstage] BF16 high part (BF16 A)
   uint16_t* DXl;        // [16][dstage] BF16 low part
-  uint32_t* cnt;        // [ntiles*64][16] draw counts (zero between iterations)
+  uint16_t* cnt;        // [16][ntiles*64] draw counts, a row per trial (rewritten whole when it samples)
   uint32_t* list;       // [ntiles*64] compacted column indices, per tile
   uint32_t* tcount;     // [ntiles] columns present per tile
   uint32_t* tpre;       // [ntiles + 1] prefix of tcount; tpre[ntiles] = |U|
@@ -169,8 +171,8 @@ hipError_t launch_bzv(const BSetup* S, BGroup g, uint32_t np, hipStream_t s);
 hipError_t launch_bbegin(const BProb* P, BGroup g, BRun r, hipStream_t s);
 hipError_t launch_btail(const BProb* P, BRun r, uint32_t op, uint32_t it, hipStream_t s);
 hipError_t launch_bnorm(const BProb* P, BGroup g, BRun r, hipStream_t s);
-// items: ceil(ntr ceil(c / kBSampleSlots) / upb) per problem (upb units each)
-hipError_t launch_bsample(const BProb* P, BGroup g, BRun r, uint32_t it, uint32_t upb, hipStream_t s);
+// items: ntr per problem (one workgroup per trial)
+hipError_t launch_bsample(const BProb* P, BGroup g, BRun r, uint32_t it, hipStream_t s);
 hipError_t launch_bprep(const BProb* P, BGroup g, BRun r, hipStream_t s);
 hipError_t launch_bscan(const BProb* P, uint32_t np, hipStream_t s);
 // items: ceil(ntiles / 4) per problem (the k_bprep group)

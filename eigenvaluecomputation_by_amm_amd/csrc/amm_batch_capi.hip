@@ -53,7 +53,7 @@ namespace {
 
 constexpr uint32_t kTargetItems = 2048;      // workgroups the first contraction aims for
 constexpr uint32_t kTargetAxpyItems = 512;   // and the second: fewer chunks, fewer partials
-constexpr uint32_t kWeightBlocks = 2048;     // weight workgroups at most (about twice resident)
+constexpr uint32_t kWeightBlocks = 4096;     // weight workgroups at most (about twice resident)
 constexpr uint32_t kFinishSlabs = 4096;      // finish workgroups of 256 threads at most
 constexpr uint32_t kMaxSlices = 16;      // contraction-1 row slices per problem
 constexpr uint32_t kZChunk = 2048;       // columns per chunk of z = A alpha
@@ -133,7 +133,6 @@ struct amm_batch {
   size_t offs_cap = 0;
   amm::BGroup g_vec{}, g_fin{}, g_sample{}, g_prep{}, g_weight{}, g_dot[2]{}, g_axpy[2][2]{};
   uint32_t fin_groups = 1;  // wave groups of the finish (4 where a problem has many chunks)
-  uint32_t sample_upb = 1;  // sampler units per workgroup
   bool plan_valid = false;
   int plan_kernel = -1;
   // run state
@@ -259,25 +258,17 @@ int upload_plan(amm_batch* b, int kernel) {
   ok &= add_group(b->g_tiles, [](const BProb& p) { return (uint32_t)amm::cdf_tiles(p.n); });
   ok &= add_group(b->g_zv, [](const BProb& p) { return (uint32_t)cdiv(rup(p.d, 8), 256); });
   ok &= add_group(b->g_vec, [](const BProb& p) { return p.nvblk; });
-  // Sampler: a workgroup per unit (trial, kBSampleSlots draws).  Several
-  // units per workgroup, sharing its start and coarse table, measured
-  // slower on 256 problems x 16 trials (350 against 266 us): each unit's
-  // barrier then waits for the previous unit's count updates.
-  b->sample_upb = 1;
-  ok &= add_group(b->g_sample, [&](const BProb& p) {
-    return kernel == AMM_KERNEL_EXACT
-               ? 0u
-               : (uint32_t)cdiv((uint64_t)b->ntr * cdiv(p.c, amm::kBSampleSlots), b->sample_upb);
-  });
-  // Compaction: a wave per tile; weights: a thread per column (the union
-  // is at most n slots).  Across many problems both counts shrink by a
+  // Sampler: a workgroup per (problem, trial), drawing all c slots
+  ok &= add_group(b->g_sample, [&](const BProb&) { return kernel == AMM_KERNEL_EXACT ? 0u : b->ntr; });
+  // Compaction: a wave per tile; weights: a lane quad per column (the
+  // union is at most n slots, a workgroup takes a tile of 64 per pass).  Across many problems both counts shrink by a
   // common factor (each wave / thread then loops over several items): a
   // workgroup's start is a chain of dependent loads (problem, descriptor,
   // union size) that few items per workgroup would not amortise.
   uint64_t tot_tiles = 0;
   for (const BProb& p : b->hp) tot_tiles += p.ntiles;
   const uint64_t f_prep = std::max<uint64_t>(1, cdiv(tot_tiles, 16384));  // <= 4096 workgroups
-  const uint64_t f_weight = std::max<uint64_t>(1, cdiv(tot_tiles, 4 * kWeightBlocks));
+  const uint64_t f_weight = std::max<uint64_t>(1, cdiv(tot_tiles, kWeightBlocks));
   ok &= add_group(b->g_prep, [f_prep](const BProb& p) { return (uint32_t)cdiv(p.ntiles, 4 * f_prep); });
   // Finish: a workgroup per 64-row slab on small batches; across many
   // problems workgroups take several slabs (about twice resident in all).
@@ -292,7 +283,7 @@ int upload_plan(amm_batch* b, int kernel) {
   b->fin_groups = max_chunks > 8 ? 4u : 1u;
   const uint64_t f_fin = std::max<uint64_t>(1, cdiv(tot_vblk, kFinishSlabs / b->fin_groups));
   ok &= add_group(b->g_fin, [f_fin](const BProb& p) { return (uint32_t)cdiv(p.nvblk, f_fin); });
-  ok &= add_group(b->g_weight, [f_weight](const BProb& p) { return (uint32_t)cdiv(p.ntiles, 4 * f_weight); });
+  ok &= add_group(b->g_weight, [f_weight](const BProb& p) { return (uint32_t)cdiv(p.ntiles, f_weight); });
   for (int dt = 0; dt < 2; ++dt)
     ok &= add_group(b->g_dot[dt], [dt](const BProb& p) {
       // vtiles * kslices(|U|) < dot_blocks + vtiles for every union size
@@ -456,7 +447,7 @@ int enq_setup(amm_batch* b, int K) {
 int enq_iteration(amm_batch* b, const amm::BRun& r, int K, uint32_t it) {
   const BProb* P = b->dP;
   hipStream_t s = b->stream;
-  if (K != AMM_KERNEL_EXACT) BHIPCHK(amm::launch_bsample(P, b->g_sample, r, it, b->sample_upb, s));
+  if (K != AMM_KERNEL_EXACT) BHIPCHK(amm::launch_bsample(P, b->g_sample, r, it, s));
   BHIPCHK(amm::launch_bprep(P, b->g_prep, r, s));
   BHIPCHK(amm::launch_bscan(P, b->np, s));
   BHIPCHK(amm::launch_bcompact(P, b->g_prep, b->np, s));
@@ -613,6 +604,8 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     if (q.ld % V) return fail(AMM_E_INVALID, w + "ld must be a multiple of 16 bytes of elements");
     if (((uintptr_t)q.A) & 15u) return fail(AMM_E_INVALID, w + "A must be 16-byte aligned");
     if (q.c > q.n) return fail(AMM_E_INVALID, w + "c must be <= n");
+    if (q.c > amm::kMaxBatchC)
+      return fail(AMM_E_INVALID, w + "c must be <= 65535 (16-bit draw counts of the batched engine)");
   }
   std::lock_guard<std::recursive_mutex> lk(amm_host::device_mutex());
   amm_batch* b = new amm_batch();
@@ -672,7 +665,7 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     f.DX = pl.take<float>(q.dtype == AMM_DTYPE_F32 ? vec : 1);
     f.DXh = pl.take<uint16_t>(q.dtype == AMM_DTYPE_BF16 ? vec : 1);
     f.DXl = pl.take<uint16_t>(q.dtype == AMM_DTYPE_BF16 ? vec : 1);
-    f.cnt = pl.take<uint32_t>(cols * amm::kBT);
+    f.cnt = pl.take<uint16_t>(cols * amm::kBT);
     f.list = pl.take<uint32_t>(cols);
     f.tcount = pl.take<uint32_t>(p.ntiles);
     f.tpre = pl.take<uint32_t>(p.ntiles + 1);
@@ -711,7 +704,7 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     p.DX = reinterpret_cast<float*>(s0 + f.DX);
     p.DXh = reinterpret_cast<uint16_t*>(s0 + f.DXh);
     p.DXl = reinterpret_cast<uint16_t*>(s0 + f.DXl);
-    p.cnt = reinterpret_cast<uint32_t*>(s0 + f.cnt);
+    p.cnt = reinterpret_cast<uint16_t*>(s0 + f.cnt);
     p.list = reinterpret_cast<uint32_t*>(s0 + f.list);
     p.tcount = reinterpret_cast<uint32_t*>(s0 + f.tcount);
     p.tpre = reinterpret_cast<uint32_t*>(s0 + f.tpre);
@@ -934,7 +927,7 @@ int amm_batch_sample(amm_batch* b, int32_t kernel, uint32_t iter, uint32_t trial
   BHIPCHK(amm::launch_bbegin(b->dP, b->g_vec, r, b->stream));
   if ((rc = enq_setup(b, kernel))) return rc;
   b->stats_valid = false;
-  BHIPCHK(amm::launch_bsample(b->dP, b->g_sample, r, iter, b->sample_upb, b->stream));
+  BHIPCHK(amm::launch_bsample(b->dP, b->g_sample, r, iter, b->stream));
   BHIPCHK(amm::launch_bprep(b->dP, b->g_prep, r, b->stream));
   BHIPCHK(amm::launch_bscan(b->dP, b->np, b->stream));
   BHIPCHK(amm::launch_bcompact(b->dP, b->g_prep, b->np, b->stream));

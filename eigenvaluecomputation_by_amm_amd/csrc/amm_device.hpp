@@ -243,7 +243,7 @@ __device__ __forceinline__ void store_vecf(float* p, const float (&v)[V]) {
 // values in LDS; a target is first located among the buckets in LDS, then
 // within its bucket (about n / K entries) in global memory.  Same result as
 // the plain search for any K: the smallest j with cdf[j] > t.
-constexpr uint32_t kCoarse = 4096;        // the batched sampler: 32 KB of LDS per 1024-thread block
+constexpr uint32_t kCoarse = 1024;        // the batched sampler: 8 KB of LDS beside its 64 KB count row
 constexpr uint32_t kCoarseSample = 8192;  // the single-vector sampler: 64 KB of LDS, buckets of n / 8192
 
 template <uint32_t K = kCoarse>
