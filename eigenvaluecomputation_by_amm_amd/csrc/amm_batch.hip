@@ -510,22 +510,46 @@ struct TileStage {
   static constexpr uint32_t kElems16 = 16 / (uint32_t)sizeof(T);
   u32x4 r[kLoads];
 
-  // cols: C column indices in LDS (0xFFFFFFFF: absent column, reads 0)
+  // cols: C column indices in LDS (0xFFFFFFFF: absent column, reads 0).
+  // load() issues every load unconditionally, from an address that is valid
+  // whatever the bounds (row 0 of column 0 for an absent column, row 0 of
+  // its column for a lane past d); store() then zeroes absent columns and
+  // reloads the lanes past or straddling d element by element, when the
+  // round's data is consumed anyway.  (A per-load branch in load() made a
+  // partial round wait for each load in turn: twice the time of k_baxpy at
+  // d not a multiple of its rows; fixing the lanes up in load() made every
+  // round wait for its loads before its MFMAs: 13% slower at full rounds.)
+  uint32_t lrow = 0;   // this lane's first row of the loaded round
+  uint32_t amask = 0;  // absent columns of the loaded round, bit q
   __device__ __forceinline__ void load(const T* __restrict__ A, uint64_t ld, const uint32_t* cols,
                                        uint32_t row0, uint32_t d) {
     const uint32_t sub = threadIdx.x % kLanesPerCol, c0 = threadIdx.x / kLanesPerCol;
-    const uint32_t row = row0 + sub * kElems16;
+    lrow = row0 + sub * kElems16;
+    const bool full = lrow + kElems16 <= d;
+    amask = 0u;
 #pragma unroll
     for (uint32_t q = 0; q < kLoads; ++q) {
-      const uint32_t j = cols[q * kColsPerPass + c0];
-      r[q] = j != 0xFFFFFFFFu ? load_a16<T, NT>(A + (uint64_t)j * ld, row, d) : u32x4{0u, 0u, 0u, 0u};
+      const uint32_t jq = cols[q * kColsPerPass + c0];
+      amask |= (jq == 0xFFFFFFFFu ? 1u : 0u) << q;
+      const uint32_t j = jq != 0xFFFFFFFFu ? jq : 0u;
+      r[q] = load16<NT>(A + (uint64_t)j * ld + (full ? lrow : 0u));
     }
   }
-  __device__ __forceinline__ void store(char* lds) const {
+  // cols: the same column indices the matching load() read
+  __device__ __forceinline__ void store(char* lds, const T* __restrict__ A, uint64_t ld, const uint32_t* cols,
+                                        uint32_t d) {
     const uint32_t sub = threadIdx.x % kLanesPerCol, c0 = threadIdx.x / kLanesPerCol;
+    const bool full = lrow + kElems16 <= d;
 #pragma unroll
-    for (uint32_t q = 0; q < kLoads; ++q)
-      *reinterpret_cast<u32x4*>(lds + (q * kColsPerPass + c0) * kPitch + sub * 16) = r[q];
+    for (uint32_t q = 0; q < kLoads; ++q) {
+      u32x4 v = r[q];
+      if ((amask >> q) & 1u) {
+        v = u32x4{0u, 0u, 0u, 0u};
+      } else if (!full) {
+        v = load_a16<T, NT>(A + (uint64_t)cols[q * kColsPerPass + c0] * ld, lrow, d);  // zeros at and beyond d
+      }
+      *reinterpret_cast<u32x4*>(lds + (q * kColsPerPass + c0) * kPitch + sub * 16) = v;
+    }
   }
 };
 
@@ -596,7 +620,7 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
     load_dx(r0);
   }
   for (uint32_t row0 = r0; row0 < r1; row0 += RB) {
-    st.store(lds);
+    st.store(lds, A, pr.ld, cols, pr.d);
     __syncthreads();
     const bool more = row0 + RB < r1;
     if (more) st.load(A, pr.ld, cols, row0 + RB, pr.d);
@@ -860,7 +884,7 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
         tf[c][k] = tt;
       }
     }
-    st.store(lds);
+    st.store(lds, A, pr.ld, cols[buf], d);
     if (threadIdx.x < C) cols[buf ^ 1u][threadIdx.x] = colnext;
     __syncthreads();
     const uint32_t sn = s + C;
