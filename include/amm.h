@@ -235,7 +235,7 @@ int amm_rank1_update_batched(void *out_dev, int32_t dtype, uint64_t d, uint64_t 
 typedef struct amm_problem {
   const void *A;   /* device matrix, d x n column-major, leading dimension ld */
   int32_t dtype;   /* AMM_DTYPE_F32 or AMM_DTYPE_BF16 */
-  uint32_t c;      /* sampled columns per trial and product (0: exact runs only) */
+  uint32_t c;      /* sampled columns per trial and product, <= min(n, 65535) (0: exact runs only) */
   uint64_t d, n, ld;
   uint64_t seed;   /* Philox key of this problem */
 } amm_problem;
