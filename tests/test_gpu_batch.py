@@ -198,6 +198,13 @@ def test_batch_invalid_arguments():
         AmmBatch([dict(A=A_t, c=10, seed=1)], trials=17)
     with pytest.raises(AmmError):
         AmmBatch([dict(A=A_t, c=129, seed=1)], trials=2)
+    # the draw counts are 16-bit: at most 65535 draws per trial
+    import torch
+    wide = torch.zeros((65536, 8), dtype=torch.float32, device="cuda")
+    with pytest.raises(AmmError):
+        AmmBatch([dict(A=wide, c=65536, seed=1)], trials=1)
+    with AmmBatch([dict(A=wide, c=65535, seed=1)], trials=1):
+        pass
     with AmmBatch([dict(A=A_t, c=10, seed=1)], trials=2) as b:
         with pytest.raises(AmmError):
             b.power_method(RunParams(kernel="previousAMM", c=10, step=3, sanity=True))
