@@ -43,12 +43,15 @@ def relmax(a, b):
 
 
 @pytest.mark.parametrize("kernel", ["previousAMM", "proposedAMM"])
-@pytest.mark.parametrize("c,trials", [(300, 16), (2000, 5), (8192, 3)])
-def test_union_sample_bitexact(kernel, c, trials):
-    banner(f"batch union sample {kernel} c={c} trials={trials}")
+@pytest.mark.parametrize("c,trials,n", [
+    (300, 16, 20000), (2000, 5, 20000), (8192, 3, 20000),   # counts built in LDS, 1-4 passes of 2048 draws
+    (300, 4, 40000), (8192, 3, 40000),                       # n > 32768: per-pass tables added into the row
+])
+def test_union_sample_bitexact(kernel, c, trials, n):
+    banner(f"batch union sample {kernel} c={c} trials={trials} n={n}")
     AmmBatch, AmmContext, *_ = _api()
-    A = heavy_columns(96, 20000, seed=11) + (0.2 if kernel == "proposedAMM" else 0.0)
-    A[:, [5, 19999]] = 0.0
+    A = heavy_columns(96, n, seed=11) + (0.2 if kernel == "proposedAMM" else 0.0)
+    A[:, [5, n - 1]] = 0.0
     seed, it, trial0 = 0xABCDEF12345, 7, 3
     A_t = dev(A)
     with AmmBatch([dict(A=A_t, c=c, seed=seed)], trials=trials) as b:
@@ -68,7 +71,7 @@ def test_union_sample_bitexact(kernel, c, trials):
     assert np.array_equal(cols, keys)
     Wref = np.stack([ref[int(j)] for j in keys])
     assert np.array_equal(W.view(np.uint32), Wref.view(np.uint32))
-    assert not np.isin([5, 19999], cols).any()
+    assert not np.isin([5, n - 1], cols).any()
 
 
 def _single(A_t, d, params, trial):
