@@ -388,7 +388,8 @@ class AmmBatch:
 
     `problems` is a list of dicts with keys A (a device tensor (n, ld) whose
     row j is column j of A, float32 or bfloat16), c (sampled columns per
-    product), seed, and optionally d (rows, default ld).  Trial k of problem
+    product, at most min(n, 65535): the engine's draw counts are 16-bit),
+    seed, and optionally d (rows, default ld).  Trial k of problem
     i reproduces the single-matrix run with seed = problem seed and trial =
     params.trial + k.
     """
