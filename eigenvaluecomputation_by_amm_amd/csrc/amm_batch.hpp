@@ -8,7 +8,8 @@
 //   T = W o (P - alpha s^T) (per-trial weights; proposed mean shift)
 //   Y = A_U T               (contraction 2, MFMA, split over column chunks)
 // Every launch covers all problems ("grouped" launches: a workgroup finds
-// its problem by binary search over per-problem item offsets).
+// its problem in a workgroup -> problem map that follows the per-problem
+// workgroup offsets).
 #pragma once
 
 #include <hip/hip_runtime.h>
