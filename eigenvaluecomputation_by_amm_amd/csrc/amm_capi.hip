@@ -381,7 +381,7 @@ bool fused_finish(const amm_ctx* c, bool sanity) { return !sanity && amm::axpy_f
 // sampled product with iteration `it`'s list at list slot `slot`; with
 // fused_finish the product also writes y = c->y and the finish partials
 int enq_amm(amm_ctx* c, uint32_t it, uint32_t slot, bool proposed, bool sanity,
-            const uint32_t* gate_mode, uint32_t gate_mask) {
+            const uint32_t* gate_mode, uint32_t gate_mask, const amm::GatherTail* tail = nullptr) {
   const uint32_t cc = c->list_c;
   const uint64_t nseg = (cc + amm::kSampleSeg - 1) / amm::kSampleSeg;
   const DotPlan dp = sanity ? c->dot_dense : c->dot_list;
@@ -402,6 +402,7 @@ int enq_amm(amm_ctx* c, uint32_t it, uint32_t slot, bool proposed, bool sanity,
   g.it = it;
   g.gate_mask = gate_mask;
   g.nt = (c->list_nt & 1u) != 0;
+  g.tail = tail;
   HIPCHK(amm::launch_gather_dot(c->dtype, g, c->stream));
   if (fused_finish(c, sanity)) {
     amm::AxpyFinishArgs f{};
@@ -488,7 +489,7 @@ enum FinishMode { kFinishAlways = 0, kFinishNone = 1, kFinishExactOnly = 2 };
 int enq_finish(amm_ctx* c, uint32_t it, bool sanity, float* y_out, const float* ym_base,
                uint32_t adapt_kernel, uint64_t period, double thr, uint32_t step,
                bool norm_inline = false, const float* zq = nullptr, bool stamp = false,
-               FinishMode fm = kFinishAlways) {
+               FinishMode fm = kFinishAlways, bool tail = true) {
   amm::FinishArgs f{};
   f.P2 = c->P2;
   f.S_exact = c->gram_S ? c->gram_S : c->axpy_dense.S;
@@ -508,6 +509,7 @@ int enq_finish(amm_ctx* c, uint32_t it, bool sanity, float* y_out, const float* 
   f.it = it;
   f.gate_mask = fm == kFinishExactOnly ? (1u << amm::kModeExactMemo) | (1u << amm::kModeExactNoMemo) : 0u;
   if (fm != kFinishNone) HIPCHK(amm::launch_finish(f, c->stream));
+  if (!tail) return AMM_OK;  // the next dot pass carries it
   amm::TailArgs t = tail_args(c, amm::kTailFinish, (uint32_t)amm::finish_blocks(c->dpad), it,
                               norm_inline, zq, stamp);
   t.y = y_out;
@@ -669,6 +671,16 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
   const uint32_t gate_exact = 1u << amm::kModeExactMemo;
   const uint32_t gate_amm = (1u << amm::kModeAmmPrevious) | (1u << amm::kModeAmmProposed);
   const bool fuse = fused_finish(c, sanity);
+  auto sampled = [&](uint32_t i) {
+    return modes[i] == amm::kModeAmmPrevious || modes[i] == amm::kModeAmmProposed;
+  };
+  // the tail of a sampled step followed by a sampled step (static schedule,
+  // fused finish, inline normalisation) rides in the next dot pass
+  const bool carry_ok = fuse && !adaptive && c->dpad <= kFuseNormRows &&
+                        c->dot_list.R <= amm::kTailMaxBlocks && c->dot_list.rows_per_split &&
+                        c->d % c->dot_list.rows_per_split == 0;
+  amm::GatherTail carried{};
+  bool have_carried = false;
   for (uint32_t i = 0; i < step; ++i) {
     bool z_after = false;
     FinishMode fm = kFinishAlways;
@@ -682,11 +694,33 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
         if ((rc = enq_exact(c, i, nullptr, 0))) return rc;
         z_after = K == AMM_KERNEL_PROPOSED_V2 && i + 1 == P && !fused_z;
       } else {
-        if ((rc = enq_amm(c, i, i, md == amm::kModeAmmProposed, sanity, nullptr, 0))) return rc;
+        if ((rc = enq_amm(c, i, i, md == amm::kModeAmmProposed, sanity, nullptr, 0,
+                          have_carried ? &carried : nullptr)))
+          return rc;
+        have_carried = false;
         if (fuse) fm = kFinishNone;
       }
     }
     const bool want_sq = K == AMM_KERNEL_PROPOSED || (K == AMM_KERNEL_PROPOSED_V2 && i + 1 >= P);
+    if (carry_ok && !z_after && i + 1 < step && sampled(i + 1)) {
+      // an exact step still runs its finish kernel (y and the block partials)
+      if (!sampled(i))
+        if ((rc = enq_finish(c, i, sanity, c->y, c->ym, 0u, P, 0.0, step, false, nullptr, false, fm, false)))
+          return rc;
+      carried.BP = c->BP;
+      carried.nb = (uint32_t)amm::finish_blocks(c->dpad);
+      carried.st = c->st;
+      carried.y = c->y;
+      carried.x = c->x;
+      carried.z = want_sq ? c->z : nullptr;
+      carried.eig = c->eig;
+      carried.err = c->err;
+      carried.ts = c->ts;
+      carried.it = i;
+      carried.exact = sampled(i) ? 0u : 1u;
+      have_carried = true;
+      continue;
+    }
     // the normalisation rides in the finish tail unless d is large or z is
     // only formed after this iteration's product (the v2 transition)
     const bool inline_norm = c->dpad <= kFuseNormRows && !z_after;

@@ -6,15 +6,18 @@
 // core_previousAMM.cuh:47,219).  Vectors of length d are FP32 and padded to
 // dpad = roundup(d, 8) with zeros, so 16-byte vector loads never leave them.
 //
-// One power iteration is four launches:
-//   gather_dot   P1[r][e] = a_{col(e)}[rows of split r]^T (x - x_memo)[rows]
-//   gather_axpy  t_e = wgt_e (sum_r P1[r][e] - alpha_col s);
-//                P2[s][rows] = sum_{e in chunk s} a_{col(e)}[rows] t_e
-//   finish       y = base + sum_s P2[s]; y^T x, y^T y, y^T v* (FP64);
-//                the last block to arrive writes lambda_i, err_i, cos_i
-//   normalize    x = y / ||y||  (+ s = 1^T dx, q = z^T dx for proposed)
-// The exact step uses the same two contraction kernels over all n columns
-// with unit weights (the sanity-mode identity of device_memory.h:500-551).
+// A sampled power iteration is two launches where d allows (8192 <= d,
+// d a multiple of the dot pass's row split, static step schedule):
+//   gather_dot   P1[r][e] = a_{col(e)}[rows of split r]^T (x - x_memo)[rows],
+//                carrying the previous iteration's tail: 1/||y||, lambda,
+//                err, x = y / ||y|| (+ s = 1^T dx, q = z^T dx for proposed)
+//   axpy_finish  t_e = wgt_e (sum_r P1[r][e] - alpha_col s); y = base +
+//                sum_e a_{col(e)} t_e; block partials of y^T x, y^T y, y^T v*
+// Otherwise the second contraction is chunked (gather_axpy + finish) and a
+// tail kernel closes every iteration.  The exact step is k_gram (A (A^T x)
+// in one read) up to d = 16384, else the same two contraction kernels over
+// all n columns with unit weights (the sanity-mode identity of
+// device_memory.h:500-551), then finish.
 // Every reduction runs in a fixed order, so results do not depend on timing
 // (the reference's kernel2 uses float atomics, core_previousAMM.cuh:259-262).
 #include "amm_device.hpp"
@@ -27,14 +30,20 @@
 
 namespace amm {
 
-// Column statistics: one workgroup per column (colstats_column).
+// Column statistics (colstats_column): workgroup b takes columns
+// [b K, (b + 1) K) in turn (K from colstats_per_block).  The 4-wave split of
+// each column and its sums do not depend on K, so neither do the bits.
 template <typename T>
 __global__ __launch_bounds__(256) void k_colstats(const T* __restrict__ A, uint64_t ld, uint32_t d,
-                                                  uint64_t n, float* __restrict__ alpha,
+                                                  uint64_t n, uint32_t K, float* __restrict__ alpha,
                                                   double* __restrict__ nrm2,
                                                   double* __restrict__ cnrm2) {
-  (void)n;
-  colstats_column<T>(A + (uint64_t)blockIdx.x * ld, d, blockIdx.x, alpha, nrm2, cnrm2);
+  const uint64_t j0 = (uint64_t)blockIdx.x * K;
+  const uint64_t j1 = j0 + K < n ? j0 + K : n;
+  for (uint64_t j = j0; j < j1; ++j) {
+    colstats_column<T>(A + j * ld, d, j, alpha, nrm2, cnrm2);
+    __syncthreads();  // the per-column exchange in LDS is reused by the next column
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -494,6 +503,63 @@ __device__ __forceinline__ bool gate_open(const uint32_t* mode, uint32_t it, uin
   return mode == nullptr || ((mask >> mode[it]) & 1u);
 }
 
+// Sums of the finish block partials (y^T x, y^T y, y^T v*: BP[3 i + k]) over
+// nb blocks, formed by one wave in a fixed order: lane l adds blocks l,
+// l + 64, l + 128, ... in turn (batches of four in flight; blocks past nb
+// read as 0), then a butterfly.  Every wave that forms them gets the same
+// bits, so the tail kernel and a dot pass that carries the tail agree
+// without a barrier.  finish_pre1 / finish_post1 form one of the three sums
+// in the same order, with the first batch loaded ahead.
+__device__ __forceinline__ double finish_part(const double* __restrict__ BP, uint32_t nb, uint32_t i, int k) {
+  const double v = BP[3 * (i < nb ? i : nb - 1) + k];  // always a valid address
+  return i < nb ? v : 0.0;
+}
+
+__device__ __forceinline__ double3 finish_sums_wave(const double* __restrict__ BP, uint32_t nb) {
+  const uint32_t lane = threadIdx.x & 63u;
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (uint32_t base = 0; base < nb; base += 256) {
+    double v[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) v[u][k] = finish_part(BP, nb, base + lane + 64 * u, k);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a += v[u][0];
+      b += v[u][1];
+      c += v[u][2];
+    }
+  }
+  return make_double3(wave_sum(a), wave_sum(b), wave_sum(c));
+}
+
+template <int K>
+__device__ __forceinline__ void finish_pre1(const double* __restrict__ BP, uint32_t nb, double (&p)[4]) {
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) p[u] = finish_part(BP, nb, lane + 64 * u, K);
+}
+
+template <int K>
+__device__ __forceinline__ double finish_post1(const double* __restrict__ BP, uint32_t nb, const double (&p)[4]) {
+  const uint32_t lane = threadIdx.x & 63u;
+  double a = 0.0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) a += p[u];
+  for (uint32_t base = 256; base < nb; base += 256) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = finish_part(BP, nb, base + lane + 64 * u, K);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a += v[u];
+  }
+  return wave_sum(a);
+}
+
+// 1/||y|| from y^T y, as every consumer of the finish sums forms it
+__device__ __forceinline__ double inv_norm(double yy) { return yy > 0.0 ? 1.0 / sqrt(yy) : 0.0; }
+
 // ---------------------------------------------------------------------------
 // Contraction 1 (previousAMM_kernel1 / proposedAMM_kernel1,
 // core_previousAMM.cuh:4-175, core_proposedAMM.cuh:17-213): for list entry e,
@@ -501,13 +567,19 @@ __device__ __forceinline__ bool gate_open(const uint32_t* mode, uint32_t it, uin
 // entries; lanes stride down the column with 16-byte loads; wave butterfly.
 // LIST: col(e) = idx[e], valid when (e % seg) < seg_count[e / seg];
 // dense: col(e) = e (exact step, sanity mode).
+// FUSE: the previous iteration's tail rides in this pass (GatherTail; list
+// mode with x_memo, d a multiple of the split): every wave
+// reduces the finish partials to 1/||y|| and forms x = (float)(y / ||y||) as
+// k_tail forms it; one extra workgroup per split writes x over the split's
+// rows and, with z, part r of s and q, and split 0's writes lambda, err,
+// 1/||y||, cos and the time stamp of the previous iteration.
 // ---------------------------------------------------------------------------
-template <typename T, int CPW, int U, bool LIST, bool NT>
+template <typename T, int CPW, int U, bool LIST, bool NT, bool FUSE>
 __global__ __launch_bounds__(256) void k_gather_dot(
     const T* __restrict__ A, uint64_t ld, uint32_t d, const uint32_t* __restrict__ idx,
     const uint32_t* __restrict__ seg_count, uint32_t cap, const float* __restrict__ x,
     const float* __restrict__ xm, uint32_t rows_per_split, float* __restrict__ P1,
-    const uint32_t* __restrict__ mode, uint32_t it, uint32_t gate_mask) {
+    const uint32_t* __restrict__ mode, uint32_t it, uint32_t gate_mask, GatherTail gt) {
   if (!gate_open(mode, it, gate_mask)) return;
   constexpr int V = Elem<T>::VEC;
   const int lane = threadIdx.x & 63;
@@ -515,6 +587,58 @@ __global__ __launch_bounds__(256) void k_gather_dot(
   const uint32_t r = blockIdx.y;
   const uint32_t rb = r * rows_per_split;
   const uint32_t re = (rb + rows_per_split < d) ? rb + rows_per_split : d;
+  if constexpr (FUSE) {
+    if (blockIdx.x == gridDim.x - 1) {
+      // the extra workgroup of split r: x over the split's rows (V rows per
+      // thread: the split is 256 V rows), with z its part of s and q; split
+      // 0's also the scalars of the previous iteration
+      __shared__ double red[2][4];
+      const uint32_t row = rb + threadIdx.x * V;
+      float yv[V], mv[V], zv[V];
+      load_vecf<V>(gt.y + row, yv);
+      load_vecf<V>(xm + row, mv);
+      if (gt.z) load_vecf<V>(gt.z + row, zv);
+      const double3 f = finish_sums_wave(gt.BP, gt.nb);
+      const double scale = inv_norm(f.y);
+      double ps = 0.0, pq = 0.0;
+#pragma unroll
+      for (int k = 0; k < V; ++k) yv[k] = (float)(scale * (double)yv[k]);
+      store_vecf<V>(gt.x + row, yv);
+      if (gt.z) {
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+          const double dd = (double)yv[k] - (double)mv[k];
+          ps += dd;
+          pq += (double)zv[k] * dd;
+        }
+        ps = wave_sum(ps);
+        pq = wave_sum(pq);
+        if (lane == 0) {
+          red[0][threadIdx.x >> 6] = ps;
+          red[1][threadIdx.x >> 6] = pq;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          gt.st->sq_part[r][0] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+          gt.st->sq_part[r][1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+        }
+      }
+      if (r == 0 && threadIdx.x == 0) {
+        if (gt.z) gt.st->sq_n = gridDim.y;
+        gt.eig[gt.it] = f.x;
+        gt.err[gt.it] = f.z * scale;
+        gt.st->scale = scale;
+        gt.st->cos_prev = f.x * scale;
+        if (gt.exact) gt.st->last_exact = gt.it;
+        if (gt.ts) gt.ts[gt.it] = wall_clock64();
+      }
+      return;
+    }
+  }
+  // FUSE: y^T y of the previous product, its first loads issued beside the
+  // sample list's (summed once the columns are in flight)
+  double yy_pre[4];
+  if constexpr (FUSE) finish_pre1<1>(gt.BP, gt.nb, yy_pre);
   const T* col[CPW];
   bool valid[CPW];
   bool any = false;
@@ -538,22 +662,18 @@ __global__ __launch_bounds__(256) void k_gather_dot(
   float acc[CPW];
 #pragma unroll
   for (int c = 0; c < CPW; ++c) acc[c] = 0.0f;
-  uint32_t row = rb + lane * V;
   constexpr uint32_t STEP = 64u * V;
-  // U wave-steps per trip: all loads of the trip issued before the FMAs
-  for (; row + (U - 1) * STEP + V <= re; row += U * STEP) {
-    float xs[U][V];
+  // FUSE: every split is exactly one trip (the host carries the tail only
+  // where d is a multiple of the split), so each wave makes one trip: the
+  // loads of y, x_memo (z for the x writer) and the columns, then x =
+  // (float)(y / ||y||) as k_tail forms it, dx = x - x_memo, the FMAs
+  if constexpr (FUSE) {
+    const uint32_t row = rb + lane * V;
+    float ys[U][V], ms[U][V];
 #pragma unroll
-    for (int u = 0; u < U; ++u) load_vecf<V>(x + row + u * STEP, xs[u]);
-    if (xm) {
+    for (int u = 0; u < U; ++u) load_vecf<V>(gt.y + row + u * STEP, ys[u]);
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        float m[V];
-        load_vecf<V>(xm + row + u * STEP, m);
-#pragma unroll
-        for (int k = 0; k < V; ++k) xs[u][k] -= m[k];
-      }
-    }
+    for (int u = 0; u < U; ++u) load_vecf<V>(xm + row + u * STEP, ms[u]);
     float a[CPW][U][V];
 #pragma unroll
     for (int c = 0; c < CPW; ++c) {
@@ -562,35 +682,78 @@ __global__ __launch_bounds__(256) void k_gather_dot(
         for (int u = 0; u < U; ++u) Elem<T>::template load_full<NT>(col[c] + row + u * STEP, a[c][u]);
       }
     }
+    const double scale = inv_norm(finish_post1<1>(gt.BP, gt.nb, yy_pre));
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int k = 0; k < V; ++k) ys[u][k] = (float)(scale * (double)ys[u][k]) - ms[u][k];
+    }
 #pragma unroll
     for (int c = 0; c < CPW; ++c) {
       if (valid[c]) {
-        float s0 = acc[c];
+        float s0 = 0.0f;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
 #pragma unroll
-          for (int k = 0; k < V; ++k) s0 = fmaf(a[c][u][k], xs[u][k], s0);
+          for (int k = 0; k < V; ++k) s0 = fmaf(a[c][u][k], ys[u][k], s0);
         }
         acc[c] = s0;
       }
     }
-  }
-  for (; row < re; row += STEP) {
-    float x0[V];
-    load_vecf<V>(x + row, x0);
-    if (xm) {
-      float m0[V];
-      load_vecf<V>(xm + row, m0);
+  } else {
+    uint32_t row = rb + lane * V;
+    // U wave-steps per trip: all loads of the trip issued before the FMAs
+    for (; row + (U - 1) * STEP + V <= re; row += U * STEP) {
+      float xs[U][V];
 #pragma unroll
-      for (int k = 0; k < V; ++k) x0[k] -= m0[k];
+      for (int u = 0; u < U; ++u) load_vecf<V>(x + row + u * STEP, xs[u]);
+      if (xm) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          float m[V];
+          load_vecf<V>(xm + row + u * STEP, m);
+#pragma unroll
+          for (int k = 0; k < V; ++k) xs[u][k] -= m[k];
+        }
+      }
+      float a[CPW][U][V];
+#pragma unroll
+      for (int c = 0; c < CPW; ++c) {
+        if (valid[c]) {
+#pragma unroll
+          for (int u = 0; u < U; ++u) Elem<T>::template load_full<NT>(col[c] + row + u * STEP, a[c][u]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < CPW; ++c) {
+        if (valid[c]) {
+          float s0 = acc[c];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int k = 0; k < V; ++k) s0 = fmaf(a[c][u][k], xs[u][k], s0);
+          }
+          acc[c] = s0;
+        }
+      }
     }
+    for (; row < re; row += STEP) {
+      float x0[V];
+      load_vecf<V>(x + row, x0);
+      if (xm) {
+        float m0[V];
+        load_vecf<V>(xm + row, m0);
 #pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-      if (valid[c]) {
-        float a0[V];
-        load_col<T>(col[c], row, re, a0);
+        for (int k = 0; k < V; ++k) x0[k] -= m0[k];
+      }
 #pragma unroll
-        for (int k = 0; k < V; ++k) acc[c] = fmaf(a0[k], x0[k], acc[c]);
+      for (int c = 0; c < CPW; ++c) {
+        if (valid[c]) {
+          float a0[V];
+          load_col<T>(col[c], row, re, a0);
+#pragma unroll
+          for (int k = 0; k < V; ++k) acc[c] = fmaf(a0[k], x0[k], acc[c]);
+        }
       }
     }
   }
@@ -1043,30 +1206,13 @@ __global__ __launch_bounds__(256) void k_tail(
     }
   }
   if (op == kTailFinish) {
-    double a = 0.0, b = 0.0, c = 0.0;
-    for (uint32_t i = t; i < nb; i += 256) {
-      a += BP[3 * i];
-      b += BP[3 * i + 1];
-      c += BP[3 * i + 2];
-    }
-    // the three block sums in one exchange (each in block256_sum's order)
-    a = wave_sum(a);
-    b = wave_sum(b);
-    c = wave_sum(c);
-    if ((t & 63) == 0) {
-      red3[0][t >> 6] = a;
-      red3[1][t >> 6] = b;
-      red3[2][t >> 6] = c;
-    }
-    __syncthreads();
-    if (t == 0) {
-      a = (red3[0][0] + red3[0][1]) + (red3[0][2] + red3[0][3]);
-      b = (red3[1][0] + red3[1][1]) + (red3[1][2] + red3[1][3]);
-      c = (red3[2][0] + red3[2][1]) + (red3[2][2] + red3[2][3]);
-    }
-    if (t == 0) {
-      const double scale = b > 0.0 ? 1.0 / sqrt(b) : 0.0;
-      scale_l = scale;
+    // wave 0 alone reduces (the order a dot pass carrying this tail repeats)
+    double a = 0.0, c = 0.0;
+    if (t < 64) {
+      const double3 f = finish_sums_wave(BP, nb);
+      a = f.x;
+      c = f.z;
+      if (t == 0) scale_l = inv_norm(f.y);
     }
     if (t == 0 && lead) {
       const uint32_t md = mode[it];
@@ -1517,16 +1663,25 @@ hipError_t launch_gram(int dtype, const void* A, uint64_t ld, uint32_t d, uint64
   return hipGetLastError();
 }
 
-// Without z the streamed form (one workgroup per column) is as fast or
-// faster at every shape measured.
+// Columns per workgroup of the statistics pass: about 512 workgroups (two
+// per CU) for FP32 columns of 64 KB or more (measured 2-6% faster at d =
+// 16384, 32768, 65536), else one workgroup per column (FP32 columns of 16 KB
+// and BF16 columns of 32-64 KB measured 5-20% slower blocked).  Without z
+// this streamed form is as fast or faster than the column-resident one at
+// every shape measured.
+static uint32_t colstats_per_block(int dtype, uint32_t d, uint64_t n) {
+  if (dtype == 1 || (uint64_t)d * 4u < 65536) return 1;
+  return (uint32_t)std::max<uint64_t>(1, n / 512);
+}
 hipError_t launch_colstats(int dtype, const void* A, uint64_t ld, uint32_t d, uint64_t n,
                            float* alpha, double* nrm2, double* cnrm2, DevState* st, hipStream_t s) {
   (void)st;
-  const dim3 g((unsigned)n);  // one workgroup per column, streamed
+  const uint32_t K = colstats_per_block(dtype, d, n);
+  const unsigned g = (unsigned)((n + K - 1) / K);
   if (dtype == 1)
-    k_colstats<__hip_bfloat16><<<g, 256, 0, s>>>((const __hip_bfloat16*)A, ld, d, n, alpha, nrm2, cnrm2);
+    k_colstats<__hip_bfloat16><<<g, 256, 0, s>>>((const __hip_bfloat16*)A, ld, d, n, K, alpha, nrm2, cnrm2);
   else
-    k_colstats<float><<<g, 256, 0, s>>>((const float*)A, ld, d, n, alpha, nrm2, cnrm2);
+    k_colstats<float><<<g, 256, 0, s>>>((const float*)A, ld, d, n, K, alpha, nrm2, cnrm2);
   return hipGetLastError();
 }
 
@@ -1565,9 +1720,19 @@ hipError_t launch_begin(float* xm, float* ym, uint64_t dpad, DevState* st, uint3
 template <typename T, int CPW, int U, bool LIST, bool NT>
 static void gd(const GatherDotArgs& a, hipStream_t s) {
   const dim3 g((a.cap + 4 * CPW - 1) / (4 * CPW), a.R);
-  k_gather_dot<T, CPW, U, LIST, NT><<<g, 256, 0, s>>>((const T*)a.A, a.ld, a.d, a.idx, a.seg_count,
-                                                      a.cap, a.x, a.xm, a.rows_per_split, a.P1, a.mode,
-                                                      a.it, a.gate_mask);
+  if constexpr (LIST) {
+    if (a.tail) {  // one more workgroup per split: the x writer
+      const dim3 gw(g.x + 1, g.y);
+      k_gather_dot<T, CPW, U, LIST, NT, true><<<gw, 256, 0, s>>>(
+          (const T*)a.A, a.ld, a.d, a.idx, a.seg_count, a.cap, a.x, a.xm, a.rows_per_split, a.P1,
+          a.mode, a.it, a.gate_mask, *a.tail);
+      return;
+    }
+  }
+  k_gather_dot<T, CPW, U, LIST, NT, false><<<g, 256, 0, s>>>((const T*)a.A, a.ld, a.d, a.idx,
+                                                             a.seg_count, a.cap, a.x, a.xm,
+                                                             a.rows_per_split, a.P1, a.mode, a.it,
+                                                             a.gate_mask, GatherTail{});
 }
 
 template <typename T>
@@ -1582,6 +1747,12 @@ static void gd_any(const GatherDotArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_gather_dot(int dtype, const GatherDotArgs& a, hipStream_t s) {
+  // the carried tail needs whole one-trip splits and writes one s, q part per
+  // split (at most kTailMaxBlocks)
+  if (a.tail && (!a.idx || !a.xm || a.R > kTailMaxBlocks || a.rows_per_split == 0 ||
+                 a.d % a.rows_per_split != 0 ||
+                 a.rows_per_split != 64u * (dtype == 1 ? 8u : 4u) * (uint32_t)kListDotSteps))
+    return hipErrorInvalidValue;
   if (dtype == 1) gd_any<__hip_bfloat16>(a, s);
   else gd_any<float>(a, s);
   return hipGetLastError();

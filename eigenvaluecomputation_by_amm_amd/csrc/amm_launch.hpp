@@ -12,6 +12,27 @@ struct DevState;
 // plan splits rows so that one wave covers one trip
 constexpr int kListDotSteps = 4;
 
+// The tail of an iteration carried by the next sampled dot pass (static
+// schedules, any step followed by a sampled step): every wave
+// reduces the finish partials BP to 1/||y|| itself (finish_sums_wave, the
+// k_tail order) and forms x = y / ||y|| for its rows on the fly; the first
+// workgroup of each row split writes x (and, with z, its part r of s = 1^T dx
+// and q = z^T dx) and the first workgroup writes the iteration's scalars --
+// what k_tail(kTailFinish, norm) writes, without its launch.
+struct GatherTail {
+  const double* BP;  // finish block partials of iteration it (y^T x, y^T y, y^T v*)
+  uint32_t nb;
+  DevState* st;
+  const float* y;    // the product of iteration it
+  float* x;          // written: y / ||y||
+  const float* z;    // proposed: z (s, q parts per row split), else null
+  double* eig;
+  double* err;
+  int64_t* ts;       // time stamp of iteration it (null: none)
+  uint32_t it;
+  uint32_t exact;    // iteration it was an exact step (last_exact = it)
+};
+
 struct GatherDotArgs {
   const void* A;
   uint64_t ld;
@@ -28,6 +49,7 @@ struct GatherDotArgs {
   uint32_t it;
   uint32_t gate_mask;
   bool nt;                    // list mode: non-temporal loads of A
+  const GatherTail* tail;     // null: x is given; else the previous iteration's tail (list mode)
 };
 
 struct GatherAxpyArgs {

@@ -533,6 +533,47 @@ def test_fused_finish_runs_against_oracle(run):
     assert np.array_equal(res.eig, eager.eig)
 
 
+@pytest.mark.parametrize("bf16", [False, True])
+@pytest.mark.parametrize("run", [dict(kernel="previousAMM", c=600, step=14, vr_period=4),
+                                 dict(kernel="proposedAMM", c=600, step=14, vr_period=4),
+                                 dict(kernel="proposedAMM_v2", c=600, step=14, vr_period=4)],
+                         ids=["previous", "proposed", "proposed_v2"])
+def test_carried_tail_runs_against_oracle(run, bf16):
+    """Runs where each iteration's tail (lambda, err, 1/||y||, x = y / ||y||
+    and the proposed s, q) rides in the next sampled dot pass: d = 8192 is a
+    multiple of the dot pass's row split (1024 FP32 / 2048 BF16 rows) and the
+    schedule is static, so sampled -> sampled and exact -> sampled steps both
+    carry it.  Same samples as the oracle: identical step kinds, Rayleigh
+    quotients and err = y^T v* / ||y|| within 1e-4; graph and eager runs bit
+    for bit."""
+    banner(f"carried tail run {run} bf16={bf16}")
+    AmmContext, RunParams, _ = _amm()
+    d, n = 8192, 3000
+    A = lowrank_plus_noise(d, n, seed=30) + 0.05
+    t, cols = dev_matrix(A, bf16=bf16)
+    colsf = bf16_to_f32(cols) if bf16 else cols
+    A64 = colsf.T.astype(np.float64)  # d x n
+    v = np.random.default_rng(31).standard_normal(d)
+    for _ in range(3):  # near the top left singular vector: err ~ 1, not ~ 0
+        v = A64 @ (A64.T @ v)
+        v /= np.linalg.norm(v)
+    with AmmContext(t) as ctx:
+        ctx.set_exact_eigenvector(v)
+        _, nrm2, cnrm2, _ = ctx.column_stats()
+        res = ctx.power_method(RunParams(seed=32, **run))
+        modes = ctx.last_modes(run["step"])
+        eager = ctx.power_method(RunParams(seed=32, use_graph=False, **run))
+    ref = O.power_method(colsf, d, seed=32, w_plain=nrm2, w_cent=cnrm2, eigvec=v, **run)
+    dm = modes.copy()
+    dm[dm == 4] = 1  # the device's exact step without memo (v2 transition) is the oracle's 1
+    assert np.array_equal(dm, ref["modes"].astype(np.uint32))
+    assert ((modes == 2) | (modes == 3)).sum() >= 8  # sampled steps carry the tails
+    assert np.max(np.abs(res.eig - ref["eig"]) / np.abs(ref["eig"])) < 1e-4
+    assert np.max(np.abs(res.err - ref["err"])) < 1e-4
+    assert np.abs(ref["err"][-1]) > 0.5  # err is not trivially zero
+    assert np.array_equal(res.eig, eager.eig) and np.array_equal(res.err, eager.err)
+
+
 def test_lanczos_ground_truth_against_eigvalsh():
     """The benchmark's FP64 ground truth (workloads.lambda_max_lanczos, GPU)
     against FP64 eigvalsh of A A^T (the computation of
