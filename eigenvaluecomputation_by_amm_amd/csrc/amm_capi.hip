@@ -666,21 +666,34 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
         if ((rc = enq_sample(c, 1, (uint32_t)P, step - (uint32_t)P, 0, p.c))) return rc;
     }
   }
-  // x0 (curandGenerateNormal + normalize<512>)
-  if ((rc = enq_x0(c, K == AMM_KERNEL_PROPOSED ? c->z : nullptr))) return rc;
   const uint32_t gate_exact = 1u << amm::kModeExactMemo;
   const uint32_t gate_amm = (1u << amm::kModeAmmPrevious) | (1u << amm::kModeAmmProposed);
   const bool fuse = fused_finish(c, sanity);
   auto sampled = [&](uint32_t i) {
     return modes[i] == amm::kModeAmmPrevious || modes[i] == amm::kModeAmmProposed;
   };
-  // the tail of a sampled step followed by a sampled step (static schedule,
-  // fused finish, inline normalisation) rides in the next dot pass
+  // the tail of a step followed by a sampled step (static schedule, fused
+  // finish, inline normalisation, d a multiple of the dot pass's row split)
+  // rides in the next dot pass, and so does x0's normalisation
   const bool carry_ok = fuse && !adaptive && c->dpad <= kFuseNormRows &&
                         c->dot_list.R <= amm::kTailMaxBlocks && c->dot_list.rows_per_split &&
                         c->d % c->dot_list.rows_per_split == 0;
   amm::GatherTail carried{};
   bool have_carried = false;
+  // x0 (curandGenerateNormal + normalize<512>)
+  const float* z0 = K == AMM_KERNEL_PROPOSED ? c->z : nullptr;
+  if (carry_ok && sampled(0)) {
+    carried.BP = c->BP;
+    carried.nb = (uint32_t)amm::x0_blocks(c->dpad);
+    carried.st = c->st;
+    carried.y = c->y;
+    carried.x = c->x;
+    carried.z = z0;
+    carried.x0 = 1;
+    have_carried = true;
+  } else if ((rc = enq_x0(c, z0))) {
+    return rc;
+  }
   for (uint32_t i = 0; i < step; ++i) {
     bool z_after = false;
     FinishMode fm = kFinishAlways;
@@ -718,6 +731,7 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
       carried.ts = c->ts;
       carried.it = i;
       carried.exact = sampled(i) ? 0u : 1u;
+      carried.x0 = 0;
       have_carried = true;
       continue;
     }

@@ -503,15 +503,17 @@ __device__ __forceinline__ bool gate_open(const uint32_t* mode, uint32_t it, uin
   return mode == nullptr || ((mask >> mode[it]) & 1u);
 }
 
-// Sums of the finish block partials (y^T x, y^T y, y^T v*: BP[3 i + k]) over
-// nb blocks, formed by one wave in a fixed order: lane l adds blocks l,
-// l + 64, l + 128, ... in turn (batches of four in flight; blocks past nb
-// read as 0), then a butterfly.  Every wave that forms them gets the same
-// bits, so the tail kernel and a dot pass that carries the tail agree
-// without a barrier.  finish_pre1 / finish_post1 form one of the three sums
-// in the same order, with the first batch loaded ahead.
-__device__ __forceinline__ double finish_part(const double* __restrict__ BP, uint32_t nb, uint32_t i, int k) {
-  const double v = BP[3 * (i < nb ? i : nb - 1) + k];  // always a valid address
+// Sums of block partials (the finish's y^T x, y^T y, y^T v*: BP[3 i + k];
+// the run prologue's sums of squares of x0: BP[i]) over nb blocks, formed by
+// one wave in a fixed order: lane l adds blocks l, l + 64, l + 128, ... in
+// turn (batches of four in flight; blocks past nb read as 0), then a
+// butterfly.  Every wave that forms them gets the same bits, so the tail
+// kernel and a dot pass that carries the tail agree without a barrier.
+// part_pre / part_post form one sum in the same order, with the first batch
+// loaded ahead.
+template <int S>
+__device__ __forceinline__ double block_part(const double* __restrict__ BP, uint32_t nb, uint32_t i, int k) {
+  const double v = BP[S * (i < nb ? i : nb - 1) + k];  // always a valid address
   return i < nb ? v : 0.0;
 }
 
@@ -523,7 +525,7 @@ __device__ __forceinline__ double3 finish_sums_wave(const double* __restrict__ B
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) v[u][k] = finish_part(BP, nb, base + lane + 64 * u, k);
+      for (int k = 0; k < 3; ++k) v[u][k] = block_part<3>(BP, nb, base + lane + 64 * u, k);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       a += v[u][0];
@@ -534,15 +536,15 @@ __device__ __forceinline__ double3 finish_sums_wave(const double* __restrict__ B
   return make_double3(wave_sum(a), wave_sum(b), wave_sum(c));
 }
 
-template <int K>
-__device__ __forceinline__ void finish_pre1(const double* __restrict__ BP, uint32_t nb, double (&p)[4]) {
+template <int S, int K>
+__device__ __forceinline__ void part_pre(const double* __restrict__ BP, uint32_t nb, double (&p)[4]) {
   const uint32_t lane = threadIdx.x & 63u;
 #pragma unroll
-  for (int u = 0; u < 4; ++u) p[u] = finish_part(BP, nb, lane + 64 * u, K);
+  for (int u = 0; u < 4; ++u) p[u] = block_part<S>(BP, nb, lane + 64 * u, K);
 }
 
-template <int K>
-__device__ __forceinline__ double finish_post1(const double* __restrict__ BP, uint32_t nb, const double (&p)[4]) {
+template <int S, int K>
+__device__ __forceinline__ double part_post(const double* __restrict__ BP, uint32_t nb, const double (&p)[4]) {
   const uint32_t lane = threadIdx.x & 63u;
   double a = 0.0;
 #pragma unroll
@@ -550,11 +552,18 @@ __device__ __forceinline__ double finish_post1(const double* __restrict__ BP, ui
   for (uint32_t base = 256; base < nb; base += 256) {
     double v[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = finish_part(BP, nb, base + lane + 64 * u, K);
+    for (int u = 0; u < 4; ++u) v[u] = block_part<S>(BP, nb, base + lane + 64 * u, K);
 #pragma unroll
     for (int u = 0; u < 4; ++u) a += v[u];
   }
   return wave_sum(a);
+}
+
+// the sum of squares of the raw x0 (stride 1), in the same order
+__device__ __forceinline__ double x0_sumsq_wave(const double* __restrict__ BP, uint32_t nb) {
+  double p[4];
+  part_pre<1, 0>(BP, nb, p);
+  return part_post<1, 0>(BP, nb, p);
 }
 
 // 1/||y|| from y^T y, as every consumer of the finish sums forms it
@@ -598,7 +607,8 @@ __global__ __launch_bounds__(256) void k_gather_dot(
       load_vecf<V>(gt.y + row, yv);
       load_vecf<V>(xm + row, mv);
       if (gt.z) load_vecf<V>(gt.z + row, zv);
-      const double3 f = finish_sums_wave(gt.BP, gt.nb);
+      const double3 f = gt.x0 ? make_double3(0.0, x0_sumsq_wave(gt.BP, gt.nb), 0.0)
+                              : finish_sums_wave(gt.BP, gt.nb);
       const double scale = inv_norm(f.y);
       double ps = 0.0, pq = 0.0;
 #pragma unroll
@@ -625,12 +635,14 @@ __global__ __launch_bounds__(256) void k_gather_dot(
       }
       if (r == 0 && threadIdx.x == 0) {
         if (gt.z) gt.st->sq_n = gridDim.y;
-        gt.eig[gt.it] = f.x;
-        gt.err[gt.it] = f.z * scale;
         gt.st->scale = scale;
-        gt.st->cos_prev = f.x * scale;
-        if (gt.exact) gt.st->last_exact = gt.it;
-        if (gt.ts) gt.ts[gt.it] = wall_clock64();
+        if (!gt.x0) {
+          gt.eig[gt.it] = f.x;
+          gt.err[gt.it] = f.z * scale;
+          gt.st->cos_prev = f.x * scale;
+          if (gt.exact) gt.st->last_exact = gt.it;
+          if (gt.ts) gt.ts[gt.it] = wall_clock64();
+        }
       }
       return;
     }
@@ -638,7 +650,12 @@ __global__ __launch_bounds__(256) void k_gather_dot(
   // FUSE: y^T y of the previous product, its first loads issued beside the
   // sample list's (summed once the columns are in flight)
   double yy_pre[4];
-  if constexpr (FUSE) finish_pre1<1>(gt.BP, gt.nb, yy_pre);
+  if constexpr (FUSE) {
+    if (gt.x0)
+      part_pre<1, 0>(gt.BP, gt.nb, yy_pre);
+    else
+      part_pre<3, 1>(gt.BP, gt.nb, yy_pre);
+  }
   const T* col[CPW];
   bool valid[CPW];
   bool any = false;
@@ -682,7 +699,8 @@ __global__ __launch_bounds__(256) void k_gather_dot(
         for (int u = 0; u < U; ++u) Elem<T>::template load_full<NT>(col[c] + row + u * STEP, a[c][u]);
       }
     }
-    const double scale = inv_norm(finish_post1<1>(gt.BP, gt.nb, yy_pre));
+    const double scale =
+        inv_norm(gt.x0 ? part_post<1, 0>(gt.BP, gt.nb, yy_pre) : part_post<3, 1>(gt.BP, gt.nb, yy_pre));
 #pragma unroll
     for (int u = 0; u < U; ++u) {
 #pragma unroll
@@ -1233,13 +1251,13 @@ __global__ __launch_bounds__(256) void k_tail(
       }
     }
   } else if (op == kTailScale) {
-    double s = 0.0;
-    for (uint32_t i = t; i < nb; i += 256) s += BP[i];
-    s = block256_sum(s, red);
-    if (t == 0) {
-      const double scale = s > 0.0 ? 1.0 / sqrt(s) : 0.0;
-      if (lead) st->scale = scale;
-      scale_l = scale;
+    // wave 0 alone reduces (the order a dot pass carrying this tail repeats)
+    if (t < 64) {
+      const double scale = inv_norm(x0_sumsq_wave(BP, nb));
+      if (t == 0) {
+        if (lead) st->scale = scale;
+        scale_l = scale;
+      }
     }
   } else if (op == kTailSQ) {
     double a = 0.0, b = 0.0;

@@ -31,6 +31,8 @@ struct GatherTail {
   int64_t* ts;       // time stamp of iteration it (null: none)
   uint32_t it;
   uint32_t exact;    // iteration it was an exact step (last_exact = it)
+  uint32_t x0;       // the raw x0 instead: BP holds its per-block sums of squares
+                     // (1 double per block); only x, s, q and 1/||x0|| are written
 };
 
 struct GatherDotArgs {
