@@ -1153,12 +1153,22 @@ __global__ __launch_bounds__(256) void k_bcolstats(const BSetup* __restrict__ S,
   const uint32_t p = find_problem(off, np, blockIdx.x);
   const BSetup& q = S[p];
   if (skip_fusable && q.fusable) return;
-  const uint64_t j = blockIdx.x - off[p];
+  const uint64_t b = blockIdx.x - off[p];
+  if (short_column_dt(q.dtype, q.d)) {  // a wave per column, four per workgroup (as k_colstats)
+    const uint64_t j = b * 4 + (threadIdx.x >> 6);
+    if (j >= q.n) return;
+    if (q.dtype == 1)
+      colstats_wave<__hip_bfloat16>(reinterpret_cast<const __hip_bfloat16*>(q.A) + j * q.ld, q.d, j, q.alpha,
+                                    q.nrm2, q.cnrm2);
+    else
+      colstats_wave<float>(reinterpret_cast<const float*>(q.A) + j * q.ld, q.d, j, q.alpha, q.nrm2, q.cnrm2);
+    return;
+  }
   if (q.dtype == 1)
-    colstats_column<__hip_bfloat16>(reinterpret_cast<const __hip_bfloat16*>(q.A) + j * q.ld, q.d, j, q.alpha,
+    colstats_column<__hip_bfloat16>(reinterpret_cast<const __hip_bfloat16*>(q.A) + b * q.ld, q.d, b, q.alpha,
                                     q.nrm2, q.cnrm2);
   else
-    colstats_column<float>(reinterpret_cast<const float*>(q.A) + j * q.ld, q.d, j, q.alpha, q.nrm2, q.cnrm2);
+    colstats_column<float>(reinterpret_cast<const float*>(q.A) + b * q.ld, q.d, b, q.alpha, q.nrm2, q.cnrm2);
 }
 
 __global__ __launch_bounds__(256) void k_bcdf_max(const BSetup* __restrict__ S, const uint32_t* off,

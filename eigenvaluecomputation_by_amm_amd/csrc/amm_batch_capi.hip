@@ -20,6 +20,7 @@
 #include "amm_batch.hpp"
 #include "amm_device.hpp"
 #include "amm_launch.hpp"
+#include "amm_setup.hpp"
 
 namespace amm_host {
 int fail(int code, const std::string& msg);
@@ -254,7 +255,10 @@ int upload_plan(amm_batch* b, int kernel) {
     return true;
   };
   bool ok = true;
-  ok &= add_group(b->g_cols, [](const BProb& p) { return p.n; });
+  // statistics: a workgroup per column, or per four short columns (a wave each)
+  ok &= add_group(b->g_cols, [](const BProb& p) {
+    return amm::short_column_dt(p.dtype, p.d) ? (p.n + 3) / 4 : p.n;
+  });
   ok &= add_group(b->g_tiles, [](const BProb& p) { return (uint32_t)amm::cdf_tiles(p.n); });
   ok &= add_group(b->g_zv, [](const BProb& p) { return (uint32_t)cdiv(rup(p.d, 8), 256); });
   ok &= add_group(b->g_vec, [](const BProb& p) { return p.nvblk; });

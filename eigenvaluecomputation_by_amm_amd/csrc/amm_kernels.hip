@@ -30,14 +30,21 @@
 
 namespace amm {
 
-// Column statistics (colstats_column): workgroup b takes columns
-// [b K, (b + 1) K) in turn (K from colstats_per_block).  The 4-wave split of
-// each column and its sums do not depend on K, so neither do the bits.
+// Column statistics: short columns (colstats_wave) a wave each, four per
+// workgroup; longer ones (colstats_column) a workgroup each, workgroup b
+// taking columns [b K, (b + 1) K) in turn (K from colstats_per_block).  The
+// 4-wave split of each column and its sums do not depend on K, so neither do
+// the bits.
 template <typename T>
 __global__ __launch_bounds__(256) void k_colstats(const T* __restrict__ A, uint64_t ld, uint32_t d,
                                                   uint64_t n, uint32_t K, float* __restrict__ alpha,
                                                   double* __restrict__ nrm2,
                                                   double* __restrict__ cnrm2) {
+  if (short_column<T>(d)) {  // a wave per column, four columns per workgroup
+    const uint64_t j = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j < n) colstats_wave<T>(A + j * ld, d, j, alpha, nrm2, cnrm2);
+    return;
+  }
   const uint64_t j0 = (uint64_t)blockIdx.x * K;
   const uint64_t j1 = j0 + K < n ? j0 + K : n;
   for (uint64_t j = j0; j < j1; ++j) {
@@ -1695,7 +1702,7 @@ hipError_t launch_colstats(int dtype, const void* A, uint64_t ld, uint32_t d, ui
                            float* alpha, double* nrm2, double* cnrm2, DevState* st, hipStream_t s) {
   (void)st;
   const uint32_t K = colstats_per_block(dtype, d, n);
-  const unsigned g = (unsigned)((n + K - 1) / K);
+  const unsigned g = (unsigned)(short_column_dt(dtype, d) ? (n + 3) / 4 : (n + K - 1) / K);
   if (dtype == 1)
     k_colstats<__hip_bfloat16><<<g, 256, 0, s>>>((const __hip_bfloat16*)A, ld, d, n, K, alpha, nrm2, cnrm2);
   else

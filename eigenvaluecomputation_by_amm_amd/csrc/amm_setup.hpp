@@ -76,6 +76,67 @@ __device__ __forceinline__ void colstats_column(const T* __restrict__ col, uint3
 }
 
 // ---------------------------------------------------------------------------
+// Column statistics of a short column (d * sizeof(T) < kShortColBytes: the
+// four-wave split would leave each wave less than one trip of four loads
+// per lane), by one wave: lane l adds rows l V + m 64 V in turn, up to eight
+// 16-byte loads in flight, then a butterfly.  Callers give each wave of a
+// workgroup its own column.  The same function for the same d in the
+// single-matrix and the grouped kernels, so both produce the same bits.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kShortColBytes = 16384;
+
+template <typename T>
+__host__ __device__ inline bool short_column(uint32_t d) {
+  return (uint64_t)d * sizeof(T) < kShortColBytes;
+}
+
+__host__ __device__ inline bool short_column_dt(int dtype, uint32_t d) {
+  return (uint64_t)d * (dtype == 1 ? 2u : 4u) < kShortColBytes;
+}
+
+template <typename T>
+__device__ __forceinline__ void colstats_wave(const T* __restrict__ col, uint32_t d, uint64_t j,
+                                              float* __restrict__ alpha, double* __restrict__ nrm2,
+                                              double* __restrict__ cnrm2) {
+  constexpr int V = Elem<T>::VEC;
+  constexpr uint32_t step = 64u * V;
+  const uint32_t lane = threadIdx.x & 63u;
+  double s = 0.0, ss = 0.0;
+  for (uint32_t row = lane * V; row < d; row += 8 * step) {
+    float a[8][V];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t r = row + u * step;
+      if (r < d) {
+        load_col<T>(col, r, d, a[u]);  // zeros at and beyond d
+      } else {
+#pragma unroll
+        for (int k = 0; k < V; ++k) a[u][k] = 0.0f;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+#pragma unroll
+      for (int k = 0; k < V; ++k) {
+        const double b = a[u][k];
+        s += b;
+        ss += b * b;
+      }
+    }
+  }
+  s = wave_sum(s);
+  ss = wave_sum(ss);
+  if (lane == 0) {
+    const double mean = s / (double)d;
+    alpha[j] = (float)mean;
+    nrm2[j] = ss;
+    double c = ss - s * mean;
+    c = c > 0.0 ? c : (c == c ? 0.0 : c);  // clamp rounding below 0, keep NaN
+    cnrm2[j] = c;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Sampling CDF (set_weights + cub::DeviceScan::InclusiveSum,
 // common.h:34-40, powerMethod_by_previous.cuh:26-32) in exact integer
 // arithmetic:
