@@ -672,10 +672,10 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
   auto sampled = [&](uint32_t i) {
     return modes[i] == amm::kModeAmmPrevious || modes[i] == amm::kModeAmmProposed;
   };
-  // the tail of a step followed by a sampled step (static schedule, fused
-  // finish, inline normalisation, d a multiple of the dot pass's row split)
+  // the tail of a step followed by a sampled step (static schedule, sample
+  // lists, inline normalisation, d a multiple of the dot pass's row split)
   // rides in the next dot pass, and so does x0's normalisation
-  const bool carry_ok = fuse && !adaptive && c->dpad <= kFuseNormRows &&
+  const bool carry_ok = !sanity && !adaptive && c->dpad <= kFuseNormRows &&
                         c->dot_list.R <= amm::kTailMaxBlocks && c->dot_list.rows_per_split &&
                         c->d % c->dot_list.rows_per_split == 0;
   amm::GatherTail carried{};
@@ -716,8 +716,9 @@ int enqueue_run(amm_ctx* c, const amm_params& p, const std::vector<uint32_t>& mo
     }
     const bool want_sq = K == AMM_KERNEL_PROPOSED || (K == AMM_KERNEL_PROPOSED_V2 && i + 1 >= P);
     if (carry_ok && !z_after && i + 1 < step && sampled(i + 1)) {
-      // an exact step still runs its finish kernel (y and the block partials)
-      if (!sampled(i))
+      // the finish kernel still runs (y and the block partials) unless the
+      // second contraction carried it
+      if (fm != kFinishNone)
         if ((rc = enq_finish(c, i, sanity, c->y, c->ym, 0u, P, 0.0, step, false, nullptr, false, fm, false)))
           return rc;
       carried.BP = c->BP;

@@ -533,22 +533,24 @@ def test_fused_finish_runs_against_oracle(run):
     assert np.array_equal(res.eig, eager.eig)
 
 
+@pytest.mark.parametrize("d", [8192, 4096])
 @pytest.mark.parametrize("bf16", [False, True])
 @pytest.mark.parametrize("run", [dict(kernel="previousAMM", c=600, step=14, vr_period=4),
                                  dict(kernel="proposedAMM", c=600, step=14, vr_period=4),
                                  dict(kernel="proposedAMM_v2", c=600, step=14, vr_period=4)],
                          ids=["previous", "proposed", "proposed_v2"])
-def test_carried_tail_runs_against_oracle(run, bf16):
+def test_carried_tail_runs_against_oracle(run, bf16, d):
     """Runs where each iteration's tail (lambda, err, 1/||y||, x = y / ||y||
-    and the proposed s, q) rides in the next sampled dot pass: d = 8192 is a
+    and the proposed s, q) rides in the next sampled dot pass: d is a
     multiple of the dot pass's row split (1024 FP32 / 2048 BF16 rows) and the
     schedule is static, so sampled -> sampled and exact -> sampled steps both
-    carry it.  Same samples as the oracle: identical step kinds, Rayleigh
-    quotients and err = y^T v* / ||y|| within 1e-4; graph and eager runs bit
-    for bit."""
-    banner(f"carried tail run {run} bf16={bf16}")
+    carry it; d = 8192 with the finish in the second contraction, d = 4096
+    with the chunked contraction and the finish kernel.  Same samples as the
+    oracle: identical step kinds, Rayleigh quotients and err = y^T v* / ||y||
+    within 1e-4; graph and eager runs bit for bit."""
+    banner(f"carried tail run {run} bf16={bf16} d={d}")
     AmmContext, RunParams, _ = _amm()
-    d, n = 8192, 3000
+    n = 3000
     A = lowrank_plus_noise(d, n, seed=30) + 0.05
     t, cols = dev_matrix(A, bf16=bf16)
     colsf = bf16_to_f32(cols) if bf16 else cols
