@@ -272,12 +272,25 @@ __device__ __forceinline__ uint32_t bcdf_search(const uint64_t* __restrict__ cdf
 }
 
 // Fill the coarse table cs[K] (LDS) by threads tid = 0, stride, ...; an
-// empty bucket before the first column ends at "-1": CDF value 0.
+// empty bucket before the first column ends at "-1": CDF value 0.  Loads in
+// batches of eight (a loop with a load per trip keeps one in flight).
 template <uint32_t K = kCoarse>
 __device__ __forceinline__ void coarse_fill(uint64_t* cs, const uint64_t* __restrict__ cdf, uint32_t n,
                                             uint32_t tid, uint32_t stride) {
-  for (uint32_t m = tid; m < K; m += stride)
-    cs[m] = ((uint64_t)(m + 1) * n) / K == 0 ? 0ull : cdf[bucket_end<K>(m, n)];
+  uint32_t m = tid;
+  for (; m + 7 * stride < K; m += 8 * stride) {
+    uint64_t v[8];
+#pragma unroll
+    for (uint32_t u = 0; u < 8; ++u) {
+      const uint32_t mu = m + u * stride;
+      const bool empty = ((uint64_t)(mu + 1) * n) / K == 0;
+      v[u] = cdf[empty ? 0u : bucket_end<K>(mu, n)];
+      v[u] = empty ? 0ull : v[u];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 8; ++u) cs[m + u * stride] = v[u];
+  }
+  for (; m < K; m += stride) cs[m] = ((uint64_t)(m + 1) * n) / K == 0 ? 0ull : cdf[bucket_end<K>(m, n)];
 }
 
 }  // namespace amm

@@ -313,6 +313,60 @@ __global__ __launch_bounds__(256) void k_cdf_offsets(uint64_t* __restrict__ cdf,
   cdf_offsets_tile(cdf, n, tile_sum, tile_lp, tile_a2, st, which, want_a2, blockIdx.x, gridDim.x);
 }
 
+// Bitonic sort of keys[0, N) in LDS (N a power of two, NE = max(1, N/1024))
+// by a 1024-thread workgroup: thread tid holds the keys at positions
+// tid + 1024 e, e < NE; partners 1024 or more apart are in the same thread
+// (registers), closer than 64 in the same wave (shuffles), else they meet in
+// LDS.  Called by the whole workgroup; keys[] holds the result on return.
+template <uint32_t NE>
+__device__ __forceinline__ void bitonic_keys(uint32_t* keys, uint32_t N, uint32_t tid) {
+  uint32_t v[NE];
+#pragma unroll
+  for (uint32_t e = 0; e < NE; ++e) v[e] = keys[tid + 1024 * e];
+  for (uint32_t k = 2; k <= N; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 1024) {
+        const uint32_t je = j >> 10;
+#pragma unroll
+        for (uint32_t e = 0; e < NE; ++e) {
+          if ((e & je) == 0) {
+            const uint32_t e2 = e | je;  // position p + j
+            const bool up = ((tid + 1024 * e) & k) == 0;
+            const uint32_t a = v[e], b = v[e2];
+            const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+            v[e] = up ? lo : hi;
+            v[e2] = up ? hi : lo;
+          }
+        }
+      } else if (j >= 64) {
+        __syncthreads();
+#pragma unroll
+        for (uint32_t e = 0; e < NE; ++e) keys[tid + 1024 * e] = v[e];
+        __syncthreads();
+#pragma unroll
+        for (uint32_t e = 0; e < NE; ++e) {
+          const uint32_t p = tid + 1024 * e;
+          const uint32_t q = keys[p ^ j];
+          const bool up = (p & k) == 0, lower = (p & j) == 0;
+          v[e] = (lower == up) ? (v[e] < q ? v[e] : q) : (v[e] < q ? q : v[e]);
+        }
+      } else {
+#pragma unroll
+        for (uint32_t e = 0; e < NE; ++e) {
+          const uint32_t p = tid + 1024 * e;
+          const uint32_t q = (uint32_t)__shfl_xor((int)v[e], (int)j);
+          const bool up = (p & k) == 0, lower = (p & j) == 0;
+          v[e] = (lower == up) ? (v[e] < q ? v[e] : q) : (v[e] < q ? q : v[e]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t e = 0; e < NE; ++e) keys[tid + 1024 * e] = v[e];
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // Sampler (get_rand + pick_index + cub SortKeys + update_weights,
 // device_memory.h:506-545, powerMethod_by_previous.cuh:56-70,
@@ -320,8 +374,9 @@ __global__ __launch_bounds__(256) void k_cdf_offsets(uint64_t* __restrict__ cdf,
 // Philox counter (k>>1, it, trial, kStreamSample), words 2(k&1), 2(k&1)+1;
 // t = floor(u * total / 2^64); column = min{j : cdf_j > t} (located in a
 // coarse table of the CDF in LDS first, then within its bucket).  One
-// workgroup per (segment, iteration) sorts its slots (bitonic: register
-// shuffles between partners of one wave, LDS between waves), merges
+// workgroup per (segment, iteration) sorts its slots (bitonic, up to eight
+// keys per thread: partners in one thread swap in registers, in one wave by
+// shuffles, between waves through LDS), merges
 // duplicates and writes the unique columns in ascending order with
 // weight mult * total / (c * q_j), q_j = cdf_j - cdf_{j-1}.
 // ---------------------------------------------------------------------------
@@ -367,48 +422,15 @@ __global__ __launch_bounds__(1024) void k_sample(const uint64_t* __restrict__ cd
   }
   __syncthreads();
 
-  // bitonic sort of the first N = next_pow2(cnt) keys (the rest are sentinels)
+  // bitonic sort of the first N = next_pow2(cnt) keys (the rest are
+  // sentinels), NE = max(1, N / 1024) keys per thread
   uint32_t N = 1;
   while (N < cnt) N <<= 1;
-  if (N <= 1024) {
-    // one key per thread: partners closer than 64 are in the same wave
-    // (register shuffles), farther ones meet in LDS
-    uint32_t v = keys[tid];
-    for (uint32_t k = 2; k <= N; k <<= 1) {
-      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-        uint32_t p;
-        if (j >= 64) {
-          __syncthreads();
-          keys[tid] = v;
-          __syncthreads();
-          p = keys[tid ^ j];
-        } else {
-          p = (uint32_t)__shfl_xor((int)v, (int)j);
-        }
-        const bool up = (tid & k) == 0, lower = (tid & j) == 0;
-        v = (lower == up) ? (v < p ? v : p) : (v < p ? p : v);
-      }
-    }
-    __syncthreads();
-    keys[tid] = v;
-    __syncthreads();
-  } else {
-    for (uint32_t k = 2; k <= N; k <<= 1) {
-      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-        for (uint32_t i = tid; i < N; i += 1024) {
-          const uint32_t ixj = i ^ j;
-          if (ixj > i) {
-            const uint32_t a = keys[i], bb = keys[ixj];
-            const bool up = (i & k) == 0;
-            if ((a > bb) == up) {
-              keys[i] = bb;
-              keys[ixj] = a;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
+  switch (N > 1024 ? N / 1024 : 1) {
+    case 1: bitonic_keys<1>(keys, N, tid); break;
+    case 2: bitonic_keys<2>(keys, N, tid); break;
+    case 4: bitonic_keys<4>(keys, N, tid); break;
+    default: bitonic_keys<8>(keys, N, tid); break;
   }
 
   // run starts: thread tid owns positions [tid*8, tid*8+8)
@@ -432,8 +454,14 @@ __global__ __launch_bounds__(1024) void k_sample(const uint64_t* __restrict__ cd
   }
   uint32_t pos = scan[cur][tid] - nf;  // exclusive prefix
   const uint64_t base = (uint64_t)(it - buf_it0) * c + b;
+  // every run start's multiplicity (LDS) and its two CDF entries first, so
+  // the thread's global loads are in flight together
+  uint32_t rkey[E], rmult[E];
+  uint64_t ca[E], cb[E];
 #pragma unroll
   for (uint32_t q = 0; q < E; ++q) {
+    rkey[q] = 0u;
+    rmult[q] = 0u;
     if (flags & (1u << q)) {
       const uint32_t p = tid * E + q;
       const uint32_t key = keys[p];
@@ -443,9 +471,18 @@ __global__ __launch_bounds__(1024) void k_sample(const uint64_t* __restrict__ cd
         const uint32_t mid = (lo + hi) >> 1;
         if (keys[mid] > key) hi = mid; else lo = mid + 1;
       }
-      const uint32_t mult = lo - p;
-      const uint64_t qj = cdf[key] - (key ? cdf[key - 1] : 0ull);
-      const double num = (double)mult * (double)total;
+      rkey[q] = key;
+      rmult[q] = lo - p;
+    }
+    ca[q] = cdf[rkey[q]];
+    cb[q] = cdf[rkey[q] ? rkey[q] - 1 : 0u];
+  }
+#pragma unroll
+  for (uint32_t q = 0; q < E; ++q) {
+    if (flags & (1u << q)) {
+      const uint32_t key = rkey[q];
+      const uint64_t qj = ca[q] - (key ? cb[q] : 0ull);
+      const double num = (double)rmult[q] * (double)total;
       const double den = (double)c * (double)qj;
       idx_out[base + pos] = key;
       wgt_out[base + pos] = den > 0.0 ? (float)(num / den) : 0.0f;

@@ -89,6 +89,30 @@ def test_sample_bitwise(c, centred, iteration, trial):
     assert not np.isin([0, 100, 24999], idx[: segc[0]]).any()
 
 
+@pytest.mark.parametrize("n", [65536, 90000, 3000])
+def test_sample_bitwise_bucket_forms(n):
+    """The sampler's in-bucket search: buckets of at most 8 columns (n <=
+    65536, including exactly full ones, and n < 8192 with empty buckets) are
+    searched by loading the whole bucket; larger ones (n = 90000) by binary
+    search.  Both bit for bit against the oracle."""
+    banner(f"sample bucket forms n={n}")
+    AmmContext, *_ = _amm()
+    A = heavy_columns(32, n, seed=41)
+    A[:, [1, n - 2]] = 0.0
+    t, _ = dev_matrix(A)
+    seed = 0x5EED_0042
+    with AmmContext(t) as ctx:
+        _, nrm2, _, _ = ctx.column_stats()
+        for c, it in ((3000, 0), (1024, 9)):
+            idx, wgt, segc = ctx.sample(c, it, seed)
+            oi, ow, os_ = O.sample(nrm2, c, seed, it)
+            assert np.array_equal(segc, os_)
+            u = int(segc[0])
+            assert np.array_equal(idx[:u], oi[:u])
+            assert np.array_equal(wgt[:u].view(np.uint32), ow[:u].view(np.uint32))
+            assert not np.isin([1, n - 2], idx[:u]).any()
+
+
 # rows per wave of the one-pass form: 1, 2, 4, 8, 16 loads per lane, and
 # d = 16392 past it (the two-pass form); ragged d and padded ld included
 @pytest.mark.parametrize("d,n,ld,bf16", [(512, 3000, None, False), (1000, 777, 1004, False),
