@@ -1629,6 +1629,26 @@ static uint32_t resident_grid(const void* fn, uint64_t n) {
   return (uint32_t)std::min<uint64_t>(n, g);
 }
 
+// Workgroups of a column-resident pass (k_gram, k_colstats_z) over columns of
+// d rows: as many as are resident, except FP32 columns of 16 KB or more,
+// where fewer column streams in flight read HBM faster -- one workgroup per
+// CU from 32 KB (16384 x 65536: 655 -> 625 us for the one-read exact
+// product, 670 -> 630 us with z; 8192 x 32768: 170 -> 160 and 185 -> 161
+// us), two per CU from 16 KB (4096 x 65536: 165 -> 158 and 191 -> 179 us).
+// BF16 columns of 8-32 KB measured as fast or faster with every resident
+// workgroup.
+static uint32_t resident_blocks(const void* fn, int dtype, uint32_t d, uint64_t n) {
+  const uint32_t g = resident_grid(fn, n);
+  if (dtype == 1) return g;
+  const uint64_t col = (uint64_t)d * 4u;
+  uint32_t per_cu = col >= 32768 ? 1u : (col >= 16384 ? 2u : 0u);
+  if (!per_cu) return g;
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) return g;
+  return std::min<uint32_t>(g, (uint32_t)std::min<uint64_t>(n, (uint64_t)per_cu * (uint32_t)cus));
+}
+
 template <typename T>
 static const void* colstats_z_fn(int L) {
   switch (L) {
@@ -1658,7 +1678,7 @@ bool colstats_fuses_z(int dtype, uint32_t d) { return resident_L(dtype, d, 16) !
 uint32_t colstats_z_blocks(int dtype, uint32_t d, uint64_t n) {
   const int L = resident_L(dtype, d, 16);
   if (!L) return 0;
-  return resident_grid(dtype == 1 ? colstats_z_fn<__hip_bfloat16>(L) : colstats_z_fn<float>(L), n);
+  return resident_blocks(dtype == 1 ? colstats_z_fn<__hip_bfloat16>(L) : colstats_z_fn<float>(L), dtype, d, n);
 }
 
 bool gram_fits(int dtype, uint32_t d) { return resident_L(dtype, d, gram_maxL(dtype)) != 0; }
@@ -1666,7 +1686,7 @@ bool gram_fits(int dtype, uint32_t d) { return resident_L(dtype, d, gram_maxL(dt
 uint32_t gram_blocks(int dtype, uint32_t d, uint64_t n) {
   const int L = resident_L(dtype, d, gram_maxL(dtype));
   if (!L) return 0;
-  return resident_grid(dtype == 1 ? gram_fn<__hip_bfloat16>(L) : gram_fn<float>(L), n);
+  return resident_blocks(dtype == 1 ? gram_fn<__hip_bfloat16>(L) : gram_fn<float>(L), dtype, d, n);
 }
 
 template <typename T>
