@@ -36,9 +36,13 @@ def banner(msg):
 
 
 # --------------------------------------------------------------------------
+# a wave per column under 16 KB, a workgroup per column from 16 KB (4100
+# FP32 rows, 8200 BF16 rows), workgroups walking runs of columns for FP32
+# columns of 64 KB or more (16384 x 2048: four columns each)
 @pytest.mark.parametrize("d,n,ld,bf16", [(256, 300, 256, False), (1000, 777, 1004, False),
                                          (37, 5, 40, False), (512, 300, 512, True),
-                                         (1001, 33, 1008, True)])
+                                         (1001, 33, 1008, True), (4100, 300, 4104, False),
+                                         (8200, 260, None, True), (16384, 2048, None, False)])
 def test_column_stats(d, n, ld, bf16):
     banner(f"column_stats d={d} n={n} ld={ld} bf16={bf16}")
     AmmContext, *_ = _amm()
@@ -91,10 +95,10 @@ def test_sample_bitwise(c, centred, iteration, trial):
 
 @pytest.mark.parametrize("n", [65536, 90000, 3000])
 def test_sample_bitwise_bucket_forms(n):
-    """The sampler's in-bucket search: buckets of at most 8 columns (n <=
-    65536, including exactly full ones, and n < 8192 with empty buckets) are
-    searched by loading the whole bucket; larger ones (n = 90000) by binary
-    search.  Both bit for bit against the oracle."""
+    """The sampler's search in its 8192-bucket coarse table, then within the
+    bucket: buckets of exactly 8 columns (n = 65536), of 10-11 (n = 90000)
+    and empty ones among single-column buckets (n = 3000 < 8192), each bit
+    for bit against the oracle."""
     banner(f"sample bucket forms n={n}")
     AmmContext, *_ = _amm()
     A = heavy_columns(32, n, seed=41)
