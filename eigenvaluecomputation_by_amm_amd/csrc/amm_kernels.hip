@@ -313,54 +313,77 @@ __global__ __launch_bounds__(256) void k_cdf_offsets(uint64_t* __restrict__ cdf,
   cdf_offsets_tile(cdf, n, tile_sum, tile_lp, tile_a2, st, which, want_a2, blockIdx.x, gridDim.x);
 }
 
-// Bitonic sort of keys[0, N) in LDS (N a power of two, NE = max(1, N/1024))
-// by a 1024-thread workgroup: thread tid holds the keys at positions
-// tid + 1024 e, e < NE; partners 1024 or more apart are in the same thread
-// (registers), closer than 64 in the same wave (shuffles), else they meet in
-// LDS.  Called by the whole workgroup; keys[] holds the result on return.
+// Value of lane (lane ^ J) within a wave, J a power of two below 64, by the
+// cheapest exchange for J: DPP quad permutes (1, 2), swizzle (4, 8, 16),
+// bpermute (32).
+template <uint32_t J>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
+  if constexpr (J == 1) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+  } else if constexpr (J == 2) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+  } else if constexpr (J < 32) {
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (J << 10));  // bit mode: lane ^ J within 32
+  } else {
+    return (uint32_t)__shfl_xor((int)v, (int)J);
+  }
+}
+
+// Bitonic sort of keys[0, 1024 NE) by a 1024-thread workgroup, the network
+// unrolled: thread tid holds the keys at positions tid + 1024 e, e < NE;
+// partners 1024 or more apart swap in registers, closer than 64 exchange
+// within the wave (xor_lane), the others through LDS.  Sorting all 1024 NE
+// positions gives the first cnt sorted keys of any shorter sort: the
+// positions past cnt hold sentinel keys, which sort last.
+template <uint32_t NE, uint32_t K, uint32_t J>
+__device__ __forceinline__ void bitonic_stage(uint32_t* keys, uint32_t tid, uint32_t (&v)[NE]) {
+  if constexpr (J >= 1024) {
+    constexpr uint32_t je = J / 1024;
+#pragma unroll
+    for (uint32_t e = 0; e < NE; ++e) {
+      if ((e & je) == 0) {
+        const bool up = ((tid + 1024 * e) & K) == 0;
+        const uint32_t a = v[e], b = v[e | je];
+        const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+        v[e] = up ? lo : hi;
+        v[e | je] = up ? hi : lo;
+      }
+    }
+  } else {
+    uint32_t q[NE];
+    if constexpr (J >= 64) {
+      __syncthreads();
+#pragma unroll
+      for (uint32_t e = 0; e < NE; ++e) keys[tid + 1024 * e] = v[e];
+      __syncthreads();
+#pragma unroll
+      for (uint32_t e = 0; e < NE; ++e) q[e] = keys[(tid + 1024 * e) ^ J];
+    } else {
+#pragma unroll
+      for (uint32_t e = 0; e < NE; ++e) q[e] = xor_lane<J>(v[e]);
+    }
+#pragma unroll
+    for (uint32_t e = 0; e < NE; ++e) {
+      const uint32_t p = tid + 1024 * e;
+      const bool up = (p & K) == 0, lower = (p & J) == 0;
+      v[e] = (lower == up) ? (v[e] < q[e] ? v[e] : q[e]) : (v[e] < q[e] ? q[e] : v[e]);
+    }
+  }
+  if constexpr (J > 1) bitonic_stage<NE, K, J / 2>(keys, tid, v);
+}
+
+template <uint32_t NE, uint32_t K>
+__device__ __forceinline__ void bitonic_merge_all(uint32_t* keys, uint32_t tid, uint32_t (&v)[NE]) {
+  bitonic_stage<NE, K, K / 2>(keys, tid, v);
+  if constexpr (K < 1024 * NE) bitonic_merge_all<NE, K * 2>(keys, tid, v);
+}
+
 template <uint32_t NE>
-__device__ __forceinline__ void bitonic_keys(uint32_t* keys, uint32_t N, uint32_t tid) {
+__device__ __forceinline__ void bitonic_sort_keys(uint32_t* keys, uint32_t tid) {
   uint32_t v[NE];
 #pragma unroll
   for (uint32_t e = 0; e < NE; ++e) v[e] = keys[tid + 1024 * e];
-  for (uint32_t k = 2; k <= N; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      if (j >= 1024) {
-        const uint32_t je = j >> 10;
-#pragma unroll
-        for (uint32_t e = 0; e < NE; ++e) {
-          if ((e & je) == 0) {
-            const uint32_t e2 = e | je;  // position p + j
-            const bool up = ((tid + 1024 * e) & k) == 0;
-            const uint32_t a = v[e], b = v[e2];
-            const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
-            v[e] = up ? lo : hi;
-            v[e2] = up ? hi : lo;
-          }
-        }
-      } else if (j >= 64) {
-        __syncthreads();
-#pragma unroll
-        for (uint32_t e = 0; e < NE; ++e) keys[tid + 1024 * e] = v[e];
-        __syncthreads();
-#pragma unroll
-        for (uint32_t e = 0; e < NE; ++e) {
-          const uint32_t p = tid + 1024 * e;
-          const uint32_t q = keys[p ^ j];
-          const bool up = (p & k) == 0, lower = (p & j) == 0;
-          v[e] = (lower == up) ? (v[e] < q ? v[e] : q) : (v[e] < q ? q : v[e]);
-        }
-      } else {
-#pragma unroll
-        for (uint32_t e = 0; e < NE; ++e) {
-          const uint32_t p = tid + 1024 * e;
-          const uint32_t q = (uint32_t)__shfl_xor((int)v[e], (int)j);
-          const bool up = (p & k) == 0, lower = (p & j) == 0;
-          v[e] = (lower == up) ? (v[e] < q ? v[e] : q) : (v[e] < q ? q : v[e]);
-        }
-      }
-    }
-  }
+  bitonic_merge_all<NE, 2>(keys, tid, v);
   __syncthreads();
 #pragma unroll
   for (uint32_t e = 0; e < NE; ++e) keys[tid + 1024 * e] = v[e];
@@ -426,11 +449,11 @@ __global__ __launch_bounds__(1024) void k_sample(const uint64_t* __restrict__ cd
   // sentinels), NE = max(1, N / 1024) keys per thread
   uint32_t N = 1;
   while (N < cnt) N <<= 1;
-  switch (N > 1024 ? N / 1024 : 1) {
-    case 1: bitonic_keys<1>(keys, N, tid); break;
-    case 2: bitonic_keys<2>(keys, N, tid); break;
-    case 4: bitonic_keys<4>(keys, N, tid); break;
-    default: bitonic_keys<8>(keys, N, tid); break;
+  switch (N > 1024 ? N / 1024 : 1) {  // sentinels beyond N sort last
+    case 1: bitonic_sort_keys<1>(keys, tid); break;
+    case 2: bitonic_sort_keys<2>(keys, tid); break;
+    case 4: bitonic_sort_keys<4>(keys, tid); break;
+    default: bitonic_sort_keys<8>(keys, tid); break;
   }
 
   // run starts: thread tid owns positions [tid*8, tid*8+8)
