@@ -127,6 +127,11 @@ struct amm_ctx {
   DevState* st = nullptr;
   double* BP = nullptr;
   uint64_t* tile_max = nullptr;
+  // per-workgroup maxima of the weights from the statistics pass ([2][wmax_n],
+  // plain then centred), valid while wmax_n > 0; the CDF then skips its
+  // maximum pass
+  uint64_t* wmax = nullptr;
+  uint32_t wmax_n = 0;
   uint64_t* tile_sum = nullptr;
   uint64_t* tile_lp = nullptr;
   double* tile_a2 = nullptr;
@@ -270,7 +275,8 @@ int ensure_lists(amm_ctx* c, uint64_t nit, uint32_t cc) {
 // ---- enqueue helpers (all asynchronous on c->stream) ----------------------
 int enq_stats(amm_ctx* c) {
   HIPCHK(amm::launch_colstats(c->dtype, c->A, c->ld, (uint32_t)c->d, c->n, c->alpha, c->nrm2,
-                              c->cnrm2, c->st, c->stream));
+                              c->cnrm2, c->st, c->wmax, c->stream));
+  c->wmax_n = amm::colstats_max_blocks(c->dtype, (uint32_t)c->d, c->n);
   return AMM_OK;
 }
 
@@ -286,6 +292,7 @@ bool z_fusable(const amm_ctx* c) {
 int enq_stats_z(amm_ctx* c) {
   HIPCHK(amm::launch_colstats_z(c->dtype, c->A, c->ld, (uint32_t)c->d, c->n, c->dpad, c->alpha, c->nrm2,
                                 c->cnrm2, c->ZP, c->st, c->stream));
+  c->wmax_n = 0;  // this pass leaves no maxima
   return AMM_OK;
 }
 
@@ -298,7 +305,7 @@ int enq_z_partials(amm_ctx* c) {
 int enq_cdf(amm_ctx* c, int which, bool /*unused*/) {
   HIPCHK(amm::launch_cdf(which ? c->cnrm2 : c->nrm2, c->n, c->cdf[which], c->st, which,
                          which ? c->alpha : nullptr, c->tile_max, c->tile_sum, c->tile_lp,
-                         c->tile_a2, c->stream));
+                         c->tile_a2, c->wmax, c->wmax_n, c->stream));
   return AMM_OK;
 }
 
@@ -841,6 +848,7 @@ int amm_ctx_create(const void* A, int32_t dtype, uint64_t d, uint64_t n, uint64_
       // block partials: 3 doubles per finish block bound every reduction kernel
       (cap = 0, rc = grow(c->BP, cap, 3 * amm::finish_blocks(dp) + 64)) ||
       (cap = 0, rc = grow(c->tile_max, cap, amm::cdf_tiles(n))) ||
+      (cap = 0, rc = grow(c->wmax, cap, 2 * (size_t)amm::kColstatsMaxBlocks)) ||
       (cap = 0, rc = grow(c->tile_sum, cap, amm::cdf_tiles(n))) ||
       (cap = 0, rc = grow(c->tile_lp, cap, amm::cdf_tiles(n))) ||
       (cap = 0, rc = grow(c->tile_a2, cap, amm::cdf_tiles(n))))
@@ -863,7 +871,7 @@ int amm_ctx_destroy(amm_ctx* c) {
                   c->x,     c->y,     c->xm,       c->ym,      c->eigvec, c->st,  c->BP,
                   c->mode,  c->eig,   c->err,      c->ts,      c->idx,    c->wgt, c->segc,
                   c->P1,    c->P2,    c->P2s,      c->tile_sum, c->tile_lp, c->tile_a2,
-                  c->tile_max, c->tvec, c->tbp, c->ZP};
+                  c->tile_max, c->wmax, c->tvec, c->tbp, c->ZP};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->ev0) (void)hipEventDestroy(c->ev0);

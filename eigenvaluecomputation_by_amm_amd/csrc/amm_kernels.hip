@@ -39,7 +39,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_colstats(const T* __restrict__ A, uint64_t ld, uint32_t d,
                                                   uint64_t n, uint32_t K, float* __restrict__ alpha,
                                                   double* __restrict__ nrm2,
-                                                  double* __restrict__ cnrm2) {
+                                                  double* __restrict__ cnrm2, uint64_t* __restrict__ wmax) {
   if (short_column<T>(d)) {  // a wave per column, four columns per workgroup
     const uint64_t j = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (j < n) colstats_wave<T>(A + j * ld, d, j, alpha, nrm2, cnrm2);
@@ -47,9 +47,20 @@ __global__ __launch_bounds__(256) void k_colstats(const T* __restrict__ A, uint6
   }
   const uint64_t j0 = (uint64_t)blockIdx.x * K;
   const uint64_t j1 = j0 + K < n ? j0 + K : n;
+  uint64_t m0 = 0, m1 = 0;  // bit patterns of the workgroup's largest weights (thread 0)
   for (uint64_t j = j0; j < j1; ++j) {
-    colstats_column<T>(A + j * ld, d, j, alpha, nrm2, cnrm2);
+    const double2 w = colstats_column<T>(A + j * ld, d, j, alpha, nrm2, cnrm2);
+    const uint64_t b0 = (uint64_t)__double_as_longlong(w.x), b1 = (uint64_t)__double_as_longlong(w.y);
+    m0 = b0 > m0 ? b0 : m0;
+    m1 = b1 > m1 ? b1 : m1;
     __syncthreads();  // the per-column exchange in LDS is reused by the next column
+  }
+  // per-workgroup maxima of both weights for the CDF (cdf_max_tile's
+  // order: non-negative doubles order like their patterns, NaN and Inf
+  // above every finite one), in place of its maximum pass
+  if (wmax && threadIdx.x == 0) {
+    wmax[blockIdx.x] = m0;
+    wmax[gridDim.x + blockIdx.x] = m1;
   }
 }
 
@@ -298,11 +309,11 @@ __global__ __launch_bounds__(256) void k_cdf_max(const double* __restrict__ w, u
 __global__ __launch_bounds__(256) void k_cdf_tiles(const double* __restrict__ w, uint64_t n,
                                                    uint64_t* __restrict__ cdf, DevState* st,
                                                    int which, const float* __restrict__ alpha,
-                                                   const uint64_t* __restrict__ tile_max,
+                                                   const uint64_t* __restrict__ tile_max, uint32_t nmax,
                                                    uint64_t* __restrict__ tile_sum,
                                                    uint64_t* __restrict__ tile_lp,
                                                    double* __restrict__ tile_a2) {
-  cdf_scan_tile(w, n, cdf, st, alpha, tile_max, gridDim.x, tile_sum, tile_lp, tile_a2, blockIdx.x);
+  cdf_scan_tile(w, n, cdf, st, alpha, tile_max, nmax, tile_sum, tile_lp, tile_a2, blockIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_cdf_offsets(uint64_t* __restrict__ cdf, uint64_t n,
@@ -1778,15 +1789,26 @@ static uint32_t colstats_per_block(int dtype, uint32_t d, uint64_t n) {
   if (dtype == 1 || (uint64_t)d * 4u < 65536) return 1;
   return (uint32_t)std::max<uint64_t>(1, n / 512);
 }
+uint32_t colstats_max_blocks(int dtype, uint32_t d, uint64_t n) {
+  if (short_column_dt(dtype, d) || n == 0) return 0;
+  const uint32_t K = colstats_per_block(dtype, d, n);
+  if (K < 2) return 0;
+  const uint64_t g = (n + K - 1) / K;
+  return g <= kColstatsMaxBlocks ? (uint32_t)g : 0u;
+}
+
 hipError_t launch_colstats(int dtype, const void* A, uint64_t ld, uint32_t d, uint64_t n,
-                           float* alpha, double* nrm2, double* cnrm2, DevState* st, hipStream_t s) {
+                           float* alpha, double* nrm2, double* cnrm2, DevState* st, uint64_t* wmax,
+                           hipStream_t s) {
   (void)st;
   const uint32_t K = colstats_per_block(dtype, d, n);
   const unsigned g = (unsigned)(short_column_dt(dtype, d) ? (n + 3) / 4 : (n + K - 1) / K);
+  if (!colstats_max_blocks(dtype, d, n)) wmax = nullptr;
   if (dtype == 1)
-    k_colstats<__hip_bfloat16><<<g, 256, 0, s>>>((const __hip_bfloat16*)A, ld, d, n, K, alpha, nrm2, cnrm2);
+    k_colstats<__hip_bfloat16><<<g, 256, 0, s>>>((const __hip_bfloat16*)A, ld, d, n, K, alpha, nrm2, cnrm2,
+                                                  wmax);
   else
-    k_colstats<float><<<g, 256, 0, s>>>((const float*)A, ld, d, n, K, alpha, nrm2, cnrm2);
+    k_colstats<float><<<g, 256, 0, s>>>((const float*)A, ld, d, n, K, alpha, nrm2, cnrm2, wmax);
   return hipGetLastError();
 }
 
@@ -1794,10 +1816,15 @@ uint64_t cdf_tiles(uint64_t n) { return grid1(n, kCdfTile); }
 
 hipError_t launch_cdf(const double* w, uint64_t n, uint64_t* cdf, DevState* st, int which,
                       const float* alpha, uint64_t* tile_max, uint64_t* tile_sum, uint64_t* tile_lp,
-                      double* tile_a2, hipStream_t s) {
+                      double* tile_a2, const uint64_t* wmax, uint32_t nwmax, hipStream_t s) {
   const unsigned nt = (unsigned)cdf_tiles(n);
-  k_cdf_max<<<nt, 256, 0, s>>>(w, n, tile_max);
-  k_cdf_tiles<<<nt, 256, 0, s>>>(w, n, cdf, st, which, alpha, tile_max, tile_sum, tile_lp, tile_a2);
+  if (wmax && nwmax) {  // the statistics pass left the maxima of its workgroups
+    k_cdf_tiles<<<nt, 256, 0, s>>>(w, n, cdf, st, which, alpha, wmax + (size_t)which * nwmax, nwmax, tile_sum,
+                                   tile_lp, tile_a2);
+  } else {
+    k_cdf_max<<<nt, 256, 0, s>>>(w, n, tile_max);
+    k_cdf_tiles<<<nt, 256, 0, s>>>(w, n, cdf, st, which, alpha, tile_max, nt, tile_sum, tile_lp, tile_a2);
+  }
   k_cdf_offsets<<<nt, 256, 0, s>>>(cdf, n, tile_sum, tile_lp, tile_a2, st, which, alpha ? 1 : 0);
   return hipGetLastError();
 }

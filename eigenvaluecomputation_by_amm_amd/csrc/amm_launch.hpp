@@ -174,8 +174,15 @@ hipError_t launch_tail(const TailArgs& a, hipStream_t s);
 // double ||alpha||^2 partial per tile)
 uint64_t cdf_tiles(uint64_t n);
 
+// Column statistics; where the pass runs as workgroups over runs of columns
+// (colstats_max_blocks(dtype, d, n) > 0) it also writes each workgroup's
+// largest plain and centred weight to wmax[2][that count] (wmax may be null)
+// for launch_cdf, which then skips its own maximum pass.
+constexpr uint32_t kColstatsMaxBlocks = 1024;
+uint32_t colstats_max_blocks(int dtype, uint32_t d, uint64_t n);
 hipError_t launch_colstats(int dtype, const void* A, uint64_t ld, uint32_t d, uint64_t n,
-                           float* alpha, double* nrm2, double* cnrm2, DevState* st, hipStream_t s);
+                           float* alpha, double* nrm2, double* cnrm2, DevState* st, uint64_t* wmax,
+                           hipStream_t s);
 // Column statistics and the z = A alpha partials
 // ZP[colstats_z_blocks(dtype, d, n)][dpad] in the same read of A (finish
 // with launch_sum_rows).  Needs colstats_fuses_z(dtype, d).
@@ -192,9 +199,11 @@ hipError_t launch_gram(int dtype, const void* A, uint64_t ld, uint32_t d, uint64
                        hipStream_t s);
 uint32_t gram_blocks(int dtype, uint32_t d, uint64_t n);
 bool gram_fits(int dtype, uint32_t d);
+// wmax / nwmax: the statistics pass's per-workgroup maxima (null / 0: the
+// CDF's own maximum pass)
 hipError_t launch_cdf(const double* w, uint64_t n, uint64_t* cdf, DevState* st, int which,
                       const float* alpha, uint64_t* tile_max, uint64_t* tile_sum, uint64_t* tile_lp,
-                      double* tile_a2, hipStream_t s);
+                      double* tile_a2, const uint64_t* wmax, uint32_t nwmax, hipStream_t s);
 hipError_t launch_sample(const uint64_t* cdf, uint64_t n, const DevState* st, int which, uint32_t it0, uint32_t nit,
                          uint32_t buf_it0, uint32_t c, uint32_t* idx, float* wgt,
                          uint32_t* seg_count, hipStream_t s);

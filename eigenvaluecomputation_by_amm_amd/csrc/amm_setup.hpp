@@ -19,9 +19,9 @@ namespace amm {
 // one wave walking the whole column.)
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void colstats_column(const T* __restrict__ col, uint32_t d, uint64_t j,
-                                                float* __restrict__ alpha, double* __restrict__ nrm2,
-                                                double* __restrict__ cnrm2) {
+__device__ __forceinline__ double2 colstats_column(const T* __restrict__ col, uint32_t d, uint64_t j,
+                                                   float* __restrict__ alpha, double* __restrict__ nrm2,
+                                                   double* __restrict__ cnrm2) {
   constexpr int V = Elem<T>::VEC;
   __shared__ double part[2][4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -63,6 +63,7 @@ __device__ __forceinline__ void colstats_column(const T* __restrict__ col, uint3
     part[1][w] = ss;
   }
   __syncthreads();
+  double2 out = make_double2(0.0, 0.0);  // (||a_j||^2, centred) in thread 0
   if (threadIdx.x == 0) {
     const double S = (part[0][0] + part[0][1]) + (part[0][2] + part[0][3]);
     const double SS = (part[1][0] + part[1][1]) + (part[1][2] + part[1][3]);
@@ -72,7 +73,9 @@ __device__ __forceinline__ void colstats_column(const T* __restrict__ col, uint3
     double c = SS - S * mean;
     c = c > 0.0 ? c : (c == c ? 0.0 : c);  // clamp rounding below 0, keep NaN
     cnrm2[j] = c;
+    out = make_double2(SS, c);
   }
+  return out;
 }
 
 // ---------------------------------------------------------------------------

@@ -57,11 +57,14 @@ def test_column_stats(d, n, ld, bf16):
     assert abs(a2 - float((alpha.astype(np.float64) ** 2).sum())) / a2 < 1e-12
 
 
+# d = 16384 FP32: the statistics pass runs over runs of columns and hands
+# its per-workgroup maxima to the CDF in place of the CDF's maximum pass
+@pytest.mark.parametrize("d,n", [(128, 5000), (16384, 2048)])
 @pytest.mark.parametrize("centred", [False, True])
-def test_cdf_bitwise(centred):
-    banner(f"cdf centred={centred}")
+def test_cdf_bitwise(centred, d, n):
+    banner(f"cdf centred={centred} d={d} n={n}")
     AmmContext, *_ = _amm()
-    A = heavy_columns(128, 5000, seed=2)
+    A = heavy_columns(d, n, seed=2)
     A[:, 17] = 0.0
     t, _ = dev_matrix(A)
     with AmmContext(t) as ctx:
