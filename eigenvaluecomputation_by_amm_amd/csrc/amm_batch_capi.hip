@@ -58,6 +58,7 @@ constexpr uint32_t kWeightBlocks = 4096;     // weight workgroups at most (about
 constexpr uint32_t kFinishSlabs = 4096;      // finish workgroups of 256 threads at most
 constexpr uint32_t kMaxSlices = 16;      // contraction-1 row slices per problem
 constexpr uint32_t kZChunk = 2048;       // columns per chunk of z = A alpha
+constexpr uint64_t kNtMinBytes = 512ull << 20;  // matrix bytes beyond which the contractions stream A non-temporally
 
 uint64_t rup(uint64_t a, uint64_t b) { return (a + b - 1) / b * b; }
 uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
@@ -239,21 +240,50 @@ int upload_plan(amm_batch* b, int kernel) {
   std::vector<uint32_t> all;
   std::vector<std::pair<amm::BGroup*, size_t>> fix;
   // a group: offsets off[0 .. np] of each problem's first workgroup, then
-  // the problem of every workgroup (find_problem reads it in one load)
-  auto add_group = [&](amm::BGroup& g, auto count_fn) {
+  // the problem of every workgroup (find_problem reads it in one load).
+  // With `order`, the problems' workgroups follow each other in that order
+  // (off[p] is still problem p's first workgroup, off[np] the total); only
+  // kernels that read off[p] alone, not off[p + 1] - off[p], take such a group.
+  auto add_group = [&](amm::BGroup& g, auto count_fn, const std::vector<uint32_t>* order = nullptr) {
     const size_t base = all.size();
+    all.resize(base + np + 1);
     uint64_t acc = 0;
-    all.push_back(0);
-    for (uint32_t i = 0; i < np; ++i) {
+    for (uint32_t k = 0; k < np; ++k) {
+      const uint32_t i = order ? (*order)[k] : k;
+      all[base + i] = (uint32_t)std::min<uint64_t>(acc, 0xFFFFFFFFu);
       acc += count_fn(b->hp[i]);
-      all.push_back((uint32_t)acc);
     }
     if (acc >= (1ull << 31)) return false;
-    for (uint32_t i = 0; i < np; ++i) all.insert(all.end(), all[base + i + 1] - all[base + i], i);
+    all[base + np] = (uint32_t)acc;
+    for (uint32_t k = 0; k < np; ++k) {
+      const uint32_t i = order ? (*order)[k] : k;
+      all.insert(all.end(), (size_t)count_fn(b->hp[i]), i);
+    }
     g.total = (uint32_t)acc;
     fix.emplace_back(&g, base);
     return true;
   };
+  // Contraction workgroups are long-lived and their sizes differ by problem
+  // (a contraction-2 workgroup of a problem with one chunk reads all n
+  // columns of its row block): dispatched in problem order, the largest
+  // could start last and run on alone at the end of the launch.  Both
+  // contraction groups therefore go largest workgroup first, sized at the
+  // dense union (every column present).
+  auto largest_first = [&](auto bytes_fn) {
+    std::vector<uint32_t> ord(np);
+    for (uint32_t i = 0; i < np; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](uint32_t x, uint32_t y) { return bytes_fn(b->hp[x]) > bytes_fn(b->hp[y]); });
+    return ord;
+  };
+  const std::vector<uint32_t> dot_order = largest_first([](const BProb& p) {
+    const amm::UPlan up = amm::union_plan(p, p.n);
+    return (uint64_t)amm::kBTile * up.kslice_rows * (p.dtype == 1 ? 2u : 4u);
+  });
+  const std::vector<uint32_t> axpy_order = largest_first([](const BProb& p) {
+    const amm::UPlan up = amm::union_plan(p, p.n);
+    return (uint64_t)std::min(up.cpc, p.n) * block_rows(p.passes) * (p.dtype == 1 ? 2u : 4u);
+  });
   bool ok = true;
   // statistics: a workgroup per column, or per four short columns (a wave each)
   ok &= add_group(b->g_cols, [](const BProb& p) {
@@ -289,15 +319,21 @@ int upload_plan(amm_batch* b, int kernel) {
   ok &= add_group(b->g_fin, [f_fin](const BProb& p) { return (uint32_t)cdiv(p.nvblk, f_fin); });
   ok &= add_group(b->g_weight, [f_weight](const BProb& p) { return (uint32_t)cdiv(p.ntiles, f_weight); });
   for (int dt = 0; dt < 2; ++dt)
-    ok &= add_group(b->g_dot[dt], [dt](const BProb& p) {
-      // vtiles * kslices(|U|) < dot_blocks + vtiles for every union size
-      return p.dtype == dt ? std::min(p.ntiles * p.kslices_max, p.dot_blocks + p.ntiles) : 0u;
-    });
+    ok &= add_group(
+        b->g_dot[dt],
+        [dt](const BProb& p) {
+          // vtiles * kslices(|U|) < dot_blocks + vtiles for every union size
+          return p.dtype == dt ? std::min(p.ntiles * p.kslices_max, p.dot_blocks + p.ntiles) : 0u;
+        },
+        &dot_order);
   for (int dt = 0; dt < 2; ++dt)
     for (int pa = 0; pa < 2; ++pa)
-      ok &= add_group(b->g_axpy[dt][pa], [dt, pa](const BProb& p) {
-        return (p.dtype == dt && p.passes == (uint32_t)pa + 1) ? p.rblocks * p.chunks : 0u;
-      });
+      ok &= add_group(
+          b->g_axpy[dt][pa],
+          [dt, pa](const BProb& p) {
+            return (p.dtype == dt && p.passes == (uint32_t)pa + 1) ? p.rblocks * p.chunks : 0u;
+          },
+          &axpy_order);
   if (!ok) return fail(AMM_E_INVALID, "a grouped launch exceeds 2^31 workgroups");
   if (all.size() > b->offs_cap) {
     if (b->offs) (void)hipFree(b->offs);
@@ -734,8 +770,20 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     p.v = s.v;
     p.eigvec = nullptr;
   }
-  b->nt_loads = true;
-  for (uint32_t i = 0; i < count; ++i) b->nt_loads = b->nt_loads && probs[i].d >= 4096;
+  // Non-temporal loads of A in the contractions where every problem has
+  // d >= 4096 (+5% on the dense passes of 16384 x 65536 and 64 x 4096^2),
+  // and wherever the batch's matrices exceed twice the 256 MB Infinity
+  // Cache, so that no column survives from one pass to the next anyway
+  // (256 matrices, 36 GB BF16: 12.32 against 13.01 ms per dense pass).
+  {
+    bool all_tall = true;
+    uint64_t bytes = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+      all_tall = all_tall && probs[i].d >= 4096;
+      bytes += probs[i].d * probs[i].n * (probs[i].dtype == AMM_DTYPE_BF16 ? 2u : 4u);
+    }
+    b->nt_loads = all_tall || bytes > kNtMinBytes;
+  }
   uint64_t tiles = 0;
   b->tiles.resize(count);
   for (uint32_t i = 0; i < count; ++i) {
