@@ -628,6 +628,40 @@ def probe_config(args):
           flush=True)
 
 
+def probe_mix_config(args):
+    """The batched contractions on the many-matrix mix (`--count` matrices,
+    d ~ U[512, 8192], n ~ U[1024, 32768], BF16, leading dimension padded to
+    512 bytes, N(0, 1) entries: traffic does not depend on the values), 16
+    trials, for hardware-counter profiling.  Launches, in order: a plain
+    device copy of 1 GiB (the scale for FETCH_SIZE / WRITE_SIZE), then
+    `--probe-reps` dense passes of both contractions (every column present:
+    k_bdot, k_bweight, k_baxpy)."""
+    import torch
+    from eigenvaluecomputation_by_amm_amd import AmmBatch
+    from eigenvaluecomputation_by_amm_amd.workloads import padded_ld, uneven_problem_shapes
+    reps = args.probe_reps
+    buf = torch.empty(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+    out = torch.empty_like(buf)
+    for _ in range(reps):
+        out.copy_(buf)
+    torch.cuda.synchronize()
+    del buf, out
+    shapes = uneven_problem_shapes(args.count)
+    mats = []
+    for i, (d, n) in enumerate(shapes):
+        g = torch.Generator(device="cuda").manual_seed(1000 + i)
+        M = torch.zeros(n, padded_ld(d), dtype=torch.bfloat16, device="cuda")
+        M[:, :d] = torch.randn(n, d, generator=g, device="cuda").to(torch.bfloat16)
+        mats.append(M)
+    with AmmBatch([dict(A=m, d=d, c=max(1, n // 8), seed=77 + i) for i, (m, (d, n)) in enumerate(zip(mats, shapes))],
+                  trials=16) as b:
+        ms = b.time_dense(reps)
+    elems = sum(d * n for d, n in shapes)
+    print(json.dumps({"probe": "done", "reps": reps, "workload": f"{len(shapes)} uneven BF16 matrices",
+                      "elements": elems, "dense_pass_ms": round(ms, 3),
+                      "algorithmic_bytes_per_contraction": elems * 2}), flush=True)
+
+
 def kernel_roofline(ctx, A, d, n, n_exact, n_amm, kernel_iters):
     """Per-kernel durations (HIP events on the context stream) and each
     kernel's share of the headline run up to tolerance; the roofline entry
@@ -715,12 +749,13 @@ def main():
     ap.add_argument("--kernel-iters", type=int, default=200)
     ap.add_argument("--batch-reps", type=int, default=3, help="batched runs (16 seeds each)")
     ap.add_argument("--probe-reps", type=int, default=3, help="launches per kernel of --config probe")
-    ap.add_argument("--config", choices=["headline", "batch", "bf16", "sigma", "seeds", "probe"],
+    ap.add_argument("--config", choices=["headline", "batch", "bf16", "sigma", "seeds", "probe", "probe-mix"],
                     default="headline",
                     help="batch: many uneven matrices x 16 seeds; bf16: image-like BF16 workload; "
                          "sigma: weight stack; seeds: distribution over --seeds seeds of every "
                          "kernel x c on the headline matrix; probe: kernels for counter profiling "
-                         "(separate JSON lines)")
+                         "(separate JSON lines); probe-mix: the batched contractions on the "
+                         "many-matrix mix, for counter profiling")
     ap.add_argument("--seeds", type=int, default=100, help="seeds per cell of --config seeds")
     ap.add_argument("--count", type=int, default=256, help="matrices of the c4 section")
     ap.add_argument("--batch-iters", type=int, default=150)
@@ -738,6 +773,9 @@ def main():
 
     if args.config == "probe":
         probe_config(args)
+        return
+    if args.config == "probe-mix":
+        probe_mix_config(args)
         return
     if args.config == "batch":
         t0 = time.perf_counter()

@@ -182,19 +182,21 @@ __global__ __launch_bounds__(256) void k_bnorm(const BProb* __restrict__ P, cons
       }
     }
     *reinterpret_cast<float4*>(pr.X + o) = make_float4(xv[0], xv[1], xv[2], xv[3]);
+    // dx in the trial-interleaved staging layout (dx_offset)
     if (pr.dtype == 1) {
+      const uint64_t od = dx_offset<32>(k, row);
       uint16_t h[4], l[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         h[e] = bf16_rne(dv[e]);
         l[e] = bf16_rne(dv[e] - bf16_bits_to_f32(h[e]));
       }
-      *reinterpret_cast<uint2*>(pr.DXh + o) =
+      *reinterpret_cast<uint2*>(pr.DXh + od) =
           make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
-      *reinterpret_cast<uint2*>(pr.DXl + o) =
+      *reinterpret_cast<uint2*>(pr.DXl + od) =
           make_uint2((uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16));
     } else {
-      *reinterpret_cast<float4*>(pr.DX + o) = make_float4(dv[0], dv[1], dv[2], dv[3]);
+      *reinterpret_cast<float4*>(pr.DX + dx_offset<16>(k, row)) = make_float4(dv[0], dv[1], dv[2], dv[3]);
     }
   }
   if (md == kModeAmmProposed) {
@@ -605,13 +607,14 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
   auto load_dx = [&](uint32_t row0) {
 #pragma unroll
     for (uint32_t s = 0; s < 2; ++s) {
+      // rows of trial li in the trial-interleaved layout: the wave's 64
+      // lanes read 1 KB of consecutive bytes
       if constexpr (BF) {
-        const uint32_t rr = row0 + wave * 64 + s * 32 + 8 * lg;
-        xa[s] = *reinterpret_cast<const u32x4*>(pr.DXh + (uint64_t)li * pr.dstage + rr);
-        xb[s] = *reinterpret_cast<const u32x4*>(pr.DXl + (uint64_t)li * pr.dstage + rr);
+        const uint64_t o = dx_offset<32>(li, row0 + wave * 64 + s * 32 + 8 * lg);
+        xa[s] = *reinterpret_cast<const u32x4*>(pr.DXh + o);
+        xb[s] = *reinterpret_cast<const u32x4*>(pr.DXl + o);
       } else {
-        const uint32_t rr = row0 + wave * 32 + s * 16 + 4 * lg;
-        xa[s] = *reinterpret_cast<const u32x4*>(pr.DX + (uint64_t)li * pr.dstage + rr);
+        xa[s] = *reinterpret_cast<const u32x4*>(pr.DX + dx_offset<16>(li, row0 + wave * 32 + s * 16 + 4 * lg));
       }
     }
   };

@@ -58,9 +58,9 @@ struct alignas(16) BProb {
   float* Y;             // [16][dstage] product
   float* XM;            // [16][dstage] x_memo
   float* YM;            // [16][dstage] y_memo
-  float* DX;            // [16][dstage] FP32 staging of x - x_memo (FP32 A)
-  uint16_t* DXh;        // [16][dstage] BF16 high part (BF16 A)
-  uint16_t* DXl;        // [16][dstage] BF16 low part
+  float* DX;            // [dstage / 16][16][16] FP32 staging of x - x_memo (FP32 A), dx_offset<16>
+  uint16_t* DXh;        // [dstage / 32][16][32] BF16 high part (BF16 A), dx_offset<32>
+  uint16_t* DXl;        // [dstage / 32][16][32] BF16 low part
   uint16_t* cnt;        // [16][ntiles*64] draw counts, a row per trial (rewritten whole when it samples)
   uint32_t* list;       // [ntiles*64] compacted column indices, per tile
   uint32_t* tcount;     // [ntiles] columns present per tile
@@ -85,6 +85,16 @@ struct alignas(16) BProb {
   uint32_t* modes;      // [16][step] step kind used at each iteration
   uint32_t* pad_ptr;
 };
+
+// Staged dx (the operand of contraction 1) is stored trial-interleaved by
+// groups of G rows: rows [G g, G g + G) of trial k are the G consecutive
+// elements at (g * 16 + k) * G, so one K step of a wave (G rows of all 16
+// trials: 1 KB) is one contiguous read.  G = 32 for the BF16 parts, 16 for
+// FP32.
+template <uint32_t G>
+__host__ __device__ inline uint64_t dx_offset(uint32_t k, uint32_t row) {
+  return ((uint64_t)(row / G) * kBT + k) * G + row % G;
+}
 
 // Scalars shared by every launch of a run.
 struct BRun {
