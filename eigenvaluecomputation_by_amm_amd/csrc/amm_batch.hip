@@ -496,6 +496,110 @@ __global__ __launch_bounds__(256) void k_bcompact(const BProb* __restrict__ P, c
 }
 
 // ---------------------------------------------------------------------------
+// The three passes above in one launch, one 1024-thread workgroup per problem
+// (for problems of at most kUnionMaxTiles tiles whose count rows are small:
+// a lone workgroup reads them all).  Wave w takes the 512-column groups w,
+// w + 16, ...: lane l finds the presence of columns 8 l .. 8 l + 7 of the
+// group as k_bprep does and keeps the bits in LDS, the lane octet of a tile
+// adds up its count; the tile counts are scanned into tpre (k_bscan); then
+// each lane writes its present columns to their union slots tpre[t] +
+// (position within the tile), the order k_bprep + k_bcompact produce.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_bunion(const BProb* __restrict__ P, BRun r) {
+  __shared__ uint32_t exact_any;
+  __shared__ uint8_t bits_l[kUnionMaxTiles * 8];  // presence of 8 columns per byte
+  __shared__ uint32_t tc[kUnionMaxTiles];         // tile counts, then their exclusive prefix
+  __shared__ uint32_t wsum[16];
+  const BProb& pr = P[blockIdx.x];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) exact_any = 0u;
+  __syncthreads();
+  if (tid < r.ntr) {
+    const uint32_t md = pr.bt[tid].mode;
+    if (md == kModeExactMemo || md == kModeExactNoMemo) exact_any = 1u;
+  }
+  __syncthreads();
+  const bool all_present = exact_any != 0u;
+  const uint32_t nt = pr.ntiles;
+  const uint32_t npad = nt * kBTile;
+  const uint32_t ngroups = (nt + 7) / 8;
+  for (uint32_t grp = wave; grp < ngroups; grp += 16) {
+    const uint32_t tile = grp * 8 + (lane >> 3);
+    const uint32_t j0 = grp * 512 + lane * 8;
+    uint32_t bits = 0u;
+    if (tile < nt) {
+      if (all_present) {
+        bits = 0xFFu;
+      } else {
+        uint4 o = make_uint4(0u, 0u, 0u, 0u);
+        for (uint32_t k = 0; k < r.ntr; ++k) {
+          const uint4 v = *reinterpret_cast<const uint4*>(pr.cnt + (uint64_t)k * npad + j0);
+          o.x |= v.x;
+          o.y |= v.y;
+          o.z |= v.z;
+          o.w |= v.w;
+        }
+        const uint32_t w4[4] = {o.x, o.y, o.z, o.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          bits |= ((w4[q] & 0xFFFFu) != 0u ? 1u : 0u) << (2 * q);
+          bits |= ((w4[q] >> 16) != 0u ? 1u : 0u) << (2 * q + 1);
+        }
+      }
+      if (j0 + 8 > pr.n) bits &= j0 >= pr.n ? 0u : (1u << (pr.n - j0)) - 1u;
+      bits_l[grp * 64 + lane] = (uint8_t)bits;
+    }
+    uint32_t incl = (uint32_t)__popc(bits);
+#pragma unroll
+    for (uint32_t o = 1; o < 8; o <<= 1) {
+      const uint32_t v = __shfl_up(incl, o, 8);
+      if ((lane & 7u) >= o) incl += v;
+    }
+    if (tile < nt && (lane & 7u) == 7u) tc[tile] = incl;
+  }
+  __syncthreads();
+  // exclusive prefix of the tile counts (nt <= 1024: a tile per thread)
+  const uint32_t v = tid < nt ? tc[tid] : 0u;
+  uint32_t incl = v;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  uint32_t base = 0, total = 0;
+  for (uint32_t w = 0; w < 16; ++w) {
+    base += w < wave ? wsum[w] : 0u;
+    total += wsum[w];
+  }
+  if (tid < nt) {
+    tc[tid] = base + incl - v;
+    pr.tpre[tid] = base + incl - v;
+  }
+  if (tid == 0) pr.tpre[nt] = total;
+  __syncthreads();
+  for (uint32_t grp = wave; grp < ngroups; grp += 16) {
+    const uint32_t tile = grp * 8 + (lane >> 3);
+    const uint32_t bits = tile < nt ? (uint32_t)bits_l[grp * 64 + lane] : 0u;
+    const uint32_t cntl = (uint32_t)__popc(bits);
+    uint32_t in8 = cntl;
+#pragma unroll
+    for (uint32_t o = 1; o < 8; o <<= 1) {
+      const uint32_t t = __shfl_up(in8, o, 8);
+      if ((lane & 7u) >= o) in8 += t;
+    }
+    if (tile < nt) {
+      uint32_t pos = tc[tile] + in8 - cntl;
+      const uint32_t j0 = grp * 512 + lane * 8;
+#pragma unroll
+      for (uint32_t b2 = 0; b2 < 8; ++b2)
+        if ((bits >> b2) & 1u) pr.colc[pos++] = j0 + b2;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // LDS tile of C columns x RB rows staged from HBM with long contiguous runs:
 // each column's RB rows are read by RB*sizeof(T)/16 consecutive lanes (512
 // or 1024 contiguous bytes), prefetched into registers one round ahead and
@@ -1261,6 +1365,17 @@ hipError_t launch_bscan(const BProb* P, uint32_t np, hipStream_t s) {
 hipError_t launch_bcompact(const BProb* P, BGroup g, uint32_t np, hipStream_t s) {
   if (g.total) k_bcompact<<<g.total, 256, 0, s>>>(P, g.off, np);
   return hipGetLastError();
+}
+
+hipError_t launch_bunion(const BProb* P, BGroup g, BRun r, bool fused, hipStream_t s) {
+  if (fused) {
+    k_bunion<<<r.np, 1024, 0, s>>>(P, r);
+    return hipGetLastError();
+  }
+  hipError_t e = launch_bprep(P, g, r, s);
+  if (e == hipSuccess) e = launch_bscan(P, r.np, s);
+  if (e == hipSuccess) e = launch_bcompact(P, g, r.np, s);
+  return e;
 }
 
 hipError_t launch_bweight(const BProb* P, BGroup g, BRun r, uint32_t proposed_any, uint32_t out_w, hipStream_t s) {

@@ -26,6 +26,7 @@ constexpr uint32_t kBStageAlign = 256;  // staged vectors are padded to this man
 constexpr uint32_t kBSampleSlots = 2048;  // draws (slots) per pass of a sampler workgroup
 constexpr uint32_t kDenseCols = 32768;    // columns whose 16-bit counts the sampler holds in LDS (64 KB)
 constexpr uint32_t kMaxBatchC = 65535;    // draws per trial and iteration (the counts are 16-bit)
+constexpr uint32_t kUnionMaxTiles = 1024;  // tiles per problem of the one-workgroup union pass
 
 // Per (problem, trial) scalar state.
 struct alignas(16) BTrial {
@@ -188,6 +189,10 @@ hipError_t launch_bprep(const BProb* P, BGroup g, BRun r, hipStream_t s);
 hipError_t launch_bscan(const BProb* P, uint32_t np, hipStream_t s);
 // items: ceil(ntiles / 4) per problem (the k_bprep group)
 hipError_t launch_bcompact(const BProb* P, BGroup g, uint32_t np, hipStream_t s);
+// the union of a product: k_bprep, k_bscan, k_bcompact (g: the k_bprep
+// group), or with fused k_bunion, one workgroup per problem (every problem
+// has at most kUnionMaxTiles tiles)
+hipError_t launch_bunion(const BProb* P, BGroup g, BRun r, bool fused, hipStream_t s);
 // items: any count per problem (the blocks stride over the union slots);
 // t of every union slot, or (out_w) the bare weights
 hipError_t launch_bweight(const BProb* P, BGroup g, BRun r, uint32_t proposed_any, uint32_t out_w, hipStream_t s);

@@ -58,7 +58,8 @@ constexpr uint32_t kWeightBlocks = 4096;     // weight workgroups at most (about
 constexpr uint32_t kFinishSlabs = 4096;      // finish workgroups of 256 threads at most
 constexpr uint32_t kMaxSlices = 16;      // contraction-1 row slices per problem
 constexpr uint32_t kZChunk = 2048;       // columns per chunk of z = A alpha
-constexpr uint64_t kNtMinBytes = 512ull << 20;  // matrix bytes beyond which the contractions stream A non-temporally
+constexpr uint64_t kNtMinBytes = 512ull << 20;
+constexpr uint64_t kUnionFusedBytes = 256u << 10;  // count rows per problem up to which one workgroup forms its union  // matrix bytes beyond which the contractions stream A non-temporally
 
 uint64_t rup(uint64_t a, uint64_t b) { return (a + b - 1) / b * b; }
 uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
@@ -126,6 +127,7 @@ struct amm_batch {
   // rows (measured: +4-6% on 16384 x 65536 and 64 x 4096 x 4096, -4% on a
   // mix of small matrices, whose second contraction finds columns in cache)
   bool nt_loads = false;
+  bool union_fused = false;  // the union pass as one workgroup per problem (k_bunion)
   // grouped setup: descriptors (device) and their launch groups
   amm::BSetup* dS = nullptr;
   std::vector<uint32_t> tiles;  // CDF tiles per problem
@@ -488,9 +490,7 @@ int enq_iteration(amm_batch* b, const amm::BRun& r, int K, uint32_t it) {
   const BProb* P = b->dP;
   hipStream_t s = b->stream;
   if (K != AMM_KERNEL_EXACT) BHIPCHK(amm::launch_bsample(P, b->g_sample, r, it, s));
-  BHIPCHK(amm::launch_bprep(P, b->g_prep, r, s));
-  BHIPCHK(amm::launch_bscan(P, b->np, s));
-  BHIPCHK(amm::launch_bcompact(P, b->g_prep, b->np, s));
+  BHIPCHK(amm::launch_bunion(P, b->g_prep, r, b->union_fused, s));
   const uint32_t prop = (K == AMM_KERNEL_PROPOSED || K == AMM_KERNEL_PROPOSED_V2) ? 1u : 0u;
   for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bdot(dt, b->nt_loads, P, b->g_dot[dt], b->np, s));
   BHIPCHK(amm::launch_bweight(P, b->g_weight, r, prop, 0u, s));
@@ -784,6 +784,14 @@ int amm_batch_create(const amm_problem* probs, uint32_t count, uint32_t trials, 
     }
     b->nt_loads = all_tall || bytes > kNtMinBytes;
   }
+  // The union of a product in one launch (a workgroup per problem) where
+  // every problem's count rows are small enough for one workgroup to read
+  // (64 x 4096^2 with one trial: three launches of about 5 us each become
+  // one); otherwise the three grid-wide passes.
+  b->union_fused = true;
+  for (uint32_t i = 0; i < count; ++i)
+    b->union_fused = b->union_fused && cdiv(probs[i].n, amm::kBTile) <= amm::kUnionMaxTiles &&
+                     (uint64_t)trials * probs[i].n * 2u <= kUnionFusedBytes;
   uint64_t tiles = 0;
   b->tiles.resize(count);
   for (uint32_t i = 0; i < count; ++i) {
@@ -980,9 +988,7 @@ int amm_batch_sample(amm_batch* b, int32_t kernel, uint32_t iter, uint32_t trial
   if ((rc = enq_setup(b, kernel))) return rc;
   b->stats_valid = false;
   BHIPCHK(amm::launch_bsample(b->dP, b->g_sample, r, iter, b->stream));
-  BHIPCHK(amm::launch_bprep(b->dP, b->g_prep, r, b->stream));
-  BHIPCHK(amm::launch_bscan(b->dP, b->np, b->stream));
-  BHIPCHK(amm::launch_bcompact(b->dP, b->g_prep, b->np, b->stream));
+  BHIPCHK(amm::launch_bunion(b->dP, b->g_prep, r, b->union_fused, b->stream));
   // the per-trial weights of every union slot, as the products form them
   BHIPCHK(amm::launch_bweight(b->dP, b->g_weight, r, 0u, 1u, b->stream));
   const BProb& q = b->hp[problem];
@@ -1038,9 +1044,7 @@ int amm_batch_sigma_max(amm_batch* b, const amm_params* p, double* sigma_host, f
   // t = A^T x for every trial: one exact half-step over the dense union
   hipError_t e = amm::launch_bset_mode(b->dP, b->np, amm::kModeExactNoMemo, b->stream);
   if (e == hipSuccess) e = amm::launch_bnorm(b->dP, b->g_vec, r, b->stream);
-  if (e == hipSuccess) e = amm::launch_bprep(b->dP, b->g_prep, r, b->stream);
-  if (e == hipSuccess) e = amm::launch_bscan(b->dP, b->np, b->stream);
-  if (e == hipSuccess) e = amm::launch_bcompact(b->dP, b->g_prep, b->np, b->stream);
+  if (e == hipSuccess) e = amm::launch_bunion(b->dP, b->g_prep, r, b->union_fused, b->stream);
   for (int dt = 0; dt < 2 && e == hipSuccess; ++dt) e = amm::launch_bdot(dt, b->nt_loads, b->dP, b->g_dot[dt], b->np, b->stream);
   if (e == hipSuccess) e = amm::launch_bweight(b->dP, b->g_weight, r, 0u, 0u, b->stream);
   if (e == hipSuccess)
@@ -1089,9 +1093,7 @@ int time_dense(amm_batch* b, uint32_t iters, double* ms_out) {
   drop_graph(b);
   const amm::BRun r = run_args(b, p);
   BHIPCHK(amm::launch_bbegin(b->dP, b->g_vec, r, b->stream));
-  BHIPCHK(amm::launch_bprep(b->dP, b->g_prep, r, b->stream));
-  BHIPCHK(amm::launch_bscan(b->dP, b->np, b->stream));
-  BHIPCHK(amm::launch_bcompact(b->dP, b->g_prep, b->np, b->stream));
+  BHIPCHK(amm::launch_bunion(b->dP, b->g_prep, r, b->union_fused, b->stream));
   for (uint32_t w = 0; w < 2; ++w) {  // warm-up, then timed
     BHIPCHK(hipEventRecord(b->ev0, b->stream));
     for (uint32_t k = 0; k < (w ? iters : 1); ++k) {
