@@ -107,14 +107,26 @@ __device__ __forceinline__ void colstats_wave(const T* __restrict__ col, uint32_
   double s = 0.0, ss = 0.0;
   for (uint32_t row = lane * V; row < d; row += 8 * step) {
     float a[8][V];
+    if (row - lane * V + 8 * step <= d) {
+      // the whole trip lies within the column (wave-uniform): its eight
+      // non-temporal loads go out together, with no bound per load
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const uint32_t r = row + u * step;
-      if (r < d) {
-        load_col<T>(col, r, d, a[u]);  // zeros at and beyond d
-      } else {
+      for (int u = 0; u < 8; ++u) Elem<T>::template load_full<true>(col + row + u * step, a[u]);
+    } else {
+      // the last, partial trip: its whole 64-lane steps load as above, the
+      // step that crosses d row by row
+      const uint32_t full = (d - (row - lane * V)) / step;  // wave-uniform, < 8
 #pragma unroll
-        for (int k = 0; k < V; ++k) a[u][k] = 0.0f;
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t r = row + u * step;
+        if ((uint32_t)u < full) {
+          Elem<T>::template load_full<true>(col + r, a[u]);
+        } else if (r < d) {
+          load_col<T>(col, r, d, a[u]);  // zeros at and beyond d
+        } else {
+#pragma unroll
+          for (int k = 0; k < V; ++k) a[u][k] = 0.0f;
+        }
       }
     }
 #pragma unroll
