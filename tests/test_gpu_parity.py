@@ -38,10 +38,14 @@ def banner(msg):
 # --------------------------------------------------------------------------
 # a wave per column under 16 KB, a workgroup per column from 16 KB (4100
 # FP32 rows, 8200 BF16 rows), workgroups walking runs of columns for FP32
-# columns of 64 KB or more (16384 x 2048: four columns each)
+# columns of 64 KB or more (16384 x 2048: four columns each).  Short
+# columns: whole trips of eight 64-lane steps (2048 FP32 / 4096 BF16 rows),
+# whole steps of a partial trip, and the step that crosses d.
 @pytest.mark.parametrize("d,n,ld,bf16", [(256, 300, 256, False), (1000, 777, 1004, False),
                                          (37, 5, 40, False), (512, 300, 512, True),
-                                         (1001, 33, 1008, True), (4100, 300, 4104, False),
+                                         (1001, 33, 1008, True), (2049, 64, 2052, False),
+                                         (3000, 200, 3000, False), (4096, 100, 4096, True),
+                                         (6000, 150, 6016, True), (4100, 300, 4104, False),
                                          (8200, 260, None, True), (16384, 2048, None, False)])
 def test_column_stats(d, n, ld, bf16):
     banner(f"column_stats d={d} n={n} ld={ld} bf16={bf16}")
