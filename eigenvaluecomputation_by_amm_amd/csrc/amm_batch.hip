@@ -6,6 +6,7 @@
 // summation order of the contractions differs.
 #include "amm_batch.hpp"
 #include "amm_setup.hpp"
+#include "amm_resident.hpp"
 
 namespace amm {
 
@@ -35,6 +36,13 @@ __device__ __forceinline__ T sum16(T v) {
 
 __device__ __forceinline__ bool is_amm(uint32_t md) {
   return md == kModeAmmPrevious || md == kModeAmmProposed;
+}
+
+// the product of this iteration is k_bgram's (a one-trial batch whose trial
+// takes an exact step)
+__device__ __forceinline__ bool gram_step(const BProb& pr) {
+  const uint32_t md = pr.bt[0].mode;
+  return pr.gram_chunks != 0 && (md == kModeExactMemo || md == kModeExactNoMemo);
 }
 
 // ---------------------------------------------------------------------------
@@ -673,7 +681,7 @@ struct TileStage {
 // ---------------------------------------------------------------------------
 template <typename T, bool NT>
 __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const uint32_t* off,
-                                              uint32_t np) {
+                                              uint32_t np, uint32_t skip_exact) {
   constexpr bool BF = Elem<T>::VEC == 8;
   constexpr uint32_t RB = BF ? 256 : 128;
   using Stage = TileStage<T, 64, RB, NT>;
@@ -681,6 +689,7 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
   __shared__ uint32_t cols[64];
   const uint32_t p = find_problem(off, np, blockIdx.x);
   const BProb& pr = P[p];
+  if (skip_exact && gram_step(pr)) return;
   const uint32_t local = blockIdx.x - off[p];
   // the grid covers the largest vtiles x kslices any union size can need
   // (< dot_blocks + ntiles); item = slice-major over this product's
@@ -801,10 +810,12 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
 // blocks stride over the union, 64 slots per workgroup and pass.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bweight(const BProb* __restrict__ P, const uint32_t* off,
-                                                 BRun r, uint32_t proposed_any, uint32_t out_w) {
+                                                 BRun r, uint32_t proposed_any, uint32_t out_w,
+                                                 uint32_t skip_exact) {
   __shared__ double s_sk[kBT];
   const uint32_t p = find_problem(off, r.np, blockIdx.x);
   const BProb& pr = P[p];
+  if (skip_exact && gram_step(pr)) return;
   const uint32_t local = blockIdx.x - off[p];
   const uint32_t nblk = off[p + 1] - off[p];
   const UPlan up = union_plan(pr, pr.tpre[pr.ntiles]);
@@ -921,7 +932,7 @@ __global__ __launch_bounds__(256) void k_bweight(const BProb* __restrict__ P, co
 // ---------------------------------------------------------------------------
 template <typename T, uint32_t PASSES, bool NT>
 __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, const uint32_t* off,
-                                               uint32_t np, uint32_t proposed_any) {
+                                               uint32_t np, uint32_t proposed_any, uint32_t skip_exact) {
   constexpr bool BF = Elem<T>::VEC == 8;
   constexpr uint32_t C = BF ? 32 : 16;
   constexpr uint32_t RB = 256 * PASSES;
@@ -940,6 +951,7 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
   __shared__ double cred[256];
   const uint32_t p = find_problem(off, np, blockIdx.x);
   const BProb& pr = P[p];
+  if (skip_exact && gram_step(pr)) return;
   const uint32_t local = blockIdx.x - off[p];
   // chunk-major order: active chunks form one contiguous range of blocks
   const uint32_t ch = local / pr.rblocks, rb = local % pr.rblocks;
@@ -1070,6 +1082,74 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
 }
 
 // ---------------------------------------------------------------------------
+// The exact step of a one-trial batch in one read of A (the two cublasSgemv
+// calls of powerMethod_by_exact.cuh:46-62, in k_gram's column-resident
+// form): workgroup (problem p, chunk ch) walks the contiguous columns
+// [n ch / G, n (ch + 1) / G) of its problem (G = gram_chunks), holds each
+// column in registers, forms c_j = a_j^T x (FP64 sum of exact FP32
+// products, rounded to FP32) with x = trial 0's iterate in LDS, and adds
+// c_j a_j into a register partial of its rows; the partial goes to
+// P2[ch][trial 0] for k_bfinish.  The next column streams in while the
+// workgroup reduces the current one.  Exits unless trial 0 takes an exact
+// step.
+// ---------------------------------------------------------------------------
+template <typename T, int L>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_bgram(
+    const BProb* __restrict__ P, const uint32_t* off, uint32_t np) {
+  constexpr int V = Elem<T>::VEC;
+  constexpr uint32_t kRows = 4u * L * 64u * V;
+  __shared__ float xs[kRows];
+  __shared__ double part[2][4];
+  const uint32_t p = find_problem(off, np, blockIdx.x);
+  const BProb& pr = P[p];
+  if (!gram_step(pr)) return;
+  const uint64_t ch = blockIdx.x - off[p];
+  const uint64_t G = pr.gram_chunks, n = pr.n;
+  const uint64_t j1 = n * (ch + 1) / G;
+  const uint32_t d = pr.d;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t q = resident_rows<V>(d);  // <= L * 64 * V (host: gram_L)
+  const uint32_t rb = w * q;
+  const uint32_t re = rb + q < d ? rb + q : d;
+  const T* A = reinterpret_cast<const T*>(pr.A);
+  ColRegs<T, L> cr;
+  uint64_t j = n * ch / G;
+  if (j < j1) cr.load(A + j * pr.ld, rb, re, lane);
+  for (uint32_t r = threadIdx.x; r < kRows; r += 256) xs[r] = r < d ? pr.X[r] : 0.0f;
+  float yp[L * V];
+#pragma unroll
+  for (int e = 0; e < L * V; ++e) yp[e] = 0.0f;
+  __syncthreads();
+  for (int par = 0; j < j1; ++j, par ^= 1) {
+    u32x4 cur[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) cur[l] = cr.raw[l];
+    if (j + 1 < j1) cr.load(A + (j + 1) * pr.ld, rb, re, lane);  // streams in meanwhile
+    double t = 0.0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      float v[V];
+      ColRegs<T, L>::unpack(cur[l], v);
+      const float* xr = xs + rb + lane * V + l * ColRegs<T, L>::step;
+#pragma unroll
+      for (int k = 0; k < V; k += 4) {
+        const float4 xv = *reinterpret_cast<const float4*>(xr + k);
+        t += (double)v[k] * (double)xv.x;
+        t += (double)v[k + 1] * (double)xv.y;
+        t += (double)v[k + 2] * (double)xv.z;
+        t += (double)v[k + 3] * (double)xv.w;
+      }
+    }
+    t = wave_sum(t);
+    if (lane == 0) part[par][w] = t;
+    __syncthreads();
+    const float tj = (float)((part[par][0] + part[par][1]) + (part[par][2] + part[par][3]));
+    resident_axpy<T, L>(yp, tj, cur);
+  }
+  resident_store<T, L>(yp, pr.P2 + ch * kBT * pr.p2pitch, rb, q, lane, pr.dstage);
+}
+
+// ---------------------------------------------------------------------------
 // Finish the products (dotProduct_and_generate_x, common.h:89-154, and the
 // proposed rank-2 part, core_proposedAMM.cuh:198-212): y_k = base_k +
 // sum_ch P2[ch][k] with base 0 (exact), y_memo (previous) or y_memo + s v + q
@@ -1095,7 +1175,9 @@ __global__ __launch_bounds__(256 * G) void k_bfinish(const BProb* __restrict__ P
   const uint32_t md = live ? pr.bt[k].mode : kModeExactNoMemo;
   const bool memo_base = is_amm(md);
   const bool prop = md == kModeAmmProposed;
-  const uint32_t active = union_plan(pr, pr.tpre[pr.ntiles]).active;
+  // chunk partials to add: k_bgram's column chunks on its steps, else the
+  // second contraction's active chunks
+  const uint32_t active = r.gram && gram_step(pr) ? pr.gram_chunks : union_plan(pr, pr.tpre[pr.ntiles]).active;
   const uint32_t c0 = (uint32_t)(((uint64_t)active * grp) / G);
   const uint32_t c1 = (uint32_t)(((uint64_t)active * (grp + 1)) / G);
   const float* __restrict__ P2 = pr.P2;
@@ -1378,51 +1460,78 @@ hipError_t launch_bunion(const BProb* P, BGroup g, BRun r, bool fused, hipStream
   return e;
 }
 
-hipError_t launch_bweight(const BProb* P, BGroup g, BRun r, uint32_t proposed_any, uint32_t out_w, hipStream_t s) {
-  if (g.total) k_bweight<<<g.total, 256, 0, s>>>(P, g.off, r, proposed_any, out_w);
+hipError_t launch_bweight(const BProb* P, BGroup g, BRun r, uint32_t proposed_any, uint32_t out_w, hipStream_t s,
+                          bool skip_exact) {
+  if (g.total) k_bweight<<<g.total, 256, 0, s>>>(P, g.off, r, proposed_any, out_w, skip_exact ? 1u : 0u);
   return hipGetLastError();
 }
 
 template <bool NT>
-static void bdot_launch(int dtype, const BProb* P, BGroup g, uint32_t np, hipStream_t s) {
+static void bdot_launch(int dtype, const BProb* P, BGroup g, uint32_t np, uint32_t skip, hipStream_t s) {
   if (dtype == 1)
-    k_bdot<__hip_bfloat16, NT><<<g.total, 256, 0, s>>>(P, g.off, np);
+    k_bdot<__hip_bfloat16, NT><<<g.total, 256, 0, s>>>(P, g.off, np, skip);
   else
-    k_bdot<float, NT><<<g.total, 256, 0, s>>>(P, g.off, np);
+    k_bdot<float, NT><<<g.total, 256, 0, s>>>(P, g.off, np, skip);
 }
 
-hipError_t launch_bdot(int dtype, bool nt, const BProb* P, BGroup g, uint32_t np, hipStream_t s) {
+hipError_t launch_bdot(int dtype, bool nt, const BProb* P, BGroup g, uint32_t np, hipStream_t s,
+                       bool skip_exact) {
   if (!g.total) return hipSuccess;
   if (nt)
-    bdot_launch<true>(dtype, P, g, np, s);
+    bdot_launch<true>(dtype, P, g, np, skip_exact ? 1u : 0u, s);
   else
-    bdot_launch<false>(dtype, P, g, np, s);
+    bdot_launch<false>(dtype, P, g, np, skip_exact ? 1u : 0u, s);
+  return hipGetLastError();
+}
+
+template <typename T>
+static void bgram_launch(int L, const BProb* P, BGroup g, uint32_t np, hipStream_t s) {
+  switch (L) {
+    case 1: k_bgram<T, 1><<<g.total, 256, 0, s>>>(P, g.off, np); break;
+    case 2: k_bgram<T, 2><<<g.total, 256, 0, s>>>(P, g.off, np); break;
+    case 4: k_bgram<T, 4><<<g.total, 256, 0, s>>>(P, g.off, np); break;
+    case 8: k_bgram<T, 8><<<g.total, 256, 0, s>>>(P, g.off, np); break;
+    default:
+      if constexpr (sizeof(T) == 4) k_bgram<T, 16><<<g.total, 256, 0, s>>>(P, g.off, np);  // FP32 only
+      break;
+  }
+}
+
+hipError_t launch_bgram(int dtype, int L, const BProb* P, BGroup g, uint32_t np, hipStream_t s) {
+  if (!g.total) return hipSuccess;
+  if (L != 1 && L != 2 && L != 4 && L != 8 && L != 16) return hipErrorInvalidValue;
+  if (dtype == 1) {
+    if (L > 8) return hipErrorInvalidValue;
+    bgram_launch<__hip_bfloat16>(L, P, g, np, s);
+  } else {
+    bgram_launch<float>(L, P, g, np, s);
+  }
   return hipGetLastError();
 }
 
 template <bool NT>
 static void baxpy_launch(int dtype, uint32_t passes, const BProb* P, BGroup g, uint32_t np,
-                         uint32_t proposed_any, hipStream_t s) {
+                         uint32_t proposed_any, uint32_t skip, hipStream_t s) {
   if (dtype == 1) {
     if (passes == 1)
-      k_baxpy<__hip_bfloat16, 1, NT><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any);
+      k_baxpy<__hip_bfloat16, 1, NT><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any, skip);
     else
-      k_baxpy<__hip_bfloat16, 2, NT><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any);
+      k_baxpy<__hip_bfloat16, 2, NT><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any, skip);
   } else {
     if (passes == 1)
-      k_baxpy<float, 1, NT><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any);
+      k_baxpy<float, 1, NT><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any, skip);
     else
-      k_baxpy<float, 2, NT><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any);
+      k_baxpy<float, 2, NT><<<g.total, 256, 0, s>>>(P, g.off, np, proposed_any, skip);
   }
 }
 
 hipError_t launch_baxpy(int dtype, uint32_t passes, bool nt, const BProb* P, BGroup g, uint32_t np,
-                        uint32_t proposed_any, hipStream_t s) {
+                        uint32_t proposed_any, hipStream_t s, bool skip_exact) {
   if (!g.total) return hipSuccess;
   if (nt)
-    baxpy_launch<true>(dtype, passes, P, g, np, proposed_any, s);
+    baxpy_launch<true>(dtype, passes, P, g, np, proposed_any, skip_exact ? 1u : 0u, s);
   else
-    baxpy_launch<false>(dtype, passes, P, g, np, proposed_any, s);
+    baxpy_launch<false>(dtype, passes, P, g, np, proposed_any, skip_exact ? 1u : 0u, s);
   return hipGetLastError();
 }
 

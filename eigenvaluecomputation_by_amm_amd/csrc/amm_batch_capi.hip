@@ -59,7 +59,11 @@ constexpr uint32_t kFinishSlabs = 4096;      // finish workgroups of 256 threads
 constexpr uint32_t kMaxSlices = 16;      // contraction-1 row slices per problem
 constexpr uint32_t kZChunk = 2048;       // columns per chunk of z = A alpha
 constexpr uint64_t kNtMinBytes = 512ull << 20;
-constexpr uint64_t kUnionFusedBytes = 256u << 10;  // count rows per problem up to which one workgroup forms its union  // matrix bytes beyond which the contractions stream A non-temporally
+constexpr uint64_t kUnionFusedBytes = 256u << 10;
+constexpr uint32_t kGramItems = 1024;  // k_bgram workgroups to aim for (a share per problem by d x n)
+// columns shorter than this keep the two contractions on exact steps: a
+// k_bgram workgroup has one column in flight, too little to stream short ones
+constexpr uint64_t kGramMinColBytes = 8192;  // count rows per problem up to which one workgroup forms its union  // matrix bytes beyond which the contractions stream A non-temporally
 
 uint64_t rup(uint64_t a, uint64_t b) { return (a + b - 1) / b * b; }
 uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
@@ -128,6 +132,9 @@ struct amm_batch {
   // mix of small matrices, whose second contraction finds columns in cache)
   bool nt_loads = false;
   bool union_fused = false;  // the union pass as one workgroup per problem (k_bunion)
+  bool gram = false;         // one-trial batch: exact steps in one read of A (k_bgram)
+  int gram_Ls[2] = {0, 0};   // k_bgram's loads per lane for the FP32 / BF16 group
+  amm::BGroup g_gram[2]{};
   // grouped setup: descriptors (device) and their launch groups
   amm::BSetup* dS = nullptr;
   std::vector<uint32_t> tiles;  // CDF tiles per problem
@@ -212,6 +219,29 @@ int upload_plan(amm_batch* b, int kernel) {
   BHIPCHK(hipStreamSynchronize(b->stream));
   Pool pl;
   std::vector<size_t> o1(np), o2(np), o2s(np);
+  // A one-trial batch takes the exact steps of its problems with columns
+  // of kGramMinColBytes or more that fit k_bgram's registers in one read of
+  // A: each such problem gets a share of kGramItems workgroups by its d x n,
+  // each a contiguous run of columns whose partial k_bfinish adds up (like
+  // the second contraction's chunks).  The other problems keep the two
+  // contractions on every step.
+  auto gram_ok = [&](const BProb& p) {
+    return b->ntr == 1 && amm::gram_L(p.dtype, p.d) != 0 &&
+           (uint64_t)p.d * (p.dtype == 1 ? 2u : 4u) >= kGramMinColBytes;
+  };
+  b->gram = false;
+  b->gram_Ls[0] = b->gram_Ls[1] = 0;
+  double gram_tot = 0.0;
+  for (const BProb& p : b->hp) {
+    if (!gram_ok(p)) continue;
+    b->gram = true;
+    b->gram_Ls[p.dtype == 1 ? 1 : 0] = std::max(b->gram_Ls[p.dtype == 1 ? 1 : 0], amm::gram_L(p.dtype, p.d));
+    gram_tot += (double)p.n * (double)p.d;
+  }
+  for (BProb& p : b->hp) {
+    const double want = std::round(kGramItems * (double)p.n * (double)p.d / std::max(gram_tot, 1.0));
+    p.gram_chunks = gram_ok(p) ? (uint32_t)std::min<double>((double)p.n, std::max(1.0, want)) : 0u;
+  }
   for (uint32_t i = 0; i < np; ++i) {
     BProb& p = b->hp[i];
     const Tuning& t = b->tune[i];
@@ -222,7 +252,7 @@ int upload_plan(amm_batch* b, int kernel) {
     p.p2pitch = p.rblocks * block_rows(t.passes);
     p.chunks = t.chunks;
     o1[i] = pl.take<float>((size_t)p.kslices_max * p.ntiles * amm::kBTile * amm::kBT);
-    o2[i] = pl.take<float>((size_t)p.chunks * amm::kBT * p.p2pitch);
+    o2[i] = pl.take<float>((size_t)std::max(p.chunks, p.gram_chunks) * amm::kBT * p.p2pitch);
     o2s[i] = pl.take<double>((size_t)p.chunks * amm::kBT);
   }
   if (pl.bytes > b->tpool_bytes) {
@@ -282,6 +312,9 @@ int upload_plan(amm_batch* b, int kernel) {
     const amm::UPlan up = amm::union_plan(p, p.n);
     return (uint64_t)amm::kBTile * up.kslice_rows * (p.dtype == 1 ? 2u : 4u);
   });
+  const std::vector<uint32_t> gram_order = largest_first([](const BProb& p) {
+    return p.gram_chunks ? (uint64_t)p.n * p.d * (p.dtype == 1 ? 2u : 4u) / p.gram_chunks : 0u;
+  });
   const std::vector<uint32_t> axpy_order = largest_first([](const BProb& p) {
     const amm::UPlan up = amm::union_plan(p, p.n);
     return (uint64_t)std::min(up.cpc, p.n) * block_rows(p.passes) * (p.dtype == 1 ? 2u : 4u);
@@ -314,7 +347,7 @@ int upload_plan(amm_batch* b, int kernel) {
   uint32_t max_chunks = 0;
   for (const BProb& p : b->hp) {
     tot_vblk += p.nvblk;
-    max_chunks = std::max(max_chunks, p.chunks);
+    max_chunks = std::max(max_chunks, std::max(p.chunks, p.gram_chunks));
   }
   b->fin_groups = max_chunks > 8 ? 4u : 1u;
   const uint64_t f_fin = std::max<uint64_t>(1, cdiv(tot_vblk, kFinishSlabs / b->fin_groups));
@@ -336,6 +369,9 @@ int upload_plan(amm_batch* b, int kernel) {
             return (p.dtype == dt && p.passes == (uint32_t)pa + 1) ? p.rblocks * p.chunks : 0u;
           },
           &axpy_order);
+  for (int dt = 0; dt < 2; ++dt)
+    ok &= add_group(
+        b->g_gram[dt], [dt](const BProb& p) { return p.dtype == dt ? p.gram_chunks : 0u; }, &gram_order);
   if (!ok) return fail(AMM_E_INVALID, "a grouped launch exceeds 2^31 workgroups");
   if (all.size() > b->offs_cap) {
     if (b->offs) (void)hipFree(b->offs);
@@ -431,6 +467,7 @@ amm::BRun run_args(amm_batch* b, const amm_params& p) {
   r.modes_static = b->modes_static;
   r.ts = b->ts;
   r.t_start = b->t_start;
+  r.gram = b->gram ? 1u : 0u;
   return r;
 }
 
@@ -492,11 +529,15 @@ int enq_iteration(amm_batch* b, const amm::BRun& r, int K, uint32_t it) {
   if (K != AMM_KERNEL_EXACT) BHIPCHK(amm::launch_bsample(P, b->g_sample, r, it, s));
   BHIPCHK(amm::launch_bunion(P, b->g_prep, r, b->union_fused, s));
   const uint32_t prop = (K == AMM_KERNEL_PROPOSED || K == AMM_KERNEL_PROPOSED_V2) ? 1u : 0u;
-  for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bdot(dt, b->nt_loads, P, b->g_dot[dt], b->np, s));
-  BHIPCHK(amm::launch_bweight(P, b->g_weight, r, prop, 0u, s));
+  // a one-trial batch's exact step is k_bgram's (each kernel checks the
+  // step kind on the device: the contractions leave such problems alone)
+  if (b->gram)
+    for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bgram(dt, b->gram_Ls[dt], P, b->g_gram[dt], b->np, s));
+  for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bdot(dt, b->nt_loads, P, b->g_dot[dt], b->np, s, b->gram));
+  BHIPCHK(amm::launch_bweight(P, b->g_weight, r, prop, 0u, s, b->gram));
   for (int dt = 0; dt < 2; ++dt)
     for (int pa = 0; pa < 2; ++pa)
-      BHIPCHK(amm::launch_baxpy(dt, (uint32_t)pa + 1, b->nt_loads, P, b->g_axpy[dt][pa], b->np, prop, s));
+      BHIPCHK(amm::launch_baxpy(dt, (uint32_t)pa + 1, b->nt_loads, P, b->g_axpy[dt][pa], b->np, prop, s, b->gram));
   BHIPCHK(amm::launch_bfinish(P, b->g_fin, r, b->fin_groups, s));
   BHIPCHK(amm::launch_btail(P, r, amm::kBTailFinish, it, s));
   BHIPCHK(amm::launch_bnorm(P, b->g_vec, r, s));

@@ -267,6 +267,43 @@ def test_batch_edge_shapes(shapes, trials, c, bf16):
             assert relmax(r.eig[i, k], e) < 1e-4, (i, k, r.eig[i, k], e)
 
 
+@pytest.mark.parametrize("kernel,extra", [
+    ("exact", {}),
+    ("previousAMM", {"vr_period": 3}),
+    ("proposedAMM", {"vr_period": 4}),
+    ("proposedAMM_v2", {"vr_period": 5}),
+    ("previousAMM", {"vr_adaptive_threshold": 1e-3}),
+])
+def test_one_trial_exact_steps_in_one_read(kernel, extra):
+    """A one-trial batch takes its exact steps in one read of A (k_bgram):
+    uneven FP32 and BF16 problems, d from 40 to 9000 rows (one to eight
+    16-byte loads per lane of a register-held column), each trajectory
+    against the single-matrix run with the same seed and trial word (step
+    kinds identical, Rayleigh quotients within 1e-5, BF16 1e-4)."""
+    banner(f"one-trial batch, exact steps in one read: {kernel} {extra}")
+    AmmBatch, _, RunParams, _ = _api()
+    shapes = [(40, 700, False), (1000, 3000, True), (2050, 1500, False), (5000, 900, True), (9000, 600, True)]
+    probs, mats = [], []
+    for i, (d, n, bf) in enumerate(shapes):
+        A = lowrank_plus_noise(d, n, seed=500 + i, noise=0.05) + 0.05
+        ld = (d + 7) // 8 * 8
+        t = dev(A, bf, ld)
+        mats.append((t, d, bf))
+        probs.append(dict(A=t, d=d, c=max(1, n // 6), seed=700 + i))
+    params = RunParams(kernel=kernel, step=14, trial=3, **extra)
+    with AmmBatch(probs, trials=1) as b:
+        r = b.power_method(params)
+        modes = b.last_modes(params.step)
+    if "vr_adaptive_threshold" not in extra:
+        assert (modes == 1).any() or (modes == 4).any()  # exact steps happened
+    for i, (t, d, bf) in enumerate(mats):
+        p = RunParams(kernel=kernel, c=probs[i]["c"] if kernel != "exact" else 0, step=14, trial=3,
+                      seed=probs[i]["seed"], **extra)
+        e, _, m = _single(t, d, p, 3)
+        assert np.array_equal(modes[i, 0], m), (i, modes[i, 0], m)
+        assert relmax(r.eig[i, 0], e) < (1e-4 if bf else 1e-5), (i, relmax(r.eig[i, 0], e))
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_batch_random_mix(seed):
     """Seeded random batches: 2-5 problems of random ragged shapes, FP32 and
