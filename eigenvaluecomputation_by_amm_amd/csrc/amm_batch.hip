@@ -38,12 +38,9 @@ __device__ __forceinline__ bool is_amm(uint32_t md) {
   return md == kModeAmmPrevious || md == kModeAmmProposed;
 }
 
-// the product of this iteration is k_bgram's (a one-trial batch whose trial
-// takes an exact step)
-__device__ __forceinline__ bool gram_step(const BProb& pr) {
-  const uint32_t md = pr.bt[0].mode;
-  return pr.gram_chunks != 0 && (md == kModeExactMemo || md == kModeExactNoMemo);
-}
+// the products of this problem are k_bgram's (a one-trial batch, columns
+// that k_bgram streams well)
+__device__ __forceinline__ bool gram_step(const BProb& pr) { return pr.gram_chunks != 0; }
 
 // ---------------------------------------------------------------------------
 // Run prologue: x_memo = y_memo = 0 (zero_clear, device_memory.h:345-355),
@@ -681,7 +678,7 @@ struct TileStage {
 // ---------------------------------------------------------------------------
 template <typename T, bool NT>
 __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const uint32_t* off,
-                                              uint32_t np, uint32_t skip_exact) {
+                                              uint32_t np, uint32_t skip_gram) {
   constexpr bool BF = Elem<T>::VEC == 8;
   constexpr uint32_t RB = BF ? 256 : 128;
   using Stage = TileStage<T, 64, RB, NT>;
@@ -689,7 +686,7 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
   __shared__ uint32_t cols[64];
   const uint32_t p = find_problem(off, np, blockIdx.x);
   const BProb& pr = P[p];
-  if (skip_exact && gram_step(pr)) return;
+  if (skip_gram && gram_step(pr)) return;
   const uint32_t local = blockIdx.x - off[p];
   // the grid covers the largest vtiles x kslices any union size can need
   // (< dot_blocks + ntiles); item = slice-major over this product's
@@ -811,11 +808,11 @@ __global__ __launch_bounds__(256) void k_bdot(const BProb* __restrict__ P, const
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bweight(const BProb* __restrict__ P, const uint32_t* off,
                                                  BRun r, uint32_t proposed_any, uint32_t out_w,
-                                                 uint32_t skip_exact) {
+                                                 uint32_t skip_gram) {
   __shared__ double s_sk[kBT];
   const uint32_t p = find_problem(off, r.np, blockIdx.x);
   const BProb& pr = P[p];
-  if (skip_exact && gram_step(pr)) return;
+  if (skip_gram && gram_step(pr)) return;
   const uint32_t local = blockIdx.x - off[p];
   const uint32_t nblk = off[p + 1] - off[p];
   const UPlan up = union_plan(pr, pr.tpre[pr.ntiles]);
@@ -932,7 +929,7 @@ __global__ __launch_bounds__(256) void k_bweight(const BProb* __restrict__ P, co
 // ---------------------------------------------------------------------------
 template <typename T, uint32_t PASSES, bool NT>
 __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, const uint32_t* off,
-                                               uint32_t np, uint32_t proposed_any, uint32_t skip_exact) {
+                                               uint32_t np, uint32_t proposed_any, uint32_t skip_gram) {
   constexpr bool BF = Elem<T>::VEC == 8;
   constexpr uint32_t C = BF ? 32 : 16;
   constexpr uint32_t RB = 256 * PASSES;
@@ -951,7 +948,7 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
   __shared__ double cred[256];
   const uint32_t p = find_problem(off, np, blockIdx.x);
   const BProb& pr = P[p];
-  if (skip_exact && gram_step(pr)) return;
+  if (skip_gram && gram_step(pr)) return;
   const uint32_t local = blockIdx.x - off[p];
   // chunk-major order: active chunks form one contiguous range of blocks
   const uint32_t ch = local / pr.rblocks, rb = local % pr.rblocks;
@@ -1082,18 +1079,27 @@ __global__ __launch_bounds__(256) void k_baxpy(const BProb* __restrict__ P, cons
 }
 
 // ---------------------------------------------------------------------------
-// The exact step of a one-trial batch in one read of A (the two cublasSgemv
-// calls of powerMethod_by_exact.cuh:46-62, in k_gram's column-resident
-// form): workgroup (problem p, chunk ch) walks the contiguous columns
-// [n ch / G, n (ch + 1) / G) of its problem (G = gram_chunks), holds each
-// column in registers, forms c_j = a_j^T x (FP64 sum of exact FP32
-// products, rounded to FP32) with x = trial 0's iterate in LDS, and adds
-// c_j a_j into a register partial of its rows; the partial goes to
-// P2[ch][trial 0] for k_bfinish.  The next column streams in while the
-// workgroup reduces the current one.  Exits unless trial 0 takes an exact
-// step.
+// The product of a one-trial batch in one read of A, for every step kind.
+// It follows k_gram's column-resident form; the exact step is the two
+// cublasSgemv calls of powerMethod_by_exact.cuh:46-62, and a sampled step is
+// previousAMM_kernel1 / proposedAMM_kernel1 with its weights
+// (core_previousAMM.cuh:4-175, core_proposedAMM.cuh:17-213, common.h:42-53).
+// Workgroup (problem p, chunk ch) walks the union slots
+// [U ch / G, U (ch + 1) / G) of its problem (G = gram_chunks; every column
+// on an exact step).  It holds each column in registers and forms
+// c_j = a_j^T dx (FP64 sum of exact FP32 products) with dx in LDS:
+// x - x_memo on a sampled step, x on an exact one.  The weight comes from
+// the draw count as k_bweight forms it: t_j = 1 * c_j on an exact step,
+// t_j = w_j (c_j - alpha_j s) on a sampled one (s = 0 unless proposed), and
+// t_j a_j goes into a register partial of the workgroup's rows.  The partial
+// goes to P2[ch][trial 0] and, proposed, sum alpha_j t_j to P2s[ch][0], both
+// for k_bfinish.  The next column (and its weight's operands) streams in
+// while the workgroup reduces the current one.  ANY = false: the host knows
+// the step is exact (static schedule), and the instance without the weight
+// path keeps fewer registers (7 against 5 workgroups per CU at 4096 BF16
+// rows).
 // ---------------------------------------------------------------------------
-template <typename T, int L>
+template <typename T, int L, bool ANY>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_bgram(
     const BProb* __restrict__ P, const uint32_t* off, uint32_t np) {
   constexpr int V = Elem<T>::VEC;
@@ -1103,29 +1109,67 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const uint32_t p = find_problem(off, np, blockIdx.x);
   const BProb& pr = P[p];
   if (!gram_step(pr)) return;
+  const uint32_t md = pr.bt[0].mode;
+  const bool exact = !ANY || md == kModeExactMemo || md == kModeExactNoMemo;
+  const bool prop = ANY && md == kModeAmmProposed;
   const uint64_t ch = blockIdx.x - off[p];
-  const uint64_t G = pr.gram_chunks, n = pr.n;
-  const uint64_t j1 = n * (ch + 1) / G;
+  const uint64_t G = pr.gram_chunks;
+  const uint64_t U = exact ? pr.n : pr.tpre[pr.ntiles];
+  const uint64_t s1 = U * (ch + 1) / G;
   const uint32_t d = pr.d;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint32_t q = resident_rows<V>(d);  // <= L * 64 * V (host: gram_L)
   const uint32_t rb = w * q;
   const uint32_t re = rb + q < d ? rb + q : d;
   const T* A = reinterpret_cast<const T*>(pr.A);
+  const uint32_t* __restrict__ colc = pr.colc;
+  // the weight's operands of column j: draw count, q_j of the step's CDF, alpha_j
+  const uint64_t* __restrict__ cdf = prop ? pr.cdf1 : pr.cdf0;
+  const double tot = exact ? 0.0 : pr.st->total[prop ? 1 : 0];
+  const double sk = prop ? pr.bt[0].s : 0.0;
+  struct Wops {
+    uint32_t mult;
+    uint64_t qj;
+    float aj;
+  };
+  auto wops = [&](uint32_t j) {
+    Wops o{1u, 1ull, 0.0f};
+    if (!exact) {
+      o.mult = pr.cnt[j];
+      o.qj = cdf[j] - (j ? cdf[j - 1] : 0ull);
+      if (prop) o.aj = pr.alpha[j];
+    }
+    return o;
+  };
   ColRegs<T, L> cr;
-  uint64_t j = n * ch / G;
-  if (j < j1) cr.load(A + j * pr.ld, rb, re, lane);
-  for (uint32_t r = threadIdx.x; r < kRows; r += 256) xs[r] = r < d ? pr.X[r] : 0.0f;
+  uint64_t sl = U * ch / G;
+  uint32_t j = 0;
+  Wops wo{};
+  if (sl < s1) {
+    j = exact ? (uint32_t)sl : colc[sl];
+    cr.load(A + (uint64_t)j * pr.ld, rb, re, lane);
+    wo = wops(j);
+  }
+  uint32_t jn = (sl + 1 < s1) ? (exact ? (uint32_t)(sl + 1) : colc[sl + 1]) : 0u;
+  const bool memo = ANY && is_amm(md);
+  for (uint32_t r = threadIdx.x; r < kRows; r += 256)
+    xs[r] = r < d ? pr.X[r] - (memo ? pr.XM[r] : 0.0f) : 0.0f;  // dx as k_bnorm stages it
   float yp[L * V];
 #pragma unroll
   for (int e = 0; e < L * V; ++e) yp[e] = 0.0f;
+  double corr = 0.0;  // proposed: sum alpha_j t_j (thread 0)
   __syncthreads();
-  for (int par = 0; j < j1; ++j, par ^= 1) {
+  for (int par = 0; sl < s1; ++sl, par ^= 1) {
     u32x4 cur[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) cur[l] = cr.raw[l];
-    if (j + 1 < j1) cr.load(A + (j + 1) * pr.ld, rb, re, lane);  // streams in meanwhile
-    double t = 0.0;
+    const Wops wc = wo;
+    if (sl + 1 < s1) {  // the next column and its weight's operands stream in meanwhile
+      cr.load(A + (uint64_t)jn * pr.ld, rb, re, lane);
+      wo = wops(jn);
+      jn = (sl + 2 < s1) ? (exact ? (uint32_t)(sl + 2) : colc[sl + 2]) : 0u;
+    }
+    double c = 0.0;
 #pragma unroll
     for (int l = 0; l < L; ++l) {
       float v[V];
@@ -1134,19 +1178,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
       for (int k = 0; k < V; k += 4) {
         const float4 xv = *reinterpret_cast<const float4*>(xr + k);
-        t += (double)v[k] * (double)xv.x;
-        t += (double)v[k + 1] * (double)xv.y;
-        t += (double)v[k + 2] * (double)xv.z;
-        t += (double)v[k + 3] * (double)xv.w;
+        c += (double)v[k] * (double)xv.x;
+        c += (double)v[k + 1] * (double)xv.y;
+        c += (double)v[k + 2] * (double)xv.z;
+        c += (double)v[k + 3] * (double)xv.w;
       }
     }
-    t = wave_sum(t);
-    if (lane == 0) part[par][w] = t;
+    c = wave_sum(c);
+    if (lane == 0) part[par][w] = c;
     __syncthreads();
-    const float tj = (float)((part[par][0] + part[par][1]) + (part[par][2] + part[par][3]));
+    const double cj = (part[par][0] + part[par][1]) + (part[par][2] + part[par][3]);
+    float tj;
+    if (exact) {
+      tj = (float)cj;
+    } else {
+      // weight mult total / (c q_j) as k_bweight forms it, then t = w (c_j - alpha_j s)
+      const double den = (double)pr.c * (double)wc.qj;
+      const float wj = wc.mult && den > 0.0 ? (float)((double)wc.mult * tot / den) : 0.0f;
+      const double ajs = sk != 0.0 ? (double)wc.aj : 0.0;
+      tj = (float)((double)wj * (cj - ajs * sk));
+      if (prop) corr += (double)wc.aj * (double)tj;
+    }
     resident_axpy<T, L>(yp, tj, cur);
   }
   resident_store<T, L>(yp, pr.P2 + ch * kBT * pr.p2pitch, rb, q, lane, pr.dstage);
+  if (prop && threadIdx.x == 0) pr.P2s[ch * kBT] = corr;
 }
 
 // ---------------------------------------------------------------------------
@@ -1461,8 +1517,8 @@ hipError_t launch_bunion(const BProb* P, BGroup g, BRun r, bool fused, hipStream
 }
 
 hipError_t launch_bweight(const BProb* P, BGroup g, BRun r, uint32_t proposed_any, uint32_t out_w, hipStream_t s,
-                          bool skip_exact) {
-  if (g.total) k_bweight<<<g.total, 256, 0, s>>>(P, g.off, r, proposed_any, out_w, skip_exact ? 1u : 0u);
+                          bool skip_gram) {
+  if (g.total) k_bweight<<<g.total, 256, 0, s>>>(P, g.off, r, proposed_any, out_w, skip_gram ? 1u : 0u);
   return hipGetLastError();
 }
 
@@ -1475,36 +1531,34 @@ static void bdot_launch(int dtype, const BProb* P, BGroup g, uint32_t np, uint32
 }
 
 hipError_t launch_bdot(int dtype, bool nt, const BProb* P, BGroup g, uint32_t np, hipStream_t s,
-                       bool skip_exact) {
+                       bool skip_gram) {
   if (!g.total) return hipSuccess;
   if (nt)
-    bdot_launch<true>(dtype, P, g, np, skip_exact ? 1u : 0u, s);
+    bdot_launch<true>(dtype, P, g, np, skip_gram ? 1u : 0u, s);
   else
-    bdot_launch<false>(dtype, P, g, np, skip_exact ? 1u : 0u, s);
+    bdot_launch<false>(dtype, P, g, np, skip_gram ? 1u : 0u, s);
   return hipGetLastError();
 }
 
-template <typename T>
+template <typename T, bool ANY>
 static void bgram_launch(int L, const BProb* P, BGroup g, uint32_t np, hipStream_t s) {
   switch (L) {
-    case 1: k_bgram<T, 1><<<g.total, 256, 0, s>>>(P, g.off, np); break;
-    case 2: k_bgram<T, 2><<<g.total, 256, 0, s>>>(P, g.off, np); break;
-    case 4: k_bgram<T, 4><<<g.total, 256, 0, s>>>(P, g.off, np); break;
-    case 8: k_bgram<T, 8><<<g.total, 256, 0, s>>>(P, g.off, np); break;
-    default:
-      if constexpr (sizeof(T) == 4) k_bgram<T, 16><<<g.total, 256, 0, s>>>(P, g.off, np);  // FP32 only
-      break;
+    case 1: k_bgram<T, 1, ANY><<<g.total, 256, 0, s>>>(P, g.off, np); break;
+    case 2: k_bgram<T, 2, ANY><<<g.total, 256, 0, s>>>(P, g.off, np); break;
+    case 4: k_bgram<T, 4, ANY><<<g.total, 256, 0, s>>>(P, g.off, np); break;
+    default: k_bgram<T, 8, ANY><<<g.total, 256, 0, s>>>(P, g.off, np); break;
   }
 }
 
-hipError_t launch_bgram(int dtype, int L, const BProb* P, BGroup g, uint32_t np, hipStream_t s) {
+hipError_t launch_bgram(int dtype, int L, bool exact_step, const BProb* P, BGroup g, uint32_t np, hipStream_t s) {
   if (!g.total) return hipSuccess;
-  if (L != 1 && L != 2 && L != 4 && L != 8 && L != 16) return hipErrorInvalidValue;
+  if (L != 1 && L != 2 && L != 4 && L != 8) return hipErrorInvalidValue;
   if (dtype == 1) {
-    if (L > 8) return hipErrorInvalidValue;
-    bgram_launch<__hip_bfloat16>(L, P, g, np, s);
+    if (exact_step) bgram_launch<__hip_bfloat16, false>(L, P, g, np, s);
+    else bgram_launch<__hip_bfloat16, true>(L, P, g, np, s);
   } else {
-    bgram_launch<float>(L, P, g, np, s);
+    if (exact_step) bgram_launch<float, false>(L, P, g, np, s);
+    else bgram_launch<float, true>(L, P, g, np, s);
   }
   return hipGetLastError();
 }
@@ -1526,12 +1580,12 @@ static void baxpy_launch(int dtype, uint32_t passes, const BProb* P, BGroup g, u
 }
 
 hipError_t launch_baxpy(int dtype, uint32_t passes, bool nt, const BProb* P, BGroup g, uint32_t np,
-                        uint32_t proposed_any, hipStream_t s, bool skip_exact) {
+                        uint32_t proposed_any, hipStream_t s, bool skip_gram) {
   if (!g.total) return hipSuccess;
   if (nt)
-    baxpy_launch<true>(dtype, passes, P, g, np, proposed_any, skip_exact ? 1u : 0u, s);
+    baxpy_launch<true>(dtype, passes, P, g, np, proposed_any, skip_gram ? 1u : 0u, s);
   else
-    baxpy_launch<false>(dtype, passes, P, g, np, proposed_any, skip_exact ? 1u : 0u, s);
+    baxpy_launch<false>(dtype, passes, P, g, np, proposed_any, skip_gram ? 1u : 0u, s);
   return hipGetLastError();
 }
 

@@ -55,7 +55,7 @@ struct alignas(16) BProb {
   uint32_t chunks;      // contraction-2 column chunks of the union (at most)
   uint32_t p2pitch;     // rows of one P2 partial (rblocks * rows per block)
   uint32_t nvblk;       // vector-kernel blocks = dstage / 64
-  uint32_t gram_chunks; // one-read exact step (k_bgram): column chunks, 0 = off (one-trial batches)
+  uint32_t gram_chunks; // one-read products (k_bgram): chunks of union slots, 0 = off (one-trial batches)
   float* X;             // [16][dstage] iterate
   float* Y;             // [16][dstage] product
   float* XM;            // [16][dstage] x_memo
@@ -101,7 +101,7 @@ __host__ __device__ inline uint64_t dx_offset(uint32_t k, uint32_t row) {
 // Scalars shared by every launch of a run.
 struct BRun {
   uint32_t np;          // problems
-  uint32_t gram;        // a run whose exact steps go through k_bgram (k_bfinish sums its partials)
+  uint32_t gram;        // a run whose products may go through k_bgram (k_bfinish sums its partials)
   uint32_t ntr;         // trials per problem (<= 16)
   uint32_t trial0;      // Philox counter word 2 of trial 0
   uint32_t kernel;      // AMM_KERNEL_* (1 previous / 2 proposed drive adaptive steps)
@@ -197,28 +197,31 @@ hipError_t launch_bcompact(const BProb* P, BGroup g, uint32_t np, hipStream_t s)
 hipError_t launch_bunion(const BProb* P, BGroup g, BRun r, bool fused, hipStream_t s);
 // items: any count per problem (the blocks stride over the union slots);
 // t of every union slot, or (out_w) the bare weights
-// skip_exact (runs of a batch with k_bgram): problems whose trial 0 takes an
-// exact step are left to k_bgram by the weight pass and both contractions
+// skip_gram (runs of a batch with k_bgram): the problems k_bgram takes are
+// left alone by the weight pass and both contractions
 hipError_t launch_bweight(const BProb* P, BGroup g, BRun r, uint32_t proposed_any, uint32_t out_w, hipStream_t s,
-                          bool skip_exact = false);
+                          bool skip_gram = false);
 // dtype-uniform groups: all problems of a group share the element type
 // nt: A read with non-temporal loads (streams that no later pass reuses)
 hipError_t launch_bdot(int dtype, bool nt, const BProb* P, BGroup g, uint32_t np, hipStream_t s,
-                       bool skip_exact = false);
+                       bool skip_gram = false);
 // passes: 1 or 2 (a group shares it)
 hipError_t launch_baxpy(int dtype, uint32_t passes, bool nt, const BProb* P, BGroup g, uint32_t np,
-                        uint32_t proposed_any, hipStream_t s, bool skip_exact = false);
-// The exact step of a one-trial batch in one read of A (k_bgram): items
+                        uint32_t proposed_any, hipStream_t s, bool skip_gram = false);
+// The products of a one-trial batch in one read of A (k_bgram): items
 // gram_chunks per problem of element type dtype; L: 16-byte loads per lane
-// that hold the group's longest column (1, 2, 4, 8 or 16)
-hipError_t launch_bgram(int dtype, int L, const BProb* P, BGroup g, uint32_t np, hipStream_t s);
-// 16-byte loads per lane of a register-held column of d rows (0: beyond 16)
+// that hold the group's longest column (1, 2, 4 or 8); exact_step: every
+// trial takes an exact step this iteration (known on the host), else the
+// step kind is read on the device
+hipError_t launch_bgram(int dtype, int L, bool exact_step, const BProb* P, BGroup g, uint32_t np, hipStream_t s);
+// 16-byte loads per lane of k_bgram's register-held column of d rows: 0
+// beyond 8 (16384 BF16 / 8192 FP32 rows), where its registers would spill
 __host__ __device__ inline int gram_L(int dtype, uint32_t d) {
   const uint32_t step = 64u * (dtype == 1 ? 8u : 4u);
   const uint32_t need = ((d + 3) / 4 + step - 1) / step;
   int L = 1;
   while ((uint32_t)L < need) L *= 2;
-  return L <= (dtype == 1 ? 8 : 16) ? L : 0;
+  return L <= 8 ? L : 0;
 }
 // groups: 1 (256 threads per workgroup) or 4 (1024: four ranges of chunks)
 hipError_t launch_bfinish(const BProb* P, BGroup g, BRun r, uint32_t groups, hipStream_t s);

@@ -61,8 +61,8 @@ constexpr uint32_t kZChunk = 2048;       // columns per chunk of z = A alpha
 constexpr uint64_t kNtMinBytes = 512ull << 20;
 constexpr uint64_t kUnionFusedBytes = 256u << 10;
 constexpr uint32_t kGramItems = 1024;  // k_bgram workgroups to aim for (a share per problem by d x n)
-// columns shorter than this keep the two contractions on exact steps: a
-// k_bgram workgroup has one column in flight, too little to stream short ones
+// columns shorter than this keep the two contractions: a k_bgram workgroup
+// has one column in flight, too little to stream short ones
 constexpr uint64_t kGramMinColBytes = 8192;  // count rows per problem up to which one workgroup forms its union  // matrix bytes beyond which the contractions stream A non-temporally
 
 uint64_t rup(uint64_t a, uint64_t b) { return (a + b - 1) / b * b; }
@@ -219,12 +219,12 @@ int upload_plan(amm_batch* b, int kernel) {
   BHIPCHK(hipStreamSynchronize(b->stream));
   Pool pl;
   std::vector<size_t> o1(np), o2(np), o2s(np);
-  // A one-trial batch takes the exact steps of its problems with columns
-  // of kGramMinColBytes or more that fit k_bgram's registers in one read of
-  // A: each such problem gets a share of kGramItems workgroups by its d x n,
-  // each a contiguous run of columns whose partial k_bfinish adds up (like
-  // the second contraction's chunks).  The other problems keep the two
-  // contractions on every step.
+  // A one-trial batch takes every product of its problems with columns of
+  // kGramMinColBytes or more that fit k_bgram's registers in one read of A:
+  // each such problem gets a share of kGramItems workgroups by its d x n,
+  // each a contiguous run of union slots whose partial k_bfinish adds up
+  // (like the second contraction's chunks).  The other problems keep the two
+  // contractions.
   auto gram_ok = [&](const BProb& p) {
     return b->ntr == 1 && amm::gram_L(p.dtype, p.d) != 0 &&
            (uint64_t)p.d * (p.dtype == 1 ? 2u : 4u) >= kGramMinColBytes;
@@ -253,7 +253,7 @@ int upload_plan(amm_batch* b, int kernel) {
     p.chunks = t.chunks;
     o1[i] = pl.take<float>((size_t)p.kslices_max * p.ntiles * amm::kBTile * amm::kBT);
     o2[i] = pl.take<float>((size_t)std::max(p.chunks, p.gram_chunks) * amm::kBT * p.p2pitch);
-    o2s[i] = pl.take<double>((size_t)p.chunks * amm::kBT);
+    o2s[i] = pl.take<double>((size_t)std::max(p.chunks, p.gram_chunks) * amm::kBT);
   }
   if (pl.bytes > b->tpool_bytes) {
     if (b->tpool) (void)hipFree(b->tpool);
@@ -523,16 +523,19 @@ int enq_setup(amm_batch* b, int K) {
 }
 
 // the iterations' kernels (after x0)
-int enq_iteration(amm_batch* b, const amm::BRun& r, int K, uint32_t it) {
+// kind: the iteration's step kind where the schedule is static, else 0
+int enq_iteration(amm_batch* b, const amm::BRun& r, int K, uint32_t it, uint32_t kind) {
   const BProb* P = b->dP;
   hipStream_t s = b->stream;
   if (K != AMM_KERNEL_EXACT) BHIPCHK(amm::launch_bsample(P, b->g_sample, r, it, s));
   BHIPCHK(amm::launch_bunion(P, b->g_prep, r, b->union_fused, s));
   const uint32_t prop = (K == AMM_KERNEL_PROPOSED || K == AMM_KERNEL_PROPOSED_V2) ? 1u : 0u;
-  // a one-trial batch's exact step is k_bgram's (each kernel checks the
-  // step kind on the device: the contractions leave such problems alone)
+  // a one-trial batch's products are k_bgram's where its columns allow (the
+  // contractions and the weight pass leave those problems alone)
   if (b->gram)
-    for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bgram(dt, b->gram_Ls[dt], P, b->g_gram[dt], b->np, s));
+    for (int dt = 0; dt < 2; ++dt)
+      BHIPCHK(amm::launch_bgram(dt, b->gram_Ls[dt], kind == amm::kModeExactMemo || kind == amm::kModeExactNoMemo, P,
+                                b->g_gram[dt], b->np, s));
   for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bdot(dt, b->nt_loads, P, b->g_dot[dt], b->np, s, b->gram));
   BHIPCHK(amm::launch_bweight(P, b->g_weight, r, prop, 0u, s, b->gram));
   for (int dt = 0; dt < 2; ++dt)
@@ -554,8 +557,9 @@ int enqueue_run(amm_batch* b, const amm_params& p, bool setup) {
   // x0 and its block sums came with the prologue (the setup leaves Y and BP alone)
   BHIPCHK(amm::launch_btail(b->dP, r, amm::kBTailScale, 0, b->stream));
   BHIPCHK(amm::launch_bnorm(b->dP, b->g_vec, r, b->stream));
+  const std::vector<uint32_t> kinds = is_adaptive(p) ? std::vector<uint32_t>(p.step, 0u) : static_modes(p);
   for (uint32_t it = 0; it < (uint32_t)p.step; ++it)
-    if ((rc = enq_iteration(b, r, K, it))) return rc;
+    if ((rc = enq_iteration(b, r, K, it, kinds[it]))) return rc;
   return AMM_OK;
 }
 

@@ -274,13 +274,15 @@ def test_batch_edge_shapes(shapes, trials, c, bf16):
     ("proposedAMM_v2", {"vr_period": 5}),
     ("previousAMM", {"vr_adaptive_threshold": 1e-3}),
 ])
-def test_one_trial_exact_steps_in_one_read(kernel, extra):
-    """A one-trial batch takes its exact steps in one read of A (k_bgram):
-    uneven FP32 and BF16 problems, d from 40 to 9000 rows (one to eight
+def test_one_trial_products_in_one_read(kernel, extra):
+    """A one-trial batch takes the products of its problems with columns of
+    8 KB or more in one read of A (k_bgram: exact and sampled steps, the
+    weights formed per column), the others through the two contractions:
+    uneven FP32 and BF16 problems, d from 40 to 9000 rows (up to eight
     16-byte loads per lane of a register-held column), each trajectory
     against the single-matrix run with the same seed and trial word (step
     kinds identical, Rayleigh quotients within 1e-5, BF16 1e-4)."""
-    banner(f"one-trial batch, exact steps in one read: {kernel} {extra}")
+    banner(f"one-trial batch, products in one read: {kernel} {extra}")
     AmmBatch, _, RunParams, _ = _api()
     shapes = [(40, 700, False), (1000, 3000, True), (2050, 1500, False), (5000, 900, True), (9000, 600, True)]
     probs, mats = [], []
