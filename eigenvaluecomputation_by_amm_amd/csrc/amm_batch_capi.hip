@@ -132,7 +132,8 @@ struct amm_batch {
   // mix of small matrices, whose second contraction finds columns in cache)
   bool nt_loads = false;
   bool union_fused = false;  // the union pass as one workgroup per problem (k_bunion)
-  bool gram = false;         // one-trial batch: exact steps in one read of A (k_bgram)
+  bool gram = false;         // one-trial batch: products in one read of A (k_bgram)
+  bool gram_all = false;     // every problem's products are k_bgram's (no contraction launches)
   int gram_Ls[2] = {0, 0};   // k_bgram's loads per lane for the FP32 / BF16 group
   amm::BGroup g_gram[2]{};
   // grouped setup: descriptors (device) and their launch groups
@@ -238,9 +239,11 @@ int upload_plan(amm_batch* b, int kernel) {
     b->gram_Ls[p.dtype == 1 ? 1 : 0] = std::max(b->gram_Ls[p.dtype == 1 ? 1 : 0], amm::gram_L(p.dtype, p.d));
     gram_tot += (double)p.n * (double)p.d;
   }
+  b->gram_all = b->gram;
   for (BProb& p : b->hp) {
     const double want = std::round(kGramItems * (double)p.n * (double)p.d / std::max(gram_tot, 1.0));
     p.gram_chunks = gram_ok(p) ? (uint32_t)std::min<double>((double)p.n, std::max(1.0, want)) : 0u;
+    b->gram_all = b->gram_all && p.gram_chunks != 0;
   }
   for (uint32_t i = 0; i < np; ++i) {
     BProb& p = b->hp[i];
@@ -527,6 +530,7 @@ int enq_setup(amm_batch* b, int K) {
 int enq_iteration(amm_batch* b, const amm::BRun& r, int K, uint32_t it, uint32_t kind) {
   const BProb* P = b->dP;
   hipStream_t s = b->stream;
+  const bool exact_kind = kind == amm::kModeExactMemo || kind == amm::kModeExactNoMemo;
   if (K != AMM_KERNEL_EXACT) BHIPCHK(amm::launch_bsample(P, b->g_sample, r, it, s));
   BHIPCHK(amm::launch_bunion(P, b->g_prep, r, b->union_fused, s));
   const uint32_t prop = (K == AMM_KERNEL_PROPOSED || K == AMM_KERNEL_PROPOSED_V2) ? 1u : 0u;
@@ -534,13 +538,14 @@ int enq_iteration(amm_batch* b, const amm::BRun& r, int K, uint32_t it, uint32_t
   // contractions and the weight pass leave those problems alone)
   if (b->gram)
     for (int dt = 0; dt < 2; ++dt)
-      BHIPCHK(amm::launch_bgram(dt, b->gram_Ls[dt], kind == amm::kModeExactMemo || kind == amm::kModeExactNoMemo, P,
-                                b->g_gram[dt], b->np, s));
-  for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bdot(dt, b->nt_loads, P, b->g_dot[dt], b->np, s, b->gram));
-  BHIPCHK(amm::launch_bweight(P, b->g_weight, r, prop, 0u, s, b->gram));
-  for (int dt = 0; dt < 2; ++dt)
-    for (int pa = 0; pa < 2; ++pa)
-      BHIPCHK(amm::launch_baxpy(dt, (uint32_t)pa + 1, b->nt_loads, P, b->g_axpy[dt][pa], b->np, prop, s, b->gram));
+      BHIPCHK(amm::launch_bgram(dt, b->gram_Ls[dt], exact_kind, P, b->g_gram[dt], b->np, s));
+  if (!b->gram_all) {
+    for (int dt = 0; dt < 2; ++dt) BHIPCHK(amm::launch_bdot(dt, b->nt_loads, P, b->g_dot[dt], b->np, s, b->gram));
+    BHIPCHK(amm::launch_bweight(P, b->g_weight, r, prop, 0u, s, b->gram));
+    for (int dt = 0; dt < 2; ++dt)
+      for (int pa = 0; pa < 2; ++pa)
+        BHIPCHK(amm::launch_baxpy(dt, (uint32_t)pa + 1, b->nt_loads, P, b->g_axpy[dt][pa], b->np, prop, s, b->gram));
+  }
   BHIPCHK(amm::launch_bfinish(P, b->g_fin, r, b->fin_groups, s));
   BHIPCHK(amm::launch_btail(P, r, amm::kBTailFinish, it, s));
   BHIPCHK(amm::launch_bnorm(P, b->g_vec, r, s));
