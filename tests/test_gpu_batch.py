@@ -279,12 +279,14 @@ def test_one_trial_products_in_one_read(kernel, extra):
     8 KB or more in one read of A (k_bgram: exact and sampled steps, the
     weights formed per column), the others through the two contractions:
     uneven FP32 and BF16 problems, d from 40 to 9000 rows (up to eight
-    16-byte loads per lane of a register-held column), each trajectory
-    against the single-matrix run with the same seed and trial word (step
-    kinds identical, Rayleigh quotients within 1e-5, BF16 1e-4)."""
+    16-byte loads per lane of a register-held column; ragged d included),
+    each trajectory against the single-matrix run with the same seed and
+    trial word (step kinds identical, Rayleigh quotients within 1e-5, BF16
+    1e-4)."""
     banner(f"one-trial batch, products in one read: {kernel} {extra}")
     AmmBatch, _, RunParams, _ = _api()
-    shapes = [(40, 700, False), (1000, 3000, True), (2050, 1500, False), (5000, 900, True), (9000, 600, True)]
+    shapes = [(40, 700, False), (1000, 3000, True), (2050, 1500, False), (5000, 900, True), (9000, 600, True),
+              (4100, 700, True), (2049, 500, False)]  # the last two: d not a multiple of a 16-byte load
     probs, mats = [], []
     for i, (d, n, bf) in enumerate(shapes):
         A = lowrank_plus_noise(d, n, seed=500 + i, noise=0.05) + 0.05
