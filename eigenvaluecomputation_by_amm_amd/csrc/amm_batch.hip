@@ -1226,7 +1226,11 @@ __global__ __launch_bounds__(256 * G) void k_bfinish(const BProb* __restrict__ P
   const uint32_t local = blockIdx.x - off[p];
   const uint32_t nblk = off[p + 1] - off[p];
   const uint32_t grp = threadIdx.x >> 8, lt = threadIdx.x & 255;
-  const uint32_t k = lt >> 4;
+  // lane group lt >> 4 takes trial lt >> 4 of a slab, or, with one trial,
+  // trial 0 of the pass's slab lt >> 4 (16 slabs per pass)
+  const bool one = r.ntr == 1;
+  const uint32_t k = one ? 0u : lt >> 4;
+  const uint32_t sub = one ? lt >> 4 : 0u, spw = one ? 16u : 1u;
   const bool live = k < r.ntr;
   const uint32_t md = live ? pr.bt[k].mode : kModeExactNoMemo;
   const bool memo_base = is_amm(md);
@@ -1250,17 +1254,19 @@ __global__ __launch_bounds__(256 * G) void k_bfinish(const BProb* __restrict__ P
     sx = pr.bt[k].s;
     qx = pr.bt[k].q;
   }
-  for (uint32_t blk = local; blk < nvblk; blk += nblk) {
+  for (uint32_t b0 = local * spw; b0 < nvblk; b0 += nblk * spw) {
+    const uint32_t blk = b0 + sub;
+    const bool in = blk < nvblk;  // uniform over the lane group
     const uint32_t row = blk * kBVecRows + (lt & 15) * 4;
     const uint64_t o = (uint64_t)k * pr.dstage + row;
     // loads that do not depend on the partials first
     float4 ym4 = make_float4(0.f, 0.f, 0.f, 0.f), x4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (grp == 0 && live) {
+    if (grp == 0 && live && in) {
       if (memo_base) ym4 = *reinterpret_cast<const float4*>(pr.YM + o);
       x4 = *reinterpret_cast<const float4*>(pr.X + o);
     }
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-    if (live) {
+    if (live && in) {
       // batches of 4 chunk partials, loads first (a loop with a bound per
       // element gets one load in flight at a time); sums in chunk order
       uint32_t ch = c0;
@@ -1310,7 +1316,9 @@ __global__ __launch_bounds__(256 * G) void k_bfinish(const BProb* __restrict__ P
       if (grp > 0) continue;
     }
     double yx = 0.0, yy = 0.0, yw = 0.0;
-    if (!live) {  // unused trial slot: zeros, no reads
+    if (!in) {
+      // past the problem's last slab (one trial): nothing to write
+    } else if (!live) {  // unused trial slot: zeros, no reads
       *reinterpret_cast<float4*>(pr.Y + o) = make_float4(0.f, 0.f, 0.f, 0.f);
     } else {
       const float ymv[4] = {ym4.x, ym4.y, ym4.z, ym4.w};
@@ -1336,7 +1344,7 @@ __global__ __launch_bounds__(256 * G) void k_bfinish(const BProb* __restrict__ P
     yx = sum16(yx);
     yy = sum16(yy);
     yw = sum16(yw);
-    if ((lt & 15) == 0) {
+    if ((lt & 15) == 0 && in) {
       double* b = pr.BP + ((uint64_t)blk * kBT + k) * 3;
       b[0] = yx;
       b[1] = yy;

@@ -353,8 +353,10 @@ int upload_plan(amm_batch* b, int kernel) {
     max_chunks = std::max(max_chunks, std::max(p.chunks, p.gram_chunks));
   }
   b->fin_groups = max_chunks > 8 ? 4u : 1u;
-  const uint64_t f_fin = std::max<uint64_t>(1, cdiv(tot_vblk, kFinishSlabs / b->fin_groups));
-  ok &= add_group(b->g_fin, [f_fin](const BProb& p) { return (uint32_t)cdiv(p.nvblk, f_fin); });
+  // one trial: a workgroup pass takes 16 slabs (its 16 lane groups)
+  const uint64_t spw = b->ntr == 1 ? 16u : 1u;
+  const uint64_t f_fin = std::max<uint64_t>(1, cdiv(cdiv(tot_vblk, spw), kFinishSlabs / b->fin_groups));
+  ok &= add_group(b->g_fin, [f_fin, spw](const BProb& p) { return (uint32_t)cdiv(cdiv(p.nvblk, spw), f_fin); });
   ok &= add_group(b->g_weight, [f_weight](const BProb& p) { return (uint32_t)cdiv(p.ntiles, f_weight); });
   for (int dt = 0; dt < 2; ++dt)
     ok &= add_group(
