@@ -160,8 +160,14 @@ __global__ __launch_bounds__(256) void k_bnorm(const BProb* __restrict__ P, cons
                                                BRun r) {
   const uint32_t p = find_problem(off, r.np, blockIdx.x);
   const BProb& pr = P[p];
-  const uint32_t blk = blockIdx.x - off[p];
-  const uint32_t k = threadIdx.x >> 4;
+  // lane group threadIdx.x >> 4 takes trial threadIdx.x >> 4 of the
+  // workgroup's slab, or, with one trial, trial 0 of slab 16 local +
+  // (threadIdx.x >> 4); a group past the last slab leaves (no barrier here)
+  const uint32_t local = blockIdx.x - off[p];
+  const bool one = r.ntr == 1;
+  const uint32_t k = one ? 0u : threadIdx.x >> 4;
+  const uint32_t blk = one ? local * 16u + (threadIdx.x >> 4) : local;
+  if (blk >= pr.nvblk) return;
   const BTrial& t = pr.bt[k];
   const double scale = k < r.ntr ? t.scale : 0.0;
   const uint32_t md = k < r.ntr ? t.mode : kModeExactNoMemo;

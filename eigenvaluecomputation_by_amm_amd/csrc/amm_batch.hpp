@@ -184,6 +184,7 @@ hipError_t launch_bzv(const BSetup* S, BGroup g, uint32_t np, hipStream_t s);
 
 hipError_t launch_bbegin(const BProb* P, BGroup g, BRun r, hipStream_t s);
 hipError_t launch_btail(const BProb* P, BRun r, uint32_t op, uint32_t it, hipStream_t s);
+// items: nvblk per problem, or ceil(nvblk / 16) with one trial (16 slabs each)
 hipError_t launch_bnorm(const BProb* P, BGroup g, BRun r, hipStream_t s);
 // items: ntr per problem (one workgroup per trial)
 hipError_t launch_bsample(const BProb* P, BGroup g, BRun r, uint32_t it, hipStream_t s);

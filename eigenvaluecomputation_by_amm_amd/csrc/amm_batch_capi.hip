@@ -143,7 +143,7 @@ struct amm_batch {
   // grouped-launch item offsets
   uint32_t* offs = nullptr;
   size_t offs_cap = 0;
-  amm::BGroup g_vec{}, g_fin{}, g_sample{}, g_prep{}, g_weight{}, g_dot[2]{}, g_axpy[2][2]{};
+  amm::BGroup g_vec{}, g_norm{}, g_fin{}, g_sample{}, g_prep{}, g_weight{}, g_dot[2]{}, g_axpy[2][2]{};
   uint32_t fin_groups = 1;  // wave groups of the finish (4 where a problem has many chunks)
   bool plan_valid = false;
   int plan_kernel = -1;
@@ -330,6 +330,7 @@ int upload_plan(amm_batch* b, int kernel) {
   ok &= add_group(b->g_tiles, [](const BProb& p) { return (uint32_t)amm::cdf_tiles(p.n); });
   ok &= add_group(b->g_zv, [](const BProb& p) { return (uint32_t)cdiv(rup(p.d, 8), 256); });
   ok &= add_group(b->g_vec, [](const BProb& p) { return p.nvblk; });
+  ok &= add_group(b->g_norm, [&](const BProb& p) { return b->ntr == 1 ? (p.nvblk + 15) / 16 : p.nvblk; });
   // Sampler: a workgroup per (problem, trial), drawing all c slots
   ok &= add_group(b->g_sample, [&](const BProb&) { return kernel == AMM_KERNEL_EXACT ? 0u : b->ntr; });
   // Compaction: a wave per tile; weights: a lane quad per column (the
@@ -550,7 +551,7 @@ int enq_iteration(amm_batch* b, const amm::BRun& r, int K, uint32_t it, uint32_t
   }
   BHIPCHK(amm::launch_bfinish(P, b->g_fin, r, b->fin_groups, s));
   BHIPCHK(amm::launch_btail(P, r, amm::kBTailFinish, it, s));
-  BHIPCHK(amm::launch_bnorm(P, b->g_vec, r, s));
+  BHIPCHK(amm::launch_bnorm(P, b->g_norm, r, s));
   return AMM_OK;
 }
 
@@ -563,7 +564,7 @@ int enqueue_run(amm_batch* b, const amm_params& p, bool setup) {
     if ((rc = enq_setup(b, K))) return rc;
   // x0 and its block sums came with the prologue (the setup leaves Y and BP alone)
   BHIPCHK(amm::launch_btail(b->dP, r, amm::kBTailScale, 0, b->stream));
-  BHIPCHK(amm::launch_bnorm(b->dP, b->g_vec, r, b->stream));
+  BHIPCHK(amm::launch_bnorm(b->dP, b->g_norm, r, b->stream));
   const std::vector<uint32_t> kinds = is_adaptive(p) ? std::vector<uint32_t>(p.step, 0u) : static_modes(p);
   for (uint32_t it = 0; it < (uint32_t)p.step; ++it)
     if ((rc = enq_iteration(b, r, K, it, kinds[it]))) return rc;
@@ -1095,7 +1096,7 @@ int amm_batch_sigma_max(amm_batch* b, const amm_params* p, double* sigma_host, f
   }
   // t = A^T x for every trial: one exact half-step over the dense union
   hipError_t e = amm::launch_bset_mode(b->dP, b->np, amm::kModeExactNoMemo, b->stream);
-  if (e == hipSuccess) e = amm::launch_bnorm(b->dP, b->g_vec, r, b->stream);
+  if (e == hipSuccess) e = amm::launch_bnorm(b->dP, b->g_norm, r, b->stream);
   if (e == hipSuccess) e = amm::launch_bunion(b->dP, b->g_prep, r, b->union_fused, b->stream);
   for (int dt = 0; dt < 2 && e == hipSuccess; ++dt) e = amm::launch_bdot(dt, b->nt_loads, b->dP, b->g_dot[dt], b->np, b->stream);
   if (e == hipSuccess) e = amm::launch_bweight(b->dP, b->g_weight, r, 0u, 0u, b->stream);
