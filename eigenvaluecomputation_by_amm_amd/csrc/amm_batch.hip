@@ -1241,8 +1241,8 @@ __global__ __launch_bounds__(256 * G) void k_bfinish(const BProb* __restrict__ P
   const uint32_t md = live ? pr.bt[k].mode : kModeExactNoMemo;
   const bool memo_base = is_amm(md);
   const bool prop = md == kModeAmmProposed;
-  // chunk partials to add: k_bgram's column chunks on its steps, else the
-  // second contraction's active chunks
+  // chunk partials to add: k_bgram's chunks for the problems it takes, else
+  // the second contraction's active chunks
   const uint32_t active = r.gram && gram_step(pr) ? pr.gram_chunks : union_plan(pr, pr.tpre[pr.ntiles]).active;
   const uint32_t c0 = (uint32_t)(((uint64_t)active * grp) / G);
   const uint32_t c1 = (uint32_t)(((uint64_t)active * (grp + 1)) / G);
